@@ -1,0 +1,326 @@
+"""ctypes binding of the C ABI in ``include/epseon_hip.h`` (libepseon_hip.so).
+
+This is the thin Python-side binding a maintainer would add next to the
+reference's pybind11 module (``cpp/gpu/source/epseon/gpu/python/api.cpp``);
+the pybind11 module ``_libepseon_gpu`` in this package wraps the same library
+for the reference's builder/handle API.  Arrays cross the boundary as numpy
+buffers (host memory); device memory stays inside the library's arena.
+
+The library is loaded from this package's ``lib/`` directory.  If it is
+missing the import fails with an error naming the build step -- there is no
+fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Sequence
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libepseon_hip.so"
+
+EPSEON_OK = 0
+EPSEON_ERR_INVALID = 1
+EPSEON_ERR_DEVICE = 2
+EPSEON_ERR_STATE = 3
+EPSEON_ERR_MEMORY = 4
+
+LEVEL_OK = 0
+LEVEL_UNBOUND = 1
+LEVEL_MAXITER = 2
+LEVEL_UNRESOLVED = 3
+
+POT_OK = 0
+POT_UNRESOLVED = 1
+
+
+class EpseonError(RuntimeError):
+    """Error reported by libepseon_hip (carries the C status code)."""
+
+    def __init__(self, code: int, message: str) -> None:
+        super().__init__(message)
+        self.code = code
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 256),
+        ("arch", ctypes.c_char * 64),
+        ("runtime_version", ctypes.c_int32),
+        ("driver_version", ctypes.c_int32),
+        ("vendor_id", ctypes.c_int32),
+        ("pci_device_id", ctypes.c_int32),
+        ("compute_units", ctypes.c_int32),
+        ("max_threads_per_block", ctypes.c_int32),
+        ("max_block_dim", ctypes.c_int32 * 3),
+        ("max_grid_dim", ctypes.c_int32 * 3),
+        ("shared_mem_per_block", ctypes.c_int64),
+        ("shared_mem_per_cu", ctypes.c_int64),
+        ("total_global_mem", ctypes.c_int64),
+        ("clock_khz", ctypes.c_int32),
+        ("warp_size", ctypes.c_int32),
+        ("uuid", ctypes.c_uint8 * 16),
+    ]
+
+
+class PotentialDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_points", ctypes.c_int64),
+        ("n_energies", ctypes.c_int64),
+        ("r_min", ctypes.c_double),
+        ("r_max", ctypes.c_double),
+        ("reduced_mass", ctypes.c_double),
+        ("e_top", ctypes.c_double),
+        ("level_min", ctypes.c_int32),
+        ("level_count", ctypes.c_int32),
+    ]
+
+
+class PotentialInfo(ctypes.Structure):
+    _fields_ = [
+        ("beta", ctypes.c_double),
+        ("h", ctypes.c_double),
+        ("e_lo", ctypes.c_double),
+        ("e_hi", ctypes.c_double),
+        ("stride", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("scan_ms", ctypes.c_double),
+        ("refine_ms", ctypes.c_double),
+        ("normalize_ms", ctypes.c_double),
+        ("scan_point_updates", ctypes.c_double),
+        ("refine_point_updates", ctypes.c_double),
+        ("normalize_point_updates", ctypes.c_double),
+        ("scan_launches", ctypes.c_int64),
+        ("levels_converged", ctypes.c_int64),
+    ]
+
+
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_int32_p = ctypes.POINTER(ctypes.c_int32)
+_c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+# Every entry point of include/epseon_hip.h with its argument types.
+SIGNATURES: dict[str, tuple[object, list[object]]] = {
+    "epseon_last_error": (ctypes.c_char_p, []),
+    "epseon_device_count": (ctypes.c_int, [_c_int32_p]),
+    "epseon_device_info_get": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(DeviceInfo)]),
+    "epseon_ctx_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "epseon_ctx_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "epseon_load_potentials": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(PotentialDesc), _c_double_p, _c_int64_p],
+    ),
+    "epseon_potential_info_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(PotentialInfo)]),
+    "epseon_scan": (ctypes.c_int, [ctypes.c_void_p]),
+    "epseon_refine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
+    "epseon_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
+    "epseon_ctx_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "epseon_get_levels": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_int32_p, _c_int32_p]),
+    "epseon_get_scan": (ctypes.c_int, [ctypes.c_void_p, _c_int32_p, _c_double_p, _c_int32_p]),
+    "epseon_get_sizes": (ctypes.c_int, [ctypes.c_void_p, _c_int32_p, _c_int64_p, _c_int64_p, _c_int64_p]),
+    "epseon_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
+}
+
+_LIB: ctypes.CDLL | None = None
+
+
+def _share_runtime_with_torch() -> None:
+    """Load torch's HIP runtime first when torch is installed.
+
+    torch ships its own libamdhip64.so with the same SONAME as ROCm's.  If
+    this library were loaded first, a later ``import torch`` would map a
+    second HIP runtime into the process; importing torch first makes both
+    resolve to one runtime.
+    """
+    if os.environ.get("EPSEON_NO_TORCH_PRELOAD"):
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
+def load_library() -> ctypes.CDLL:
+    """Load libepseon_hip.so (once) and declare every C entry point."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        msg = (
+            f"{LIB_PATH} is missing: build the HIP library first "
+            "(python -m epseon_backend_amd._build or __graft_entry__.build())"
+        )
+        raise ImportError(msg)
+    _share_runtime_with_torch()
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (restype, argtypes) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _LIB = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != EPSEON_OK:
+        lib = load_library()
+        raise EpseonError(rc, lib.epseon_last_error().decode(errors="replace"))
+
+
+def device_count() -> int:
+    lib = load_library()
+    n = ctypes.c_int32(0)
+    _check(lib.epseon_device_count(ctypes.byref(n)))
+    return int(n.value)
+
+
+def device_info(device: int) -> DeviceInfo:
+    lib = load_library()
+    info = DeviceInfo()
+    _check(lib.epseon_device_info_get(device, ctypes.byref(info)))
+    return info
+
+
+@dataclass
+class Potential:
+    """One potential of a batch: its V table on the uniform grid and options."""
+
+    values: np.ndarray
+    r_min: float
+    r_max: float
+    reduced_mass: float
+    e_top: float
+    n_energies: int
+    level_min: int = 0
+    level_count: int = 1
+
+
+def _as_f64(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One device context (stream + arena) of libepseon_hip."""
+
+    def __init__(self, device: int = 0, block_bytes: int = 0) -> None:
+        self._lib = load_library()
+        handle = ctypes.c_void_p()
+        _check(self._lib.epseon_ctx_create(device, block_bytes, ctypes.byref(handle)))
+        self._h = handle
+        self.n_potentials = 0
+        self.level_counts: list[int] = []
+        self.energy_counts: list[int] = []
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.epseon_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self) -> Context:
+        return self
+
+    def __exit__(self, *exc: object) -> None:
+        self.close()
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def load(self, potentials: Sequence[Potential]) -> None:
+        """Upload a batch (epseon_load_potentials)."""
+        n = len(potentials)
+        descs = (PotentialDesc * max(n, 1))()
+        offsets = np.zeros(max(n, 1), dtype=np.int64)
+        total = 0
+        for i, p in enumerate(potentials):
+            offsets[i] = total
+            total += len(p.values)
+        v = np.zeros(max(total, 1), dtype=np.float64)
+        for i, p in enumerate(potentials):
+            vals = _as_f64(p.values)
+            v[offsets[i] : offsets[i] + len(vals)] = vals
+            d = descs[i]
+            d.n_points = len(vals)
+            d.n_energies = int(p.n_energies)
+            d.r_min = float(p.r_min)
+            d.r_max = float(p.r_max)
+            d.reduced_mass = float(p.reduced_mass)
+            d.e_top = float(p.e_top)
+            d.level_min = int(p.level_min)
+            d.level_count = int(p.level_count)
+        _check(
+            self._lib.epseon_load_potentials(
+                self._h,
+                n,
+                descs,
+                v.ctypes.data_as(_c_double_p),
+                offsets.ctypes.data_as(_c_int64_p),
+            )
+        )
+        self.n_potentials = n
+        self.level_counts = [int(p.level_count) for p in potentials]
+        self.energy_counts = [int(p.n_energies) for p in potentials]
+
+    def potential_info(self, p: int) -> PotentialInfo:
+        info = PotentialInfo()
+        _check(self._lib.epseon_potential_info_get(self._h, p, ctypes.byref(info)))
+        return info
+
+    def scan(self) -> None:
+        _check(self._lib.epseon_scan(self._h))
+
+    def refine(self, rel_tol: float = 1e-10, max_iter: int = 200) -> None:
+        _check(self._lib.epseon_refine(self._h, rel_tol, max_iter))
+
+    def solve(self, rel_tol: float = 1e-10, max_iter: int = 200) -> None:
+        _check(self._lib.epseon_solve(self._h, rel_tol, max_iter))
+
+    def synchronize(self) -> None:
+        _check(self._lib.epseon_ctx_synchronize(self._h))
+
+    def levels(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(levels, status, iterations), concatenated in potential order."""
+        n = sum(self.level_counts)
+        lv = np.empty(max(n, 1), dtype=np.float64)
+        st = np.empty(max(n, 1), dtype=np.int32)
+        it = np.empty(max(n, 1), dtype=np.int32)
+        _check(
+            self._lib.epseon_get_levels(
+                self._h,
+                lv.ctypes.data_as(_c_double_p),
+                st.ctypes.data_as(_c_int32_p),
+                it.ctypes.data_as(_c_int32_p),
+            )
+        )
+        return lv[:n], st[:n], it[:n]
+
+    def scan_results(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(node counts, far-end values, rescale counts) in potential order."""
+        n = sum(self.energy_counts)
+        nodes = np.empty(max(n, 1), dtype=np.int32)
+        far = np.empty(max(n, 1), dtype=np.float64)
+        resc = np.empty(max(n, 1), dtype=np.int32)
+        _check(
+            self._lib.epseon_get_scan(
+                self._h,
+                nodes.ctypes.data_as(_c_int32_p),
+                far.ctypes.data_as(_c_double_p),
+                resc.ctypes.data_as(_c_int32_p),
+            )
+        )
+        return nodes[:n], far[:n], resc[:n]
+
+    def stats(self) -> Stats:
+        st = Stats()
+        _check(self._lib.epseon_get_stats(self._h, ctypes.byref(st)))
+        return st

@@ -1,0 +1,644 @@
+// epseon_capi.hip -- host side of libepseon_hip.so: contexts, device arena,
+// batch layout and the extern "C" entry points declared in include/epseon_hip.h.
+//
+// Stands in for what VibwaAlgorithm<FP>::run sets up per task in the
+// reference (logical device, VMA allocator, per-slot buffers and descriptor
+// sets: cpp/gpu/include/epseon/gpu/algorithms/vibwa.hpp:605-637) and for the
+// dispatch that never happens there.  No Vulkan objects, no descriptors: the
+// kernels take raw device pointers into one arena plus a table of
+// PotentialRecord offsets.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "epseon_hip.h"
+#include "records.hpp"
+
+namespace epseon::hip {
+
+hipError_t launch_shift_potential(hipStream_t, const double*, const PotentialRecord*, int32_t,
+                                  int32_t, double*);
+hipError_t launch_scan(hipStream_t, const double*, const PotentialRecord*, const WaveTask*, int32_t,
+                       int32_t, int32_t*, double*, int32_t*);
+hipError_t launch_refine(hipStream_t, const double*, const PotentialRecord*, const WaveTask*,
+                         int32_t, int32_t, const int32_t*, const double*, const int32_t*, double,
+                         int32_t, double*, int32_t*, int32_t*);
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(EPSEON_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define EPSEON_HIP_TRY(expr)                                 \
+    do {                                                     \
+        const hipError_t epseon_e_ = (expr);                 \
+        if (epseon_e_ != hipSuccess)                         \
+            return hip_fail(epseon_e_, #expr);               \
+    } while (0)
+
+// Device arena: hipMalloc'd blocks of a fixed size carved by a bump pointer
+// (256-byte aligned); requests larger than a block get a block of their own.
+// reset() keeps the blocks for the next batch.
+class Arena {
+  public:
+    explicit Arena(size_t block_bytes) : block_bytes_(block_bytes) {}
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+    ~Arena() { release(); }
+
+    hipError_t alloc(size_t bytes, void** out)
+    {
+        bytes = (bytes + 255) & ~size_t(255);
+        if (bytes == 0)
+            bytes = 256;
+        for (size_t b = cursor_block_; b < blocks_.size(); ++b) {
+            Block& blk = blocks_[b];
+            if (blk.size - blk.used >= bytes) {
+                *out = static_cast<char*>(blk.ptr) + blk.used;
+                blk.used += bytes;
+                cursor_block_ = b;
+                return hipSuccess;
+            }
+        }
+        Block blk{nullptr, std::max(bytes, block_bytes_), 0};
+        const hipError_t e = hipMalloc(&blk.ptr, blk.size);
+        if (e != hipSuccess)
+            return e;
+        blk.used = bytes;
+        blocks_.push_back(blk);
+        cursor_block_ = blocks_.size() - 1;
+        *out = blk.ptr;
+        return hipSuccess;
+    }
+
+    void reset()
+    {
+        for (Block& b : blocks_)
+            b.used = 0;
+        cursor_block_ = 0;
+    }
+
+    void release()
+    {
+        for (Block& b : blocks_)
+            (void)hipFree(b.ptr);
+        blocks_.clear();
+        cursor_block_ = 0;
+    }
+
+    size_t reserved_bytes() const
+    {
+        size_t s = 0;
+        for (const Block& b : blocks_)
+            s += b.size;
+        return s;
+    }
+
+  private:
+    struct Block {
+        void* ptr;
+        size_t size;
+        size_t used;
+    };
+    size_t block_bytes_;
+    std::vector<Block> blocks_;
+    size_t cursor_block_ = 0;
+};
+
+template <typename T>
+hipError_t arena_array(Arena& a, size_t count, T** out)
+{
+    void* p = nullptr;
+    const hipError_t e = a.alloc(count * sizeof(T), &p);
+    *out = static_cast<T*>(p);
+    return e;
+}
+
+// Node-sampling stride: the largest power of two B <= 32 with B * theta_max
+// <= pi/2, theta_max being the discrete phase advance per step at the
+// highest kinetic energy of the window.
+int32_t node_stride(double beta, double e_hi, double v_min)
+{
+    const double f = 1.0 + beta * (e_hi - v_min);
+    const double u = 12.0 / f - 10.0;
+    if (!(u > -2.0))
+        return 1;
+    if (u >= 2.0)
+        return kMaxStride;
+    const double theta = std::acos(0.5 * u);
+    const double bmax = std::floor(1.5707963267948966 / theta);
+    int32_t b = 1;
+    while (b * 2 <= kMaxStride && static_cast<double>(b * 2) <= bmax)
+        b *= 2;
+    return b;
+}
+
+}  // namespace
+}  // namespace epseon::hip
+
+using namespace epseon::hip;
+
+struct epseon_ctx {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    Arena arena;
+    int32_t waves_per_block = 4;
+
+    std::vector<epseon_potential_desc> desc;
+    std::vector<PotentialRecord> rec;
+    int64_t total_points = 0, total_energies = 0, total_levels = 0;
+    int32_t max_points = 0;
+
+    double* d_v = nullptr;
+    double* d_ut = nullptr;
+    PotentialRecord* d_rec = nullptr;
+    WaveTask* d_scan_tasks = nullptr;
+    WaveTask* d_refine_tasks = nullptr;
+    int32_t n_scan_tasks = 0, n_refine_tasks = 0;
+    int32_t* d_nodes = nullptr;
+    double* d_far = nullptr;
+    int32_t* d_resc = nullptr;
+    double* d_levels = nullptr;
+    int32_t* d_status = nullptr;
+    int32_t* d_iters = nullptr;
+
+    bool loaded = false, scanned = false, refined = false;
+    bool scan_timed = false, refine_timed = false;
+    double scan_point_updates = 0.0;
+
+    explicit epseon_ctx(size_t block_bytes) : arena(block_bytes) {}
+};
+
+namespace {
+
+int set_device(epseon_ctx* ctx)
+{
+    EPSEON_HIP_TRY(hipSetDevice(ctx->device));
+    return EPSEON_OK;
+}
+
+int check_ctx(epseon_ctx* ctx)
+{
+    if (ctx == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null context");
+    return set_device(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* epseon_last_error(void) { return g_last_error.c_str(); }
+
+int epseon_device_count(int32_t* count)
+{
+    if (count == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null count pointer");
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    if (e == hipErrorNoDevice) {
+        *count = 0;
+        return EPSEON_OK;
+    }
+    if (e != hipSuccess) {
+        *count = 0;
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    *count = n;
+    return EPSEON_OK;
+}
+
+int epseon_device_info_get(int32_t device, epseon_device_info* info)
+{
+    if (info == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null info pointer");
+    int32_t n = 0;
+    if (const int rc = epseon_device_count(&n); rc != EPSEON_OK)
+        return rc;
+    if (device < 0 || device >= n)
+        return fail(EPSEON_ERR_DEVICE, "Device not available.");
+    hipDeviceProp_t p;
+    EPSEON_HIP_TRY(hipGetDeviceProperties(&p, device));
+    std::memset(info, 0, sizeof *info);
+    std::snprintf(info->name, sizeof info->name, "%s", p.name);
+    std::snprintf(info->arch, sizeof info->arch, "%s", p.gcnArchName);
+    int rv = 0, dv = 0;
+    (void)hipRuntimeGetVersion(&rv);
+    (void)hipDriverGetVersion(&dv);
+    info->runtime_version = rv;
+    info->driver_version = dv;
+    info->vendor_id = 0x1002;
+    info->pci_device_id = p.pciDeviceID;
+    info->compute_units = p.multiProcessorCount;
+    info->max_threads_per_block = p.maxThreadsPerBlock;
+    for (int i = 0; i < 3; ++i) {
+        info->max_block_dim[i] = p.maxThreadsDim[i];
+        info->max_grid_dim[i] = p.maxGridSize[i];
+    }
+    info->shared_mem_per_block = static_cast<int64_t>(p.sharedMemPerBlock);
+    info->shared_mem_per_cu = static_cast<int64_t>(p.maxSharedMemoryPerMultiProcessor);
+    info->total_global_mem = static_cast<int64_t>(p.totalGlobalMem);
+    info->clock_khz = p.clockRate;
+    info->warp_size = p.warpSize;
+    std::memcpy(info->uuid, p.uuid.bytes, 16);
+    return EPSEON_OK;
+}
+
+int epseon_ctx_create(int32_t device, uint64_t block_bytes, epseon_ctx** out)
+{
+    if (out == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    int32_t n = 0;
+    if (const int rc = epseon_device_count(&n); rc != EPSEON_OK)
+        return rc;
+    if (device < 0 || device >= n)
+        return fail(EPSEON_ERR_DEVICE, "Device not available.");
+    const size_t bb = block_bytes ? static_cast<size_t>(block_bytes) : (size_t(64) << 20);
+    epseon_ctx* ctx = new (std::nothrow) epseon_ctx(bb);
+    if (ctx == nullptr)
+        return fail(EPSEON_ERR_MEMORY, "host allocation failed");
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    for (int i = 0; i < 6 && e == hipSuccess; ++i)
+        e = hipEventCreate(&ctx->ev[i]);
+    if (e != hipSuccess) {
+        const int rc = hip_fail(e, "context creation");
+        epseon_ctx_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return EPSEON_OK;
+}
+
+int epseon_ctx_destroy(epseon_ctx* ctx)
+{
+    if (ctx == nullptr)
+        return EPSEON_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream != nullptr)
+        (void)hipStreamSynchronize(ctx->stream);
+    ctx->arena.release();
+    for (hipEvent_t& ev : ctx->ev)
+        if (ev != nullptr)
+            (void)hipEventDestroy(ev);
+    if (ctx->stream != nullptr)
+        (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return EPSEON_OK;
+}
+
+int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc,
+                           const double* v, const int64_t* v_offsets)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (count < 0 || (count > 0 && (desc == nullptr || v == nullptr || v_offsets == nullptr)))
+        return fail(EPSEON_ERR_INVALID, "invalid potential batch");
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->loaded = ctx->scanned = ctx->refined = false;
+    ctx->scan_timed = ctx->refine_timed = false;
+    ctx->desc.assign(desc, desc + count);
+    ctx->rec.assign(static_cast<size_t>(count), PotentialRecord{});
+
+    int64_t pts = 0, en = 0, lv = 0;
+    int32_t max_points = 0;
+    double scan_pu = 0.0;
+    for (int32_t p = 0; p < count; ++p) {
+        const epseon_potential_desc& d = desc[p];
+        char msg[256];
+        if (d.n_points < 4 || d.n_points > (int64_t(1) << 30)) {
+            std::snprintf(msg, sizeof msg, "potential %d: point count %lld outside [4, 2^30]", p,
+                          static_cast<long long>(d.n_points));
+            return fail(EPSEON_ERR_INVALID, msg);
+        }
+        if (d.n_energies < 1 || d.n_energies > (int64_t(1) << 30)) {
+            std::snprintf(msg, sizeof msg, "potential %d: trial energy count %lld outside [1, 2^30]",
+                          p, static_cast<long long>(d.n_energies));
+            return fail(EPSEON_ERR_INVALID, msg);
+        }
+        if (!(std::isfinite(d.r_min) && std::isfinite(d.r_max) && d.r_max > d.r_min)) {
+            std::snprintf(msg, sizeof msg, "potential %d: need finite min_r < max_r", p);
+            return fail(EPSEON_ERR_INVALID, msg);
+        }
+        if (!(std::isfinite(d.reduced_mass) && d.reduced_mass > 0.0)) {
+            std::snprintf(msg, sizeof msg, "potential %d: reduced mass must be positive", p);
+            return fail(EPSEON_ERR_INVALID, msg);
+        }
+        if (d.level_min < 0 || d.level_count < 0) {
+            std::snprintf(msg, sizeof msg, "potential %d: negative level range", p);
+            return fail(EPSEON_ERR_INVALID, msg);
+        }
+        const double* vp = v + v_offsets[p];
+        double vmin = vp[0], vmax = vp[0];
+        for (int64_t i = 0; i < d.n_points; ++i) {
+            const double x = vp[i];
+            if (!std::isfinite(x)) {
+                std::snprintf(msg, sizeof msg, "potential %d: V[%lld] is not finite", p,
+                              static_cast<long long>(i));
+                return fail(EPSEON_ERR_INVALID, msg);
+            }
+            vmin = std::min(vmin, x);
+            vmax = std::max(vmax, x);
+        }
+        if (!(std::isfinite(d.e_top) && d.e_top > vmin)) {
+            std::snprintf(msg, sizeof msg,
+                          "potential %d: energy window top %.17g is not above min V %.17g", p,
+                          d.e_top, vmin);
+            return fail(EPSEON_ERR_INVALID, msg);
+        }
+        PotentialRecord& r = ctx->rec[p];
+        r.h = (d.r_max - d.r_min) / static_cast<double>(d.n_points - 1);
+        r.beta = (r.h * r.h * d.reduced_mass) / (12.0 * kHbar2Over2Amu);
+        r.e_lo = vmin;
+        r.e_hi = d.e_top;
+        r.de = (r.e_hi - r.e_lo) / static_cast<double>(d.n_energies + 1);
+        r.abs_tol = 1e-15 * (r.e_hi - r.e_lo);
+        r.n_points = static_cast<int32_t>(d.n_points);
+        r.n_energies = static_cast<int32_t>(d.n_energies);
+        r.level_min = d.level_min;
+        r.level_count = d.level_count;
+        r.stride = node_stride(r.beta, r.e_hi, vmin);
+        const double f_min = 1.0 + r.beta * (r.e_lo - vmax);
+        const double f_max = 1.0 + r.beta * (r.e_hi - vmin);
+        r.flags = (f_min > 0.0 && f_max <= 2.0) ? EPSEON_POT_OK : EPSEON_POT_UNRESOLVED;
+        r.ut_offset = pts;
+        r.energy_offset = en;
+        r.level_offset = lv;
+        // 64-byte aligned rows with 64 points of slack: the shot prefetches up
+        // to 48 points past the last full 32-point super-block.
+        pts += ((d.n_points + 7) & ~int64_t(7)) + 64;
+        en += d.n_energies;
+        lv += d.level_count;
+        max_points = std::max(max_points, r.n_points);
+        if (r.flags == EPSEON_POT_OK)
+            scan_pu += static_cast<double>(d.n_points - 2) * static_cast<double>(d.n_energies);
+    }
+    ctx->total_points = pts;
+    ctx->total_energies = en;
+    ctx->total_levels = lv;
+    ctx->max_points = max_points;
+    ctx->scan_point_updates = scan_pu;
+
+    // Work lists: one wavefront per 64 trial energies (scan) or 64 levels
+    // (refine), longest grids first so the dispatcher drains large items early.
+    std::vector<int32_t> order(static_cast<size_t>(count));
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return ctx->rec[a].n_points > ctx->rec[b].n_points;
+    });
+    std::vector<WaveTask> scan_tasks, refine_tasks;
+    for (int32_t p : order) {
+        const PotentialRecord& r = ctx->rec[p];
+        if (r.flags == EPSEON_POT_OK)
+            for (int32_t j = 0; j < r.n_energies; j += 64)
+                scan_tasks.push_back({p, j});
+        for (int32_t l = 0; l < r.level_count; l += 64)
+            refine_tasks.push_back({p, l});
+    }
+    ctx->n_scan_tasks = static_cast<int32_t>(scan_tasks.size());
+    ctx->n_refine_tasks = static_cast<int32_t>(refine_tasks.size());
+
+    ctx->arena.reset();
+    Arena& a = ctx->arena;
+    const size_t np = static_cast<size_t>(std::max<int64_t>(pts, 1));
+    const size_t ne = static_cast<size_t>(std::max<int64_t>(en, 1));
+    const size_t nl = static_cast<size_t>(std::max<int64_t>(lv, 1));
+    EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_v));
+    EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_ut));
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(count, 1), &ctx->d_rec));
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(scan_tasks.size(), 1), &ctx->d_scan_tasks));
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(refine_tasks.size(), 1), &ctx->d_refine_tasks));
+    EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_nodes));
+    EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_far));
+    EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_resc));
+    EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_levels));
+    EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_status));
+    EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_iters));
+
+    for (int32_t p = 0; p < count; ++p)
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_v + ctx->rec[p].ut_offset, v + v_offsets[p],
+                                      static_cast<size_t>(desc[p].n_points) * sizeof(double),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    if (count > 0)
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_rec, ctx->rec.data(),
+                                      static_cast<size_t>(count) * sizeof(PotentialRecord),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    if (!scan_tasks.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_scan_tasks, scan_tasks.data(),
+                                      scan_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice,
+                                      ctx->stream));
+    if (!refine_tasks.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_refine_tasks, refine_tasks.data(),
+                                      refine_tasks.size() * sizeof(WaveTask),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_nodes, 0, ne * sizeof(int32_t), ctx->stream));
+    EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_far, 0, ne * sizeof(double), ctx->stream));
+    EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_resc, 0, ne * sizeof(int32_t), ctx->stream));
+    EPSEON_HIP_TRY(launch_shift_potential(ctx->stream, ctx->d_v, ctx->d_rec, count, max_points,
+                                          ctx->d_ut));
+    // The host buffers are pageable and owned by the caller: finish the
+    // copies before returning.
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->loaded = true;
+    return EPSEON_OK;
+}
+
+int epseon_potential_info_get(epseon_ctx* ctx, int32_t p, epseon_potential_info* info)
+{
+    if (ctx == nullptr || info == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null argument");
+    if (!ctx->loaded)
+        return fail(EPSEON_ERR_STATE, "no potentials loaded");
+    if (p < 0 || p >= static_cast<int32_t>(ctx->rec.size()))
+        return fail(EPSEON_ERR_INVALID, "potential index out of range");
+    const PotentialRecord& r = ctx->rec[p];
+    info->beta = r.beta;
+    info->h = r.h;
+    info->e_lo = r.e_lo;
+    info->e_hi = r.e_hi;
+    info->stride = r.stride;
+    info->flags = r.flags;
+    return EPSEON_OK;
+}
+
+int epseon_scan(epseon_ctx* ctx)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (!ctx->loaded)
+        return fail(EPSEON_ERR_STATE, "epseon_scan before epseon_load_potentials");
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+    EPSEON_HIP_TRY(launch_scan(ctx->stream, ctx->d_ut, ctx->d_rec, ctx->d_scan_tasks,
+                               ctx->n_scan_tasks, ctx->waves_per_block, ctx->d_nodes, ctx->d_far,
+                               ctx->d_resc));
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+    ctx->scanned = true;
+    ctx->scan_timed = true;
+    ctx->refined = false;
+    return EPSEON_OK;
+}
+
+int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (!ctx->scanned)
+        return fail(EPSEON_ERR_STATE, "epseon_refine before epseon_scan");
+    if (!(rel_tol >= 0.0 && std::isfinite(rel_tol)))
+        return fail(EPSEON_ERR_INVALID, "relative tolerance must be finite and >= 0");
+    if (max_iter < 1)
+        return fail(EPSEON_ERR_INVALID, "max_iter must be >= 1");
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+    EPSEON_HIP_TRY(launch_refine(ctx->stream, ctx->d_ut, ctx->d_rec, ctx->d_refine_tasks,
+                                 ctx->n_refine_tasks, ctx->waves_per_block, ctx->d_nodes,
+                                 ctx->d_far, ctx->d_resc, rel_tol, max_iter, ctx->d_levels,
+                                 ctx->d_status, ctx->d_iters));
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
+    ctx->refined = true;
+    ctx->refine_timed = true;
+    return EPSEON_OK;
+}
+
+int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
+{
+    if (const int rc = epseon_scan(ctx); rc != EPSEON_OK)
+        return rc;
+    return epseon_refine(ctx, rel_tol, max_iter);
+}
+
+int epseon_ctx_synchronize(epseon_ctx* ctx)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return EPSEON_OK;
+}
+
+int epseon_get_levels(epseon_ctx* ctx, double* levels, int32_t* status, int32_t* iterations)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (!ctx->refined)
+        return fail(EPSEON_ERR_STATE, "epseon_get_levels before epseon_refine");
+    const size_t n = static_cast<size_t>(ctx->total_levels);
+    if (n > 0) {
+        if (levels)
+            EPSEON_HIP_TRY(hipMemcpyAsync(levels, ctx->d_levels, n * sizeof(double),
+                                          hipMemcpyDeviceToHost, ctx->stream));
+        if (status)
+            EPSEON_HIP_TRY(hipMemcpyAsync(status, ctx->d_status, n * sizeof(int32_t),
+                                          hipMemcpyDeviceToHost, ctx->stream));
+        if (iterations)
+            EPSEON_HIP_TRY(hipMemcpyAsync(iterations, ctx->d_iters, n * sizeof(int32_t),
+                                          hipMemcpyDeviceToHost, ctx->stream));
+    }
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return EPSEON_OK;
+}
+
+int epseon_get_scan(epseon_ctx* ctx, int32_t* nodes, double* far_end, int32_t* rescales)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (!ctx->scanned)
+        return fail(EPSEON_ERR_STATE, "epseon_get_scan before epseon_scan");
+    const size_t n = static_cast<size_t>(ctx->total_energies);
+    if (n > 0) {
+        if (nodes)
+            EPSEON_HIP_TRY(hipMemcpyAsync(nodes, ctx->d_nodes, n * sizeof(int32_t),
+                                          hipMemcpyDeviceToHost, ctx->stream));
+        if (far_end)
+            EPSEON_HIP_TRY(hipMemcpyAsync(far_end, ctx->d_far, n * sizeof(double),
+                                          hipMemcpyDeviceToHost, ctx->stream));
+        if (rescales)
+            EPSEON_HIP_TRY(hipMemcpyAsync(rescales, ctx->d_resc, n * sizeof(int32_t),
+                                          hipMemcpyDeviceToHost, ctx->stream));
+    }
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return EPSEON_OK;
+}
+
+int epseon_get_sizes(epseon_ctx* ctx, int32_t* potentials, int64_t* points, int64_t* energies,
+                     int64_t* levels)
+{
+    if (ctx == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null context");
+    if (potentials)
+        *potentials = static_cast<int32_t>(ctx->rec.size());
+    if (points) {
+        int64_t s = 0;
+        for (const epseon_potential_desc& d : ctx->desc)
+            s += d.n_points;
+        *points = s;
+    }
+    if (energies)
+        *energies = ctx->total_energies;
+    if (levels)
+        *levels = ctx->total_levels;
+    return EPSEON_OK;
+}
+
+int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (stats == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null stats pointer");
+    std::memset(stats, 0, sizeof *stats);
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    if (ctx->scan_timed) {
+        EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        stats->scan_ms = ms;
+        stats->scan_point_updates = ctx->scan_point_updates;
+        stats->scan_launches = ctx->n_scan_tasks > 0 ? 1 : 0;
+    }
+    if (ctx->refine_timed) {
+        EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+        stats->refine_ms = ms;
+        const size_t n = static_cast<size_t>(ctx->total_levels);
+        std::vector<int32_t> it(n), st(n);
+        if (n > 0) {
+            EPSEON_HIP_TRY(hipMemcpy(it.data(), ctx->d_iters, n * sizeof(int32_t),
+                                     hipMemcpyDeviceToHost));
+            EPSEON_HIP_TRY(hipMemcpy(st.data(), ctx->d_status, n * sizeof(int32_t),
+                                     hipMemcpyDeviceToHost));
+        }
+        double pu = 0.0;
+        int64_t ok = 0;
+        for (size_t p = 0; p < ctx->rec.size(); ++p) {
+            const PotentialRecord& r = ctx->rec[p];
+            for (int32_t l = 0; l < r.level_count; ++l) {
+                const size_t o = static_cast<size_t>(r.level_offset + l);
+                pu += static_cast<double>(it[o]) * static_cast<double>(r.n_points - 2);
+                ok += st[o] == EPSEON_LEVEL_OK;
+            }
+        }
+        stats->refine_point_updates = pu;
+        stats->levels_converged = ok;
+    }
+    return EPSEON_OK;
+}
+
+}  // extern "C"

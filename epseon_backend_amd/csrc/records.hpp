@@ -1,0 +1,49 @@
+// records.hpp -- host/device records shared by the kernels and the C ABI.
+//
+// One PotentialRecord per potential of a batch replaces the per-slot buffer
+// set of the reference (staging + 5 device-only + 1 output buffer per
+// group_size slot, task_configurator/algorithm_config.hpp:173-198): all
+// potentials live in single CSR-style device arrays and the record holds the
+// offsets into them.
+#pragma once
+
+#include <cstdint>
+
+namespace epseon::hip {
+
+// Rescale threshold of the recurrence: when |w| reaches 2^480 the last two
+// values are multiplied by 2^-480 (exact) and the rescale counter grows.
+constexpr double kRescaleAt = 0x1p480;
+constexpr double kRescaleBy = 0x1p-480;
+constexpr int kRescaleLog2 = 480;
+constexpr int kMaxStride = 32;
+// hbar^2 / (2 amu A^2) in cm^-1 (CODATA 2018 h, c and atomic mass unit).
+constexpr double kHbar2Over2Amu = 16.857629191640175;
+
+struct alignas(16) PotentialRecord {
+    int64_t ut_offset;      // first grid point of this potential in the u table
+    int64_t energy_offset;  // first trial energy in the scan arrays
+    int64_t level_offset;   // first level in the level arrays
+    int32_t n_points;       // N
+    int32_t n_energies;     // M
+    int32_t level_min;      // first level v
+    int32_t level_count;    // levels requested
+    int32_t stride;         // node-sampling stride B (power of two <= 32)
+    int32_t flags;          // EPSEON_POT_*
+    double beta;            // h^2 mu / (12 K)
+    double e_lo;            // window floor (min V)
+    double de;              // trial spacing (e_hi - e_lo)/(M + 1)
+    double e_hi;            // window top
+    double abs_tol;         // absolute refinement tolerance
+    double h;               // grid step
+};
+static_assert(sizeof(PotentialRecord) == 96, "record layout");
+
+// Work item of one wavefront: potential index and the first trial energy
+// (scan) or first level (refine) of its 64 lanes.
+struct WaveTask {
+    int32_t potential;
+    int32_t first;
+};
+
+}  // namespace epseon::hip

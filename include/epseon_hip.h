@@ -1,0 +1,190 @@
+/*
+ * epseon_hip.h -- C ABI of the MI355X Numerov-shooting library (libepseon_hip.so).
+ *
+ * The reference hosts this path behind its pybind11 module _libepseon_gpu
+ * (cpp/gpu/source/epseon/gpu/python/api.cpp:114-494) and the plugin call
+ * Algorithm<FP>::run(stop_token, TaskHandle*) (cpp/gpu/include/epseon/gpu/
+ * algorithms/algorithm.hpp:34), whose only implementation,
+ * VibwaAlgorithm<FP>::run (algorithms/vibwa.hpp:605-637), creates a device,
+ * allocates buffers and descriptor sets and returns without dispatching.
+ * Every entry point below replaces one step of that pipeline; the comment on
+ * each names the reference code it stands in for.
+ *
+ * Conventions: plain C types only; every call returns EPSEON_OK (0) or an
+ * error code, and epseon_last_error() gives the message of the last failing
+ * call on the calling thread.  Host pointers everywhere unless a name ends in
+ * _device.  A context owns one HIP stream and one device arena; calls on one
+ * context must not overlap (the pybind layer runs one context per task).
+ *
+ * Units: cm^-1 for energies, Angstrom for distances, amu for masses.
+ */
+#ifndef EPSEON_HIP_H
+#define EPSEON_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define EPSEON_OK 0
+#define EPSEON_ERR_INVALID 1   /* bad argument */
+#define EPSEON_ERR_DEVICE 2    /* HIP runtime error or no such device */
+#define EPSEON_ERR_STATE 3     /* call out of order (e.g. refine before scan) */
+#define EPSEON_ERR_MEMORY 4    /* device allocation failed */
+
+/* ---- per-level status ------------------------------------------------ */
+#define EPSEON_LEVEL_OK 0         /* converged to the requested tolerance */
+#define EPSEON_LEVEL_UNBOUND 1    /* no level with this node count below e_top */
+#define EPSEON_LEVEL_MAXITER 2    /* iteration cap reached; value is the bracket midpoint */
+#define EPSEON_LEVEL_UNRESOLVED 3 /* potential rejected: Numerov factor outside (0, 2] */
+
+/* ---- per-potential flags ----------------------------------------------- */
+#define EPSEON_POT_OK 0
+#define EPSEON_POT_UNRESOLVED 1   /* 1 + beta (E - V_i) leaves (0, 2] inside the window */
+
+/* Device description.  Replaces the Vulkan property structs the reference
+ * exposes through ComputeContext::getPhysicalDevicesInfo
+ * (cpp/gpu/source/epseon/gpu/compute_context.cpp:94-104; bound at
+ * python/api.cpp:118-329). */
+typedef struct epseon_device_info {
+    char name[256];
+    char arch[64];                     /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+    int32_t runtime_version;           /* hipRuntimeGetVersion */
+    int32_t driver_version;            /* hipDriverGetVersion */
+    int32_t vendor_id;                 /* 0x1002 */
+    int32_t pci_device_id;             /* PCI device number of the card */
+    int32_t compute_units;
+    int32_t max_threads_per_block;
+    int32_t max_block_dim[3];
+    int32_t max_grid_dim[3];
+    int64_t shared_mem_per_block;      /* bytes */
+    int64_t shared_mem_per_cu;         /* bytes */
+    int64_t total_global_mem;          /* bytes */
+    int32_t clock_khz;
+    int32_t warp_size;
+    uint8_t uuid[16];
+} epseon_device_info;
+
+/* One potential of a batch.  Replaces the per-slot inputs of the reference:
+ * the Morse record's grid (MorsePotentialConfig min_r/max_r/point_count,
+ * task_configurator/potential_source.hpp:102-183) and the Vibwa record's
+ * masses, asymptote margin and level range (task_configurator/
+ * algorithm_config.hpp:75-99). */
+typedef struct epseon_potential_desc {
+    int64_t n_points;      /* N >= 4; grid r_i = r_min + i*h, h = (r_max - r_min)/(N - 1) */
+    int64_t n_energies;    /* M >= 1 trial energies, E_j = E_lo + (j+1)(e_top - E_lo)/(M+1) */
+    double r_min;
+    double r_max;
+    double reduced_mass;   /* mu = m0*m1/(m0+m1), amu */
+    double e_top;          /* top of the energy window; E_lo is min_i V_i */
+    int32_t level_min;     /* first level (node count) to refine */
+    int32_t level_count;   /* number of levels; the reference's max_level - min_level + 1
+                              (algorithm_config.hpp:177) */
+} epseon_potential_desc;
+
+/* Per-potential values the library derived at load time. */
+typedef struct epseon_potential_info {
+    double beta;           /* h^2 mu / (12 K) */
+    double h;              /* grid step */
+    double e_lo;           /* window floor = min_i V_i */
+    double e_hi;           /* window top = e_top */
+    int32_t stride;        /* node-sampling stride B */
+    int32_t flags;         /* EPSEON_POT_* */
+} epseon_potential_info;
+
+/* Counters and device times of the last calls on a context. */
+typedef struct epseon_stats {
+    double scan_ms;               /* device time of the last epseon_scan (HIP events) */
+    double refine_ms;             /* device time of the last epseon_refine */
+    double normalize_ms;          /* device time of the last epseon_normalize */
+    double scan_point_updates;    /* sum over potentials of (N - 2) * M */
+    double refine_point_updates;  /* sum over levels of shots * (N - 2) */
+    double normalize_point_updates;
+    int64_t scan_launches;        /* kernel launches of the last scan */
+    int64_t levels_converged;     /* levels with EPSEON_LEVEL_OK after the last refine */
+} epseon_stats;
+
+typedef struct epseon_ctx epseon_ctx;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* epseon_last_error(void);
+
+/* Number of visible HIP devices.  Replaces the physical-device enumeration of
+ * ComputeContext::getPhysicalDevicesInfo (compute_context.cpp:94-104). */
+int epseon_device_count(int32_t* count);
+
+/* Properties of device `device` (an ordinal).  Same source as above. */
+int epseon_device_info_get(int32_t device, epseon_device_info* info);
+
+/* Open a context on device `device`: one HIP stream plus a device arena that
+ * allocates in blocks of `block_bytes` (0 = 64 MiB).  Replaces the logical
+ * device and VMA allocator VibwaAlgorithm::run builds per task
+ * (vibwa.hpp:611 and :654-674, :621-626), sized by HardwareConfig's
+ * allocation_block_size (task_configurator/hardware_config.hpp:11-76). */
+int epseon_ctx_create(int32_t device, uint64_t block_bytes, epseon_ctx** out);
+
+/* Release the context, its stream and all device memory. */
+int epseon_ctx_destroy(epseon_ctx* ctx);
+
+/* Upload a batch of `count` potentials: `v` holds the concatenated V tables
+ * (cm^-1), potential p at v[v_offsets[p] .. v_offsets[p] + n_points).  The
+ * library derives beta, the window and the node stride, rejects potentials
+ * whose Numerov factor leaves (0, 2] (flag EPSEON_POT_UNRESOLVED), builds the
+ * shifted table u_i = beta V_i - 1 on the device and lays out every result
+ * buffer.  Replaces the staging upload plus the five device-only buffers and
+ * the output buffer per slot (ShaderResources::create, vibwa.hpp:153-233;
+ * layout from getShaderBufferRequirements, algorithm_config.hpp:173-198),
+ * and MorsePotentialGenerator::get_potential_data, which returns {} in the
+ * reference (potential_source.hpp:223-225). */
+int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc,
+                           const double* v, const int64_t* v_offsets);
+
+/* Derived values of potential p after epseon_load_potentials. */
+int epseon_potential_info_get(epseon_ctx* ctx, int32_t p, epseon_potential_info* info);
+
+/* Numerov scan: shoot every trial energy of every potential, recording node
+ * count, far-end value w_{N-1} and rescale count.  Stream-ordered; returns
+ * after the launch (epseon_ctx_synchronize waits).  This is the dispatch that
+ * would sit between descriptor update and "Thread finished" in
+ * VibwaAlgorithm::run (vibwa.hpp:630-636). */
+int epseon_scan(epseon_ctx* ctx);
+
+/* Bracket every requested level between adjacent trial energies by node count
+ * and refine it (node-count bisection until isolated, then Illinois regula
+ * falsi on the far-end value) until the bracket is narrower than
+ * rel_tol*|E| + abs_tol, abs_tol = 1e-15 (e_top - E_lo).  Requires a prior
+ * epseon_scan.  Same dispatch site as above; its result is the reference's
+ * output buffer of level_count values per slot (algorithm_config.hpp:189). */
+int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter);
+
+/* epseon_scan followed by epseon_refine. */
+int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter);
+
+/* Wait for all work queued on the context's stream. */
+int epseon_ctx_synchronize(epseon_ctx* ctx);
+
+/* Copy the refined levels out, in potential order, level_count per potential
+ * (the reference's per-slot output buffer, algorithm_config.hpp:189).  Any of
+ * the pointers may be NULL.  Synchronises the stream. */
+int epseon_get_levels(epseon_ctx* ctx, double* levels, int32_t* status, int32_t* iterations);
+
+/* Copy the scan results out, in potential order, n_energies per potential.
+ * Any of the pointers may be NULL.  Synchronises the stream. */
+int epseon_get_scan(epseon_ctx* ctx, int32_t* nodes, double* far_end, int32_t* rescales);
+
+/* Totals: number of potentials, grid points, trial energies and levels of the
+ * loaded batch (any pointer may be NULL). */
+int epseon_get_sizes(epseon_ctx* ctx, int32_t* potentials, int64_t* points, int64_t* energies,
+                     int64_t* levels);
+
+/* Counters and device times of the last phases. */
+int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EPSEON_HIP_H */

@@ -1,0 +1,254 @@
+"""CPU oracle for the Numerov-shooting path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package; the product path
+(``epseon_backend_amd``) never does.
+
+It wraps ``numerov_oracle.c`` (built by ``oracle/Makefile`` into
+``libnumerov_oracle.so``), a plain-C FP64 restatement of the numerics the HIP
+kernels implement.  The reference ships no implementation of this path
+(``VibwaAlgorithm<FP>::run``, ``cpp/gpu/include/epseon/gpu/algorithms/
+vibwa.hpp:605-637``, never dispatches), so the C file writes the
+specification down; its functions are pinned in ``tests/`` against closed
+forms (the discrete particle in a box, analytic Morse levels), dense
+symmetric eigensolves of the discrete problem, and high-precision mpmath
+trajectories committed under ``tests/golden/``.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIB_PATH = ORACLE_DIR / "libnumerov_oracle.so"
+K_CM = 16.857629191640175  # hbar^2/(2 amu A^2) in cm^-1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_LIB: ctypes.CDLL | None = None
+
+
+class Bracket(ctypes.Structure):
+    _fields_ = [
+        ("e_lo", ctypes.c_double),
+        ("e_hi", ctypes.c_double),
+        ("w_lo", ctypes.c_double),
+        ("w_hi", ctypes.c_double),
+        ("k_lo", ctypes.c_int32),
+        ("k_hi", ctypes.c_int32),
+        ("c_lo", ctypes.c_int32),
+        ("c_hi", ctypes.c_int32),
+        ("lo_valid", ctypes.c_int32),
+        ("found", ctypes.c_int32),
+    ]
+
+
+def build() -> Path:
+    """Compile the oracle with its Makefile (no-op when up to date)."""
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True, capture_output=True)
+    return LIB_PATH
+
+
+def _lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        build()
+    lib = ctypes.CDLL(str(LIB_PATH))
+    d, i32, i64 = ctypes.c_double, ctypes.c_int32, ctypes.c_int64
+    sig = {
+        "eo_grid_step": (d, [d, d, i64]),
+        "eo_beta": (d, [d, d]),
+        "eo_shifted_potential": (None, [_dp, i64, d, _dp]),
+        "eo_morse": (None, [d, d, d, d, d, i64, _dp]),
+        "eo_trial_energy": (d, [i64, d, d]),
+        "eo_trial_spacing": (d, [d, d, i64]),
+        "eo_node_stride": (i32, [d, d, d]),
+        "eo_shoot": (i32, [_dp, i64, d, i32, _dp, _ip]),
+        "eo_shoot_full_count": (i32, [_dp, i64, d]),
+        "eo_trajectory": (None, [_dp, i64, d, _dp]),
+        "eo_scan": (None, [_dp, i64, d, d, d, i64, i32, _ip, _dp, _ip]),
+        "eo_bracket_level": (None, [_ip, _dp, _ip, i64, d, d, i32, ctypes.POINTER(Bracket)]),
+        "eo_refine_level": (
+            i32,
+            [_dp, i64, d, i32, i32, ctypes.POINTER(Bracket), d, d, i32, _dp],
+        ),
+        "eo_solve": (
+            None,
+            [_dp, i64, d, d, d, i64, i32, i32, i32, d, d, i32, _ip, _dp, _ip, _dp, _ip],
+        ),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def _f64(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def grid_step(r_min: float, r_max: float, n: int) -> float:
+    return float(_lib().eo_grid_step(r_min, r_max, n))
+
+
+def beta(h: float, mu: float) -> float:
+    return float(_lib().eo_beta(h, mu))
+
+
+def morse(de: float, re: float, a: float, r_min: float, h: float, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.float64)
+    _lib().eo_morse(de, re, a, r_min, h, n, out.ctypes.data_as(_dp))
+    return out
+
+
+def shifted_potential(v: np.ndarray, b: float) -> np.ndarray:
+    v = _f64(v)
+    out = np.empty_like(v)
+    _lib().eo_shifted_potential(v.ctypes.data_as(_dp), len(v), b, out.ctypes.data_as(_dp))
+    return out
+
+
+def trial_spacing(e_lo: float, e_hi: float, m: int) -> float:
+    return float(_lib().eo_trial_spacing(e_lo, e_hi, m))
+
+
+def trial_energies(e_lo: float, e_hi: float, m: int) -> np.ndarray:
+    de = trial_spacing(e_lo, e_hi, m)
+    lib = _lib()
+    return np.array([lib.eo_trial_energy(j, e_lo, de) for j in range(m)], dtype=np.float64)
+
+
+def node_stride(b: float, e_hi: float, v_min: float) -> int:
+    return int(_lib().eo_node_stride(b, e_hi, v_min))
+
+
+def shoot(ut: np.ndarray, e: float, stride: int) -> tuple[int, float, int]:
+    """(node count, w_{N-1}, rescales) of one shot at probe e = beta*E."""
+    ut = _f64(ut)
+    w = ctypes.c_double()
+    k = ctypes.c_int32()
+    c = _lib().eo_shoot(ut.ctypes.data_as(_dp), len(ut), e, stride, ctypes.byref(w), ctypes.byref(k))
+    return int(c), float(w.value), int(k.value)
+
+
+def shoot_full_count(ut: np.ndarray, e: float) -> int:
+    ut = _f64(ut)
+    return int(_lib().eo_shoot_full_count(ut.ctypes.data_as(_dp), len(ut), e))
+
+
+def trajectory(ut: np.ndarray, e: float) -> np.ndarray:
+    ut = _f64(ut)
+    out = np.empty_like(ut)
+    _lib().eo_trajectory(ut.ctypes.data_as(_dp), len(ut), e, out.ctypes.data_as(_dp))
+    return out
+
+
+def scan(ut: np.ndarray, b: float, e_lo: float, e_hi: float, m: int, stride: int):
+    """(nodes, far-end values, rescales) of every trial energy."""
+    ut = _f64(ut)
+    nodes = np.empty(m, dtype=np.int32)
+    far = np.empty(m, dtype=np.float64)
+    resc = np.empty(m, dtype=np.int32)
+    _lib().eo_scan(
+        ut.ctypes.data_as(_dp), len(ut), b, e_lo, e_hi, m, stride,
+        nodes.ctypes.data_as(_ip), far.ctypes.data_as(_dp), resc.ctypes.data_as(_ip),
+    )
+    return nodes, far, resc
+
+
+def bracket_level(nodes, far, resc, e_lo: float, e_hi: float, v: int) -> Bracket:
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    far = _f64(far)
+    resc = np.ascontiguousarray(resc, dtype=np.int32)
+    br = Bracket()
+    m = len(nodes)
+    _lib().eo_bracket_level(
+        nodes.ctypes.data_as(_ip), far.ctypes.data_as(_dp), resc.ctypes.data_as(_ip), m,
+        e_lo, trial_spacing(e_lo, e_hi, m), v, ctypes.byref(br),
+    )
+    return br
+
+
+def refine_level(ut, b: float, stride: int, v: int, br: Bracket, rel_tol: float, abs_tol: float,
+                 max_iter: int = 200) -> tuple[float, int]:
+    ut = _f64(ut)
+    lv = ctypes.c_double()
+    it = _lib().eo_refine_level(
+        ut.ctypes.data_as(_dp), len(ut), b, stride, v, ctypes.byref(br), rel_tol, abs_tol,
+        max_iter, ctypes.byref(lv),
+    )
+    return float(lv.value), int(it)
+
+
+@dataclass
+class Problem:
+    """One potential prepared exactly as the library prepares it."""
+
+    v: np.ndarray
+    r_min: float
+    r_max: float
+    mu: float
+    e_top: float
+    m: int
+
+    @property
+    def n(self) -> int:
+        return len(self.v)
+
+    @property
+    def h(self) -> float:
+        return grid_step(self.r_min, self.r_max, self.n)
+
+    @property
+    def beta(self) -> float:
+        return beta(self.h, self.mu)
+
+    @property
+    def e_lo(self) -> float:
+        return float(np.min(self.v))
+
+    @property
+    def ut(self) -> np.ndarray:
+        return shifted_potential(self.v, self.beta)
+
+    @property
+    def stride(self) -> int:
+        return node_stride(self.beta, self.e_top, self.e_lo)
+
+    @property
+    def abs_tol(self) -> float:
+        return 1e-15 * (self.e_top - self.e_lo)
+
+
+def solve(prob: Problem, v_min: int, n_levels: int, rel_tol: float = 1e-10, max_iter: int = 200):
+    """Scan + bracket + refine, as epseon_solve does: returns
+    (nodes, far, resc, levels, iterations)."""
+    ut = prob.ut
+    m = prob.m
+    nodes = np.empty(m, dtype=np.int32)
+    far = np.empty(m, dtype=np.float64)
+    resc = np.empty(m, dtype=np.int32)
+    lv = np.empty(max(n_levels, 1), dtype=np.float64)
+    it = np.empty(max(n_levels, 1), dtype=np.int32)
+    _lib().eo_solve(
+        ut.ctypes.data_as(_dp), len(ut), prob.beta, prob.e_lo, prob.e_top, m, prob.stride,
+        v_min, n_levels, rel_tol, prob.abs_tol, max_iter,
+        nodes.ctypes.data_as(_ip), far.ctypes.data_as(_dp), resc.ctypes.data_as(_ip),
+        lv.ctypes.data_as(_dp), it.ctypes.data_as(_ip),
+    )
+    return nodes, far, resc, lv[:n_levels], it[:n_levels]
+
+
+def morse_levels(mu: float, de: float, a: float, v: np.ndarray | int) -> np.ndarray:
+    """Analytic Morse levels E_v = 2a sqrt(K' D_e)(v+1/2) - a^2 K' (v+1/2)^2, K' = K/mu."""
+    kp = K_CM / mu
+    x = np.asarray(v, dtype=np.float64) + 0.5
+    return 2.0 * a * np.sqrt(kp * de) * x - a * a * kp * x * x

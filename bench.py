@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the Numerov-shooting hot path on one MI355X.
+
+Workload (default ``single``): one heavy Morse potential (mu = 63.452 amu,
+D_e = 12440 cm^-1, a = 1.8676 1/A, r_e = 2.666 A), an FP64 grid of 200,000
+points on r in [2, 8] A, 65,536 trial energies uniform in (min V, D_e), every
+bound level refined to 1e-10 relative.  A step is one pass of the hot path over
+that input: Numerov scan of all trial energies, node-count bracketing and
+refinement of every level, levels copied back.  The potential table is
+uploaded once before timing (inputs resident in HBM).
+
+``--workload batched`` runs the batched case instead: 4,096 seeded Morse
+potentials with ragged grids (2k-64k points) and trial counts (256-8192).
+
+Prints one JSON line: metric/value (whole-job Numerov point-updates per
+second over scan + refinement shots), eigenlevels/s, the roofline of the
+dominant kernel (the scan, timed with HIP events on its own stream), and a
+CPU baseline (the C oracle on the host cores, on a bounded sample of the same
+workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+FP64_VECTOR_PEAK_TFLOPS = 78.6  # MI355X: 256 CU x 128 FP64 FLOP/clk x 2.4 GHz
+FLOP_PER_POINT_UPDATE = 7  # sub + FMA + mul + mul + FMA (FMA = 2)
+METRIC = "Numerov point-updates/s & eigenlevels/s at 1e-9 rel err; grad/fwd time ratio"
+
+SINGLE = dict(mu=63.452, de=12440.0, a=1.8676, re=2.666, r_min=2.0, r_max=8.0, n=200_000, m=65_536)
+
+
+def morse_table(de: float, re: float, a: float, r_min: float, r_max: float, n: int) -> np.ndarray:
+    h = (r_max - r_min) / (n - 1)
+    r = r_min + np.arange(n, dtype=np.float64) * h
+    x = 1.0 - np.exp(-a * (r - re))
+    return de * (x * x)
+
+
+def single_workload(capi):
+    c = SINGLE
+    v = morse_table(c["de"], c["re"], c["a"], c["r_min"], c["r_max"], c["n"])
+    # Level count: every level below D_e (the analytic Morse count is 116).
+    pots = [
+        capi.Potential(
+            values=v, r_min=c["r_min"], r_max=c["r_max"], reduced_mass=c["mu"], e_top=c["de"],
+            n_energies=c["m"], level_min=0, level_count=120,
+        )
+    ]
+    config = {
+        "workload": "single heavy Morse potential (mu=63.452 amu, De=12440 cm-1, a=1.8676/A, "
+        "re=2.666 A), FP64 grid of 200000 points on [2,8] A, 65536 trial energies in (min V, De), "
+        "all bound levels refined to 1e-10 rel",
+        "potentials": 1,
+        "grid_points": c["n"],
+        "trial_energies": c["m"],
+    }
+    return pots, config
+
+
+def batched_workload(capi, seed: int = 20261202, count: int = 4096):
+    rng = np.random.default_rng(seed)
+    mu = np.exp(rng.uniform(np.log(0.5), np.log(100.0), count))
+    de = rng.uniform(5e3, 5e4, count)
+    a = rng.uniform(1.0, 2.5, count)
+    re = rng.uniform(0.7, 3.0, count)
+    n = np.floor(np.exp(rng.uniform(np.log(2000), np.log(64000), count))).astype(np.int64)
+    m = np.floor(np.exp(rng.uniform(np.log(256), np.log(8192), count))).astype(np.int64)
+    pots = []
+    for i in range(count):
+        lo, hi = re[i] - 0.6, re[i] + 6.0 / a[i]
+        v = morse_table(de[i], re[i], a[i], lo, hi, int(n[i]))
+        kp = 16.857629191640175 / mu[i]
+        n_bound = int(np.floor(np.sqrt(de[i] / kp) / a[i] - 0.5)) + 1
+        pots.append(
+            capi.Potential(
+                values=v, r_min=lo, r_max=hi, reduced_mass=float(mu[i]), e_top=float(de[i]),
+                n_energies=int(m[i]), level_min=0, level_count=n_bound + 2,
+            )
+        )
+    config = {
+        "workload": f"batched: {count} seeded Morse potentials (seed {seed}), grids log-U[2000,64000] "
+        "points on [re-0.6, re+6/a] A, trial energies log-U[256,8192], all bound levels to 1e-10 rel",
+        "potentials": count,
+        "grid_points": int(n.sum()),
+        "trial_energies": int(m.sum()),
+    }
+    return pots, config
+
+
+def cpu_baseline(target_seconds: float = 12.0) -> dict:
+    """The C oracle (OpenMP over trial energies) on a bounded sample of the
+    single-potential workload: the same grid with fewer trial energies."""
+    import oracle
+
+    c = SINGLE
+    h = oracle.grid_step(c["r_min"], c["r_max"], c["n"])
+    v = oracle.morse(c["de"], c["re"], c["a"], c["r_min"], h, c["n"])
+    prob = oracle.Problem(v=v, r_min=c["r_min"], r_max=c["r_max"], mu=c["mu"], e_top=c["de"], m=64)
+    ut, b, stride = prob.ut, prob.beta, prob.stride
+    threads = oracle.num_threads()
+    m = 64 * threads
+    t0 = time.perf_counter()
+    oracle.scan(ut, b, prob.e_lo, prob.e_top, m, stride)
+    dt = time.perf_counter() - t0
+    m = int(min(c["m"], max(m, m * target_seconds / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    oracle.scan(ut, b, prob.e_lo, prob.e_top, m, stride)
+    dt = time.perf_counter() - t0
+    pu = (c["n"] - 2) * m
+    return {
+        "value": pu / dt,
+        "unit": "point-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"Numerov scan of {m} of the 65536 trial energies over the full 200000-point grid "
+        f"({pu:.3e} point-updates in {dt:.2f} s)",
+    }
+
+
+def load_traffic(workload: str) -> float | None:
+    """HBM bytes per scan launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), if committed."""
+    path = REPO / "profiles" / f"scan_{workload}_traffic.json"
+    if not path.exists():
+        return None
+    try:
+        return float(json.loads(path.read_text())["bytes_per_launch"])
+    except (KeyError, ValueError, OSError):
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["single", "batched"], default="single")
+    ap.add_argument("--rel-tol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        sys.stderr.write(
+            "bench.py measures one GPU only: the Numerov path shards potentials (or trial "
+            "energies) across GPUs with no exchange, and this build runs on one MI355X. "
+            f"--gpus {args.gpus} is not supported.\n"
+        )
+        return 2
+
+    from epseon_backend_amd import capi
+
+    if capi.device_count() < 1:
+        sys.stderr.write("no HIP device visible\n")
+        return 1
+
+    pots, config = single_workload(capi) if args.workload == "single" else batched_workload(capi)
+    ctx = capi.Context(0)
+    ctx.load(pots)
+
+    for _ in range(args.warmup):
+        ctx.solve(args.rel_tol, 200)
+        ctx.levels()
+
+    scan_ms: list[float] = []
+    scan_pu = refine_pu = 0.0
+    levels_ok = 0
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.solve(args.rel_tol, 200)
+        lv, st, it = ctx.levels()
+        s = ctx.stats()
+        scan_ms.append(s.scan_ms)
+        scan_pu += s.scan_point_updates
+        refine_pu += s.refine_point_updates
+        levels_ok += int(s.levels_converged)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.close()
+
+    ms_per_step = wall / args.steps * 1e3
+    value = (scan_pu + refine_pu) / wall
+    avg_scan_s = float(np.mean(scan_ms)) * 1e-3
+    achieved = FLOP_PER_POINT_UPDATE * (scan_pu / args.steps) / avg_scan_s / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": config,
+        "eigenlevels_per_s": levels_ok / wall,
+        "levels_per_step": levels_ok // args.steps,
+        "scan_ms": float(np.mean(scan_ms)),
+        "scan_point_updates_per_step": scan_pu / args.steps,
+        "refine_point_updates_per_step": refine_pu / args.steps,
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP64_VECTOR_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
+            "traffic": load_traffic(args.workload),
+        },
+    }
+    if not args.no_cpu_baseline and args.workload == "single":
+        out["cpu_baseline"] = cpu_baseline()
+    sys.stdout.write(json.dumps(out) + "\n")
+    sys.stdout.flush()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

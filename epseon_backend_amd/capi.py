@@ -87,6 +87,8 @@ class PotentialInfo(ctypes.Structure):
         ("e_hi", ctypes.c_double),
         ("stride", ctypes.c_int32),
         ("flags", ctypes.c_int32),
+        ("first_point", ctypes.c_int64),
+        ("point_count", ctypes.c_int64),
     ]
 
 
@@ -110,6 +112,7 @@ _c_int64_p = ctypes.POINTER(ctypes.c_int64)
 # Every entry point of include/epseon_hip.h with its argument types.
 SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "epseon_last_error": (ctypes.c_char_p, []),
+    "epseon_struct_sizes": (ctypes.c_int, [_c_int64_p, ctypes.c_int32]),
     "epseon_device_count": (ctypes.c_int, [_c_int32_p]),
     "epseon_device_info_get": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(DeviceInfo)]),
     "epseon_ctx_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
@@ -165,8 +168,19 @@ def load_library() -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = restype
         fn.argtypes = argtypes
+    sizes = (ctypes.c_int64 * len(ABI_STRUCTS))()
+    if lib.epseon_struct_sizes(sizes, len(ABI_STRUCTS)) != EPSEON_OK:
+        raise ImportError("libepseon_hip.so does not report its struct sizes")
+    for cls, size in zip(ABI_STRUCTS, sizes):
+        if ctypes.sizeof(cls) != size:
+            msg = f"ABI mismatch: {cls.__name__} is {ctypes.sizeof(cls)} bytes here, {size} in the library"
+            raise ImportError(msg)
     _LIB = lib
     return lib
+
+
+# Public structs in the order epseon_struct_sizes reports them.
+ABI_STRUCTS = (DeviceInfo, PotentialDesc, PotentialInfo, Stats)
 
 
 def _check(rc: int) -> None:

@@ -208,6 +208,17 @@ extern "C" {
 
 const char* epseon_last_error(void) { return g_last_error.c_str(); }
 
+int epseon_struct_sizes(int64_t* sizes, int32_t count)
+{
+    if (sizes == nullptr || count < 0)
+        return fail(EPSEON_ERR_INVALID, "invalid size buffer");
+    const int64_t all[] = {sizeof(epseon_device_info), sizeof(epseon_potential_desc),
+                           sizeof(epseon_potential_info), sizeof(epseon_stats)};
+    for (int32_t i = 0; i < count && i < 4; ++i)
+        sizes[i] = all[i];
+    return EPSEON_OK;
+}
+
 int epseon_device_count(int32_t* count)
 {
     if (count == nullptr)
@@ -350,7 +361,7 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
             return fail(EPSEON_ERR_INVALID, msg);
         }
         const double* vp = v + v_offsets[p];
-        double vmin = vp[0], vmax = vp[0];
+        int64_t imin = 0;
         for (int64_t i = 0; i < d.n_points; ++i) {
             const double x = vp[i];
             if (!std::isfinite(x)) {
@@ -358,9 +369,10 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
                               static_cast<long long>(i));
                 return fail(EPSEON_ERR_INVALID, msg);
             }
-            vmin = std::min(vmin, x);
-            vmax = std::max(vmax, x);
+            if (x < vp[imin])
+                imin = i;
         }
+        const double vmin = vp[imin];
         if (!(std::isfinite(d.e_top) && d.e_top > vmin)) {
             std::snprintf(msg, sizeof msg,
                           "potential %d: energy window top %.17g is not above min V %.17g", p,
@@ -370,19 +382,39 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
         PotentialRecord& r = ctx->rec[p];
         r.h = (d.r_max - d.r_min) / static_cast<double>(d.n_points - 1);
         r.beta = (r.h * r.h * d.reduced_mass) / (12.0 * kHbar2Over2Amu);
+        // Wall trimming: the Dirichlet points move in to the last point left
+        // of the minimum and the first point right of it where V >= V_min +
+        // 1/(2 beta).  Inside, 1 + beta (E - V_i) > 1/2 for every E >= V_min.
+        const double wall = vmin + 0.5 / r.beta;
+        int64_t first = 0, last = d.n_points - 1;
+        for (int64_t i = imin; i >= 0; --i)
+            if (vp[i] >= wall) {
+                first = i;
+                break;
+            }
+        for (int64_t i = imin; i < d.n_points; ++i)
+            if (vp[i] >= wall) {
+                last = i;
+                break;
+            }
+        r.first_point = first;
         r.e_lo = vmin;
         r.e_hi = d.e_top;
         r.de = (r.e_hi - r.e_lo) / static_cast<double>(d.n_energies + 1);
         r.abs_tol = 1e-15 * (r.e_hi - r.e_lo);
-        r.n_points = static_cast<int32_t>(d.n_points);
+        r.n_points = static_cast<int32_t>(last - first + 1);
         r.n_energies = static_cast<int32_t>(d.n_energies);
         r.level_min = d.level_min;
         r.level_count = d.level_count;
         r.stride = node_stride(r.beta, r.e_hi, vmin);
-        const double f_min = 1.0 + r.beta * (r.e_lo - vmax);
+        // f_max = 1 + beta (E_hi - V_min) <= 3/2 keeps E_hi below the wall
+        // threshold and bounds the per-step growth of the recurrence.
         const double f_max = 1.0 + r.beta * (r.e_hi - vmin);
-        r.flags = (f_min > 0.0 && f_max <= 2.0) ? EPSEON_POT_OK : EPSEON_POT_UNRESOLVED;
-        r.ut_offset = pts;
+        r.flags = (f_max <= 1.5 && r.n_points >= 4) ? EPSEON_POT_OK : EPSEON_POT_UNRESOLVED;
+        if (r.n_points < 4)
+            r.n_points = 4;
+        r.v_offset = pts;
+        r.ut_offset = pts + first;
         r.energy_offset = en;
         r.level_offset = lv;
         // 64-byte aligned rows with 64 points of slack: the shot prefetches up
@@ -390,9 +422,9 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
         pts += ((d.n_points + 7) & ~int64_t(7)) + 64;
         en += d.n_energies;
         lv += d.level_count;
-        max_points = std::max(max_points, r.n_points);
+        max_points = std::max(max_points, static_cast<int32_t>(d.n_points));
         if (r.flags == EPSEON_POT_OK)
-            scan_pu += static_cast<double>(d.n_points - 2) * static_cast<double>(d.n_energies);
+            scan_pu += static_cast<double>(r.n_points - 2) * static_cast<double>(d.n_energies);
     }
     ctx->total_points = pts;
     ctx->total_energies = en;
@@ -437,7 +469,7 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_iters));
 
     for (int32_t p = 0; p < count; ++p)
-        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_v + ctx->rec[p].ut_offset, v + v_offsets[p],
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_v + ctx->rec[p].v_offset, v + v_offsets[p],
                                       static_cast<size_t>(desc[p].n_points) * sizeof(double),
                                       hipMemcpyHostToDevice, ctx->stream));
     if (count > 0)
@@ -479,6 +511,8 @@ int epseon_potential_info_get(epseon_ctx* ctx, int32_t p, epseon_potential_info*
     info->e_hi = r.e_hi;
     info->stride = r.stride;
     info->flags = r.flags;
+    info->first_point = r.first_point;
+    info->point_count = r.n_points;
     return EPSEON_OK;
 }
 

@@ -21,10 +21,12 @@ constexpr int kMaxStride = 32;
 constexpr double kHbar2Over2Amu = 16.857629191640175;
 
 struct alignas(16) PotentialRecord {
-    int64_t ut_offset;      // first grid point of this potential in the u table
+    int64_t v_offset;       // grid point 0 of this potential in the V and u tables
+    int64_t ut_offset;      // first point of the trimmed grid (v_offset + first_point)
     int64_t energy_offset;  // first trial energy in the scan arrays
     int64_t level_offset;   // first level in the level arrays
-    int32_t n_points;       // N
+    int64_t first_point;    // index of the left Dirichlet point on the full grid
+    int32_t n_points;       // points of the trimmed grid (both Dirichlet points included)
     int32_t n_energies;     // M
     int32_t level_min;      // first level v
     int32_t level_count;    // levels requested
@@ -37,7 +39,7 @@ struct alignas(16) PotentialRecord {
     double abs_tol;         // absolute refinement tolerance
     double h;               // grid step
 };
-static_assert(sizeof(PotentialRecord) == 96, "record layout");
+static_assert(sizeof(PotentialRecord) == 112, "record layout");
 
 // Work item of one wavefront: potential index and the first trial energy
 // (scan) or first level (refine) of its 64 lanes.

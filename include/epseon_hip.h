@@ -39,11 +39,11 @@ extern "C" {
 #define EPSEON_LEVEL_OK 0         /* converged to the requested tolerance */
 #define EPSEON_LEVEL_UNBOUND 1    /* no level with this node count below e_top */
 #define EPSEON_LEVEL_MAXITER 2    /* iteration cap reached; value is the bracket midpoint */
-#define EPSEON_LEVEL_UNRESOLVED 3 /* potential rejected: Numerov factor outside (0, 2] */
+#define EPSEON_LEVEL_UNRESOLVED 3 /* potential rejected: grid too coarse for the window */
 
 /* ---- per-potential flags ----------------------------------------------- */
 #define EPSEON_POT_OK 0
-#define EPSEON_POT_UNRESOLVED 1   /* 1 + beta (E - V_i) leaves (0, 2] inside the window */
+#define EPSEON_POT_UNRESOLVED 1   /* 1 + beta (E_top - V_min) > 3/2, or fewer than 4 points left */
 
 /* Device description.  Replaces the Vulkan property structs the reference
  * exposes through ComputeContext::getPhysicalDevicesInfo
@@ -93,6 +93,8 @@ typedef struct epseon_potential_info {
     double e_hi;           /* window top = e_top */
     int32_t stride;        /* node-sampling stride B */
     int32_t flags;         /* EPSEON_POT_* */
+    int64_t first_point;   /* left Dirichlet point of the trimmed grid (index on the full grid) */
+    int64_t point_count;   /* points of the trimmed grid, both Dirichlet points included */
 } epseon_potential_info;
 
 /* Counters and device times of the last calls on a context. */
@@ -111,6 +113,13 @@ typedef struct epseon_ctx epseon_ctx;
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* epseon_last_error(void);
+
+/* sizeof of the public structs, in this order: epseon_device_info,
+ * epseon_potential_desc, epseon_potential_info, epseon_stats (at most
+ * `count` values are written).  Lets a foreign-function binding verify its
+ * struct layouts at load time.  No counterpart in the reference (its binding
+ * is compiled against the headers). */
+int epseon_struct_sizes(int64_t* sizes, int32_t count);
 
 /* Number of visible HIP devices.  Replaces the physical-device enumeration of
  * ComputeContext::getPhysicalDevicesInfo (compute_context.cpp:94-104). */
@@ -131,8 +140,10 @@ int epseon_ctx_destroy(epseon_ctx* ctx);
 
 /* Upload a batch of `count` potentials: `v` holds the concatenated V tables
  * (cm^-1), potential p at v[v_offsets[p] .. v_offsets[p] + n_points).  The
- * library derives beta, the window and the node stride, rejects potentials
- * whose Numerov factor leaves (0, 2] (flag EPSEON_POT_UNRESOLVED), builds the
+ * library derives beta, the window and the node stride, moves the Dirichlet
+ * points in from the walls to the last/first points where V >= V_min +
+ * 1/(2 beta) (so 1 + beta (E - V_i) > 1/2 on the grid that is shot), rejects
+ * potentials with 1 + beta (e_top - V_min) > 3/2 (flag EPSEON_POT_UNRESOLVED), builds the
  * shifted table u_i = beta V_i - 1 on the device and lays out every result
  * buffer.  Replaces the staging upload plus the five device-only buffers and
  * the output buffer per slot (ShaderResources::create, vibwa.hpp:153-233;

@@ -62,10 +62,12 @@ def _lib() -> ctypes.CDLL:
     lib = ctypes.CDLL(str(LIB_PATH))
     d, i32, i64 = ctypes.c_double, ctypes.c_int32, ctypes.c_int64
     sig = {
+        "eo_num_threads": (i32, []),
         "eo_grid_step": (d, [d, d, i64]),
         "eo_beta": (d, [d, d]),
         "eo_shifted_potential": (None, [_dp, i64, d, _dp]),
         "eo_morse": (None, [d, d, d, d, d, i64, _dp]),
+        "eo_trim": (i32, [_dp, i64, d, d, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "eo_trial_energy": (d, [i64, d, d]),
         "eo_trial_spacing": (d, [d, d, i64]),
         "eo_node_stride": (i32, [d, d, d]),
@@ -95,6 +97,11 @@ def _f64(a: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def num_threads() -> int:
+    """OpenMP threads the oracle's parallel loops use."""
+    return int(_lib().eo_num_threads())
+
+
 def grid_step(r_min: float, r_max: float, n: int) -> float:
     return float(_lib().eo_grid_step(r_min, r_max, n))
 
@@ -114,6 +121,15 @@ def shifted_potential(v: np.ndarray, b: float) -> np.ndarray:
     out = np.empty_like(v)
     _lib().eo_shifted_potential(v.ctypes.data_as(_dp), len(v), b, out.ctypes.data_as(_dp))
     return out
+
+
+def trim(v: np.ndarray, b: float, e_hi: float) -> tuple[int, int, bool]:
+    """(first, last, usable): the Dirichlet points after wall trimming."""
+    v = _f64(v)
+    first = ctypes.c_int64()
+    last = ctypes.c_int64()
+    ok = _lib().eo_trim(v.ctypes.data_as(_dp), len(v), b, e_hi, ctypes.byref(first), ctypes.byref(last))
+    return int(first.value), int(last.value), bool(ok)
 
 
 def trial_spacing(e_lo: float, e_hi: float, m: int) -> float:
@@ -216,8 +232,19 @@ class Problem:
         return float(np.min(self.v))
 
     @property
+    def trimmed(self) -> tuple[int, int, bool]:
+        """(first, last, usable) of the wall-trimmed grid."""
+        return trim(self.v, self.beta, self.e_top)
+
+    @property
+    def usable(self) -> bool:
+        return self.trimmed[2]
+
+    @property
     def ut(self) -> np.ndarray:
-        return shifted_potential(self.v, self.beta)
+        """beta V - 1 on the trimmed grid (the array every shot runs over)."""
+        first, last, _ = self.trimmed
+        return shifted_potential(self.v[first : last + 1], self.beta)
 
     @property
     def stride(self) -> int:

@@ -39,16 +39,34 @@
  *               are multiplied by 2^-480 (exact) and the rescale counter is
  *               incremented.  (Growth per step is below 16, so 47 steps
  *               between tests cannot overflow.)
- *   trial E     E_j = fma(j+1, dE, E_lo), dE = (E_hi - E_lo)/(M+1)
+ *   trial E     E_j = fma(j+1, dE, E_lo), dE = (E_hi - E_lo)/(M+1), E_lo = min V
+ *   walls       the Dirichlet points are the last point left of argmin V and the
+ *               first point right of it with V >= V_min + 1/(2 beta) (grid ends
+ *               if none); the shot runs on that sub-grid, where
+ *               1 + beta (E - V_i) > 1/2 for all E >= V_min.  A potential with
+ *               1 + beta (E_hi - V_min) > 3/2 is rejected as unresolved.
  */
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define EO_K 16.857629191640175        /* cm^-1 amu A^2 (CODATA 2018 h, c, u) */
 #define EO_RESCALE_AT 0x1p480
 #define EO_RESCALE_BY 0x1p-480
 #define EO_MAX_STRIDE 32
+
+/* Threads the OpenMP loops below use (1 without OpenMP). */
+int32_t eo_num_threads(void)
+{
+#ifdef _OPENMP
+    return (int32_t)omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 /* ------------------------------------------------------------------ grid */
 
@@ -76,6 +94,33 @@ void eo_morse(double de, double re, double a, double r_min, double h, int64_t n,
         const double x = 1.0 - exp(-a * (r - re));
         v[i] = de * (x * x);
     }
+}
+
+/* Wall trimming (see header).  Returns 1 when the potential is usable
+ * (at least 4 points remain and 1 + beta (e_hi - V_min) <= 3/2). */
+int32_t eo_trim(const double* v, int64_t n, double beta, double e_hi, int64_t* first, int64_t* last)
+{
+    int64_t imin = 0;
+    for (int64_t i = 1; i < n; ++i)
+        if (v[i] < v[imin])
+            imin = i;
+    const double vmin = v[imin];
+    const double wall = vmin + 0.5 / beta;
+    int64_t a = 0, b = n - 1;
+    for (int64_t i = imin; i >= 0; --i)
+        if (v[i] >= wall) {
+            a = i;
+            break;
+        }
+    for (int64_t i = imin; i < n; ++i)
+        if (v[i] >= wall) {
+            b = i;
+            break;
+        }
+    *first = a;
+    *last = b;
+    const double f_max = 1.0 + beta * (e_hi - vmin);
+    return (f_max <= 1.5 && b - a + 1 >= 4) ? 1 : 0;
 }
 
 double eo_trial_energy(int64_t j, double e_lo, double de)

@@ -1,0 +1,313 @@
+"""Contract of the drop-in Python surface, re-expressed from the reference's
+own tests (python/test/test_device/test_gpu/test_libepseon_gpu.py and
+test_cpu/test_libepseon_cpu.py).
+
+The parts that need a device are marked ``gpu``; everything else (builder
+identity, precision parsing, error messages, C ABI exports) runs on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import gc
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+REPO = Path(__file__).resolve().parents[1]
+HEADER = REPO / "include" / "epseon_hip.h"
+
+COMPUTE_GROUP_AXES_COUNT = 3
+
+
+def _gpu():
+    from epseon_backend_amd.device.gpu import _libepseon_gpu
+
+    return _libepseon_gpu
+
+
+def _morse(m, n=16500):
+    return m.MorsePotentialConfig(
+        dissociation_energy=5500.0,
+        equilibrium_bond_distance=0.6,
+        well_width=10,
+        min_r=0.0,
+        max_r=10.0,
+        point_count=n,
+    )
+
+
+def _configure(m, configurator):
+    """The reference test's canonical builder chain (test_libepseon_gpu.py:172-209)."""
+    return (
+        configurator.set_hardware_config(
+            potential_buffer_size=16500,
+            group_size=512,
+            allocation_block_size=16 * 1024 * 1024,
+        )
+        .set_morse_potential([_morse(m), _morse(m)])
+        .set_vibwa_algorithm(
+            mass_atom_0=87.62,
+            mass_atom_1=87.62,
+            integration_step=0.1,
+            min_distance_to_asymptote=0.1,
+            min_level=0,
+            max_level=0,
+        )
+    )
+
+
+# ---------------------------------------------------------------- CPU side
+
+
+def test_greet() -> None:
+    from epseon_backend_amd.device.cpu._libepseon_cpu import greet
+
+    assert greet() == "Hello, World from C++!"
+
+
+def test_context_creates_without_device_error() -> None:
+    m = _gpu()
+    ctx = m.EpseonComputeContext.create()
+    assert repr(ctx)
+    assert re.match(r"\d+\.\d+\.\d+\.\d+", ctx.get_vulkan_version()) is not None
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64", "FLOAT32", "Float64"])
+def test_configurator_builder_returns_same_object(precision) -> None:
+    m = _gpu()
+    cls = m.TaskConfiguratorFloat32 if precision.lower() == "float32" else m.TaskConfiguratorFloat64
+    c = cls()
+    a = c.set_hardware_config(potential_buffer_size=16500, group_size=4096, allocation_block_size=64 << 20)
+    b = a.set_morse_potential([_morse(m), _morse(m)])
+    d = b.set_vibwa_algorithm(
+        mass_atom_0=87.62,
+        mass_atom_1=87.62,
+        integration_step=0.1,
+        min_distance_to_asymptote=0.1,
+        min_level=0,
+        max_level=0,
+    )
+    assert id(c) == id(a) == id(b) == id(d)
+    assert d.is_configured()
+
+
+def test_configurator_is_configured_stages() -> None:
+    m = _gpu()
+    c = m.TaskConfiguratorFloat64()
+    assert not c.is_configured()
+    c.set_hardware_config(potential_buffer_size=10, group_size=1, allocation_block_size=1 << 20)
+    assert not c.is_configured()
+    c.set_morse_potential([_morse(m, 10)])
+    assert not c.is_configured()
+    c.set_vibwa_algorithm(
+        mass_atom_0=1.0, mass_atom_1=1.0, integration_step=0.1, min_distance_to_asymptote=0.0,
+        min_level=0, max_level=1,
+    )
+    assert c.is_configured()
+
+
+def test_mismatched_point_count_message() -> None:
+    m = _gpu()
+    c = m.TaskConfiguratorFloat32()
+    with pytest.raises(RuntimeError, match="All Morse potentials must have same point count, but previous ones had 16500"):
+        c.set_morse_potential([_morse(m, 16500), _morse(m, 16400)])
+    # The batch setter (an addition) accepts ragged grids.
+    c.set_morse_potential_batch([_morse(m, 16500), _morse(m, 16400)])
+
+
+def test_morse_config_fields() -> None:
+    m = _gpu()
+    c = _morse(m)
+    assert (c.dissociation_energy, c.equilibrium_bond_distance, c.well_width) == (5500.0, 0.6, 10.0)
+    assert (c.min_r, c.max_r, c.point_count) == (0.0, 10.0, 16500)
+
+
+def test_invalid_options_raise() -> None:
+    m = _gpu()
+    c = m.TaskConfiguratorFloat64()
+    with pytest.raises(RuntimeError):
+        c.set_trial_energies(0)
+    with pytest.raises(RuntimeError):
+        c.set_reduced_masses([1.0, -2.0])
+    with pytest.raises(RuntimeError):
+        c.set_tolerance(-1.0)
+    with pytest.raises(RuntimeError):
+        m.DoubleWellPotentialConfig(1.0, 1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 1.0, 100)
+    with pytest.raises(RuntimeError):
+        c.set_tabulated_potential([m.TabulatedPotentialConfig([1.0, 2.0], 0.0, 1.0, 3.0)])
+
+
+def test_no_device_interface_without_device() -> None:
+    m = _gpu()
+    ctx = m.EpseonComputeContext.create()
+    n = len(list(ctx.get_physical_device_info()))
+    with pytest.raises(RuntimeError, match="Device not available."):
+        ctx.get_device_interface(n)
+
+
+def _declared_functions() -> list[str]:
+    text = HEADER.read_text()
+    return re.findall(r"^(?:int|const char\*)\s+(epseon_\w+)\(", text, flags=re.M)
+
+
+def test_capi_exports_every_declared_symbol() -> None:
+    from epseon_backend_amd import capi
+
+    lib = capi.load_library()
+    names = _declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in capi.SIGNATURES, f"{name} missing from the ctypes binding"
+    # and the binding declares nothing the header does not
+    assert set(capi.SIGNATURES) == set(names)
+
+
+def test_capi_errors_without_device() -> None:
+    from epseon_backend_amd import capi
+
+    if capi.device_count() > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(capi.EpseonError, match="Device not available"):
+        capi.Context(0)
+    lib = capi.load_library()
+    out = ctypes.c_void_p()
+    assert lib.epseon_ctx_create(0, 0, ctypes.byref(out)) == capi.EPSEON_ERR_DEVICE
+    assert lib.epseon_scan(None) == capi.EPSEON_ERR_INVALID
+    assert b"null context" in lib.epseon_last_error()
+
+
+# ---------------------------------------------------------------- GPU side
+
+
+@pytest.mark.gpu
+def test_get_physical_device_info_device_properties() -> None:
+    m = _gpu()
+    ctx = m.EpseonComputeContext.create()
+    infos = list(ctx.get_physical_device_info())
+    assert len(infos) > 0
+    for device in infos:
+        p = device.device_properties
+        assert isinstance(p.api_version, str)
+        assert isinstance(p.driver_version, str)
+        assert isinstance(p.vendor_id, int) and p.vendor_id == 0x1002
+        assert isinstance(p.device_id, int)
+        assert p.device_type in ("INTEGRATED_GPU", "DISCRETE_GPU", "VIRTUAL_GPU", "CPU", "OTHER")
+        assert isinstance(p.device_name, str) and len(p.device_name) > 0
+        assert isinstance(p.pipeline_cache_uuid, list)
+        assert isinstance(p.limits.max_compute_shared_memory_size, int)
+        assert isinstance(p.limits.max_compute_work_group_count, tuple)
+        assert len(p.limits.max_compute_work_group_count) == COMPUTE_GROUP_AXES_COUNT
+        assert isinstance(p.limits.max_compute_work_group_invocations, int)
+        assert isinstance(p.limits.max_compute_work_group_size, int)
+        assert hasattr(p, "sparse_properties")
+        assert p.arch.startswith("gfx950")
+
+
+@pytest.mark.gpu
+def test_get_physical_device_info_memory_properties() -> None:
+    m = _gpu()
+    ctx = m.EpseonComputeContext.create()
+    for device in ctx.get_physical_device_info():
+        heaps = device.memory_properties.memory_heaps
+        assert heaps and heaps[0].size > 100 * 2**30
+        assert "DEVICE_LOCAL" in heaps[0].flags
+        assert device.memory_properties.memory_types[0].heap_index == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_get_and_use_task_configurator(precision) -> None:
+    m = _gpu()
+    ctx = m.EpseonComputeContext.create()
+    info = next(iter(ctx.get_physical_device_info()))
+    interface = ctx.get_device_interface(info.device_properties.device_id)
+    cfg = _configure(m, interface.get_task_configurator(precision))
+    assert id(cfg)
+
+
+@pytest.mark.gpu
+def test_unknown_precision() -> None:
+    m = _gpu()
+    ctx = m.EpseonComputeContext.create()
+    interface = ctx.get_device_interface(0)
+    with pytest.raises(ValueError, match='Invalid PrecisionType literal in string: "float80"'):
+        interface.get_task_configurator("float80")
+
+
+@pytest.mark.gpu
+def test_submit_unconfigured_raises() -> None:
+    m = _gpu()
+    interface = m.EpseonComputeContext.create().get_device_interface(0)
+    c = interface.get_task_configurator("float64")
+    with pytest.raises(RuntimeError, match="TaskConfigurator submitted for execution before fully configured."):
+        interface.submit_task(c)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_submit_task_with_ctx_going_out_of_scope(precision) -> None:
+    m = _gpu()
+    ctx = m.EpseonComputeContext.create()
+    interface = ctx.get_device_interface(0)
+    del ctx
+    gc.collect(0)
+    gc.collect(1)
+    gc.collect(2)
+    cfg = _configure(m, interface.get_task_configurator(precision))
+    handle = interface.submit_task(cfg)
+    assert isinstance(handle.get_status_message(), str)
+    handle.wait()
+    assert handle.is_done()
+    assert handle.get_status_message() == "done"
+    levels = handle.get_levels()
+    assert levels.shape == (2, 1)
+    assert levels.dtype == (np.float32 if precision == "float32" else np.float64)
+    # Heavy Sr2-like Morse: the ground level is below the first excited one
+    # and close to the harmonic estimate.
+    assert np.all(handle.get_level_status() == 0)
+    mu = 87.62 / 2
+    kp = 16.857629191640175 / mu
+    e0 = 2 * 10 * np.sqrt(kp * 5500.0) * 0.5 - 100 * kp * 0.25
+    np.testing.assert_allclose(levels[:, 0], e0, rtol=1e-4)
+
+
+@pytest.mark.gpu
+def test_submit_and_wait_every_device() -> None:
+    m = _gpu()
+    ctx = m.EpseonComputeContext.create()
+    for info in ctx.get_physical_device_info():
+        interface = ctx.get_device_interface(info.device_properties.device_id)
+        handle = interface.submit_task(_configure(m, interface.get_task_configurator("float32")))
+        handle.wait()
+        assert handle.is_done()
+
+
+@pytest.mark.gpu
+def test_builder_snapshot_at_submit() -> None:
+    """Changing the builder after submit does not change the running task."""
+    m = _gpu()
+    interface = m.EpseonComputeContext.create().get_device_interface(0)
+    c = _configure(m, interface.get_task_configurator("float64"))
+    handle = interface.submit_task(c)
+    c.set_vibwa_algorithm(
+        mass_atom_0=1.0, mass_atom_1=1.0, integration_step=0.1, min_distance_to_asymptote=0.1,
+        min_level=0, max_level=5,
+    )
+    handle.wait()
+    assert handle.get_levels().shape == (2, 1)
+
+
+@pytest.mark.gpu
+def test_worker_error_is_raised_on_wait() -> None:
+    m = _gpu()
+    interface = m.EpseonComputeContext.create().get_device_interface(0)
+    c = _configure(m, interface.get_task_configurator("float64"))
+    c.set_hardware_config(potential_buffer_size=100, group_size=1, allocation_block_size=1 << 20)
+    handle = interface.submit_task(c)
+    with pytest.raises(RuntimeError, match="exceeds potential_buffer_size"):
+        handle.wait()
+    assert handle.is_done()
+    assert handle.get_status_message().startswith("failed")

@@ -100,6 +100,7 @@ def test_ragged_batch_parity(capi):
     eo = lo = 0
     for i, p in enumerate(probs):
         assert strides[i] == p.stride
+        assert flags[i] == (0 if p.usable else 1)
         if flags[i] != 0:
             assert np.all(st[lo : lo + 30] == 3)
             eo += p.m

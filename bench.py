@@ -147,7 +147,25 @@ def main() -> int:
     ap.add_argument("--workload", choices=["single", "batched"], default="single")
     ap.add_argument("--rel-tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument(
+        "--copy-calibration",
+        action="store_true",
+        help="only run a plain 1 GiB device-to-device copy 5 times (the known-size kernel whose "
+        "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic of the scan)",
+    )
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        n = (1 << 30) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        sys.stdout.write(json.dumps({"copy_calibration_bytes_read": n * 8, "bytes_written": n * 8, "launches": 5}) + "\n")
+        return 0
 
     if args.gpus != 1:
         sys.stderr.write(

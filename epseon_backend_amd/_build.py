@@ -45,6 +45,11 @@ HIPCC_FLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    # Schedule for instruction-level parallelism: the heavy single-potential
+    # scan runs one wavefront per SIMD, so FP64 latency must be hidden inside
+    # the wavefront (dependent ops >= 4 instructions apart).
+    "-mllvm",
+    "-amdgpu-sched-strategy=max-ilp",
     "-Wall",
 ]
 

@@ -36,6 +36,10 @@ LEVEL_UNRESOLVED = 3
 POT_OK = 0
 POT_UNRESOLVED = 1
 
+REFINE_AUTO = 0
+REFINE_LANE = 1
+REFINE_SEGMENTED = 2
+
 
 class EpseonError(RuntimeError):
     """Error reported by libepseon_hip (carries the C status code)."""
@@ -102,6 +106,8 @@ class Stats(ctypes.Structure):
         ("normalize_point_updates", ctypes.c_double),
         ("scan_launches", ctypes.c_int64),
         ("levels_converged", ctypes.c_int64),
+        ("refine_mode", ctypes.c_int32),
+        ("refine_lanes", ctypes.c_int32),
     ]
 
 
@@ -125,6 +131,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "epseon_scan": (ctypes.c_int, [ctypes.c_void_p]),
     "epseon_refine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
     "epseon_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
+    "epseon_set_refine_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "epseon_ctx_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "epseon_get_levels": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_int32_p, _c_int32_p]),
     "epseon_get_scan": (ctypes.c_int, [ctypes.c_void_p, _c_int32_p, _c_double_p, _c_int32_p]),
@@ -295,6 +302,9 @@ class Context:
 
     def refine(self, rel_tol: float = 1e-10, max_iter: int = 200) -> None:
         _check(self._lib.epseon_refine(self._h, rel_tol, max_iter))
+
+    def set_refine_mode(self, mode: int) -> None:
+        _check(self._lib.epseon_set_refine_mode(self._h, mode))
 
     def solve(self, rel_tol: float = 1e-10, max_iter: int = 200) -> None:
         _check(self._lib.epseon_solve(self._h, rel_tol, max_iter))

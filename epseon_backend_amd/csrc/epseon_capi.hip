@@ -24,12 +24,15 @@
 namespace epseon::hip {
 
 hipError_t launch_shift_potential(hipStream_t, const double*, const PotentialRecord*, int32_t,
-                                  int32_t, double*);
+                                  int32_t, double*, double*);
 hipError_t launch_scan(hipStream_t, const double*, const PotentialRecord*, const WaveTask*, int32_t,
                        int32_t, int32_t*, double*, int32_t*);
 hipError_t launch_refine(hipStream_t, const double*, const PotentialRecord*, const WaveTask*,
                          int32_t, int32_t, const int32_t*, const double*, const int32_t*, double,
                          int32_t, double*, int32_t*, int32_t*);
+hipError_t launch_refine_segmented(hipStream_t, int32_t, const double*, const PotentialRecord*,
+                                   const WaveTask*, int32_t, const int32_t*, const double*,
+                                   const int32_t*, double, int32_t, double*, int32_t*, int32_t*);
 
 namespace {
 
@@ -131,6 +134,11 @@ hipError_t arena_array(Arena& a, size_t count, T** out)
     return e;
 }
 
+// Batches with at least this many levels fill the GPU with one lane per
+// level (2048 wavefronts); smaller ones refine with one segmented wavefront
+// per level.
+constexpr int64_t kLanePerLevelMinLevels = int64_t(64) * 2048;
+
 // Node-sampling stride: the largest power of two B <= 32 with B * theta_max
 // <= pi/2, theta_max being the discrete phase advance per step at the
 // highest kinetic energy of the window.
@@ -160,7 +168,7 @@ struct epseon_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
     Arena arena;
-    int32_t waves_per_block = 4;
+    int32_t waves_per_block = 1;
 
     std::vector<epseon_potential_desc> desc;
     std::vector<PotentialRecord> rec;
@@ -169,10 +177,15 @@ struct epseon_ctx {
 
     double* d_v = nullptr;
     double* d_ut = nullptr;
+    double* d_uv = nullptr;
     PotentialRecord* d_rec = nullptr;
     WaveTask* d_scan_tasks = nullptr;
     WaveTask* d_refine_tasks = nullptr;
-    int32_t n_scan_tasks = 0, n_refine_tasks = 0;
+    WaveTask* d_level_tasks = nullptr;
+    int32_t n_scan_tasks = 0, n_refine_tasks = 0, n_level_tasks = 0;
+    int32_t refine_mode = EPSEON_REFINE_AUTO;
+    int32_t last_refine_mode = EPSEON_REFINE_AUTO;
+    int32_t last_refine_lanes = 0;
     int32_t* d_nodes = nullptr;
     double* d_far = nullptr;
     int32_t* d_resc = nullptr;
@@ -439,7 +452,8 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
         return ctx->rec[a].n_points > ctx->rec[b].n_points;
     });
-    std::vector<WaveTask> scan_tasks, refine_tasks;
+    // Segmented refinement uses one 64-lane block per level.
+    std::vector<WaveTask> scan_tasks, refine_tasks, level_tasks;
     for (int32_t p : order) {
         const PotentialRecord& r = ctx->rec[p];
         if (r.flags == EPSEON_POT_OK)
@@ -447,9 +461,12 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
                 scan_tasks.push_back({p, j});
         for (int32_t l = 0; l < r.level_count; l += 64)
             refine_tasks.push_back({p, l});
+        for (int32_t l = 0; l < r.level_count; ++l)
+            level_tasks.push_back({p, l});
     }
     ctx->n_scan_tasks = static_cast<int32_t>(scan_tasks.size());
     ctx->n_refine_tasks = static_cast<int32_t>(refine_tasks.size());
+    ctx->n_level_tasks = static_cast<int32_t>(level_tasks.size());
 
     ctx->arena.reset();
     Arena& a = ctx->arena;
@@ -458,9 +475,11 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     const size_t nl = static_cast<size_t>(std::max<int64_t>(lv, 1));
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_v));
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_ut));
+    EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_uv));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(count, 1), &ctx->d_rec));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(scan_tasks.size(), 1), &ctx->d_scan_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(refine_tasks.size(), 1), &ctx->d_refine_tasks));
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(level_tasks.size(), 1), &ctx->d_level_tasks));
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_nodes));
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_far));
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_resc));
@@ -484,11 +503,15 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
         EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_refine_tasks, refine_tasks.data(),
                                       refine_tasks.size() * sizeof(WaveTask),
                                       hipMemcpyHostToDevice, ctx->stream));
+    if (!level_tasks.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_level_tasks, level_tasks.data(),
+                                      level_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice,
+                                      ctx->stream));
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_nodes, 0, ne * sizeof(int32_t), ctx->stream));
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_far, 0, ne * sizeof(double), ctx->stream));
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_resc, 0, ne * sizeof(int32_t), ctx->stream));
     EPSEON_HIP_TRY(launch_shift_potential(ctx->stream, ctx->d_v, ctx->d_rec, count, max_points,
-                                          ctx->d_ut));
+                                          ctx->d_ut, ctx->d_uv));
     // The host buffers are pageable and owned by the caller: finish the
     // copies before returning.
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -543,14 +566,40 @@ int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
         return fail(EPSEON_ERR_INVALID, "relative tolerance must be finite and >= 0");
     if (max_iter < 1)
         return fail(EPSEON_ERR_INVALID, "max_iter must be >= 1");
+    int32_t mode = ctx->refine_mode;
+    if (mode == EPSEON_REFINE_AUTO)
+        mode = ctx->total_levels >= kLanePerLevelMinLevels ? EPSEON_REFINE_LANE : EPSEON_REFINE_SEGMENTED;
+    ctx->last_refine_mode = mode;
+    // Segmented blocks: up to 4 wavefronts per level while the batch has too
+    // few levels to give every SIMD a wavefront.
+    int32_t lanes = 64;
+    while (lanes < 256 && ctx->total_levels * (lanes / 64) * 2 <= 2048)
+        lanes *= 2;
+    ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
-    EPSEON_HIP_TRY(launch_refine(ctx->stream, ctx->d_ut, ctx->d_rec, ctx->d_refine_tasks,
-                                 ctx->n_refine_tasks, ctx->waves_per_block, ctx->d_nodes,
-                                 ctx->d_far, ctx->d_resc, rel_tol, max_iter, ctx->d_levels,
-                                 ctx->d_status, ctx->d_iters));
+    if (mode == EPSEON_REFINE_LANE)
+        EPSEON_HIP_TRY(launch_refine(ctx->stream, ctx->d_uv, ctx->d_rec, ctx->d_refine_tasks,
+                                     ctx->n_refine_tasks, ctx->waves_per_block, ctx->d_nodes,
+                                     ctx->d_far, ctx->d_resc, rel_tol, max_iter, ctx->d_levels,
+                                     ctx->d_status, ctx->d_iters));
+    else
+        EPSEON_HIP_TRY(launch_refine_segmented(ctx->stream, lanes, ctx->d_uv, ctx->d_rec,
+                                               ctx->d_level_tasks, ctx->n_level_tasks, ctx->d_nodes,
+                                               ctx->d_far, ctx->d_resc, rel_tol, max_iter,
+                                               ctx->d_levels, ctx->d_status, ctx->d_iters));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
     ctx->refined = true;
     ctx->refine_timed = true;
+    return EPSEON_OK;
+}
+
+int epseon_set_refine_mode(epseon_ctx* ctx, int32_t mode)
+{
+    if (ctx == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null context");
+    if (mode != EPSEON_REFINE_AUTO && mode != EPSEON_REFINE_LANE && mode != EPSEON_REFINE_SEGMENTED)
+        return fail(EPSEON_ERR_INVALID, "unknown refine mode");
+    ctx->refine_mode = mode;
     return EPSEON_OK;
 }
 
@@ -671,6 +720,8 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
         }
         stats->refine_point_updates = pu;
         stats->levels_converged = ok;
+        stats->refine_mode = ctx->last_refine_mode;
+        stats->refine_lanes = ctx->last_refine_lanes;
     }
     return EPSEON_OK;
 }

@@ -35,56 +35,96 @@ __device__ __forceinline__ uint32_t sign_bit(double x)
     return static_cast<uint32_t>(__double_as_longlong(x) >> 63) & 1u;
 }
 
-struct ShotState {
-    double wm, w, fm;
-    int32_t nodes, rescales;
-    uint32_t last_sign;
-};
-
-// Loop constants of t = fma(-10, f, 12).  Neither is an inline constant of
-// the FP64 VALU, so the compiler would otherwise rebuild 12.0 in the
-// accumulator of a v_fmac every step (two extra v_mov per step); held opaque
-// in VGPRs the step is a single three-VGPR v_fma_f64.
+// Opaque loop constants.  -10, 12 and 10 are not inline constants of the
+// FP64 VALU; held in VGPRs the compiler cannot rebuild them every step.
 struct StepConsts {
-    double m10, c12;
+    double m10, c12, c10;
 };
 
 __device__ __forceinline__ StepConsts step_consts()
 {
-    StepConsts k{-10.0, 12.0};
-    asm volatile("" : "+v"(k.m10), "+v"(k.c12));
+    StepConsts k{-10.0, 12.0, 10.0};
+    asm volatile("" : "+v"(k.m10), "+v"(k.c12), "+v"(k.c10));
     return k;
 }
 
-__device__ __forceinline__ void numerov_step(const double u, const double e, const StepConsts& k,
-                                             ShotState& s)
-{
-    const double f = e - u;
-    const double t = __builtin_fma(k.m10, f, k.c12);
-    const double g = f * s.fm;
-    const double wn = __builtin_fma(t, s.w, -(g * s.wm));
-    s.wm = s.w;
-    s.w = wn;
-    s.fm = f;
-}
+// Scan form: division-free homogeneous Numerov on u_i = beta V_i - 1,
+// f_i = e - u_i, 5 FP64 VALU ops per step.
+struct ScanForm {
+    double wm, w, fm;
+    __device__ __forceinline__ void init(const double e, const double u0)
+    {
+        wm = 0.0;
+        w = 1.0;
+        fm = e - u0;
+    }
+    __device__ __forceinline__ void step(const double u, const double e, const StepConsts& k)
+    {
+        const double f = e - u;
+        const double t = __builtin_fma(k.m10, f, k.c12);
+        const double g = f * fm;
+        const double wn = __builtin_fma(t, w, -(g * wm));
+        wm = w;
+        w = wn;
+        fm = f;
+    }
+    __device__ __forceinline__ void rescale()
+    {
+        const bool big = fmax(fabs(w), fabs(wm)) >= kRescaleAt;
+        const int32_t sh = big ? -kRescaleLog2 : 0;
+        w = __builtin_ldexp(w, sh);
+        wm = __builtin_ldexp(wm, sh);
+        rescales += big ? 1 : 0;
+    }
+    int32_t rescales;
+};
 
-__device__ __forceinline__ void sample_sign(ShotState& s)
-{
-    const uint32_t sb = sign_bit(s.w);
-    s.nodes += static_cast<int32_t>(sb ^ s.last_sign);
-    s.last_sign = sb;
-}
+// Difference form on uv_i = beta V_i (refinement): eps_i = e - uv_i carried
+// at full relative precision, 8 FP64 VALU ops per step:
+//   sig = fma(eps_i, eps_{i-1}, eps_i + eps_{i-1}),
+//   c = fma(10 eps_i, w_i, sig w_{i-1}), d -= c, w += d.
+struct DiffForm {
+    double w, d, wm, em;
+    __device__ __forceinline__ void init(const double e, const double u0)
+    {
+        w = 1.0;
+        d = 1.0;
+        wm = 0.0;
+        em = e - u0;
+    }
+    __device__ __forceinline__ void step(const double u, const double e, const StepConsts& k)
+    {
+        const double ep = e - u;
+        const double e10 = ep * k.c10;
+        const double sg = __builtin_fma(ep, em, ep + em);
+        const double c = __builtin_fma(e10, w, sg * wm);
+        d = d - c;
+        wm = w;
+        w = w + d;
+        em = ep;
+    }
+    __device__ __forceinline__ void rescale()
+    {
+        const bool big = fmax(fabs(w), fabs(wm)) >= kRescaleAt;
+        const int32_t sh = big ? -kRescaleLog2 : 0;
+        w = __builtin_ldexp(w, sh);
+        d = __builtin_ldexp(d, sh);
+        wm = __builtin_ldexp(wm, sh);
+        rescales += big ? 1 : 0;
+    }
+    int32_t rescales;
+};
 
-// Branch-free overflow guard: a branch here would split the super-block and
-// let the compiler sink half of the next prefetch behind it.
-__device__ __forceinline__ void rescale(ShotState& s)
-{
-    const bool big = fmax(fabs(s.w), fabs(s.wm)) >= kRescaleAt;
-    const int32_t sh = big ? -kRescaleLog2 : 0;
-    s.w = __builtin_ldexp(s.w, sh);
-    s.wm = __builtin_ldexp(s.wm, sh);
-    s.rescales += big ? 1 : 0;
-}
+struct SignCounter {
+    int32_t nodes;
+    uint32_t last;
+    __device__ __forceinline__ void sample(const double w)
+    {
+        const uint32_t sb = sign_bit(w);
+        nodes += static_cast<int32_t>(sb ^ last);
+        last = sb;
+    }
+};
 
 // Scheduling fence: nothing moves across it, so the next buffer's scalar
 // loads issue right after the first use of the current buffer.  The wait the
@@ -93,37 +133,40 @@ __device__ __forceinline__ void rescale(ShotState& s)
 // order, so every wait is lgkmcnt(0)).
 __device__ __forceinline__ void fence_schedule() { __builtin_amdgcn_sched_barrier(0); }
 
-template <int B, int Q>
+template <int B, int Q, class Form>
 __device__ __forceinline__ void step_at(const double u, const double e, const StepConsts& k,
-                                        ShotState& s)
+                                        Form& s, SignCounter& c)
 {
-    numerov_step(u, e, k, s);
+    s.step(u, e, k);
     if constexpr ((Q + 1) % B == 0)
-        sample_sign(s);
+        c.sample(s.w);
 }
 
-template <int B, int Q0, int... Is>
+template <int B, int Q0, class Form, int... Is>
 __device__ __forceinline__ void steps_1_15(const double (&buf)[16], const double e,
-                                           const StepConsts& k, ShotState& s,
+                                           const StepConsts& k, Form& s, SignCounter& c,
                                            std::integer_sequence<int, Is...>)
 {
-    (step_at<B, Q0 + Is + 1>(buf[Is + 1], e, k, s), ...);
+    (step_at<B, Q0 + Is + 1>(buf[Is + 1], e, k, s, c), ...);
 }
 
-// One outward shot over n points at probe e = beta E with node stride B.
-// The grid is consumed in super-blocks of 32 steps fed from two 16-point
-// SGPR buffers: each buffer's scalar loads issue right after the first step
-// of the other buffer, so a load has 15 steps to land.  The rescale test runs
-// once per super-block, after its 17th step; the sign is sampled after every
-// B-th step.  The table is padded so the prefetch past the last super-block
-// stays inside the allocation.
-template <int B>
+// One outward shot over n points at probe e with node stride B, the grid
+// table u being wave-uniform.  The grid is consumed in super-blocks of 32
+// steps fed from two 16-point SGPR buffers: each buffer's scalar loads issue
+// right after the first step of the other buffer, so a load has 15 steps to
+// land.  The rescale test runs once per super-block, after its 17th step; the
+// sign is sampled after every B-th step and at the last point.  The table is
+// padded so the prefetch past the last super-block stays in the allocation.
+template <int B, class Form>
 __device__ __forceinline__ int32_t shoot_fixed(const double* __restrict__ u, const int32_t n,
                                                const double e, double& w_last, int32_t& rescales)
 {
     static_assert(32 % B == 0, "stride must divide the super-block");
     const StepConsts k = step_consts();
-    ShotState s{0.0, 1.0, e - u[0], 0, 0, 0u};
+    Form s;
+    s.init(e, u[0]);
+    s.rescales = 0;
+    SignCounter c{0, 0u};
     const int32_t steps = n - 2;
     const int32_t nsb = steps >> 5;
     const double* p = u + 1;
@@ -132,47 +175,48 @@ __device__ __forceinline__ int32_t shoot_fixed(const double* __restrict__ u, con
     for (int q = 0; q < 16; ++q)
         a[q] = p[q];
     for (int32_t sb = 0; sb < nsb; ++sb) {
-        step_at<B, 0>(a[0], e, k, s);
+        step_at<B, 0>(a[0], e, k, s, c);
         fence_schedule();
 #pragma unroll
         for (int q = 0; q < 16; ++q)
             b[q] = p[16 + q];
         fence_schedule();
-        steps_1_15<B, 0>(a, e, k, s, std::make_integer_sequence<int, 15>{});
-        step_at<B, 16>(b[0], e, k, s);
-        rescale(s);
+        steps_1_15<B, 0>(a, e, k, s, c, std::make_integer_sequence<int, 15>{});
+        step_at<B, 16>(b[0], e, k, s, c);
+        s.rescale();
         fence_schedule();
 #pragma unroll
         for (int q = 0; q < 16; ++q)
             a[q] = p[32 + q];
         fence_schedule();
-        steps_1_15<B, 16>(b, e, k, s, std::make_integer_sequence<int, 15>{});
+        steps_1_15<B, 16>(b, e, k, s, c, std::make_integer_sequence<int, 15>{});
         p += 32;
     }
     const int32_t rem = steps - (nsb << 5);
     for (int32_t q = 0; q < rem; ++q) {
-        numerov_step(p[q], e, k, s);
+        s.step(p[q], e, k);
         if ((q + 1) % B == 0)
-            sample_sign(s);
+            c.sample(s.w);
     }
     if (rem % B != 0)
-        sample_sign(s);
+        c.sample(s.w);
     w_last = s.w;
     rescales = s.rescales;
-    return s.nodes;
+    return c.nodes;
 }
 
+template <class Form>
 __device__ __forceinline__ int32_t shoot(const double* __restrict__ u, const int32_t n,
                                          const double e, const int32_t stride, double& w_last,
                                          int32_t& rescales)
 {
     switch (stride) {
-    case 32: return shoot_fixed<32>(u, n, e, w_last, rescales);
-    case 16: return shoot_fixed<16>(u, n, e, w_last, rescales);
-    case 8: return shoot_fixed<8>(u, n, e, w_last, rescales);
-    case 4: return shoot_fixed<4>(u, n, e, w_last, rescales);
-    case 2: return shoot_fixed<2>(u, n, e, w_last, rescales);
-    default: return shoot_fixed<1>(u, n, e, w_last, rescales);
+    case 32: return shoot_fixed<32, Form>(u, n, e, w_last, rescales);
+    case 16: return shoot_fixed<16, Form>(u, n, e, w_last, rescales);
+    case 8: return shoot_fixed<8, Form>(u, n, e, w_last, rescales);
+    case 4: return shoot_fixed<4, Form>(u, n, e, w_last, rescales);
+    case 2: return shoot_fixed<2, Form>(u, n, e, w_last, rescales);
+    default: return shoot_fixed<1, Form>(u, n, e, w_last, rescales);
     }
 }
 
@@ -185,16 +229,21 @@ __device__ __forceinline__ int32_t wave_task_index()
 
 }  // namespace
 
-// u_i = fma(beta_p, V_i, -1) for every point of every potential.
+// u_i = fma(beta_p, V_i, -1) (scan table) and uv_i = beta_p V_i (refinement
+// table) for every point of the trimmed grid of every potential.
 __global__ __launch_bounds__(256) void shift_potential_kernel(const double* __restrict__ v,
                                                               const PotentialRecord* __restrict__ pots,
-                                                              double* __restrict__ ut)
+                                                              double* __restrict__ ut,
+                                                              double* __restrict__ uv)
 {
     const PotentialRecord& P = pots[blockIdx.y];
     const double beta = P.beta;
     for (int32_t i = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x); i < P.n_points;
-         i += static_cast<int32_t>(gridDim.x * blockDim.x))
-        ut[P.ut_offset + i] = __builtin_fma(beta, v[P.ut_offset + i], -1.0);
+         i += static_cast<int32_t>(gridDim.x * blockDim.x)) {
+        const double x = v[P.ut_offset + i];
+        ut[P.ut_offset + i] = __builtin_fma(beta, x, -1.0);
+        uv[P.ut_offset + i] = beta * x;
+    }
 }
 
 // Scan: lane = trial energy E_j = fma(j+1, dE, E_lo); outputs node count,
@@ -219,21 +268,146 @@ __global__ __launch_bounds__(256) void numerov_scan_kernel(const double* __restr
     const double e = P.beta * E;
     double w;
     int32_t k;
-    const int32_t c = shoot(ut + P.ut_offset, P.n_points, e, P.stride, w, k);
+    const int32_t c = shoot<ScanForm>(ut + P.ut_offset, P.n_points, e, P.stride, w, k);
     const int64_t o = P.energy_offset + j;
     nodes[o] = c;
     far_end[o] = w;
     rescales[o] = k;
 }
 
-// Bracket + refine: lane = level v.  The bracket is the first trial energy
-// with more than v nodes and the one below it (or the window floor).  While
-// the bracket holds more than level v it is halved by node count; then the
-// Illinois variant of regula falsi runs on F(E) = w_{N-1} 2^(480 k), each new
-// point kept at least tol/2 inside the bracket.  The level is the midpoint of
-// the final bracket, whose width is <= rel_tol max(|E_lo|, |E_hi|) + abs_tol.
+namespace {
+
+// Bracket of level v from the scan: the first trial energy with more than v
+// nodes and the one below it (or the window floor, below which no level
+// exists since every eigenvalue exceeds min V).  Exponents are binary.
+struct Bracket {
+    double el, eh, wl, wh;
+    int32_t kl, kh, cl, ch, lo_ok, found;
+};
+
+__device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const int32_t v,
+                                                const int32_t* __restrict__ nodes,
+                                                const double* __restrict__ far_end,
+                                                const int32_t* __restrict__ rescales)
+{
+    Bracket b{};
+    const int32_t m = P.n_energies;
+    const int32_t* nd = nodes + P.energy_offset;
+    int32_t lo = 0, hi = m;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (nd[mid] > v)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    const int32_t j = lo;
+    if (j == m) {
+        b.found = 0;
+        return b;
+    }
+    b.found = 1;
+    b.eh = __builtin_fma(static_cast<double>(j + 1), P.de, P.e_lo);
+    b.ch = nd[j];
+    b.wh = far_end[P.energy_offset + j];
+    b.kh = rescales[P.energy_offset + j] * kRescaleLog2;
+    if (j == 0 || nd[j - 1] > v) {
+        b.el = P.e_lo;
+        b.cl = 0;
+        b.lo_ok = 0;
+    } else {
+        b.el = __builtin_fma(static_cast<double>(j), P.de, P.e_lo);
+        b.cl = nd[j - 1];
+        b.wl = far_end[P.energy_offset + j - 1];
+        b.kl = rescales[P.energy_offset + j - 1] * kRescaleLog2;
+        b.lo_ok = 1;
+    }
+    return b;
+}
+
+// Refinement state machine shared by both refine kernels.  While the
+// bracket holds more than level v (or its floor has no shot) the probe is the
+// midpoint and needs a counted shot; once it holds exactly level v the probe
+// is the Illinois regula-falsi point on F = w 2^k, kept at least tol/2
+// inside, and the probe's count follows from the sign of F.  Stops when the
+// width is <= rel_tol max(|E_lo|, |E_hi|) + abs_tol.
+struct Refiner {
+    Bracket b;
+    int32_t v, side, it;
+    bool isolated;
+
+    __device__ __forceinline__ bool converged(const double rel_tol, const double abs_tol) const
+    {
+        const double tol = __builtin_fma(rel_tol, fmax(fabs(b.el), fabs(b.eh)), abs_tol);
+        return (b.eh - b.el) <= tol;
+    }
+
+    __device__ __forceinline__ double probe(const double rel_tol, const double abs_tol)
+    {
+        const double width = b.eh - b.el;
+        const double tol = __builtin_fma(rel_tol, fmax(fabs(b.el), fabs(b.eh)), abs_tol);
+        isolated = b.lo_ok && b.cl == v && b.ch == v + 1;
+        double em;
+        if (isolated) {
+            const double r = __builtin_ldexp(b.wh / b.wl, b.kh - b.kl);
+            const double s = 1.0 / (1.0 - r);
+            em = __builtin_fma(s, width, b.el);
+            const double d = 0.5 * tol;
+            if (!(em >= b.el + d))
+                em = b.el + d;
+            if (!(em <= b.eh - d))
+                em = b.eh - d;
+        } else {
+            em = b.el + 0.5 * width;
+            side = 0;
+        }
+        return em;
+    }
+
+    // Take the probe's result; `c` is ignored when the bracket is isolated.
+    __device__ __forceinline__ void update(const double em, const double w, const int32_t k, int32_t c)
+    {
+        if (isolated)
+            c = (static_cast<int32_t>(sign_bit(w)) == (v & 1)) ? v : v + 1;
+        ++it;
+        if (c <= v) {
+            b.el = em;
+            b.cl = c;
+            b.wl = w;
+            b.kl = k;
+            b.lo_ok = 1;
+            if (isolated && side == -1)
+                b.wh *= 0.5;
+            side = isolated ? -1 : 0;
+        } else {
+            b.eh = em;
+            b.ch = c;
+            b.wh = w;
+            b.kh = k;
+            if (isolated && side == 1)
+                b.wl *= 0.5;
+            side = isolated ? 1 : 0;
+        }
+    }
+
+    __device__ __forceinline__ double level() const { return b.el + 0.5 * (b.eh - b.el); }
+};
+
+__device__ __forceinline__ void write_level(double* levels, int32_t* status, int32_t* iterations,
+                                            const int64_t out, const double lv, const int32_t st,
+                                            const int32_t it)
+{
+    levels[out] = lv;
+    status[out] = st;
+    iterations[out] = it;
+}
+
+}  // namespace
+
+// Lane-per-level refinement: lane = level v of the wavefront's potential,
+// sequential difference-form shots over the wave-uniform uv table.
 __global__ __launch_bounds__(256) void numerov_refine_kernel(
-    const double* __restrict__ ut, const PotentialRecord* __restrict__ pots,
+    const double* __restrict__ uv, const PotentialRecord* __restrict__ pots,
     const WaveTask* __restrict__ tasks, const int32_t n_tasks, const int32_t* __restrict__ nodes,
     const double* __restrict__ far_end, const int32_t* __restrict__ rescales, const double rel_tol,
     const int32_t max_iter, double* __restrict__ levels, int32_t* __restrict__ status,
@@ -249,116 +423,343 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
         return;
     const int64_t out = P.level_offset + l;
     if (P.flags != 0) {
-        levels[out] = __builtin_nan("");
-        status[out] = 3;  // EPSEON_LEVEL_UNRESOLVED
-        iterations[out] = 0;
+        write_level(levels, status, iterations, out, __builtin_nan(""), 3, 0);
         return;
     }
-    const int32_t v = P.level_min + l;
-    const int32_t m = P.n_energies;
-    const int32_t* nd = nodes + P.energy_offset;
-    int32_t lo = 0, hi = m;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (nd[mid] > v)
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
-    const int32_t j = lo;
-    if (j == m) {
-        levels[out] = __builtin_nan("");
-        status[out] = 1;  // EPSEON_LEVEL_UNBOUND
-        iterations[out] = 0;
+    Refiner r{};
+    r.v = P.level_min + l;
+    r.b = find_bracket(P, r.v, nodes, far_end, rescales);
+    if (!r.b.found) {
+        write_level(levels, status, iterations, out, __builtin_nan(""), 1, 0);
         return;
     }
-    double eh = __builtin_fma(static_cast<double>(j + 1), P.de, P.e_lo);
-    int32_t ch = nd[j];
-    double wh = far_end[P.energy_offset + j];
-    int32_t kh = rescales[P.energy_offset + j];
-    double el, wl = 0.0;
-    int32_t cl, kl = 0, lo_ok;
-    if (j == 0 || nd[j - 1] > v) {
-        el = P.e_lo;
-        cl = 0;
-        lo_ok = 0;
-    } else {
-        el = __builtin_fma(static_cast<double>(j), P.de, P.e_lo);
-        cl = nd[j - 1];
-        wl = far_end[P.energy_offset + j - 1];
-        kl = rescales[P.energy_offset + j - 1];
-        lo_ok = 1;
-    }
-
-    const double* u = ut + P.ut_offset;
-    const int32_t n = P.n_points;
-    const int32_t stride = P.stride;
-    const double beta = P.beta;
-    int32_t side = 0;
-    int32_t it = 0;
+    const double* u = uv + P.ut_offset;
     int32_t st = 0;
     for (;;) {
-        const double width = eh - el;
-        const double tol = __builtin_fma(rel_tol, fmax(fabs(el), fabs(eh)), P.abs_tol);
-        if (width <= tol)
+        if (r.converged(rel_tol, P.abs_tol))
             break;
-        if (it >= max_iter) {
-            st = 2;  // EPSEON_LEVEL_MAXITER
+        if (r.it >= max_iter) {
+            st = 2;
             break;
         }
-        double em;
-        const bool isolated = lo_ok && cl == v && ch == v + 1;
-        if (isolated) {
-            const double r = __builtin_ldexp(wh / wl, kRescaleLog2 * (kh - kl));
-            const double s = 1.0 / (1.0 - r);
-            em = __builtin_fma(s, width, el);
-            const double d = 0.5 * tol;
-            if (!(em >= el + d))
-                em = el + d;
-            if (!(em <= eh - d))
-                em = eh - d;
-        } else {
-            em = el + 0.5 * width;
-            side = 0;
-        }
+        const double em = r.probe(rel_tol, P.abs_tol);
         double w;
         int32_t k;
-        const int32_t c = shoot(u, n, beta * em, stride, w, k);
-        ++it;
-        if (c <= v) {
-            el = em;
-            cl = c;
-            wl = w;
-            kl = k;
-            lo_ok = 1;
-            if (isolated && side == -1)
-                wh *= 0.5;
-            side = isolated ? -1 : 0;
-        } else {
-            eh = em;
-            ch = c;
-            wh = w;
-            kh = k;
-            if (isolated && side == 1)
-                wl *= 0.5;
-            side = isolated ? 1 : 0;
+        const int32_t c = shoot<DiffForm>(u, P.n_points, P.beta * em, P.stride, w, k);
+        r.update(em, w, k * kRescaleLog2, c);
+    }
+    write_level(levels, status, iterations, out, r.level(), st, r.it);
+}
+
+namespace {
+
+// ---- segmented difference-form shots (one block per level) -------------
+
+__device__ __forceinline__ int32_t frexp_exponent(const double m)
+{
+    int e = 0;
+    (void)__builtin_frexp(m, &e);
+    return e;
+}
+
+__device__ __forceinline__ int64_t segment_start(const int32_t q, const int32_t segs, const int32_t s)
+{
+    return (static_cast<int64_t>(s) * q) / segs;
+}
+
+struct SegCount {
+    int32_t internal, first, last, has;
+};
+
+template <class Body>
+__device__ __forceinline__ void walk_chunk(const double (&buf)[16], const int32_t j, Body& body)
+{
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        body(buf[k], j + k);
+}
+
+template <class Body>
+__device__ __forceinline__ void walk_tail(const double (&buf)[16], const int32_t j, const int32_t len,
+                                          Body& body)
+{
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        if (j + k < len)
+            body(buf[k], j + k);
+}
+
+template <int K>
+__device__ __forceinline__ void load16(double (&buf)[16], const double* __restrict__ p)
+{
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        buf[k] = p[k];
+}
+
+// Visit the `len` grid values p[0..len) in order, body(value, index).  Three
+// 16-value VGPR buffers rotate so that every load is issued two chunks (32
+// steps) before its use; vector-memory loads return in order, so the waits
+// are partial.  Reads up to 48 values past `len` (the tables are padded).
+template <class Body>
+__device__ __forceinline__ void walk_segment(const double* __restrict__ p, const int32_t len, Body& body)
+{
+    double A[16], B[16], C[16];
+    load16<0>(A, p);
+    load16<0>(B, p + 16);
+    int32_t j = 0;
+    for (; j + 48 <= len; j += 48) {
+        load16<0>(C, p + j + 32);
+        walk_chunk(A, j, body);
+        load16<0>(A, p + j + 48);
+        walk_chunk(B, j + 16, body);
+        load16<0>(B, p + j + 64);
+        walk_chunk(C, j + 32, body);
+    }
+    const int32_t rem = len - j;
+    if (rem >= 32) {
+        load16<0>(C, p + j + 32);
+        walk_chunk(A, j, body);
+        walk_chunk(B, j + 16, body);
+        walk_tail(C, j + 32, len, body);
+    } else if (rem >= 16) {
+        walk_chunk(A, j, body);
+        walk_tail(B, j + 16, len, body);
+    } else {
+        walk_tail(A, j, len, body);
+    }
+}
+
+// Two difference-form columns through one segment: the segment's transfer
+// matrix on (w, d).  All six carried values are renormalised after every
+// 32nd step.
+struct MatrixBody {
+    double e, em;
+    double wa, da, pa, wb, db, pb;
+    int32_t ex;
+    double c10;
+    __device__ __forceinline__ void operator()(const double u, const int32_t idx)
+    {
+        const double ep = e - u;
+        const double e10 = ep * c10;
+        const double sg = __builtin_fma(ep, em, ep + em);
+        const double ca = __builtin_fma(e10, wa, sg * pa);
+        const double cb = __builtin_fma(e10, wb, sg * pb);
+        da = da - ca;
+        db = db - cb;
+        pa = wa;
+        pb = wb;
+        wa = wa + da;
+        wb = wb + db;
+        em = ep;
+        if ((idx & 31) == 31) {
+            const double mx = fmax(fmax(fmax(fabs(wa), fabs(da)), fmax(fabs(pa), fabs(wb))),
+                                   fmax(fabs(db), fabs(pb)));
+            const int32_t sh = frexp_exponent(mx);
+            wa = __builtin_ldexp(wa, -sh);
+            da = __builtin_ldexp(da, -sh);
+            pa = __builtin_ldexp(pa, -sh);
+            wb = __builtin_ldexp(wb, -sh);
+            db = __builtin_ldexp(db, -sh);
+            pb = __builtin_ldexp(pb, -sh);
+            ex += sh;
         }
     }
-    levels[out] = el + 0.5 * (eh - el);
-    status[out] = st;
-    iterations[out] = it;
+};
+
+// One difference-form solution through a segment, sampling signs at the
+// global sample positions (q + 1) % stride == 0 and the last point.
+struct CountBody {
+    double e, em, w, d, wm;
+    int32_t q0, qn, stride;
+    SegCount sc;
+    double c10;
+    __device__ __forceinline__ void operator()(const double u, const int32_t idx)
+    {
+        const double ep = e - u;
+        const double e10 = ep * c10;
+        const double sg = __builtin_fma(ep, em, ep + em);
+        const double cc = __builtin_fma(e10, w, sg * wm);
+        d = d - cc;
+        wm = w;
+        w = w + d;
+        em = ep;
+        const int32_t q = q0 + idx;
+        if ((q + 1) % stride == 0 || (q == qn - 1 && qn % stride != 0)) {
+            const int32_t sb = static_cast<int32_t>(sign_bit(w));
+            if (sc.has)
+                sc.internal += sb ^ sc.last;
+            else
+                sc.first = sb;
+            sc.last = sb;
+            sc.has = 1;
+        }
+        if ((idx & 31) == 31) {
+            const int32_t sh = frexp_exponent(fmax(fmax(fabs(w), fabs(d)), fabs(wm)));
+            w = __builtin_ldexp(w, -sh);
+            d = __builtin_ldexp(d, -sh);
+            wm = __builtin_ldexp(wm, -sh);
+        }
+    }
+};
+
+}  // namespace
+
+// Segmented refinement: one T-thread block per level.  Lane s owns steps
+// [q_s, q_{s+1}) of the trimmed grid (S = clamp(Q/32, 1, T) segments) and
+// builds that segment's 2x2 difference-form transfer matrix (columns (w, d) =
+// (1, 0) and (0, 1), renormalised every 32 steps and at the end).  The
+// far-end value is the binary-tree product of the matrices (M_i <- M_{i+d} M_i
+// at d = 1, 2, 4, ...) applied to (w_1, d_1) = (1, 1).  Counted shots chain
+// the matrices sequentially (lane 0) for the segment start vectors and re-run
+// every segment.
+template <int T>
+__global__ __launch_bounds__(T) void numerov_refine_segmented_kernel(
+    const double* __restrict__ uv, const PotentialRecord* __restrict__ pots,
+    const WaveTask* __restrict__ tasks, const int32_t n_tasks, const int32_t* __restrict__ nodes,
+    const double* __restrict__ far_end, const int32_t* __restrict__ rescales, const double rel_tol,
+    const int32_t max_iter, double* __restrict__ levels, int32_t* __restrict__ status,
+    int32_t* __restrict__ iterations)
+{
+    __shared__ double s_m[4][T];
+    __shared__ int32_t s_e[T];
+    __shared__ double s_start[2][T];
+    __shared__ SegCount s_cnt[T];
+    __shared__ int32_t s_c;
+
+    const int32_t tid = static_cast<int32_t>(blockIdx.x);
+    if (tid >= n_tasks)
+        return;
+    const WaveTask task = tasks[tid];
+    const PotentialRecord& P = pots[task.potential];
+    const int32_t lane = static_cast<int32_t>(threadIdx.x);
+    const int64_t out = P.level_offset + task.first;
+    if (P.flags != 0) {
+        if (lane == 0)
+            write_level(levels, status, iterations, out, __builtin_nan(""), 3, 0);
+        return;
+    }
+    Refiner r{};
+    r.v = P.level_min + task.first;
+    r.b = find_bracket(P, r.v, nodes, far_end, rescales);
+    if (!r.b.found) {
+        if (lane == 0)
+            write_level(levels, status, iterations, out, __builtin_nan(""), 1, 0);
+        return;
+    }
+    const double* u = uv + P.ut_offset;
+    const int32_t n = P.n_points;
+    const int32_t qn = n - 2;
+    const int32_t segs = min(T, max(1, qn / 32));
+    const bool active = lane < segs;
+    const int32_t q0 = active ? static_cast<int32_t>(segment_start(qn, segs, lane)) : 0;
+    const int32_t q1 = active ? static_cast<int32_t>(segment_start(qn, segs, lane + 1)) : 0;
+    const StepConsts k = step_consts();
+
+    int32_t st = 0;
+    for (;;) {
+        if (r.converged(rel_tol, P.abs_tol))
+            break;
+        if (r.it >= max_iter) {
+            st = 2;
+            break;
+        }
+        const double em = r.probe(rel_tol, P.abs_tol);
+        const double e = P.beta * em;
+
+        if (active) {
+            MatrixBody mb{e, e - u[q0], 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10};
+            walk_segment(u + q0 + 1, q1 - q0, mb);
+            const double mx = fmax(fmax(fabs(mb.wa), fabs(mb.da)), fmax(fabs(mb.wb), fabs(mb.db)));
+            const int32_t sh = frexp_exponent(mx);
+            s_m[0][lane] = __builtin_ldexp(mb.wa, -sh);
+            s_m[1][lane] = __builtin_ldexp(mb.wb, -sh);
+            s_m[2][lane] = __builtin_ldexp(mb.da, -sh);
+            s_m[3][lane] = __builtin_ldexp(mb.db, -sh);
+            s_e[lane] = mb.ex + sh;
+        }
+        __syncthreads();
+
+        const bool counted = !r.isolated;
+        if (counted) {
+            if (lane == 0) {
+                double v0 = 1.0, v1 = 1.0;
+                for (int32_t s = 0; s < segs; ++s) {
+                    s_start[0][s] = v0;
+                    s_start[1][s] = v1;
+                    const double a = __builtin_fma(s_m[0][s], v0, s_m[1][s] * v1);
+                    const double b = __builtin_fma(s_m[2][s], v0, s_m[3][s] * v1);
+                    const int32_t sh = frexp_exponent(fmax(fabs(a), fabs(b)));
+                    v0 = __builtin_ldexp(a, -sh);
+                    v1 = __builtin_ldexp(b, -sh);
+                }
+            }
+            __syncthreads();
+        }
+
+        for (int32_t d = 1; d < segs; d *= 2) {
+            if ((lane & (2 * d - 1)) == 0 && lane + d < segs) {
+                const double a00 = s_m[0][lane + d], a01 = s_m[1][lane + d];
+                const double a10 = s_m[2][lane + d], a11 = s_m[3][lane + d];
+                const double b00 = s_m[0][lane], b01 = s_m[1][lane];
+                const double b10 = s_m[2][lane], b11 = s_m[3][lane];
+                const double c00 = __builtin_fma(a00, b00, a01 * b10);
+                const double c01 = __builtin_fma(a00, b01, a01 * b11);
+                const double c10 = __builtin_fma(a10, b00, a11 * b10);
+                const double c11 = __builtin_fma(a10, b01, a11 * b11);
+                const int32_t sh = frexp_exponent(
+                    fmax(fmax(fabs(c00), fabs(c01)), fmax(fabs(c10), fabs(c11))));
+                s_m[0][lane] = __builtin_ldexp(c00, -sh);
+                s_m[1][lane] = __builtin_ldexp(c01, -sh);
+                s_m[2][lane] = __builtin_ldexp(c10, -sh);
+                s_m[3][lane] = __builtin_ldexp(c11, -sh);
+                s_e[lane] = s_e[lane + d] + s_e[lane] + sh;
+            }
+            __syncthreads();
+        }
+        const double fw = __builtin_fma(s_m[0][0], 1.0, s_m[1][0] * 1.0);
+        const int32_t fk = s_e[0];
+
+        int32_t c = 0;
+        if (counted) {
+            if (active) {
+                const double w0 = s_start[0][lane], d0 = s_start[1][lane];
+                CountBody cb{e, e - u[q0], w0, d0, w0 - d0, q0, qn, P.stride, {0, 0, 0, 0}, k.c10};
+                walk_segment(u + q0 + 1, q1 - q0, cb);
+                s_cnt[lane] = cb.sc;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                int32_t total = 0, prev = 0;
+                for (int32_t s = 0; s < segs; ++s) {
+                    const SegCount sc = s_cnt[s];
+                    if (sc.has) {
+                        total += sc.internal + (sc.first ^ prev);
+                        prev = sc.last;
+                    }
+                }
+                s_c = total;
+            }
+            __syncthreads();
+            c = s_c;
+        }
+        r.update(em, fw, fk, c);
+        __syncthreads();
+    }
+    if (lane == 0)
+        write_level(levels, status, iterations, out, r.level(), st, r.it);
 }
 
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_shift_potential(hipStream_t stream, const double* v, const PotentialRecord* pots,
-                                  int32_t n_pots, int32_t max_points, double* ut)
+                                  int32_t n_pots, int32_t max_points, double* ut, double* uv)
 {
     if (n_pots == 0)
         return hipSuccess;
     const int32_t bx = (max_points + 255) / 256;
     dim3 grid(static_cast<unsigned>(bx < 1024 ? bx : 1024), static_cast<unsigned>(n_pots));
-    hipLaunchKernelGGL(shift_potential_kernel, grid, dim3(256), 0, stream, v, pots, ut);
+    hipLaunchKernelGGL(shift_potential_kernel, grid, dim3(256), 0, stream, v, pots, ut, uv);
     return hipGetLastError();
 }
 
@@ -375,7 +776,7 @@ hipError_t launch_scan(hipStream_t stream, const double* ut, const PotentialReco
     return hipGetLastError();
 }
 
-hipError_t launch_refine(hipStream_t stream, const double* ut, const PotentialRecord* pots,
+hipError_t launch_refine(hipStream_t stream, const double* uv, const PotentialRecord* pots,
                          const WaveTask* tasks, int32_t n_tasks, int32_t waves_per_block,
                          const int32_t* nodes, const double* far_end, const int32_t* rescales,
                          double rel_tol, int32_t max_iter, double* levels, int32_t* status,
@@ -385,9 +786,40 @@ hipError_t launch_refine(hipStream_t stream, const double* ut, const PotentialRe
         return hipSuccess;
     const int32_t blocks = (n_tasks + waves_per_block - 1) / waves_per_block;
     hipLaunchKernelGGL(numerov_refine_kernel, dim3(static_cast<unsigned>(blocks)),
-                       dim3(static_cast<unsigned>(64 * waves_per_block)), 0, stream, ut, pots,
+                       dim3(static_cast<unsigned>(64 * waves_per_block)), 0, stream, uv, pots,
                        tasks, n_tasks, nodes, far_end, rescales, rel_tol, max_iter, levels, status,
                        iterations);
+    return hipGetLastError();
+}
+
+hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, const double* uv,
+                                   const PotentialRecord* pots, const WaveTask* tasks,
+                                   int32_t n_tasks, const int32_t* nodes, const double* far_end,
+                                   const int32_t* rescales, double rel_tol, int32_t max_iter,
+                                   double* levels, int32_t* status, int32_t* iterations)
+{
+    if (n_tasks == 0)
+        return hipSuccess;
+    const dim3 grid(static_cast<unsigned>(n_tasks));
+    switch (lanes) {
+    case 256:
+        hipLaunchKernelGGL(numerov_refine_segmented_kernel<256>, grid, dim3(256), 0, stream, uv, pots,
+                           tasks, n_tasks, nodes, far_end, rescales, rel_tol, max_iter, levels,
+                           status, iterations);
+        break;
+    case 128:
+        hipLaunchKernelGGL(numerov_refine_segmented_kernel<128>, grid, dim3(128), 0, stream, uv, pots,
+                           tasks, n_tasks, nodes, far_end, rescales, rel_tol, max_iter, levels,
+                           status, iterations);
+        break;
+    case 64:
+        hipLaunchKernelGGL(numerov_refine_segmented_kernel<64>, grid, dim3(64), 0, stream, uv, pots,
+                           tasks, n_tasks, nodes, far_end, rescales, rel_tol, max_iter, levels,
+                           status, iterations);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -41,6 +41,11 @@ extern "C" {
 #define EPSEON_LEVEL_MAXITER 2    /* iteration cap reached; value is the bracket midpoint */
 #define EPSEON_LEVEL_UNRESOLVED 3 /* potential rejected: grid too coarse for the window */
 
+/* ---- refinement modes (epseon_set_refine_mode) ------------------------- */
+#define EPSEON_REFINE_AUTO 0       /* LANE for batches of >= 131072 levels, else SEGMENTED */
+#define EPSEON_REFINE_LANE 1       /* one lane per level, sequential shots */
+#define EPSEON_REFINE_SEGMENTED 2  /* one wavefront per level, shots split over 64 grid segments */
+
 /* ---- per-potential flags ----------------------------------------------- */
 #define EPSEON_POT_OK 0
 #define EPSEON_POT_UNRESOLVED 1   /* 1 + beta (E_top - V_min) > 3/2, or fewer than 4 points left */
@@ -107,6 +112,8 @@ typedef struct epseon_stats {
     double normalize_point_updates;
     int64_t scan_launches;        /* kernel launches of the last scan */
     int64_t levels_converged;     /* levels with EPSEON_LEVEL_OK after the last refine */
+    int32_t refine_mode;          /* EPSEON_REFINE_LANE or _SEGMENTED used by the last refine */
+    int32_t refine_lanes;         /* threads per level of the last segmented refine (1: lane mode) */
 } epseon_stats;
 
 typedef struct epseon_ctx epseon_ctx;
@@ -166,10 +173,17 @@ int epseon_scan(epseon_ctx* ctx);
 /* Bracket every requested level between adjacent trial energies by node count
  * and refine it (node-count bisection until isolated, then Illinois regula
  * falsi on the far-end value) until the bracket is narrower than
- * rel_tol*|E| + abs_tol, abs_tol = 1e-15 (e_top - E_lo).  Requires a prior
- * epseon_scan.  Same dispatch site as above; its result is the reference's
- * output buffer of level_count values per slot (algorithm_config.hpp:189). */
+ * rel_tol*|E| + abs_tol, abs_tol = 1e-15 (e_top - E_lo).  Refinement shots
+ * use the difference form of the recurrence (on beta V, carrying
+ * beta (E - V) at full relative precision), in the mode chosen with
+ * epseon_set_refine_mode.  Requires a prior epseon_scan.  Same dispatch site
+ * as above; its result is the reference's output buffer of level_count
+ * values per slot (algorithm_config.hpp:189). */
 int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter);
+
+/* Select the refinement mode (EPSEON_REFINE_*).  No counterpart in the
+ * reference, whose algorithm record has no such knob. */
+int epseon_set_refine_mode(epseon_ctx* ctx, int32_t mode);
 
 /* epseon_scan followed by epseon_refine. */
 int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter);

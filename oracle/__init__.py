@@ -66,6 +66,8 @@ def _lib() -> ctypes.CDLL:
         "eo_grid_step": (d, [d, d, i64]),
         "eo_beta": (d, [d, d]),
         "eo_shifted_potential": (None, [_dp, i64, d, _dp]),
+        "eo_scaled_potential": (None, [_dp, i64, d, _dp]),
+        "eo_shoot_diff": (i32, [_dp, i64, d, i32, _dp, _ip]),
         "eo_morse": (None, [d, d, d, d, d, i64, _dp]),
         "eo_trim": (i32, [_dp, i64, d, d, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "eo_trial_energy": (d, [i64, d, d]),
@@ -78,12 +80,15 @@ def _lib() -> ctypes.CDLL:
         "eo_bracket_level": (None, [_ip, _dp, _ip, i64, d, d, i32, ctypes.POINTER(Bracket)]),
         "eo_refine_level": (
             i32,
-            [_dp, i64, d, i32, i32, ctypes.POINTER(Bracket), d, d, i32, _dp],
+            [_dp, i64, d, i32, i32, ctypes.POINTER(Bracket), d, d, i32, i32, i32, _dp],
         ),
         "eo_solve": (
             None,
-            [_dp, i64, d, d, d, i64, i32, i32, i32, d, d, i32, _ip, _dp, _ip, _dp, _ip],
+            [_dp, _dp, i64, d, d, d, i64, i32, i32, i32, d, d, i32, i32, i32, _ip, _dp, _ip, _dp, _ip],
         ),
+        "eo_segment_count": (i32, [i64, i32]),
+        "eo_segment_start": (i64, [i64, i32, i32]),
+        "eo_shoot_segmented": (None, [_dp, i64, d, i32, i32, _dp, _ip, _ip]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -132,6 +137,13 @@ def trim(v: np.ndarray, b: float, e_hi: float) -> tuple[int, int, bool]:
     return int(first.value), int(last.value), bool(ok)
 
 
+def scaled_potential(v: np.ndarray, b: float) -> np.ndarray:
+    v = _f64(v)
+    out = np.empty_like(v)
+    _lib().eo_scaled_potential(v.ctypes.data_as(_dp), len(v), b, out.ctypes.data_as(_dp))
+    return out
+
+
 def trial_spacing(e_lo: float, e_hi: float, m: int) -> float:
     return float(_lib().eo_trial_spacing(e_lo, e_hi, m))
 
@@ -152,6 +164,15 @@ def shoot(ut: np.ndarray, e: float, stride: int) -> tuple[int, float, int]:
     w = ctypes.c_double()
     k = ctypes.c_int32()
     c = _lib().eo_shoot(ut.ctypes.data_as(_dp), len(ut), e, stride, ctypes.byref(w), ctypes.byref(k))
+    return int(c), float(w.value), int(k.value)
+
+
+def shoot_diff(uv: np.ndarray, e: float, stride: int) -> tuple[int, float, int]:
+    """(node count, w_{N-1}, rescales) of one difference-form shot."""
+    uv = _f64(uv)
+    w = ctypes.c_double()
+    k = ctypes.c_int32()
+    c = _lib().eo_shoot_diff(uv.ctypes.data_as(_dp), len(uv), e, stride, ctypes.byref(w), ctypes.byref(k))
     return int(c), float(w.value), int(k.value)
 
 
@@ -193,13 +214,13 @@ def bracket_level(nodes, far, resc, e_lo: float, e_hi: float, v: int) -> Bracket
     return br
 
 
-def refine_level(ut, b: float, stride: int, v: int, br: Bracket, rel_tol: float, abs_tol: float,
-                 max_iter: int = 200) -> tuple[float, int]:
-    ut = _f64(ut)
+def refine_level(uv, b: float, stride: int, v: int, br: Bracket, rel_tol: float, abs_tol: float,
+                 max_iter: int = 200, segmented: bool = False, lanes: int = 64) -> tuple[float, int]:
+    uv = _f64(uv)
     lv = ctypes.c_double()
     it = _lib().eo_refine_level(
-        ut.ctypes.data_as(_dp), len(ut), b, stride, v, ctypes.byref(br), rel_tol, abs_tol,
-        max_iter, ctypes.byref(lv),
+        uv.ctypes.data_as(_dp), len(uv), b, stride, v, ctypes.byref(br), rel_tol, abs_tol,
+        max_iter, int(segmented), lanes, ctypes.byref(lv),
     )
     return float(lv.value), int(it)
 
@@ -247,6 +268,12 @@ class Problem:
         return shifted_potential(self.v[first : last + 1], self.beta)
 
     @property
+    def uv(self) -> np.ndarray:
+        """beta V on the trimmed grid (the refinement shots' table)."""
+        first, last, _ = self.trimmed
+        return scaled_potential(self.v[first : last + 1], self.beta)
+
+    @property
     def stride(self) -> int:
         return node_stride(self.beta, self.e_top, self.e_lo)
 
@@ -255,10 +282,31 @@ class Problem:
         return 1e-15 * (self.e_top - self.e_lo)
 
 
-def solve(prob: Problem, v_min: int, n_levels: int, rel_tol: float = 1e-10, max_iter: int = 200):
+def segment_count(n: int, lanes: int = 64) -> int:
+    return int(_lib().eo_segment_count(n, lanes))
+
+
+def shoot_segmented(uv: np.ndarray, e: float, stride: int, count: bool = True, lanes: int = 64):
+    """(far-end mantissa, binary exponent, node count or None) of a segmented
+    difference-form shot."""
+    uv = _f64(uv)
+    w = ctypes.c_double()
+    k = ctypes.c_int32()
+    c = ctypes.c_int32()
+    _lib().eo_shoot_segmented(
+        uv.ctypes.data_as(_dp), len(uv), e, stride, lanes, ctypes.byref(w), ctypes.byref(k),
+        ctypes.byref(c) if count else None,
+    )
+    return float(w.value), int(k.value), (int(c.value) if count else None)
+
+
+def solve(prob: Problem, v_min: int, n_levels: int, rel_tol: float = 1e-10, max_iter: int = 200,
+          segmented: bool = False, lanes: int = 64):
     """Scan + bracket + refine, as epseon_solve does: returns
-    (nodes, far, resc, levels, iterations)."""
+    (nodes, far, resc, levels, iterations).  ``segmented`` selects the
+    refinement with segmented shots over blocks of ``lanes`` threads."""
     ut = prob.ut
+    uv = prob.uv
     m = prob.m
     nodes = np.empty(m, dtype=np.int32)
     far = np.empty(m, dtype=np.float64)
@@ -266,8 +314,8 @@ def solve(prob: Problem, v_min: int, n_levels: int, rel_tol: float = 1e-10, max_
     lv = np.empty(max(n_levels, 1), dtype=np.float64)
     it = np.empty(max(n_levels, 1), dtype=np.int32)
     _lib().eo_solve(
-        ut.ctypes.data_as(_dp), len(ut), prob.beta, prob.e_lo, prob.e_top, m, prob.stride,
-        v_min, n_levels, rel_tol, prob.abs_tol, max_iter,
+        ut.ctypes.data_as(_dp), uv.ctypes.data_as(_dp), len(ut), prob.beta, prob.e_lo, prob.e_top, m, prob.stride,
+        v_min, n_levels, rel_tol, prob.abs_tol, max_iter, int(segmented), lanes,
         nodes.ctypes.data_as(_ip), far.ctypes.data_as(_dp), resc.ctypes.data_as(_ip),
         lv.ctypes.data_as(_dp), it.ctypes.data_as(_ip),
     )

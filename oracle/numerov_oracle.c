@@ -22,7 +22,8 @@
  * Numerics (cm^-1, Angstrom, amu; everything FP64, no contraction, explicit fma):
  *   grid        r_i = r_min + i*h,  h = (r_max - r_min)/(N-1)
  *   beta        = (h*h*mu) / (12*K),  K = hbar^2/(2 amu A^2) in cm^-1
- *   shifted pot ut_i = fma(beta, V_i, -1)            (= beta*V_i - 1)
+ *   shifted pot ut_i = fma(beta, V_i, -1)            (= beta*V_i - 1; scan)
+ *   scaled pot  uv_i = beta*V_i                       (refinement, see below)
  *   probe       e = beta*E
  *   Numerov f   f_i = e - ut_i                       (= 1 + beta*(E - V_i))
  *   step        t = fma(-10, f_i, 12); g = f_i*f_{i-1};
@@ -84,6 +85,12 @@ void eo_shifted_potential(const double* v, int64_t n, double beta, double* ut)
 {
     for (int64_t i = 0; i < n; ++i)
         ut[i] = fma(beta, v[i], -1.0);
+}
+
+void eo_scaled_potential(const double* v, int64_t n, double beta, double* uv)
+{
+    for (int64_t i = 0; i < n; ++i)
+        uv[i] = beta * v[i];
 }
 
 /* Morse V(r) = D_e (1 - exp(-a (r - r_e)))^2 on the grid r_i = r_min + i h. */
@@ -308,22 +315,283 @@ void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t*
     }
 }
 
+/* --------------------------------------------------- difference form
+ *
+ * Refinement shots use the same recurrence written on (w_i, d_i = w_i -
+ * w_{i-1}) with the small quantities kept at full relative precision:
+ *   eps_i = e - uv_i  (uv_i = beta V_i, so eps_i = beta (E - V_i))
+ *   sig_i = fma(eps_i, eps_{i-1}, eps_i + eps_{i-1})      (g_i = 1 + sig_i)
+ *   c     = fma(10 eps_i, w_i, sig_i w_{i-1})
+ *   d_{i+1} = d_i - c,  w_{i+1} = w_i + d_{i+1}
+ * (from w_{i+1} = (2 - 10 eps_i) w_i - (1 + sig_i) w_{i-1}).  With f = 1 + eps
+ * held near 1, FP64 quantises E in steps of 2^-52/beta; here the root of
+ * w_{N-1}(E) is resolved to ~1e-15 relative.  w_0 = 0, w_1 = d_1 = 1; the
+ * sign sampling and the 2^480 rescale test (on max(|w_i|, |w_{i-1}|), scaling
+ * w, d and w_{i-1}) sit where they sit in eo_shoot.
+ */
+
+int32_t eo_shoot_diff(const double* uv, int64_t n, double e, int32_t stride, double* w_last,
+                      int32_t* rescales)
+{
+    double w = 1.0, d = 1.0, wm = 0.0, em = e - uv[0];
+    int32_t nodes = 0, k = 0;
+    uint32_t sp = 0;
+    const int64_t steps = n - 2;
+    for (int64_t q = 0; q < steps; ++q) {
+        const double ep = e - uv[q + 1];
+        const double e10 = ep * 10.0;
+        const double sg = fma(ep, em, ep + em);
+        const double c = fma(e10, w, sg * wm);
+        d = d - c;
+        wm = w;
+        w = w + d;
+        em = ep;
+        if ((q + 1) % stride == 0) {
+            const uint32_t sb = eo_signbit(w);
+            nodes += (int32_t)(sb ^ sp);
+            sp = sb;
+        }
+        if ((q & 31) == 16 && q < (steps & ~(int64_t)31) && fmax(fabs(w), fabs(wm)) >= EO_RESCALE_AT) {
+            w *= EO_RESCALE_BY;
+            d *= EO_RESCALE_BY;
+            wm *= EO_RESCALE_BY;
+            ++k;
+        }
+    }
+    if (steps % stride != 0) {
+        const uint32_t sb = eo_signbit(w);
+        nodes += (int32_t)(sb ^ sp);
+    }
+    *w_last = w;
+    *rescales = k;
+    return nodes;
+}
+
+/* ------------------------------------------------------ segmented shots
+ *
+ * A difference-form shot split over S segments of the Q = N-2 steps:
+ * segment s covers steps [q_s, q_{s+1}), q_s = (s*Q)/S.  Each segment yields
+ * the 2x2 matrix 2^E [[m00 m01][m10 m11]] taking (w, d) at the segment start
+ * to (w, d) at its end: the two columns start from (w, d) = (1, 0) and (0, 1)
+ * with w_prev = w - d, and after every 32nd step of the segment and at its end
+ * all carried values are divided by 2^x, x the frexp exponent of their
+ * largest magnitude.  The chain v <- M_s v from v = (w_1, d_1) = (1, 1),
+ * renormalised the same way after each product, ends at F = w_{N-1} = v0 2^E.
+ * The far-end value comes from the product of all segment matrices, formed
+ * as a fixed binary tree (at distance d = 1, 2, 4, ...: M_i <- M_{i+d} M_i for
+ * i a multiple of 2d, each product renormalised by the frexp exponent of its
+ * largest entry), applied to (w_1, d_1) = (1, 1).  For node counts the start
+ * vectors come from the sequential chain v <- M_s v (renormalised after every
+ * product), each segment is re-run from its start vector with signs sampled
+ * at the global positions of the sequential shot, and the segment counts are
+ * joined across segment borders.  S = clamp(Q/32, 1, lanes) for a block of
+ * `lanes` threads per level.
+ */
+
+typedef struct eo_mat {
+    double m00, m01, m10, m11;
+    int32_t e;
+} eo_mat;
+
+int32_t eo_segment_count(int64_t n, int32_t lanes)
+{
+    const int64_t q = n - 2;
+    int64_t s = q / 32;
+    if (s < 1)
+        s = 1;
+    if (s > lanes)
+        s = lanes;
+    return (int32_t)s;
+}
+
+int64_t eo_segment_start(int64_t n, int32_t segments, int32_t s)
+{
+    return ((int64_t)s * (n - 2)) / segments;
+}
+
+static int32_t eo_max_exponent(const double* x, int32_t k)
+{
+    double m = 0.0;
+    for (int32_t i = 0; i < k; ++i)
+        m = fmax(m, fabs(x[i]));
+    int e = 0;
+    frexp(m, &e);
+    return e;
+}
+
+static void eo_scale(double* x, int32_t k, int32_t e)
+{
+    for (int32_t i = 0; i < k; ++i)
+        x[i] = ldexp(x[i], -e);
+}
+
+void eo_segment_matrix(const double* uv, int64_t q0, int64_t q1, double e, eo_mat* out)
+{
+    /* x = {wA, dA, wmA, wB, dB, wmB} */
+    double x[6] = {1.0, 0.0, 1.0, 0.0, 1.0, -1.0};
+    double em = e - uv[q0];
+    int32_t ex = 0;
+    for (int64_t q = q0; q < q1; ++q) {
+        const double ep = e - uv[q + 1];
+        const double e10 = ep * 10.0;
+        const double sg = fma(ep, em, ep + em);
+        for (int c = 0; c < 6; c += 3) {
+            const double cc = fma(e10, x[c], sg * x[c + 2]);
+            x[c + 1] = x[c + 1] - cc;
+            x[c + 2] = x[c];
+            x[c] = x[c] + x[c + 1];
+        }
+        em = ep;
+        if (((q - q0) & 31) == 31) {
+            const int32_t sh = eo_max_exponent(x, 6);
+            eo_scale(x, 6, sh);
+            ex += sh;
+        }
+    }
+    double y[4] = {x[0], x[1], x[3], x[4]};
+    const int32_t sh = eo_max_exponent(y, 4);
+    eo_scale(y, 4, sh);
+    out->m00 = y[0];
+    out->m10 = y[1];
+    out->m01 = y[2];
+    out->m11 = y[3];
+    out->e = ex + sh;
+}
+
+/* Chain the segment matrices from (w_1, d_1) = (1, 1).  starts[2s], [2s+1]
+ * receive the normalised (w, d) at the start of segment s (may be NULL). */
+void eo_chain(const eo_mat* m, int32_t segments, double* f_mant, int32_t* f_exp, double* starts)
+{
+    double v[2] = {1.0, 1.0};
+    int32_t ex = 0;
+    for (int32_t s = 0; s < segments; ++s) {
+        if (starts) {
+            starts[2 * s] = v[0];
+            starts[2 * s + 1] = v[1];
+        }
+        const double a = fma(m[s].m00, v[0], m[s].m01 * v[1]);
+        const double b = fma(m[s].m10, v[0], m[s].m11 * v[1]);
+        v[0] = a;
+        v[1] = b;
+        const int32_t sh = eo_max_exponent(v, 2);
+        eo_scale(v, 2, sh);
+        ex += m[s].e + sh;
+    }
+    *f_mant = v[0];
+    *f_exp = ex;
+}
+
+typedef struct eo_seg_count {
+    int32_t internal, first, last, has;
+} eo_seg_count;
+
+void eo_segment_count_nodes(const double* uv, int64_t n, int64_t q0, int64_t q1, double e,
+                            int32_t stride, double w, double d, eo_seg_count* out)
+{
+    const int64_t qn = n - 2;
+    double x[3] = {w, d, w - d};
+    double em = e - uv[q0];
+    eo_seg_count c = {0, 0, 0, 0};
+    for (int64_t q = q0; q < q1; ++q) {
+        const double ep = e - uv[q + 1];
+        const double e10 = ep * 10.0;
+        const double sg = fma(ep, em, ep + em);
+        const double cc = fma(e10, x[0], sg * x[2]);
+        x[1] = x[1] - cc;
+        x[2] = x[0];
+        x[0] = x[0] + x[1];
+        em = ep;
+        if ((q + 1) % stride == 0 || (q == qn - 1 && qn % stride != 0)) {
+            const uint32_t sb = eo_signbit(x[0]);
+            if (c.has)
+                c.internal += (int32_t)(sb ^ (uint32_t)c.last);
+            else
+                c.first = (int32_t)sb;
+            c.last = (int32_t)sb;
+            c.has = 1;
+        }
+        if (((q - q0) & 31) == 31)
+            eo_scale(x, 3, eo_max_exponent(x, 3));
+    }
+    *out = c;
+}
+
+/* C = A B (A the later segment), renormalised. */
+static eo_mat eo_matmul(const eo_mat* a, const eo_mat* b)
+{
+    double c[4];
+    c[0] = fma(a->m00, b->m00, a->m01 * b->m10);
+    c[1] = fma(a->m00, b->m01, a->m01 * b->m11);
+    c[2] = fma(a->m10, b->m00, a->m11 * b->m10);
+    c[3] = fma(a->m10, b->m01, a->m11 * b->m11);
+    const int32_t sh = eo_max_exponent(c, 4);
+    eo_scale(c, 4, sh);
+    eo_mat r = {c[0], c[1], c[2], c[3], a->e + b->e + sh};
+    return r;
+}
+
+/* Far-end value by the binary-tree product of the segment matrices. */
+void eo_tree_far_end(eo_mat* m, int32_t segments, double* f_mant, int32_t* f_exp)
+{
+    for (int32_t d = 1; d < segments; d *= 2)
+        for (int32_t i = 0; i + d < segments; i += 2 * d)
+            m[i] = eo_matmul(&m[i + d], &m[i]);
+    *f_mant = fma(m[0].m00, 1.0, m[0].m01 * 1.0);
+    *f_exp = m[0].e;
+}
+
+#define EO_MAX_SEGMENTS 1024
+
+/* Far-end value (and, when nodes != NULL, the node count) of a segmented
+ * difference-form shot at probe e, `lanes` threads per level. */
+void eo_shoot_segmented(const double* uv, int64_t n, double e, int32_t stride, int32_t lanes,
+                        double* f_mant, int32_t* f_exp, int32_t* nodes)
+{
+    const int32_t segs = eo_segment_count(n, lanes > EO_MAX_SEGMENTS ? EO_MAX_SEGMENTS : lanes);
+    eo_mat m[EO_MAX_SEGMENTS], t[EO_MAX_SEGMENTS];
+    double starts[2 * EO_MAX_SEGMENTS];
+    for (int32_t s = 0; s < segs; ++s)
+        eo_segment_matrix(uv, eo_segment_start(n, segs, s), eo_segment_start(n, segs, s + 1), e, &m[s]);
+    memcpy(t, m, sizeof(eo_mat) * (size_t)segs);
+    eo_tree_far_end(t, segs, f_mant, f_exp);
+    if (nodes == NULL)
+        return;
+    double cm;
+    int32_t ce;
+    eo_chain(m, segs, &cm, &ce, starts);
+    int32_t total = 0;
+    uint32_t prev = 0;
+    for (int32_t s = 0; s < segs; ++s) {
+        eo_seg_count c;
+        eo_segment_count_nodes(uv, n, eo_segment_start(n, segs, s), eo_segment_start(n, segs, s + 1), e,
+                               stride, starts[2 * s], starts[2 * s + 1], &c);
+        if (c.has) {
+            total += c.internal + (int32_t)((uint32_t)c.first ^ prev);
+            prev = (uint32_t)c.last;
+        }
+    }
+    *nodes = total;
+}
+
 /* ---------------------------------------------------------------- refine */
 
-/* Refine level v inside its bracket.  While the bracket holds more than one
- * level (or one end has no shot) it is halved by node count; once it holds
- * exactly level v the Illinois variant of regula falsi runs on the far-end
- * value F(E) = w_{N-1} * 2^(480 k), with every new point kept at least half a
- * tolerance inside the bracket.  Stops when the width is <= tol =
- * rel_tol*max(|E_lo|,|E_hi|) + abs_tol; the level is the final midpoint.
- * Returns the number of shots (or -1 when max_iter ran out). */
-int32_t eo_refine_level(const double* ut, int64_t n, double beta, int32_t stride, int32_t v,
+/* Refine level v inside its bracket (the bracket ends carry the scan's
+ * far-end values).  While the bracket holds more than level v (or its floor
+ * has no shot) it is halved with counted shots; once it holds exactly level v
+ * the Illinois variant of regula falsi runs on F(E) = w_{N-1} 2^k, each new
+ * point kept at least tol/2 inside the bracket, and a probe's count follows
+ * from the sign of F (count parity = sign bit of w_{N-1}).  Stops when the
+ * width is <= tol = rel_tol*max(|E_lo|,|E_hi|) + abs_tol; the level is the
+ * final midpoint.  Shots are difference-form: sequential (segmented == 0) or
+ * segmented over a block of `lanes` threads (segmented != 0).  Returns the
+ * number of shots, -1 when max_iter ran out. */
+int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride, int32_t v,
                         const eo_bracket* br, double rel_tol, double abs_tol, int32_t max_iter,
-                        double* level)
+                        int32_t segmented, int32_t lanes, double* level)
 {
     double el = br->e_lo, eh = br->e_hi;
     double wl = br->w_lo, wh = br->w_hi;
-    int32_t kl = br->k_lo, kh = br->k_hi;
+    int32_t kl = br->k_lo * 480, kh = br->k_hi * 480; /* binary exponents */
     int32_t cl = br->c_lo, ch = br->c_hi;
     int32_t lo_ok = br->lo_valid;
     int32_t side = 0;
@@ -340,21 +608,28 @@ int32_t eo_refine_level(const double* ut, int64_t n, double beta, int32_t stride
         double em;
         const int32_t isolated = lo_ok && cl == v && ch == v + 1;
         if (isolated) {
-            const double r = ldexp(wh / wl, 480 * (kh - kl));
+            const double r = ldexp(wh / wl, kh - kl);
             const double s = 1.0 / (1.0 - r);
             em = fma(s, width, el);
-            const double d = 0.5 * tol;
-            if (!(em >= el + d))
-                em = el + d;
-            if (!(em <= eh - d))
-                em = eh - d;
+            const double dd = 0.5 * tol;
+            if (!(em >= el + dd))
+                em = el + dd;
+            if (!(em <= eh - dd))
+                em = eh - dd;
         } else {
             em = el + 0.5 * width;
             side = 0;
         }
         double w;
-        int32_t k;
-        const int32_t c = eo_shoot(ut, n, beta * em, stride, &w, &k);
+        int32_t k, c;
+        if (segmented) {
+            eo_shoot_segmented(uv, n, beta * em, stride, lanes, &w, &k, isolated ? NULL : &c);
+        } else {
+            c = eo_shoot_diff(uv, n, beta * em, stride, &w, &k);
+            k *= 480;
+        }
+        if (isolated)
+            c = ((int32_t)eo_signbit(w) == (v & 1)) ? v : v + 1;
         ++it;
         if (c <= v) {
             el = em; cl = c; wl = w; kl = k; lo_ok = 1;
@@ -374,10 +649,10 @@ int32_t eo_refine_level(const double* ut, int64_t n, double beta, int32_t stride
 
 /* Scan + bracket + refine of levels [v_min, v_min + n_levels) for one
  * potential.  levels[] gets NaN for levels that are not below e_hi. */
-void eo_solve(const double* ut, int64_t n, double beta, double e_lo, double e_hi, int64_t m,
-              int32_t stride, int32_t v_min, int32_t n_levels, double rel_tol, double abs_tol,
-              int32_t max_iter, int32_t* nodes, double* w_last, int32_t* rescales, double* levels,
-              int32_t* iterations)
+void eo_solve(const double* ut, const double* uv, int64_t n, double beta, double e_lo, double e_hi,
+              int64_t m, int32_t stride, int32_t v_min, int32_t n_levels, double rel_tol,
+              double abs_tol, int32_t max_iter, int32_t segmented, int32_t lanes, int32_t* nodes,
+              double* w_last, int32_t* rescales, double* levels, int32_t* iterations)
 {
     eo_scan(ut, n, beta, e_lo, e_hi, m, stride, nodes, w_last, rescales);
     const double de = eo_trial_spacing(e_lo, e_hi, m);
@@ -390,7 +665,7 @@ void eo_solve(const double* ut, int64_t n, double beta, double e_lo, double e_hi
             iterations[l] = 0;
             continue;
         }
-        iterations[l] = eo_refine_level(ut, n, beta, stride, v_min + l, &br, rel_tol, abs_tol,
-                                        max_iter, &levels[l]);
+        iterations[l] = eo_refine_level(uv, n, beta, stride, v_min + l, &br, rel_tol, abs_tol,
+                                        max_iter, segmented, lanes, &levels[l]);
     }
 }

@@ -68,14 +68,20 @@ def test_scan_parity_h2(capi):
     assert nodes.max() == 17
 
 
-def test_solve_parity_h2(capi):
-    _log("case: solve parity, H2 Morse, levels 0..19")
+@pytest.mark.parametrize("mode", ["lane", "segmented"])
+def test_solve_parity_h2(capi, mode):
+    _log(f"case: solve parity ({mode} refine), H2 Morse, levels 0..19")
     prob = morse_problem(**CFG_H2)
+    seg = mode == "segmented"
     with capi.Context(0) as ctx:
+        ctx.set_refine_mode(capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE)
         ctx.load([_to_capi(capi, prob, 0, 20)])
         ctx.solve(1e-10, 200)
         lv, st, it = ctx.levels()
-    _, _, _, olv, oit = oracle.solve(prob, 0, 20, 1e-10, 200)
+        stats = ctx.stats()
+        assert stats.refine_mode == (capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE)
+    _log(f"  refine lanes per level: {stats.refine_lanes}")
+    _, _, _, olv, oit = oracle.solve(prob, 0, 20, 1e-10, 200, segmented=seg, lanes=stats.refine_lanes)
     np.testing.assert_array_equal(st[:17], 0)
     np.testing.assert_array_equal(st[17:], 1)
     np.testing.assert_array_equal(lv[:17].view(np.int64), olv[:17].view(np.int64))
@@ -87,15 +93,19 @@ def test_solve_parity_h2(capi):
     np.testing.assert_allclose(lv[:10], ana, rtol=1e-7)
 
 
-def test_ragged_batch_parity(capi):
-    _log("case: ragged batch of 12 random Morse potentials (seed 7)")
+@pytest.mark.parametrize("mode", ["lane", "segmented"])
+def test_ragged_batch_parity(capi, mode):
+    _log(f"case: ragged batch of 12 random Morse potentials (seed 7), {mode} refine")
     probs = random_morse_batch(12, seed=7, n_range=(300, 6000), m_range=(40, 700))
+    seg = mode == "segmented"
     with capi.Context(0) as ctx:
+        ctx.set_refine_mode(capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE)
         ctx.load([_to_capi(capi, p, 0, 30) for p in probs])
         ctx.solve(1e-10, 200)
         nodes, far, resc = ctx.scan_results()
         lv, st, it = ctx.levels()
         strides = [ctx.potential_info(i).stride for i in range(len(probs))]
+        lanes = ctx.stats().refine_lanes
         flags = [ctx.potential_info(i).flags for i in range(len(probs))]
     eo = lo = 0
     for i, p in enumerate(probs):
@@ -106,7 +116,7 @@ def test_ragged_batch_parity(capi):
             eo += p.m
             lo += 30
             continue
-        on, of, orr, olv, oit = oracle.solve(p, 0, 30, 1e-10, 200)
+        on, of, orr, olv, oit = oracle.solve(p, 0, 30, 1e-10, 200, segmented=seg, lanes=lanes)
         np.testing.assert_array_equal(nodes[eo : eo + p.m], on)
         np.testing.assert_array_equal(resc[eo : eo + p.m], orr)
         np.testing.assert_array_equal(far[eo : eo + p.m].view(np.int64), of.view(np.int64))

@@ -36,24 +36,14 @@ FP64_VECTOR_PEAK_TFLOPS = 78.6  # MI355X: 256 CU x 128 FP64 FLOP/clk x 2.4 GHz
 FLOP_PER_POINT_UPDATE = 7  # sub + FMA + mul + mul + FMA (FMA = 2)
 METRIC = "Numerov point-updates/s & eigenlevels/s at 1e-9 rel err; grad/fwd time ratio"
 
-SINGLE = dict(mu=63.452, de=12440.0, a=1.8676, re=2.666, r_min=2.0, r_max=8.0, n=200_000, m=65_536)
-
-
-def morse_table(de: float, re: float, a: float, r_min: float, r_max: float, n: int) -> np.ndarray:
-    h = (r_max - r_min) / (n - 1)
-    r = r_min + np.arange(n, dtype=np.float64) * h
-    x = 1.0 - np.exp(-a * (r - re))
-    return de * (x * x)
-
-
 def single_workload(capi):
-    c = SINGLE
-    v = morse_table(c["de"], c["re"], c["a"], c["r_min"], c["r_max"], c["n"])
+    from epseon_backend_amd.workloads import I2
+
     # Level count: every level below D_e (the analytic Morse count is 116).
     pots = [
         capi.Potential(
-            values=v, r_min=c["r_min"], r_max=c["r_max"], reduced_mass=c["mu"], e_top=c["de"],
-            n_energies=c["m"], level_min=0, level_count=120,
+            values=I2.table(), r_min=I2.r_min, r_max=I2.r_max, reduced_mass=I2.mu, e_top=I2.de,
+            n_energies=I2.m, level_min=0, level_count=120,
         )
     ]
     config = {
@@ -61,38 +51,29 @@ def single_workload(capi):
         "re=2.666 A), FP64 grid of 200000 points on [2,8] A, 65536 trial energies in (min V, De), "
         "all bound levels refined to 1e-10 rel",
         "potentials": 1,
-        "grid_points": c["n"],
-        "trial_energies": c["m"],
+        "grid_points": I2.n,
+        "trial_energies": I2.m,
     }
     return pots, config
 
 
-def batched_workload(capi, seed: int = 20261202, count: int = 4096):
-    rng = np.random.default_rng(seed)
-    mu = np.exp(rng.uniform(np.log(0.5), np.log(100.0), count))
-    de = rng.uniform(5e3, 5e4, count)
-    a = rng.uniform(1.0, 2.5, count)
-    re = rng.uniform(0.7, 3.0, count)
-    n = np.floor(np.exp(rng.uniform(np.log(2000), np.log(64000), count))).astype(np.int64)
-    m = np.floor(np.exp(rng.uniform(np.log(256), np.log(8192), count))).astype(np.int64)
-    pots = []
-    for i in range(count):
-        lo, hi = re[i] - 0.6, re[i] + 6.0 / a[i]
-        v = morse_table(de[i], re[i], a[i], lo, hi, int(n[i]))
-        kp = 16.857629191640175 / mu[i]
-        n_bound = int(np.floor(np.sqrt(de[i] / kp) / a[i] - 0.5)) + 1
-        pots.append(
-            capi.Potential(
-                values=v, r_min=lo, r_max=hi, reduced_mass=float(mu[i]), e_top=float(de[i]),
-                n_energies=int(m[i]), level_min=0, level_count=n_bound + 2,
-            )
+def batched_workload(capi, count: int = 4096):
+    from epseon_backend_amd.workloads import BATCH_SEED, batched
+
+    specs = batched(count)
+    pots = [
+        capi.Potential(
+            values=s.table(), r_min=s.r_min, r_max=s.r_max, reduced_mass=s.mu, e_top=s.de,
+            n_energies=s.m, level_min=0, level_count=s.bound_levels() + 2,
         )
+        for s in specs
+    ]
     config = {
-        "workload": f"batched: {count} seeded Morse potentials (seed {seed}), grids log-U[2000,64000] "
+        "workload": f"batched: {count} seeded Morse potentials (seed {BATCH_SEED}), grids log-U[2000,64000] "
         "points on [re-0.6, re+6/a] A, trial energies log-U[256,8192], all bound levels to 1e-10 rel",
         "potentials": count,
-        "grid_points": int(n.sum()),
-        "trial_energies": int(m.sum()),
+        "grid_points": int(sum(s.n for s in specs)),
+        "trial_energies": int(sum(s.m for s in specs)),
     }
     return pots, config
 
@@ -101,11 +82,10 @@ def cpu_baseline(target_seconds: float = 12.0) -> dict:
     """The C oracle (OpenMP over trial energies) on a bounded sample of the
     single-potential workload: the same grid with fewer trial energies."""
     import oracle
+    from epseon_backend_amd.workloads import I2
 
-    c = SINGLE
-    h = oracle.grid_step(c["r_min"], c["r_max"], c["n"])
-    v = oracle.morse(c["de"], c["re"], c["a"], c["r_min"], h, c["n"])
-    prob = oracle.Problem(v=v, r_min=c["r_min"], r_max=c["r_max"], mu=c["mu"], e_top=c["de"], m=64)
+    c = {"n": I2.n, "m": I2.m}
+    prob = oracle.Problem(v=I2.table(), r_min=I2.r_min, r_max=I2.r_max, mu=I2.mu, e_top=I2.de, m=64)
     ut, b, stride = prob.ut, prob.beta, prob.stride
     threads = oracle.num_threads()
     m = 64 * threads
@@ -190,6 +170,7 @@ def main() -> int:
         ctx.levels()
 
     scan_ms: list[float] = []
+    refine_ms: list[float] = []
     scan_pu = refine_pu = 0.0
     levels_ok = 0
     ctx.synchronize()
@@ -199,6 +180,7 @@ def main() -> int:
         lv, st, it = ctx.levels()
         s = ctx.stats()
         scan_ms.append(s.scan_ms)
+        refine_ms.append(s.refine_ms)
         scan_pu += s.scan_point_updates
         refine_pu += s.refine_point_updates
         levels_ok += int(s.levels_converged)
@@ -227,6 +209,7 @@ def main() -> int:
         "eigenlevels_per_s": levels_ok / wall,
         "levels_per_step": levels_ok // args.steps,
         "scan_ms": float(np.mean(scan_ms)),
+        "refine_ms": float(np.mean(refine_ms)),
         "scan_point_updates_per_step": scan_pu / args.steps,
         "refine_point_updates_per_step": refine_pu / args.steps,
         "roofline": {

@@ -193,6 +193,11 @@ struct epseon_ctx {
     int32_t* d_status = nullptr;
     int32_t* d_iters = nullptr;
 
+    // Host copies of the last refine's status and iteration arrays, filled
+    // by epseon_get_levels so that epseon_get_stats needs no extra copy.
+    std::vector<int32_t> h_status, h_iters;
+    bool host_levels_valid = false;
+
     bool loaded = false, scanned = false, refined = false;
     bool scan_timed = false, refine_timed = false;
     double scan_point_updates = 0.0;
@@ -590,6 +595,7 @@ int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
     ctx->refined = true;
     ctx->refine_timed = true;
+    ctx->host_levels_valid = false;
     return EPSEON_OK;
 }
 
@@ -618,6 +624,24 @@ int epseon_ctx_synchronize(epseon_ctx* ctx)
     return EPSEON_OK;
 }
 
+namespace {
+
+int fetch_level_meta(epseon_ctx* ctx)
+{
+    const size_t n = static_cast<size_t>(ctx->total_levels);
+    ctx->h_status.resize(n);
+    ctx->h_iters.resize(n);
+    if (n > 0) {
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->h_status.data(), ctx->d_status, n * sizeof(int32_t),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->h_iters.data(), ctx->d_iters, n * sizeof(int32_t),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    }
+    return EPSEON_OK;
+}
+
+}  // namespace
+
 int epseon_get_levels(epseon_ctx* ctx, double* levels, int32_t* status, int32_t* iterations)
 {
     if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
@@ -625,18 +649,17 @@ int epseon_get_levels(epseon_ctx* ctx, double* levels, int32_t* status, int32_t*
     if (!ctx->refined)
         return fail(EPSEON_ERR_STATE, "epseon_get_levels before epseon_refine");
     const size_t n = static_cast<size_t>(ctx->total_levels);
-    if (n > 0) {
-        if (levels)
-            EPSEON_HIP_TRY(hipMemcpyAsync(levels, ctx->d_levels, n * sizeof(double),
-                                          hipMemcpyDeviceToHost, ctx->stream));
-        if (status)
-            EPSEON_HIP_TRY(hipMemcpyAsync(status, ctx->d_status, n * sizeof(int32_t),
-                                          hipMemcpyDeviceToHost, ctx->stream));
-        if (iterations)
-            EPSEON_HIP_TRY(hipMemcpyAsync(iterations, ctx->d_iters, n * sizeof(int32_t),
-                                          hipMemcpyDeviceToHost, ctx->stream));
-    }
+    if (n > 0 && levels)
+        EPSEON_HIP_TRY(hipMemcpyAsync(levels, ctx->d_levels, n * sizeof(double), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    if (const int rc = fetch_level_meta(ctx); rc != EPSEON_OK)
+        return rc;
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->host_levels_valid = true;
+    if (n > 0 && status)
+        std::memcpy(status, ctx->h_status.data(), n * sizeof(int32_t));
+    if (n > 0 && iterations)
+        std::memcpy(iterations, ctx->h_iters.data(), n * sizeof(int32_t));
     return EPSEON_OK;
 }
 
@@ -700,14 +723,14 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
     if (ctx->refine_timed) {
         EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
         stats->refine_ms = ms;
-        const size_t n = static_cast<size_t>(ctx->total_levels);
-        std::vector<int32_t> it(n), st(n);
-        if (n > 0) {
-            EPSEON_HIP_TRY(hipMemcpy(it.data(), ctx->d_iters, n * sizeof(int32_t),
-                                     hipMemcpyDeviceToHost));
-            EPSEON_HIP_TRY(hipMemcpy(st.data(), ctx->d_status, n * sizeof(int32_t),
-                                     hipMemcpyDeviceToHost));
+        if (!ctx->host_levels_valid) {
+            if (const int rc = fetch_level_meta(ctx); rc != EPSEON_OK)
+                return rc;
+            EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+            ctx->host_levels_valid = true;
         }
+        const std::vector<int32_t>& it = ctx->h_iters;
+        const std::vector<int32_t>& st = ctx->h_status;
         double pu = 0.0;
         int64_t ok = 0;
         for (size_t p = 0; p < ctx->rec.size(); ++p) {

@@ -17,20 +17,14 @@ def morse_problem(mu, de, a, re, r_min, r_max, n, m) -> oracle.Problem:
     return oracle.Problem(v=v, r_min=r_min, r_max=r_max, mu=mu, e_top=de, m=m)
 
 
+def problem_from_spec(spec) -> oracle.Problem:
+    """oracle.Problem on the numpy-tabulated table of a workloads.MorseSpec."""
+    return oracle.Problem(v=spec.table(), r_min=spec.r_min, r_max=spec.r_max, mu=spec.mu,
+                          e_top=spec.de, m=spec.m)
+
+
 def random_morse_batch(count: int, seed: int, n_range=(2000, 64000), m_range=(256, 8192)):
-    """Seeded batch drawn as in the batched workload: mu log-uniform [0.5, 100],
-    D_e U[5e3, 5e4], a U[1, 2.5], r_e U[0.7, 3], N and M log-uniform; grid
-    [r_e - 0.6, r_e + 6/a].  Draw order: mu, D_e, a, r_e, N, M."""
-    rng = np.random.default_rng(seed)
-    mu = np.exp(rng.uniform(np.log(0.5), np.log(100.0), count))
-    de = rng.uniform(5e3, 5e4, count)
-    a = rng.uniform(1.0, 2.5, count)
-    re = rng.uniform(0.7, 3.0, count)
-    n = np.floor(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), count))).astype(np.int64)
-    m = np.floor(np.exp(rng.uniform(np.log(m_range[0]), np.log(m_range[1]), count))).astype(np.int64)
-    out = []
-    for i in range(count):
-        out.append(
-            morse_problem(mu[i], de[i], a[i], re[i], re[i] - 0.6, re[i] + 6.0 / a[i], int(n[i]), int(m[i]))
-        )
-    return out
+    """Seeded batch drawn as in the batched workload (epseon_backend_amd.workloads.batched)."""
+    from epseon_backend_amd.workloads import batched
+
+    return [problem_from_spec(s) for s in batched(count, seed, n_range, m_range)]

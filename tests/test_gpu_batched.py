@@ -1,0 +1,146 @@
+"""Batched workload on the GPU: 4,096 seeded Morse potentials with ragged grids.
+
+* a subset of the batch at full size, bit-exact against the oracle, in both
+  refinement modes;
+* the whole batch through the default (auto) mode, checked by properties
+  that do not depend on size (monotone counts, increasing levels, one level
+  per node count below the top trial energy, analytic Morse agreement where
+  the grid resolves the well) plus a random sample of potentials bit-exact
+  against the oracle.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from epseon_backend_amd.workloads import batched
+from tests.helpers import problem_from_spec
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg: str) -> None:
+    sys.stdout.write(msg + "\n")
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from epseon_backend_amd import capi as c
+
+    if c.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return c
+
+
+def _pot(capi, spec, table, levels):
+    return capi.Potential(values=table, r_min=spec.r_min, r_max=spec.r_max, reduced_mass=spec.mu,
+                          e_top=spec.de, n_energies=spec.m, level_min=0, level_count=levels)
+
+
+def _compare(prob, nodes, far, resc, lv, it, levels, seg, lanes):
+    on, of, orr, olv, oit = oracle.solve(prob, 0, levels, 1e-10, 200, segmented=seg, lanes=lanes)
+    np.testing.assert_array_equal(nodes, on)
+    np.testing.assert_array_equal(resc, orr)
+    np.testing.assert_array_equal(far.view(np.int64), of.view(np.int64))
+    ok = ~np.isnan(olv)
+    np.testing.assert_array_equal(np.isnan(lv), ~ok)
+    np.testing.assert_array_equal(lv[ok].view(np.int64), olv[ok].view(np.int64))
+    np.testing.assert_array_equal(it[ok], oit[ok])
+
+
+@pytest.mark.parametrize("mode", ["lane", "segmented"])
+def test_batched_subset_parity(capi, mode):
+    specs = batched(4096)[:40]
+    _log(f"case: first 40 potentials of the batched workload at full size, {mode} refine")
+    seg = mode == "segmented"
+    tables = [s.table() for s in specs]
+    counts = [s.bound_levels() + 2 for s in specs]
+    with capi.Context(0) as ctx:
+        ctx.set_refine_mode(capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE)
+        ctx.load([_pot(capi, s, t, c) for s, t, c in zip(specs, tables, counts)])
+        ctx.solve(1e-10, 200)
+        nodes, far, resc = ctx.scan_results()
+        lv, st, it = ctx.levels()
+        lanes = ctx.stats().refine_lanes
+        flags = [ctx.potential_info(i).flags for i in range(len(specs))]
+    eo = lo = 0
+    for i, s in enumerate(specs):
+        prob = problem_from_spec(s)
+        prob.v = tables[i]
+        assert flags[i] == (0 if prob.usable else 1)
+        if flags[i] == 0:
+            _compare(prob, nodes[eo : eo + s.m], far[eo : eo + s.m], resc[eo : eo + s.m],
+                     lv[lo : lo + counts[i]], it[lo : lo + counts[i]], counts[i], seg, lanes)
+        else:
+            assert np.all(st[lo : lo + counts[i]] == 3)
+        eo += s.m
+        lo += counts[i]
+
+
+def test_batched_full_scale(capi):
+    specs = batched(4096)
+    _log("case: full batched workload, 4096 potentials, auto refine mode")
+    tables = [s.table() for s in specs]
+    counts = [s.bound_levels() + 2 for s in specs]
+    with capi.Context(0) as ctx:
+        ctx.load([_pot(capi, s, t, c) for s, t, c in zip(specs, tables, counts)])
+        ctx.solve(1e-10, 200)
+        nodes, far, resc = ctx.scan_results()
+        lv, st, it = ctx.levels()
+        stats = ctx.stats()
+        flags = np.array([ctx.potential_info(i).flags for i in range(len(specs))])
+    _log(f"  refine mode {stats.refine_mode}, lanes {stats.refine_lanes}, scan {stats.scan_ms:.2f} ms, "
+         f"refine {stats.refine_ms:.2f} ms, levels converged {stats.levels_converged}, "
+         f"unresolved potentials {int(flags.sum())}")
+    assert stats.refine_mode == capi.REFINE_LANE
+    assert np.all(st != 2), "a level hit the iteration cap"
+    eo = lo = 0
+    resolved_checked = 0
+    starts_e, starts_l = [], []
+    for i, s in enumerate(specs):
+        starts_e.append(eo)
+        starts_l.append(lo)
+        nd = nodes[eo : eo + s.m]
+        lev = lv[lo : lo + counts[i]]
+        if flags[i] == 0:
+            assert np.all(np.diff(nd) >= 0), f"potential {i}: node counts not monotone"
+            found = lev[~np.isnan(lev)]
+            assert np.all(np.diff(found) > 0), f"potential {i}: levels not increasing"
+            # Sturm property: the node count at the top trial energy (D_e) is
+            # the number of box levels below it, and every one of them that was
+            # requested is found.  The box (Dirichlet wall at V(r_max) =
+            # 0.995 D_e) shifts the top levels, so the analytic Morse count is
+            # only a lower bound well below the top.
+            assert len(found) == min(counts[i], int(nd[-1])), (i, len(found), int(nd[-1]))
+            nb_low = int(np.sum(s.analytic(np.arange(s.bound_levels())) < 0.9 * s.de))
+            assert int(nd[-1]) >= nb_low - 1, (i, int(nd[-1]), nb_low)
+            # Low levels against analytic Morse where the grid resolves the well
+            # and the wall at r_e - 0.6 A is opaque (WKB: 2 int kappa > 40 from
+            # the wall to the inner turning point of v = 2).
+            h = (s.r_max - s.r_min) / (s.n - 1)
+            kh = np.sqrt(s.de * s.mu / 16.857629191640175) * h
+            ana = s.analytic(np.arange(3))
+            vt = tables[i]
+            left = vt[: int(np.argmin(vt)) + 1]
+            kappa = np.sqrt(np.clip(left - ana[2], 0.0, None) * s.mu / 16.857629191640175)
+            if kh < 0.2 and 2.0 * kappa.sum() * h > 40.0:
+                np.testing.assert_allclose(found[:3], ana, rtol=max(1e-8, kh**4 / 10))
+                resolved_checked += 1
+        eo += s.m
+        lo += counts[i]
+    _log(f"  analytic Morse check on {resolved_checked} potentials")
+    assert resolved_checked > 1500
+    rng = np.random.default_rng(1)
+    for i in rng.choice(len(specs), 6, replace=False):
+        if flags[i]:
+            continue
+        s = specs[i]
+        prob = problem_from_spec(s)
+        prob.v = tables[i]
+        e0, l0 = starts_e[i], starts_l[i]
+        _compare(prob, nodes[e0 : e0 + s.m], far[e0 : e0 + s.m], resc[e0 : e0 + s.m],
+                 lv[l0 : l0 + counts[i]], it[l0 : l0 + counts[i]], counts[i], False, 1)

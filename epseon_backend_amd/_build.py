@@ -77,7 +77,7 @@ def build_hip_library() -> Path:
     """hipcc: kernels + C ABI -> lib/libepseon_hip.so."""
     LIB_DIR.mkdir(parents=True, exist_ok=True)
     tmp = HIP_LIB.with_suffix(".so.tmp")
-    srcs = [CSRC / "numerov_kernels.hip", CSRC / "epseon_capi.hip"]
+    srcs = [CSRC / "numerov_kernels.hip", CSRC / "eigenfunction_kernels.hip", CSRC / "epseon_capi.hip"]
     cmd = [
         _hipcc(),
         *HIPCC_FLAGS,

@@ -40,6 +40,9 @@ REFINE_AUTO = 0
 REFINE_LANE = 1
 REFINE_SEGMENTED = 2
 
+MEM_HOST = 0
+MEM_DEVICE = 1
+
 
 class EpseonError(RuntimeError):
     """Error reported by libepseon_hip (carries the C status code)."""
@@ -108,6 +111,8 @@ class Stats(ctypes.Structure):
         ("levels_converged", ctypes.c_int64),
         ("refine_mode", ctypes.c_int32),
         ("refine_lanes", ctypes.c_int32),
+        ("vjp_ms", ctypes.c_double),
+        ("vjp_point_updates", ctypes.c_double),
     ]
 
 
@@ -137,6 +142,9 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "epseon_get_scan": (ctypes.c_int, [ctypes.c_void_p, _c_int32_p, _c_double_p, _c_int32_p]),
     "epseon_get_sizes": (ctypes.c_int, [ctypes.c_void_p, _c_int32_p, _c_int64_p, _c_int64_p, _c_int64_p]),
     "epseon_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
+    "epseon_normalize": (ctypes.c_int, [ctypes.c_void_p]),
+    "epseon_get_wavefunctions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]),
+    "epseon_level_vjp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
 }
 
 _LIB: ctypes.CDLL | None = None
@@ -239,6 +247,8 @@ class Context:
         self.n_potentials = 0
         self.level_counts: list[int] = []
         self.energy_counts: list[int] = []
+        self.point_counts: list[int] = []
+        self.v_offsets: list[int] = []
 
     def close(self) -> None:
         if self._h:
@@ -291,6 +301,8 @@ class Context:
         self.n_potentials = n
         self.level_counts = [int(p.level_count) for p in potentials]
         self.energy_counts = [int(p.n_energies) for p in potentials]
+        self.point_counts = [len(p.values) for p in potentials]
+        self.v_offsets = [int(o) for o in offsets[:n]]
 
     def potential_info(self, p: int) -> PotentialInfo:
         info = PotentialInfo()
@@ -348,3 +360,39 @@ class Context:
         st = Stats()
         _check(self._lib.epseon_get_stats(self._h, ctypes.byref(st)))
         return st
+
+    def normalize(self) -> None:
+        """Eigenfunction normalisation of every converged level (epseon_normalize)."""
+        _check(self._lib.epseon_normalize(self._h))
+
+    def wavefunctions(self, p: int) -> np.ndarray:
+        """Normalised eigenfunctions of potential p, shape (level_count, n_points)."""
+        out = np.empty((self.level_counts[p], self.point_counts[p]), dtype=np.float64)
+        _check(self._lib.epseon_get_wavefunctions(self._h, p, out.ctypes.data, MEM_HOST))
+        return out
+
+    def wavefunctions_device(self, p: int, out_ptr: int) -> None:
+        """Write potential p's eigenfunctions to device memory at out_ptr
+        (level_count * n_points doubles)."""
+        _check(self._lib.epseon_get_wavefunctions(self._h, p, ctypes.c_void_p(out_ptr), MEM_DEVICE))
+
+    @property
+    def v_span(self) -> int:
+        """Length of the V layout passed to load (and of an adjoint output)."""
+        return max((o + n for o, n in zip(self.v_offsets, self.point_counts)), default=0)
+
+    def level_vjp(self, grad_levels: np.ndarray) -> np.ndarray:
+        """sum_l grad_levels[l] dE_l/dV in the layout of the loaded V batch
+        (epseon_level_vjp, host memory)."""
+        g = _as_f64(grad_levels)
+        if g.shape != (sum(self.level_counts),):
+            raise ValueError(f"grad_levels must have shape ({sum(self.level_counts)},), got {g.shape}")
+        out = np.zeros(max(self.v_span, 1), dtype=np.float64)
+        _check(self._lib.epseon_level_vjp(self._h, g.ctypes.data, out.ctypes.data, MEM_HOST))
+        return out[: self.v_span]
+
+    def level_vjp_device(self, grad_levels_ptr: int, grad_v_ptr: int) -> None:
+        """Device-memory form of level_vjp: both pointers address device
+        buffers of this context's device."""
+        _check(self._lib.epseon_level_vjp(self._h, ctypes.c_void_p(grad_levels_ptr),
+                                          ctypes.c_void_p(grad_v_ptr), MEM_DEVICE))

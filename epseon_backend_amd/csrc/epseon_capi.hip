@@ -24,7 +24,7 @@
 namespace epseon::hip {
 
 hipError_t launch_shift_potential(hipStream_t, const double*, const PotentialRecord*, int32_t,
-                                  int32_t, double*, double*);
+                                  int32_t, double*, double*, double*);
 hipError_t launch_scan(hipStream_t, const double*, const PotentialRecord*, const WaveTask*, int32_t,
                        int32_t, int32_t*, double*, int32_t*);
 hipError_t launch_refine(hipStream_t, const double*, const PotentialRecord*, const WaveTask*,
@@ -33,6 +33,14 @@ hipError_t launch_refine(hipStream_t, const double*, const PotentialRecord*, con
 hipError_t launch_refine_segmented(hipStream_t, int32_t, const double*, const PotentialRecord*,
                                    const WaveTask*, int32_t, const int32_t*, const double*,
                                    const int32_t*, double, int32_t, double*, int32_t*, int32_t*);
+hipError_t launch_normalize(hipStream_t, const double*, const double*, const PotentialRecord*,
+                            const WaveTask*, int32_t, const double*, const int32_t*, LevelNorm*);
+hipError_t launch_eigenfunctions(hipStream_t, const double*, const double*, const PotentialRecord*,
+                                 int32_t, int32_t, const double*, const int32_t*, const LevelNorm*,
+                                 double*);
+hipError_t launch_level_vjp(hipStream_t, int32_t, const double*, const double*,
+                            const PotentialRecord*, const int32_t*, int32_t, const double*,
+                            const int32_t*, const LevelNorm*, const double*, double*);
 
 namespace {
 
@@ -166,7 +174,7 @@ using namespace epseon::hip;
 struct epseon_ctx {
     int32_t device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[8] = {};
     Arena arena;
     int32_t waves_per_block = 1;
 
@@ -178,6 +186,7 @@ struct epseon_ctx {
     double* d_v = nullptr;
     double* d_ut = nullptr;
     double* d_uv = nullptr;
+    double* d_uvr = nullptr;
     PotentialRecord* d_rec = nullptr;
     WaveTask* d_scan_tasks = nullptr;
     WaveTask* d_refine_tasks = nullptr;
@@ -192,14 +201,23 @@ struct epseon_ctx {
     double* d_levels = nullptr;
     int32_t* d_status = nullptr;
     int32_t* d_iters = nullptr;
+    LevelNorm* d_norms = nullptr;
+    double* d_grad_levels = nullptr;
+    // Potentials grouped by adjoint block width (1..4 wavefronts).
+    int32_t* d_vjp_lists[4] = {};
+    int32_t n_vjp[4] = {};
+    // Scratch for host-side outputs, carved from the arena on first use.
+    double* d_stage = nullptr;
+    size_t stage_elems = 0;
+    int64_t user_span = 0;
 
     // Host copies of the last refine's status and iteration arrays, filled
     // by epseon_get_levels so that epseon_get_stats needs no extra copy.
     std::vector<int32_t> h_status, h_iters;
     bool host_levels_valid = false;
 
-    bool loaded = false, scanned = false, refined = false;
-    bool scan_timed = false, refine_timed = false;
+    bool loaded = false, scanned = false, refined = false, normalized = false;
+    bool scan_timed = false, refine_timed = false, normalize_timed = false, vjp_timed = false;
     double scan_point_updates = 0.0;
 
     explicit epseon_ctx(size_t block_bytes) : arena(block_bytes) {}
@@ -309,7 +327,7 @@ int epseon_ctx_create(int32_t device, uint64_t block_bytes, epseon_ctx** out)
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    for (int i = 0; i < 6 && e == hipSuccess; ++i)
+    for (int i = 0; i < 8 && e == hipSuccess; ++i)
         e = hipEventCreate(&ctx->ev[i]);
     if (e != hipSuccess) {
         const int rc = hip_fail(e, "context creation");
@@ -345,12 +363,14 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     if (count < 0 || (count > 0 && (desc == nullptr || v == nullptr || v_offsets == nullptr)))
         return fail(EPSEON_ERR_INVALID, "invalid potential batch");
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->loaded = ctx->scanned = ctx->refined = false;
-    ctx->scan_timed = ctx->refine_timed = false;
+    ctx->loaded = ctx->scanned = ctx->refined = ctx->normalized = false;
+    ctx->scan_timed = ctx->refine_timed = ctx->normalize_timed = ctx->vjp_timed = false;
+    ctx->d_stage = nullptr;
+    ctx->stage_elems = 0;
     ctx->desc.assign(desc, desc + count);
     ctx->rec.assign(static_cast<size_t>(count), PotentialRecord{});
 
-    int64_t pts = 0, en = 0, lv = 0;
+    int64_t pts = 0, en = 0, lv = 0, user_span = 0;
     int32_t max_points = 0;
     double scan_pu = 0.0;
     for (int32_t p = 0; p < count; ++p) {
@@ -376,6 +396,10 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
         }
         if (d.level_min < 0 || d.level_count < 0) {
             std::snprintf(msg, sizeof msg, "potential %d: negative level range", p);
+            return fail(EPSEON_ERR_INVALID, msg);
+        }
+        if (v_offsets[p] < 0) {
+            std::snprintf(msg, sizeof msg, "potential %d: negative V offset", p);
             return fail(EPSEON_ERR_INVALID, msg);
         }
         const double* vp = v + v_offsets[p];
@@ -431,6 +455,13 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
         r.flags = (f_max <= 1.5 && r.n_points >= 4) ? EPSEON_POT_OK : EPSEON_POT_UNRESOLVED;
         if (r.n_points < 4)
             r.n_points = 4;
+        // Two-sided eigenfunction shots meet at min V (inside the classically
+        // allowed region of every level); points m and m+1 must be interior.
+        r.match = static_cast<int32_t>(
+            std::clamp<int64_t>(imin - first, 1, static_cast<int64_t>(r.n_points) - 3));
+        r.full_points = static_cast<int32_t>(d.n_points);
+        r.user_offset = v_offsets[p];
+        user_span = std::max<int64_t>(user_span, v_offsets[p] + d.n_points);
         r.v_offset = pts;
         r.ut_offset = pts + first;
         r.energy_offset = en;
@@ -449,6 +480,7 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     ctx->total_levels = lv;
     ctx->max_points = max_points;
     ctx->scan_point_updates = scan_pu;
+    ctx->user_span = user_span;
 
     // Work lists: one wavefront per 64 trial energies (scan) or 64 levels
     // (refine), longest grids first so the dispatcher drains large items early.
@@ -459,7 +491,10 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     });
     // Segmented refinement uses one 64-lane block per level.
     std::vector<WaveTask> scan_tasks, refine_tasks, level_tasks;
+    std::vector<int32_t> vjp_lists[4];
     for (int32_t p : order) {
+        const int32_t groups = (ctx->rec[p].level_count + 63) / 64;
+        vjp_lists[std::clamp(groups, 1, 4) - 1].push_back(p);
         const PotentialRecord& r = ctx->rec[p];
         if (r.flags == EPSEON_POT_OK)
             for (int32_t j = 0; j < r.n_energies; j += 64)
@@ -481,6 +516,7 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_v));
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_ut));
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_uv));
+    EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_uvr));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(count, 1), &ctx->d_rec));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(scan_tasks.size(), 1), &ctx->d_scan_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(refine_tasks.size(), 1), &ctx->d_refine_tasks));
@@ -491,6 +527,16 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_levels));
     EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_status));
     EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_iters));
+    EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_norms));
+    EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_grad_levels));
+    for (int w = 0; w < 4; ++w) {
+        ctx->n_vjp[w] = static_cast<int32_t>(vjp_lists[w].size());
+        EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(vjp_lists[w].size(), 1), &ctx->d_vjp_lists[w]));
+        if (!vjp_lists[w].empty())
+            EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_vjp_lists[w], vjp_lists[w].data(),
+                                          vjp_lists[w].size() * sizeof(int32_t),
+                                          hipMemcpyHostToDevice, ctx->stream));
+    }
 
     for (int32_t p = 0; p < count; ++p)
         EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_v + ctx->rec[p].v_offset, v + v_offsets[p],
@@ -516,7 +562,7 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_far, 0, ne * sizeof(double), ctx->stream));
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_resc, 0, ne * sizeof(int32_t), ctx->stream));
     EPSEON_HIP_TRY(launch_shift_potential(ctx->stream, ctx->d_v, ctx->d_rec, count, max_points,
-                                          ctx->d_ut, ctx->d_uv));
+                                          ctx->d_ut, ctx->d_uv, ctx->d_uvr));
     // The host buffers are pageable and owned by the caller: finish the
     // copies before returning.
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -558,6 +604,7 @@ int epseon_scan(epseon_ctx* ctx)
     ctx->scanned = true;
     ctx->scan_timed = true;
     ctx->refined = false;
+    ctx->normalized = false;
     return EPSEON_OK;
 }
 
@@ -595,6 +642,7 @@ int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
     ctx->refined = true;
     ctx->refine_timed = true;
+    ctx->normalized = false;
     ctx->host_levels_valid = false;
     return EPSEON_OK;
 }
@@ -641,6 +689,124 @@ int fetch_level_meta(epseon_ctx* ctx)
 }
 
 }  // namespace
+
+namespace {
+
+int ensure_stage(epseon_ctx* ctx, size_t elems)
+{
+    if (ctx->stage_elems >= elems && ctx->d_stage != nullptr)
+        return EPSEON_OK;
+    EPSEON_HIP_TRY(arena_array(ctx->arena, std::max<size_t>(elems, 1), &ctx->d_stage));
+    ctx->stage_elems = elems;
+    return EPSEON_OK;
+}
+
+int check_mem(int32_t mem)
+{
+    if (mem != EPSEON_MEM_HOST && mem != EPSEON_MEM_DEVICE)
+        return fail(EPSEON_ERR_INVALID, "memory kind must be EPSEON_MEM_HOST or EPSEON_MEM_DEVICE");
+    return EPSEON_OK;
+}
+
+}  // namespace
+
+int epseon_normalize(epseon_ctx* ctx)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (!ctx->refined)
+        return fail(EPSEON_ERR_STATE, "epseon_normalize before epseon_refine");
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
+    EPSEON_HIP_TRY(launch_normalize(ctx->stream, ctx->d_uv, ctx->d_uvr, ctx->d_rec,
+                                    ctx->d_refine_tasks, ctx->n_refine_tasks, ctx->d_levels,
+                                    ctx->d_status, ctx->d_norms));
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
+    ctx->normalized = true;
+    ctx->normalize_timed = true;
+    return EPSEON_OK;
+}
+
+int epseon_get_wavefunctions(epseon_ctx* ctx, int32_t potential, double* psi, int32_t mem)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (const int rc = check_mem(mem); rc != EPSEON_OK)
+        return rc;
+    if (!ctx->refined)
+        return fail(EPSEON_ERR_STATE, "epseon_get_wavefunctions before epseon_refine");
+    if (potential < 0 || potential >= static_cast<int32_t>(ctx->rec.size()))
+        return fail(EPSEON_ERR_INVALID, "potential index out of range");
+    if (psi == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null output pointer");
+    if (!ctx->normalized)
+        if (const int rc = epseon_normalize(ctx); rc != EPSEON_OK)
+            return rc;
+    const PotentialRecord& r = ctx->rec[potential];
+    const size_t elems = static_cast<size_t>(r.level_count) * static_cast<size_t>(r.full_points);
+    if (elems == 0)
+        return EPSEON_OK;
+    double* dst = psi;
+    if (mem == EPSEON_MEM_HOST) {
+        if (const int rc = ensure_stage(ctx, elems); rc != EPSEON_OK)
+            return rc;
+        dst = ctx->d_stage;
+    }
+    EPSEON_HIP_TRY(launch_eigenfunctions(ctx->stream, ctx->d_uv, ctx->d_uvr, ctx->d_rec, potential,
+                                         r.level_count, ctx->d_levels, ctx->d_status, ctx->d_norms,
+                                         dst));
+    if (mem == EPSEON_MEM_HOST)
+        EPSEON_HIP_TRY(hipMemcpyAsync(psi, dst, elems * sizeof(double), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return EPSEON_OK;
+}
+
+int epseon_level_vjp(epseon_ctx* ctx, const double* grad_levels, double* grad_v, int32_t mem)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (const int rc = check_mem(mem); rc != EPSEON_OK)
+        return rc;
+    if (!ctx->refined)
+        return fail(EPSEON_ERR_STATE, "epseon_level_vjp before epseon_refine");
+    if (grad_v == nullptr || (grad_levels == nullptr && ctx->total_levels > 0))
+        return fail(EPSEON_ERR_INVALID, "null gradient pointer");
+    if (!ctx->normalized)
+        if (const int rc = epseon_normalize(ctx); rc != EPSEON_OK)
+            return rc;
+    const double* g = grad_levels;
+    double* out = grad_v;
+    if (mem == EPSEON_MEM_HOST) {
+        if (ctx->total_levels > 0)
+            EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_grad_levels, grad_levels,
+                                          static_cast<size_t>(ctx->total_levels) * sizeof(double),
+                                          hipMemcpyHostToDevice, ctx->stream));
+        g = ctx->d_grad_levels;
+        if (const int rc = ensure_stage(ctx, static_cast<size_t>(ctx->user_span)); rc != EPSEON_OK)
+            return rc;
+        out = ctx->d_stage;
+    }
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
+    for (int w = 0; w < 4; ++w)
+        EPSEON_HIP_TRY(launch_level_vjp(ctx->stream, w + 1, ctx->d_uv, ctx->d_uvr, ctx->d_rec,
+                                        ctx->d_vjp_lists[w], ctx->n_vjp[w], ctx->d_levels,
+                                        ctx->d_status, ctx->d_norms, g, out));
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev[7], ctx->stream));
+    ctx->vjp_timed = true;
+    if (mem == EPSEON_MEM_HOST) {
+        std::vector<double> tmp(static_cast<size_t>(ctx->user_span));
+        if (!tmp.empty())
+            EPSEON_HIP_TRY(hipMemcpyAsync(tmp.data(), out, tmp.size() * sizeof(double),
+                                          hipMemcpyDeviceToHost, ctx->stream));
+        EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (const PotentialRecord& r : ctx->rec)
+            std::memcpy(grad_v + r.user_offset, tmp.data() + r.user_offset,
+                        static_cast<size_t>(r.full_points) * sizeof(double));
+    } else {
+        EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return EPSEON_OK;
+}
 
 int epseon_get_levels(epseon_ctx* ctx, double* levels, int32_t* status, int32_t* iterations)
 {
@@ -745,6 +911,25 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
         stats->levels_converged = ok;
         stats->refine_mode = ctx->last_refine_mode;
         stats->refine_lanes = ctx->last_refine_lanes;
+        // Eigenfunction passes shoot every converged level once over its
+        // trimmed grid (n - 2 steps split between the two inward shots).
+        double shot_pu = 0.0;
+        for (size_t p = 0; p < ctx->rec.size(); ++p) {
+            const PotentialRecord& r = ctx->rec[p];
+            for (int32_t l = 0; l < r.level_count; ++l)
+                if (st[static_cast<size_t>(r.level_offset + l)] == EPSEON_LEVEL_OK)
+                    shot_pu += static_cast<double>(r.n_points - 2);
+        }
+        if (ctx->normalize_timed) {
+            EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+            stats->normalize_ms = ms;
+            stats->normalize_point_updates = shot_pu;
+        }
+        if (ctx->vjp_timed) {
+            EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]));
+            stats->vjp_ms = ms;
+            stats->vjp_point_updates = shot_pu;
+        }
     }
     return EPSEON_OK;
 }

@@ -229,20 +229,25 @@ __device__ __forceinline__ int32_t wave_task_index()
 
 }  // namespace
 
-// u_i = fma(beta_p, V_i, -1) (scan table) and uv_i = beta_p V_i (refinement
-// table) for every point of the trimmed grid of every potential.
+// u_i = fma(beta_p, V_i, -1) (scan table), uv_i = beta_p V_i (refinement
+// table) and the reversed refinement table uvr_i = uv_{n-1-i} (inward shots
+// from the right wall) for every point of the trimmed grid of every potential.
 __global__ __launch_bounds__(256) void shift_potential_kernel(const double* __restrict__ v,
                                                               const PotentialRecord* __restrict__ pots,
                                                               double* __restrict__ ut,
-                                                              double* __restrict__ uv)
+                                                              double* __restrict__ uv,
+                                                              double* __restrict__ uvr)
 {
     const PotentialRecord& P = pots[blockIdx.y];
     const double beta = P.beta;
-    for (int32_t i = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x); i < P.n_points;
+    const int32_t n = P.n_points;
+    for (int32_t i = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x); i < n;
          i += static_cast<int32_t>(gridDim.x * blockDim.x)) {
         const double x = v[P.ut_offset + i];
+        const double b = beta * x;
         ut[P.ut_offset + i] = __builtin_fma(beta, x, -1.0);
-        uv[P.ut_offset + i] = beta * x;
+        uv[P.ut_offset + i] = b;
+        uvr[P.ut_offset + (n - 1 - i)] = b;
     }
 }
 
@@ -753,13 +758,14 @@ __global__ __launch_bounds__(T) void numerov_refine_segmented_kernel(
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_shift_potential(hipStream_t stream, const double* v, const PotentialRecord* pots,
-                                  int32_t n_pots, int32_t max_points, double* ut, double* uv)
+                                  int32_t n_pots, int32_t max_points, double* ut, double* uv,
+                                  double* uvr)
 {
     if (n_pots == 0)
         return hipSuccess;
     const int32_t bx = (max_points + 255) / 256;
     dim3 grid(static_cast<unsigned>(bx < 1024 ? bx : 1024), static_cast<unsigned>(n_pots));
-    hipLaunchKernelGGL(shift_potential_kernel, grid, dim3(256), 0, stream, v, pots, ut, uv);
+    hipLaunchKernelGGL(shift_potential_kernel, grid, dim3(256), 0, stream, v, pots, ut, uv, uvr);
     return hipGetLastError();
 }
 

@@ -38,8 +38,27 @@ struct alignas(16) PotentialRecord {
     double e_hi;            // window top
     double abs_tol;         // absolute refinement tolerance
     double h;               // grid step
+    int64_t user_offset;    // point 0 of this potential in the caller's V layout
+    int32_t match;          // matching point of the two-sided eigenfunction shots
+                            // (min V, relative to first_point, in [1, n_points - 3])
+    int32_t full_points;    // points of the untrimmed grid
 };
-static_assert(sizeof(PotentialRecord) == 112, "record layout");
+static_assert(sizeof(PotentialRecord) == 128, "record layout");
+
+// Normalisation of one level's eigenfunction, from two inward shots that meet
+// at PotentialRecord::match: psi_i = y_i * ldexp(a, 240 (k - k_final)) with
+// (a, k_final) = (a_left, k_left) left of the match and (a_right, k_right)
+// right of it, k being the shot's rescale count at point i.  Zero for levels
+// that did not converge.
+struct LevelNorm {
+    double a_left, a_right;
+    int32_t k_left, k_right;
+};
+
+// Rescaling of the eigenfunction shots: 2^240 threshold, checked after every
+// 32 steps, keeps y^2 sums far from overflow.
+constexpr double kPsiRescaleAt = 0x1p240;
+constexpr int kPsiRescaleLog2 = 240;
 
 // Work item of one wavefront: potential index and the first trial energy
 // (scan) or first level (refine) of its 64 lanes.

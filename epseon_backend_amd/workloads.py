@@ -65,3 +65,29 @@ def batched(count: int = 4096, seed: int = BATCH_SEED,
                   float(re[i] + 6.0 / a[i]), int(n[i]), int(m[i]))
         for i in range(count)
     ]
+
+
+DIFF_SEED = BATCH_SEED + 1
+
+
+def differentiation(count: int = 256, seed: int = DIFF_SEED, n_points: int = 50_000) -> list[MorseSpec]:
+    """Fitting batch: potentials drawn as in ``batched`` (seed + 1 by default),
+    every grid replaced by ``n_points`` points on the same [r_e - 0.6, r_e + 6/a]."""
+    specs = batched(count, seed)
+    for s in specs:
+        s.n = n_points
+    return specs
+
+
+def perturbed(specs: list[MorseSpec], seed: int, rel: float = 0.02) -> list[MorseSpec]:
+    """Copies with D_e, a, r_e each scaled by (1 + rel U[-1, 1]) (draw order
+    D_e, a, r_e, each a vector over the batch); grids, masses and counts kept."""
+    rng = np.random.default_rng(seed)
+    k = len(specs)
+    f_de = 1.0 + rel * rng.uniform(-1.0, 1.0, k)
+    f_a = 1.0 + rel * rng.uniform(-1.0, 1.0, k)
+    f_re = 1.0 + rel * rng.uniform(-1.0, 1.0, k)
+    return [
+        MorseSpec(s.mu, s.de * f_de[i], s.a * f_a[i], s.re * f_re[i], s.r_min, s.r_max, s.n, s.m)
+        for i, s in enumerate(specs)
+    ]

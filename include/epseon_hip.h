@@ -46,6 +46,10 @@ extern "C" {
 #define EPSEON_REFINE_LANE 1       /* one lane per level, sequential shots */
 #define EPSEON_REFINE_SEGMENTED 2  /* one wavefront per level, shots split over 64 grid segments */
 
+/* ---- memory kinds of output/input pointers ------------------------------ */
+#define EPSEON_MEM_HOST 0          /* pageable or pinned host memory */
+#define EPSEON_MEM_DEVICE 1        /* device memory of the context's device */
+
 /* ---- per-potential flags ----------------------------------------------- */
 #define EPSEON_POT_OK 0
 #define EPSEON_POT_UNRESOLVED 1   /* 1 + beta (E_top - V_min) > 3/2, or fewer than 4 points left */
@@ -109,11 +113,13 @@ typedef struct epseon_stats {
     double normalize_ms;          /* device time of the last epseon_normalize */
     double scan_point_updates;    /* sum over potentials of (N - 2) * M */
     double refine_point_updates;  /* sum over levels of shots * (N - 2) */
-    double normalize_point_updates;
+    double normalize_point_updates; /* sum over converged levels of (N - 2) */
     int64_t scan_launches;        /* kernel launches of the last scan */
     int64_t levels_converged;     /* levels with EPSEON_LEVEL_OK after the last refine */
     int32_t refine_mode;          /* EPSEON_REFINE_LANE or _SEGMENTED used by the last refine */
     int32_t refine_lanes;         /* threads per level of the last segmented refine (1: lane mode) */
+    double vjp_ms;                /* device time of the last epseon_level_vjp */
+    double vjp_point_updates;     /* sum over converged levels of (N - 2) per adjoint pass */
 } epseon_stats;
 
 typedef struct epseon_ctx epseon_ctx;
@@ -190,6 +196,37 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter);
 
 /* Wait for all work queued on the context's stream. */
 int epseon_ctx_synchronize(epseon_ctx* ctx);
+
+/* Eigenfunction normalisation of every converged level: two inward shots at
+ * the refined E (difference form of the Numerov recurrence on y), matched by
+ * least squares at the point of min V and normalised to sum_i psi_i^2 h = 1,
+ * psi > 0 next to the left wall.  Stream-ordered.  epseon_get_wavefunctions
+ * and epseon_level_vjp run it when the levels changed since the last call.
+ * The reference has no eigenfunction step; it is the "normalise psi" stage
+ * its algorithm record is configured for (algorithm_config.hpp:75-199) but
+ * never runs (vibwa.hpp:605-637). */
+int epseon_normalize(epseon_ctx* ctx);
+
+/* Normalised eigenfunctions of potential `potential`: level_count rows of
+ * n_points values (the full grid passed at load), row l = psi of level
+ * level_min + l, zero outside the trimmed window and for levels whose status
+ * is not EPSEON_LEVEL_OK.  `mem` says whether `psi` is host or device memory.
+ * Synchronises the stream.  The per-slot output buffer of the reference
+ * (algorithm_config.hpp:189) holds one value per level; this is the
+ * eigenfunction output that layout has no room for. */
+int epseon_get_wavefunctions(epseon_ctx* ctx, int32_t potential, double* psi, int32_t mem);
+
+/* Adjoint of the levels with respect to the tabulated potential (Hellmann-
+ * Feynman, exact for the discrete Numerov eigenproblem):
+ *   grad_v[v_offsets[p] + i] = sum_l grad_levels[level_offset(p) + l] * psi_{p,l}(i)^2 * h_p
+ * over the converged levels l of each potential p; every element of
+ * [v_offsets[p], v_offsets[p] + n_points) is written (zero outside the
+ * trimmed window) and nothing else.  grad_levels has the layout of
+ * epseon_get_levels; grad_v the layout of the V batch passed to
+ * epseon_load_potentials.  Both are host or both device memory (`mem`).
+ * Eigenfunctions are recomputed, never stored.  Synchronises the stream.
+ * No counterpart in the reference (it has no gradient path). */
+int epseon_level_vjp(epseon_ctx* ctx, const double* grad_levels, double* grad_v, int32_t mem);
 
 /* Copy the refined levels out, in potential order, level_count per potential
  * (the reference's per-slot output buffer, algorithm_config.hpp:189).  Any of

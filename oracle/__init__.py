@@ -89,6 +89,10 @@ def _lib() -> ctypes.CDLL:
         "eo_segment_count": (i32, [i64, i32]),
         "eo_segment_start": (i64, [i64, i32, i32]),
         "eo_shoot_segmented": (None, [_dp, i64, d, i32, i32, _dp, _ip, _ip]),
+        "eo_match_point": (i64, [_dp, i64, i64]),
+        "eo_psi_norm": (None, [_dp, i64, d, i64, d, _dp, _ip]),
+        "eo_psi": (None, [_dp, i64, d, i64, d, _dp]),
+        "eo_level_vjp": (None, [_dp, i64, _dp, _dp, i32, i64, d, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -281,6 +285,13 @@ class Problem:
     def abs_tol(self) -> float:
         return 1e-15 * (self.e_top - self.e_lo)
 
+    @property
+    def match(self) -> int:
+        """Matching point of the two inward eigenfunction shots (min V,
+        relative to the trimmed grid, clamped to [1, n_trim - 3])."""
+        first, last, _ = self.trimmed
+        return int(_lib().eo_match_point(_f64(self.v).ctypes.data_as(_dp), first, last - first + 1))
+
 
 def segment_count(n: int, lanes: int = 64) -> int:
     return int(_lib().eo_segment_count(n, lanes))
@@ -327,3 +338,40 @@ def morse_levels(mu: float, de: float, a: float, v: np.ndarray | int) -> np.ndar
     kp = K_CM / mu
     x = np.asarray(v, dtype=np.float64) + 0.5
     return 2.0 * a * np.sqrt(kp * de) * x - a * a * kp * x * x
+
+
+def psi_norm(prob: Problem, level: float) -> tuple[float, float, int, int]:
+    """(a_left, a_right, k_left, k_right) of the eigenfunction at E = level."""
+    uv = prob.uv
+    a = np.zeros(2)
+    k = np.zeros(2, dtype=np.int32)
+    _lib().eo_psi_norm(uv.ctypes.data_as(_dp), len(uv), prob.beta * level, prob.match, prob.h,
+                       a.ctypes.data_as(_dp), k.ctypes.data_as(_ip))
+    return float(a[0]), float(a[1]), int(k[0]), int(k[1])
+
+
+def eigenfunction(prob: Problem, level: float) -> np.ndarray:
+    """Normalised eigenfunction at E = level on the full grid (zero outside
+    the open trimmed window), sum psi^2 h = 1."""
+    uv = prob.uv
+    first, last, _ = prob.trimmed
+    out = np.zeros(prob.n)
+    win = np.zeros(len(uv))
+    _lib().eo_psi(uv.ctypes.data_as(_dp), len(uv), prob.beta * level, prob.match, prob.h,
+                  win.ctypes.data_as(_dp))
+    out[first : last + 1] = win
+    return out
+
+
+def level_vjp(prob: Problem, levels: np.ndarray, grad: np.ndarray) -> np.ndarray:
+    """sum_l grad[l] psi_l^2 h on the full grid over the finite levels."""
+    uv = prob.uv
+    first, last, _ = prob.trimmed
+    e = _f64(np.where(np.isfinite(levels), prob.beta * np.asarray(levels, dtype=np.float64), np.nan))
+    g = _f64(grad)
+    win = np.zeros(len(uv))
+    _lib().eo_level_vjp(uv.ctypes.data_as(_dp), len(uv), e.ctypes.data_as(_dp), g.ctypes.data_as(_dp),
+                        len(e), prob.match, prob.h, win.ctypes.data_as(_dp))
+    out = np.zeros(prob.n)
+    out[first : last + 1] = win
+    return out

@@ -49,6 +49,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -668,4 +669,169 @@ void eo_solve(const double* ut, const double* uv, int64_t n, double beta, double
         iterations[l] = eo_refine_level(uv, n, beta, stride, v_min + l, &br, rel_tol, abs_tol,
                                         max_iter, segmented, lanes, &levels[l]);
     }
+}
+
+/* ---- eigenfunctions and the adjoint ------------------------------------
+ *
+ * Discrete problem on the trimmed grid (Dirichlet points 0 and n-1), with
+ * f_i = beta (E - V_i) = e - uv_i:
+ *   (1 + f_{i+1}) y_{i+1} = (2 - 10 f_i) y_i - (1 + f_{i-1}) y_{i-1}.
+ * In matrix form A y + B diag(f) y = 0, A = tridiag(1, -2, 1),
+ * B = tridiag(1, 10, 1); A and B commute, so diag(V) - B^-1 A / beta is
+ * symmetric and dE/dV_i = y_i^2 / sum_j y_j^2 (Hellmann-Feynman for the
+ * discrete eigenpair).
+ *
+ * Shot (difference form on y; f carried at full relative precision):
+ *   fn = e - uv_{i+1};  r = 1 / (1 + fn);  s = fma(10, f_i, fn);
+ *   c = fma(s, y_i, f_{i-1} y_{i-1});  d = (d - c) r;  y_{i+1} = y_i + d,
+ * from (y_0, y_1, d_1) = (0, 1, 1).  Each shot visits its points in blocks of
+ * 32 steps; after every full block, if max(|y_i|, |y_{i-1}|) >= 2^240, y, y_m
+ * and d are multiplied by 2^-240 and the shot's rescale count k grows.
+ *
+ * Two shots meet at the matching point m (min V on the trimmed grid, clamped
+ * to [1, n-3]): the left one from point 0 visits points 1..m (m steps), the
+ * right one runs over the reversed table from point n-1 and visits points
+ * n-2..m+1 (n-2-m steps).  Sums of y^2 over the visited points are rescaled
+ * with the shot.  The right shot is scaled by the least-squares factor
+ *   c = (yl_m yr_m + yl_{m+1} yr_{m+1}) / (yr_m^2 + yr_{m+1}^2)
+ * and S = fma(c*c, S_right, S_left), a = 1/sqrt(S h): psi_i = y_i a 2^{240 (k_i - k_left)}
+ * on the left, y_i (c a) 2^{240 (k_i - k_right)} on the right; sum psi^2 h = 1.
+ */
+typedef struct {
+    double y, ym, d, f, fm;
+    int32_t k;
+} eo_yshot;
+
+static void eo_ys_init(eo_yshot* s, double e, double u0, double u1)
+{
+    s->ym = 0.0;
+    s->y = 1.0;
+    s->d = 1.0;
+    s->fm = e - u0;
+    s->f = e - u1;
+    s->k = 0;
+}
+
+static void eo_ys_step(eo_yshot* s, double u, double e)
+{
+    const double fn = e - u;
+    const double r = 1.0 / (1.0 + fn);
+    const double sg = fma(10.0, s->f, fn);
+    const double c = fma(sg, s->y, s->fm * s->ym);
+    s->d = (s->d - c) * r;
+    s->ym = s->y;
+    s->y = s->y + s->d;
+    s->fm = s->f;
+    s->f = fn;
+}
+
+static int32_t eo_ys_rescale(eo_yshot* s)
+{
+    const int big = fmax(fabs(s->y), fabs(s->ym)) >= 0x1p240;
+    const int32_t sh = big ? -240 : 0;
+    s->y = ldexp(s->y, sh);
+    s->ym = ldexp(s->ym, sh);
+    s->d = ldexp(s->d, sh);
+    s->k += big;
+    return sh;
+}
+
+int64_t eo_match_point(const double* v, int64_t first, int64_t n)
+{
+    int64_t imin = 0;
+    for (int64_t i = 1; i < n; ++i)
+        if (v[first + i] < v[first + imin])
+            imin = i;
+    if (imin < 1)
+        imin = 1;
+    if (imin > n - 3)
+        imin = n - 3;
+    return imin;
+}
+
+/* One inward shot over `steps` steps of table u (visiting u's points
+ * 1..steps); returns the rescaled sum of y^2 over the visited points.  When
+ * psi is given, psi[i0 + dir*j] = y_{j+1} a 2^{240 (k - kf)} for the j-th
+ * visited point. */
+static double eo_inward(const double* u, int64_t steps, double e, eo_yshot* s, double* psi,
+                        int64_t i0, int64_t dir, double a, int32_t kf)
+{
+    double sum = 0.0;
+    double mult = ldexp(a, -240 * kf);
+    eo_ys_init(s, e, u[0], u[1]);
+    for (int64_t j = 0; j < steps; ++j) {
+        if (psi)
+            psi[i0 + dir * j] = s->y * mult;
+        sum = fma(s->y, s->y, sum);
+        eo_ys_step(s, u[2 + j], e);
+        if ((j + 1) % 32 == 0) {
+            const int32_t sh = eo_ys_rescale(s);
+            sum = ldexp(sum, 2 * sh);
+            mult = ldexp(a, 240 * (s->k - kf));
+        }
+    }
+    return sum;
+}
+
+static double* eo_reversed(const double* uv, int64_t n)
+{
+    double* r = (double*)malloc((size_t)n * sizeof(double));
+    for (int64_t i = 0; i < n; ++i)
+        r[i] = uv[n - 1 - i];
+    return r;
+}
+
+/* Normalisation of the eigenfunction at e = beta E: a[0..1] = (a_left,
+ * a_right), k[0..1] = (k_left, k_right). */
+void eo_psi_norm(const double* uv, int64_t n, double e, int64_t m, double h, double* a, int32_t* k)
+{
+    double* ur = eo_reversed(uv, n);
+    eo_yshot L, R;
+    const double sl = eo_inward(uv, m, e, &L, NULL, 0, 0, 0.0, 0);
+    const double sr = eo_inward(ur, n - 2 - m, e, &R, NULL, 0, 0, 0.0, 0);
+    free(ur);
+    const double yl0 = L.ym, yl1 = L.y;
+    const double yr1 = R.ym, yr0 = R.y;
+    const double num = fma(yl0, yr0, yl1 * yr1);
+    const double den = fma(yr0, yr0, yr1 * yr1);
+    const double c = num / den;
+    const double s = fma(c * c, sr, sl);
+    const double an = 1.0 / sqrt(s * h);
+    a[0] = an;
+    a[1] = c * an;
+    k[0] = L.k;
+    k[1] = R.k;
+}
+
+/* Normalised eigenfunction on the trimmed grid (psi[0] = psi[n-1] = 0). */
+void eo_psi(const double* uv, int64_t n, double e, int64_t m, double h, double* psi)
+{
+    double a[2];
+    int32_t k[2];
+    eo_psi_norm(uv, n, e, m, h, a, k);
+    double* ur = eo_reversed(uv, n);
+    eo_yshot L, R;
+    psi[0] = 0.0;
+    psi[n - 1] = 0.0;
+    (void)eo_inward(uv, m, e, &L, psi, 1, 1, a[0], k[0]);
+    (void)eo_inward(ur, n - 2 - m, e, &R, psi, n - 2, -1, a[1], k[1]);
+    free(ur);
+}
+
+/* Adjoint on the trimmed grid: grad[i] = sum_l g[l] psi_l(i)^2 h over the
+ * levels with finite energy (levels in e-units: e_l = beta E_l). */
+void eo_level_vjp(const double* uv, int64_t n, const double* e, const double* g, int32_t count,
+                  int64_t m, double h, double* grad)
+{
+    double* psi = (double*)malloc((size_t)n * sizeof(double));
+    for (int64_t i = 0; i < n; ++i)
+        grad[i] = 0.0;
+    for (int32_t l = 0; l < count; ++l) {
+        if (!isfinite(e[l]))
+            continue;
+        eo_psi(uv, n, e[l], m, h, psi);
+        for (int64_t i = 0; i < n; ++i)
+            grad[i] = fma(g[l] * h * psi[i], psi[i], grad[i]);
+    }
+    free(psi);
 }

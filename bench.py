@@ -10,7 +10,16 @@ refinement of every level, levels copied back.  The potential table is
 uploaded once before timing (inputs resident in HBM).
 
 ``--workload batched`` runs the batched case instead: 4,096 seeded Morse
-potentials with ragged grids (2k-64k points) and trial counts (256-8192).
+potentials with ragged grids (2k-64k points) and trial counts (256-8192); it
+also times the same solve done potential by potential (one context and one
+launch sequence per potential) on a seeded sample of 64 potentials.
+
+``--workload fit`` runs the differentiation case: 256 potentials drawn as in
+the batched case (seed + 1) on 50,000-point grids, levels v = 0..40, a
+least-squares loss against levels of +-2 % perturbed parameters plus
+N(0, 0.01^2) noise; a step is forward (Morse tabulation in torch, solve)
+plus backward (adjoint kernel, chain to D_e, a, r_e in torch), and the line
+carries the grad/fwd wall-time ratio.
 
 Prints one JSON line: metric/value (whole-job Numerov point-updates per
 second over scan + refinement shots), eigenlevels/s, the roofline of the
@@ -107,6 +116,123 @@ def cpu_baseline(target_seconds: float = 12.0) -> dict:
     }
 
 
+def per_potential_launches(capi, pots, rel_tol: float, sample: int = 64) -> dict:
+    """The batched workload solved one potential at a time: each potential of
+    a seeded sample in its own context (uploaded before timing), then a timed
+    loop of solve + levels per potential."""
+    rng = np.random.default_rng(1)
+    idx = sorted(int(i) for i in rng.choice(len(pots), sample, replace=False))
+    ctxs = []
+    for i in idx:
+        c = capi.Context(0, 8 << 20)
+        c.load([pots[i]])
+        c.solve(rel_tol, 200)
+        c.levels()
+        ctxs.append(c)
+    levels = 0
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.solve(rel_tol, 200)
+        _, st, _ = c.levels()
+        levels += int(np.sum(st == capi.LEVEL_OK))
+    wall = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    return {"eigenlevels_per_s": levels / wall, "potentials": sample, "levels": levels, "wall_s": wall}
+
+
+def run_fit(capi, args) -> int:
+    """Differentiation workload: forward + backward through the adjoint."""
+    import torch
+
+    from epseon_backend_amd.autograd import LevelSolver, LevelSpec, morse_table, numerov_levels, uniform_grid
+    from epseon_backend_amd.workloads import DIFF_SEED, differentiation, perturbed
+
+    levels = 41
+    specs = differentiation(256)
+    obs = perturbed(specs, seed=DIFF_SEED + 1)
+    lspecs = [LevelSpec(s.r_min, s.r_max, s.mu, s.de, s.m, 0, levels) for s in specs]
+    dev = torch.device("cuda:0")
+    t = lambda x: torch.tensor(x, dtype=torch.float64, device=dev)  # noqa: E731
+    r = uniform_grid(t([s.r_min for s in specs]), t([s.r_max for s in specs]), specs[0].n)
+    solver = LevelSolver(0, rel_tol=args.rel_tol)
+    with torch.no_grad():
+        e_obs = numerov_levels(morse_table(r, t([s.de for s in obs]), t([s.a for s in obs]),
+                                           t([s.re for s in obs])), lspecs, solver)
+        gen = torch.Generator(device=dev).manual_seed(DIFF_SEED)
+        e_obs = e_obs + 0.01 * torch.randn(e_obs.shape, dtype=torch.float64, device=dev, generator=gen)
+    theta0 = [t([s.de for s in specs]), t([s.a for s in specs]), t([s.re for s in specs])]
+
+    def step():
+        th = [x.clone().requires_grad_() for x in theta0]
+        torch.cuda.synchronize()
+        a0 = time.perf_counter()
+        e = numerov_levels(morse_table(r, *th), lspecs, solver)
+        ok = torch.isfinite(e) & torch.isfinite(e_obs)
+        loss = torch.where(ok, (e - e_obs) ** 2, torch.zeros_like(e)).sum()
+        torch.cuda.synchronize()
+        a1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        a2 = time.perf_counter()
+        return a1 - a0, a2 - a1, int(ok.sum())
+
+    for _ in range(args.warmup):
+        step()
+    fwd, bwd = [], []
+    pu = 0.0
+    levels_ok = 0
+    solve_ms, vjp_ms, norm_ms = [], [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f, b, n_ok = step()
+        fwd.append(f)
+        bwd.append(b)
+        st = solver.last_stats
+        pu += st.scan_point_updates + st.refine_point_updates + st.normalize_point_updates + st.vjp_point_updates
+        solve_ms.append(st.solve_ms)
+        vjp_ms.append(st.vjp_ms)
+        norm_ms.append(st.normalize_ms)
+        levels_ok += n_ok
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    solver.close()
+    n_points = int(sum(s.n for s in specs))
+    out = {
+        "metric": METRIC,
+        "value": pu / wall,
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"fit: 256 Morse potentials drawn as the batched case (seed {DIFF_SEED}) on "
+            "50000-point grids, levels v=0..40, loss sum (E - E_obs)^2 against +-2% perturbed parameters "
+            "+ N(0, 0.01^2); forward (tabulate, solve) + backward (adjoint kernel, chain to De, a, re)",
+            "potentials": len(specs),
+            "grid_points": n_points,
+        },
+        "grad_fwd_ratio": float(np.mean(bwd) / np.mean(fwd)),
+        "forward_ms": float(np.mean(fwd)) * 1e3,
+        "backward_ms": float(np.mean(bwd)) * 1e3,
+        "solve_ms": float(np.mean(solve_ms)),
+        "normalize_ms": float(np.mean(norm_ms)),
+        "vjp_ms": float(np.mean(vjp_ms)),
+        "eigenlevels_per_s": levels_ok / wall,
+        "levels_per_step": levels_ok // args.steps,
+    }
+    sys.stdout.write(json.dumps(out) + "\n")
+    sys.stdout.flush()
+    return 0
+
+
 def load_traffic(workload: str) -> float | None:
     """HBM bytes per scan launch measured by rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration), if committed."""
@@ -124,9 +250,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["single", "batched"], default="single")
+    ap.add_argument("--workload", choices=["single", "batched", "fit"], default="single")
     ap.add_argument("--rel-tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="batched: keep the default launch configuration")
     ap.add_argument(
         "--copy-calibration",
         action="store_true",
@@ -161,16 +288,21 @@ def main() -> int:
         sys.stderr.write("no HIP device visible\n")
         return 1
 
+    if args.workload == "fit":
+        return run_fit(capi, args)
     pots, config = single_workload(capi) if args.workload == "single" else batched_workload(capi)
     ctx = capi.Context(0)
     ctx.load(pots)
+    tuning = None
+    if args.workload == "batched" and not args.no_autotune:
+        tuning = ctx.autotune(args.rel_tol, 200).as_dict()
 
     for _ in range(args.warmup):
         ctx.solve(args.rel_tol, 200)
         ctx.levels()
 
-    scan_ms: list[float] = []
-    refine_ms: list[float] = []
+    solve_ms: list[float] = []
+    solve_split = 0.0
     scan_pu = refine_pu = 0.0
     levels_ok = 0
     ctx.synchronize()
@@ -179,17 +311,29 @@ def main() -> int:
         ctx.solve(args.rel_tol, 200)
         lv, st, it = ctx.levels()
         s = ctx.stats()
-        scan_ms.append(s.scan_ms)
-        refine_ms.append(s.refine_ms)
+        solve_ms.append(s.solve_ms)
+        solve_split = s.solve_split
         scan_pu += s.scan_point_updates
         refine_pu += s.refine_point_updates
         levels_ok += int(s.levels_converged)
     ctx.synchronize()
     wall = time.perf_counter() - t0
+    # Roofline of the scan kernel: isolated scans (one launch each), timed
+    # with HIP events on the context's stream; the solve above may overlap
+    # scan and refinement on two streams.
+    iso_scan_ms, iso_refine_ms = [], []
+    for _ in range(3):
+        ctx.scan()
+        ctx.refine(args.rel_tol, 200)
+        s = ctx.stats()
+        iso_scan_ms.append(s.scan_ms)
+        iso_refine_ms.append(s.refine_ms)
     ctx.close()
 
     ms_per_step = wall / args.steps * 1e3
     value = (scan_pu + refine_pu) / wall
+    scan_ms = iso_scan_ms
+    refine_ms = iso_refine_ms
     avg_scan_s = float(np.mean(scan_ms)) * 1e-3
     achieved = FLOP_PER_POINT_UPDATE * (scan_pu / args.steps) / avg_scan_s / 1e12
     out = {
@@ -208,6 +352,8 @@ def main() -> int:
         "config": config,
         "eigenlevels_per_s": levels_ok / wall,
         "levels_per_step": levels_ok // args.steps,
+        "solve_ms": float(np.mean(solve_ms)),
+        "solve_split": solve_split,
         "scan_ms": float(np.mean(scan_ms)),
         "refine_ms": float(np.mean(refine_ms)),
         "scan_point_updates_per_step": scan_pu / args.steps,
@@ -223,6 +369,12 @@ def main() -> int:
     }
     if not args.no_cpu_baseline and args.workload == "single":
         out["cpu_baseline"] = cpu_baseline()
+    if tuning is not None:
+        out["tuning"] = tuning
+    if args.workload == "batched":
+        pp = per_potential_launches(capi, pots, args.rel_tol)
+        out["per_potential_launches"] = pp
+        out["batched_vs_per_potential"] = out["eigenlevels_per_s"] / pp["eigenlevels_per_s"]
     sys.stdout.write(json.dumps(out) + "\n")
     sys.stdout.flush()
     return 0

@@ -113,8 +113,25 @@ class Stats(ctypes.Structure):
         ("refine_lanes", ctypes.c_int32),
         ("vjp_ms", ctypes.c_double),
         ("vjp_point_updates", ctypes.c_double),
+        ("solve_ms", ctypes.c_double),
+        ("solve_split", ctypes.c_double),
     ]
 
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("refine_mode", ctypes.c_int32),
+        ("levels_per_lane", ctypes.c_int32),
+        ("scan_waves_per_block", ctypes.c_int32),
+        ("segmented_lanes", ctypes.c_int32),
+        ("solve_split", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+TUNE_KEEP_RESULTS = 1
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int32_p = ctypes.POINTER(ctypes.c_int32)
@@ -137,6 +154,13 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "epseon_refine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
     "epseon_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
     "epseon_set_refine_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "epseon_set_solve_split": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "epseon_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Tuning)]),
+    "epseon_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Tuning)]),
+    "epseon_autotune": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Tuning)],
+    ),
     "epseon_ctx_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "epseon_get_levels": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_int32_p, _c_int32_p]),
     "epseon_get_scan": (ctypes.c_int, [ctypes.c_void_p, _c_int32_p, _c_double_p, _c_int32_p]),
@@ -195,7 +219,7 @@ def load_library() -> ctypes.CDLL:
 
 
 # Public structs in the order epseon_struct_sizes reports them.
-ABI_STRUCTS = (DeviceInfo, PotentialDesc, PotentialInfo, Stats)
+ABI_STRUCTS = (DeviceInfo, PotentialDesc, PotentialInfo, Stats, Tuning)
 
 
 def _check(rc: int) -> None:
@@ -317,6 +341,26 @@ class Context:
 
     def set_refine_mode(self, mode: int) -> None:
         _check(self._lib.epseon_set_refine_mode(self._h, mode))
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        _check(self._lib.epseon_get_tuning(self._h, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, t: Tuning) -> None:
+        _check(self._lib.epseon_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, rel_tol: float = 1e-10, max_iter: int = 200, keep_results: bool = False) -> Tuning:
+        """Time candidate launch configurations on the loaded batch and keep
+        the fastest (epseon_autotune)."""
+        t = Tuning()
+        flags = TUNE_KEEP_RESULTS if keep_results else 0
+        _check(self._lib.epseon_autotune(self._h, rel_tol, max_iter, flags, ctypes.byref(t)))
+        return t
+
+    def set_solve_split(self, fraction: float) -> None:
+        """Scan-work fraction of the first group of the pipelined solve (0: sequential)."""
+        _check(self._lib.epseon_set_solve_split(self._h, fraction))
 
     def solve(self, rel_tol: float = 1e-10, max_iter: int = 200) -> None:
         _check(self._lib.epseon_solve(self._h, rel_tol, max_iter))

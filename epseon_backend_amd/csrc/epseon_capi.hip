@@ -27,9 +27,9 @@ hipError_t launch_shift_potential(hipStream_t, const double*, const PotentialRec
                                   int32_t, double*, double*, double*);
 hipError_t launch_scan(hipStream_t, const double*, const PotentialRecord*, const WaveTask*, int32_t,
                        int32_t, int32_t*, double*, int32_t*);
-hipError_t launch_refine(hipStream_t, const double*, const PotentialRecord*, const WaveTask*,
-                         int32_t, int32_t, const int32_t*, const double*, const int32_t*, double,
-                         int32_t, double*, int32_t*, int32_t*);
+hipError_t launch_refine(hipStream_t, int32_t, const double*, const PotentialRecord*,
+                         const WaveTask*, int32_t, int32_t, const int32_t*, const double*,
+                         const int32_t*, double, int32_t, double*, int32_t*, int32_t*);
 hipError_t launch_refine_segmented(hipStream_t, int32_t, const double*, const PotentialRecord*,
                                    const WaveTask*, int32_t, const int32_t*, const double*,
                                    const int32_t*, double, int32_t, double*, int32_t*, int32_t*);
@@ -174,7 +174,10 @@ using namespace epseon::hip;
 struct epseon_ctx {
     int32_t device = 0;
     hipStream_t stream = nullptr;
+    // Second stream of the pipelined solve (default priority, like the first).
+    hipStream_t stream2 = nullptr;
     hipEvent_t ev[8] = {};
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_solve0 = nullptr, ev_solve1 = nullptr;
     Arena arena;
     int32_t waves_per_block = 1;
 
@@ -191,7 +194,21 @@ struct epseon_ctx {
     WaveTask* d_scan_tasks = nullptr;
     WaveTask* d_refine_tasks = nullptr;
     WaveTask* d_level_tasks = nullptr;
+    // Lane-mode refinement tasks: 64 levels (one per lane) for potentials
+    // with at most 64 levels, 128 levels (two per lane) for the others.
+    WaveTask* d_lane1_tasks = nullptr;
+    WaveTask* d_lane2_tasks = nullptr;
     int32_t n_scan_tasks = 0, n_refine_tasks = 0, n_level_tasks = 0;
+    // Cumulative task counts and scan work over the potentials in task order
+    // (longest grids first): entry k covers the first k potentials.
+    std::vector<int32_t> cum_scan_tasks, cum_refine_tasks, cum_level_tasks, cum_lane1_tasks,
+        cum_lane2_tasks;
+    std::vector<double> cum_scan_work;
+    double solve_split = 0.0;
+    int32_t levels_per_lane = 1;
+    int32_t segmented_lanes = 0;  // 0: chosen from the batch size
+    bool solve_timed = false;
+    double last_split_used = 0.0;
     int32_t refine_mode = EPSEON_REFINE_AUTO;
     int32_t last_refine_mode = EPSEON_REFINE_AUTO;
     int32_t last_refine_lanes = 0;
@@ -249,8 +266,9 @@ int epseon_struct_sizes(int64_t* sizes, int32_t count)
     if (sizes == nullptr || count < 0)
         return fail(EPSEON_ERR_INVALID, "invalid size buffer");
     const int64_t all[] = {sizeof(epseon_device_info), sizeof(epseon_potential_desc),
-                           sizeof(epseon_potential_info), sizeof(epseon_stats)};
-    for (int32_t i = 0; i < count && i < 4; ++i)
+                           sizeof(epseon_potential_info), sizeof(epseon_stats),
+                           sizeof(epseon_tuning)};
+    for (int32_t i = 0; i < count && i < 5; ++i)
         sizes[i] = all[i];
     return EPSEON_OK;
 }
@@ -327,8 +345,18 @@ int epseon_ctx_create(int32_t device, uint64_t block_bytes, epseon_ctx** out)
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
     for (int i = 0; i < 8 && e == hipSuccess; ++i)
         e = hipEventCreate(&ctx->ev[i]);
+    if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipEventCreate(&ctx->ev_solve0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&ctx->ev_solve1);
     if (e != hipSuccess) {
         const int rc = hip_fail(e, "context creation");
         epseon_ctx_destroy(ctx);
@@ -345,12 +373,19 @@ int epseon_ctx_destroy(epseon_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream != nullptr)
         (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream2 != nullptr)
+        (void)hipStreamSynchronize(ctx->stream2);
     ctx->arena.release();
     for (hipEvent_t& ev : ctx->ev)
         if (ev != nullptr)
             (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : {ctx->ev_fork, ctx->ev_join, ctx->ev_solve0, ctx->ev_solve1})
+        if (ev != nullptr)
+            (void)hipEventDestroy(ev);
     if (ctx->stream != nullptr)
         (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream2 != nullptr)
+        (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
     return EPSEON_OK;
 }
@@ -365,6 +400,7 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->loaded = ctx->scanned = ctx->refined = ctx->normalized = false;
     ctx->scan_timed = ctx->refine_timed = ctx->normalize_timed = ctx->vjp_timed = false;
+    ctx->solve_timed = false;
     ctx->d_stage = nullptr;
     ctx->stage_elems = 0;
     ctx->desc.assign(desc, desc + count);
@@ -490,8 +526,14 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
         return ctx->rec[a].n_points > ctx->rec[b].n_points;
     });
     // Segmented refinement uses one 64-lane block per level.
-    std::vector<WaveTask> scan_tasks, refine_tasks, level_tasks;
+    std::vector<WaveTask> scan_tasks, refine_tasks, level_tasks, lane1_tasks, lane2_tasks;
     std::vector<int32_t> vjp_lists[4];
+    ctx->cum_lane1_tasks.assign(1, 0);
+    ctx->cum_lane2_tasks.assign(1, 0);
+    ctx->cum_scan_tasks.assign(1, 0);
+    ctx->cum_refine_tasks.assign(1, 0);
+    ctx->cum_level_tasks.assign(1, 0);
+    ctx->cum_scan_work.assign(1, 0.0);
     for (int32_t p : order) {
         const int32_t groups = (ctx->rec[p].level_count + 63) / 64;
         vjp_lists[std::clamp(groups, 1, 4) - 1].push_back(p);
@@ -503,6 +545,19 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
             refine_tasks.push_back({p, l});
         for (int32_t l = 0; l < r.level_count; ++l)
             level_tasks.push_back({p, l});
+        if (r.level_count > 64)
+            for (int32_t l = 0; l < r.level_count; l += 128)
+                lane2_tasks.push_back({p, l});
+        else if (r.level_count > 0)
+            lane1_tasks.push_back({p, 0});
+        ctx->cum_lane1_tasks.push_back(static_cast<int32_t>(lane1_tasks.size()));
+        ctx->cum_lane2_tasks.push_back(static_cast<int32_t>(lane2_tasks.size()));
+        ctx->cum_scan_tasks.push_back(static_cast<int32_t>(scan_tasks.size()));
+        ctx->cum_refine_tasks.push_back(static_cast<int32_t>(refine_tasks.size()));
+        ctx->cum_level_tasks.push_back(static_cast<int32_t>(level_tasks.size()));
+        ctx->cum_scan_work.push_back(
+            ctx->cum_scan_work.back() +
+            (r.flags == EPSEON_POT_OK ? static_cast<double>(r.n_points) * r.n_energies : 0.0));
     }
     ctx->n_scan_tasks = static_cast<int32_t>(scan_tasks.size());
     ctx->n_refine_tasks = static_cast<int32_t>(refine_tasks.size());
@@ -521,6 +576,8 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(scan_tasks.size(), 1), &ctx->d_scan_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(refine_tasks.size(), 1), &ctx->d_refine_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(level_tasks.size(), 1), &ctx->d_level_tasks));
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(lane1_tasks.size(), 1), &ctx->d_lane1_tasks));
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(lane2_tasks.size(), 1), &ctx->d_lane2_tasks));
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_nodes));
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_far));
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_resc));
@@ -558,6 +615,14 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
         EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_level_tasks, level_tasks.data(),
                                       level_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice,
                                       ctx->stream));
+    if (!lane1_tasks.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_lane1_tasks, lane1_tasks.data(),
+                                      lane1_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice,
+                                      ctx->stream));
+    if (!lane2_tasks.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_lane2_tasks, lane2_tasks.data(),
+                                      lane2_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice,
+                                      ctx->stream));
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_nodes, 0, ne * sizeof(int32_t), ctx->stream));
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_far, 0, ne * sizeof(double), ctx->stream));
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_resc, 0, ne * sizeof(int32_t), ctx->stream));
@@ -590,6 +655,100 @@ int epseon_potential_info_get(epseon_ctx* ctx, int32_t p, epseon_potential_info*
     return EPSEON_OK;
 }
 
+namespace {
+
+int32_t choose_refine_mode(const epseon_ctx* ctx)
+{
+    int32_t mode = ctx->refine_mode;
+    if (mode == EPSEON_REFINE_AUTO)
+        mode = ctx->total_levels >= kLanePerLevelMinLevels ? EPSEON_REFINE_LANE : EPSEON_REFINE_SEGMENTED;
+    return mode;
+}
+
+// Segmented blocks: up to 4 wavefronts per level while the batch has too few
+// levels to give every SIMD a wavefront (unless fixed by the tuning).
+int32_t segmented_lanes(const epseon_ctx* ctx)
+{
+    if (ctx->segmented_lanes != 0)
+        return ctx->segmented_lanes;
+    int32_t lanes = 64;
+    while (lanes < 256 && ctx->total_levels * (lanes / 64) * 2 <= 2048)
+        lanes *= 2;
+    return lanes;
+}
+
+// Scan of the potentials [k0, k1) in task order on `stream`.
+hipError_t run_scan(epseon_ctx* ctx, hipStream_t stream, size_t k0, size_t k1)
+{
+    const int32_t t0 = ctx->cum_scan_tasks[k0];
+    const int32_t t1 = ctx->cum_scan_tasks[k1];
+    return launch_scan(stream, ctx->d_ut, ctx->d_rec, ctx->d_scan_tasks + t0, t1 - t0,
+                       ctx->waves_per_block, ctx->d_nodes, ctx->d_far, ctx->d_resc);
+}
+
+// Refinement of the levels of potentials [k0, k1) in task order.  Lane mode
+// with one level per lane runs the 64-level tasks on `stream`; with two, the
+// paired tasks (potentials with more than 64 levels) run on `stream` and the
+// single ones on `stream_b`, which may be the same stream.
+hipError_t run_refine(epseon_ctx* ctx, hipStream_t stream, hipStream_t stream_b, int32_t mode,
+                      int32_t lanes, double rel_tol, int32_t max_iter, size_t k0, size_t k1)
+{
+    if (mode == EPSEON_REFINE_LANE && ctx->levels_per_lane == 1) {
+        const int32_t t0 = ctx->cum_refine_tasks[k0];
+        const int32_t t1 = ctx->cum_refine_tasks[k1];
+        return launch_refine(stream, 1, ctx->d_uv, ctx->d_rec, ctx->d_refine_tasks + t0, t1 - t0,
+                             ctx->waves_per_block, ctx->d_nodes, ctx->d_far, ctx->d_resc, rel_tol,
+                             max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters);
+    }
+    if (mode == EPSEON_REFINE_LANE) {
+        const int32_t a0 = ctx->cum_lane2_tasks[k0];
+        const int32_t a1 = ctx->cum_lane2_tasks[k1];
+        const int32_t b0 = ctx->cum_lane1_tasks[k0];
+        const int32_t b1 = ctx->cum_lane1_tasks[k1];
+        hipError_t e = launch_refine(stream, 2, ctx->d_uv, ctx->d_rec, ctx->d_lane2_tasks + a0, a1 - a0,
+                                     ctx->waves_per_block, ctx->d_nodes, ctx->d_far, ctx->d_resc,
+                                     rel_tol, max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters);
+        if (e != hipSuccess)
+            return e;
+        return launch_refine(stream_b, 1, ctx->d_uv, ctx->d_rec, ctx->d_lane1_tasks + b0, b1 - b0,
+                             ctx->waves_per_block, ctx->d_nodes, ctx->d_far, ctx->d_resc, rel_tol,
+                             max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters);
+    }
+    const int32_t t0 = ctx->cum_level_tasks[k0];
+    const int32_t t1 = ctx->cum_level_tasks[k1];
+    return launch_refine_segmented(stream, lanes, ctx->d_uv, ctx->d_rec, ctx->d_level_tasks + t0,
+                                   t1 - t0, ctx->d_nodes, ctx->d_far, ctx->d_resc, rel_tol, max_iter,
+                                   ctx->d_levels, ctx->d_status, ctx->d_iters);
+}
+
+int check_refine_args(double rel_tol, int32_t max_iter)
+{
+    if (!(rel_tol >= 0.0 && std::isfinite(rel_tol)))
+        return fail(EPSEON_ERR_INVALID, "relative tolerance must be finite and >= 0");
+    if (max_iter < 1)
+        return fail(EPSEON_ERR_INVALID, "max_iter must be >= 1");
+    return EPSEON_OK;
+}
+
+size_t n_potentials(const epseon_ctx* ctx) { return ctx->cum_scan_tasks.size() - 1; }
+
+// Number of potentials (in task order) of the first pipeline group: the
+// longest grids holding up to `fraction` of the scan work.  0 disables the
+// pipeline.
+size_t split_point(const epseon_ctx* ctx, double fraction)
+{
+    const size_t np = n_potentials(ctx);
+    if (!(fraction > 0.0) || np < 2)
+        return 0;
+    const double limit = fraction * ctx->cum_scan_work.back();
+    size_t k = 0;
+    while (k + 1 < np && ctx->cum_scan_work[k + 1] <= limit)
+        ++k;
+    return k;
+}
+
+}  // namespace
+
 int epseon_scan(epseon_ctx* ctx)
 {
     if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
@@ -597,9 +756,7 @@ int epseon_scan(epseon_ctx* ctx)
     if (!ctx->loaded)
         return fail(EPSEON_ERR_STATE, "epseon_scan before epseon_load_potentials");
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
-    EPSEON_HIP_TRY(launch_scan(ctx->stream, ctx->d_ut, ctx->d_rec, ctx->d_scan_tasks,
-                               ctx->n_scan_tasks, ctx->waves_per_block, ctx->d_nodes, ctx->d_far,
-                               ctx->d_resc));
+    EPSEON_HIP_TRY(run_scan(ctx, ctx->stream, 0, n_potentials(ctx)));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
     ctx->scanned = true;
     ctx->scan_timed = true;
@@ -614,35 +771,26 @@ int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
         return rc;
     if (!ctx->scanned)
         return fail(EPSEON_ERR_STATE, "epseon_refine before epseon_scan");
-    if (!(rel_tol >= 0.0 && std::isfinite(rel_tol)))
-        return fail(EPSEON_ERR_INVALID, "relative tolerance must be finite and >= 0");
-    if (max_iter < 1)
-        return fail(EPSEON_ERR_INVALID, "max_iter must be >= 1");
-    int32_t mode = ctx->refine_mode;
-    if (mode == EPSEON_REFINE_AUTO)
-        mode = ctx->total_levels >= kLanePerLevelMinLevels ? EPSEON_REFINE_LANE : EPSEON_REFINE_SEGMENTED;
+    if (const int rc = check_refine_args(rel_tol, max_iter); rc != EPSEON_OK)
+        return rc;
+    const int32_t mode = choose_refine_mode(ctx);
+    const int32_t lanes = segmented_lanes(ctx);
     ctx->last_refine_mode = mode;
-    // Segmented blocks: up to 4 wavefronts per level while the batch has too
-    // few levels to give every SIMD a wavefront.
-    int32_t lanes = 64;
-    while (lanes < 256 && ctx->total_levels * (lanes / 64) * 2 <= 2048)
-        lanes *= 2;
     ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
-    if (mode == EPSEON_REFINE_LANE)
-        EPSEON_HIP_TRY(launch_refine(ctx->stream, ctx->d_uv, ctx->d_rec, ctx->d_refine_tasks,
-                                     ctx->n_refine_tasks, ctx->waves_per_block, ctx->d_nodes,
-                                     ctx->d_far, ctx->d_resc, rel_tol, max_iter, ctx->d_levels,
-                                     ctx->d_status, ctx->d_iters));
-    else
-        EPSEON_HIP_TRY(launch_refine_segmented(ctx->stream, lanes, ctx->d_uv, ctx->d_rec,
-                                               ctx->d_level_tasks, ctx->n_level_tasks, ctx->d_nodes,
-                                               ctx->d_far, ctx->d_resc, rel_tol, max_iter,
-                                               ctx->d_levels, ctx->d_status, ctx->d_iters));
+    // Paired and single lane tasks (two levels per lane) run concurrently on
+    // the two streams.
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+    EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+    EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream2, mode, lanes, rel_tol, max_iter, 0,
+                              n_potentials(ctx)));
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev_join, ctx->stream2));
+    EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
     ctx->refined = true;
     ctx->refine_timed = true;
     ctx->normalized = false;
+    ctx->normalize_timed = ctx->vjp_timed = false;
     ctx->host_levels_valid = false;
     return EPSEON_OK;
 }
@@ -657,11 +805,193 @@ int epseon_set_refine_mode(epseon_ctx* ctx, int32_t mode)
     return EPSEON_OK;
 }
 
+int epseon_set_solve_split(epseon_ctx* ctx, double fraction)
+{
+    if (ctx == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null context");
+    if (!(fraction >= 0.0 && fraction < 1.0))
+        return fail(EPSEON_ERR_INVALID, "split fraction must be in [0, 1)");
+    ctx->solve_split = fraction;
+    return EPSEON_OK;
+}
+
 int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
 {
-    if (const int rc = epseon_scan(ctx); rc != EPSEON_OK)
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
         return rc;
-    return epseon_refine(ctx, rel_tol, max_iter);
+    if (!ctx->loaded)
+        return fail(EPSEON_ERR_STATE, "epseon_solve before epseon_load_potentials");
+    if (const int rc = check_refine_args(rel_tol, max_iter); rc != EPSEON_OK)
+        return rc;
+    const size_t np = n_potentials(ctx);
+    const size_t k = split_point(ctx, ctx->solve_split);
+    if (k == 0) {
+        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve0, ctx->stream));
+        if (const int rc = epseon_scan(ctx); rc != EPSEON_OK)
+            return rc;
+        if (const int rc = epseon_refine(ctx, rel_tol, max_iter); rc != EPSEON_OK)
+            return rc;
+        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve1, ctx->stream));
+        ctx->last_split_used = 0.0;
+        ctx->solve_timed = true;
+        return EPSEON_OK;
+    }
+    // Pipelined solve: the longest grids (potentials [0, k) in task order)
+    // are scanned and refined on the first stream while the rest are scanned
+    // and refined on the second, so the long refinement shots of the first
+    // group overlap the scan of the second.  Every potential still runs the
+    // same kernels on the same data: results are identical to epseon_scan +
+    // epseon_refine.
+    const int32_t mode = choose_refine_mode(ctx);
+    const int32_t lanes = segmented_lanes(ctx);
+    ctx->last_refine_mode = mode;
+    ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve0, ctx->stream));
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+    EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+    EPSEON_HIP_TRY(run_scan(ctx, ctx->stream, 0, k));
+    EPSEON_HIP_TRY(run_scan(ctx, ctx->stream2, k, np));
+    EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream, mode, lanes, rel_tol, max_iter, 0, k));
+    EPSEON_HIP_TRY(run_refine(ctx, ctx->stream2, ctx->stream2, mode, lanes, rel_tol, max_iter, k, np));
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev_join, ctx->stream2));
+    EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve1, ctx->stream));
+    ctx->scanned = ctx->refined = true;
+    ctx->scan_timed = ctx->refine_timed = false;
+    ctx->normalized = false;
+    ctx->normalize_timed = ctx->vjp_timed = false;
+    ctx->host_levels_valid = false;
+    ctx->solve_timed = true;
+    ctx->last_split_used = ctx->cum_scan_work[k] / std::max(ctx->cum_scan_work.back(), 1.0);
+    return EPSEON_OK;
+}
+
+int epseon_get_tuning(epseon_ctx* ctx, epseon_tuning* t)
+{
+    if (ctx == nullptr || t == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null argument");
+    t->refine_mode = ctx->refine_mode;
+    t->levels_per_lane = ctx->levels_per_lane;
+    t->scan_waves_per_block = ctx->waves_per_block;
+    t->segmented_lanes = ctx->segmented_lanes;
+    t->solve_split = ctx->solve_split;
+    return EPSEON_OK;
+}
+
+int epseon_set_tuning(epseon_ctx* ctx, const epseon_tuning* t)
+{
+    if (ctx == nullptr || t == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null argument");
+    if (t->refine_mode != EPSEON_REFINE_AUTO && t->refine_mode != EPSEON_REFINE_LANE &&
+        t->refine_mode != EPSEON_REFINE_SEGMENTED)
+        return fail(EPSEON_ERR_INVALID, "unknown refine mode");
+    if (t->levels_per_lane != 1 && t->levels_per_lane != 2)
+        return fail(EPSEON_ERR_INVALID, "levels_per_lane must be 1 or 2");
+    if (t->scan_waves_per_block < 1 || t->scan_waves_per_block > 4)
+        return fail(EPSEON_ERR_INVALID, "scan_waves_per_block must be in [1, 4]");
+    if (t->segmented_lanes != 0 && t->segmented_lanes != 64 && t->segmented_lanes != 128 &&
+        t->segmented_lanes != 256)
+        return fail(EPSEON_ERR_INVALID, "segmented_lanes must be 0, 64, 128 or 256");
+    if (!(t->solve_split >= 0.0 && t->solve_split < 1.0))
+        return fail(EPSEON_ERR_INVALID, "solve_split must be in [0, 1)");
+    ctx->refine_mode = t->refine_mode;
+    ctx->levels_per_lane = t->levels_per_lane;
+    ctx->waves_per_block = t->scan_waves_per_block;
+    ctx->segmented_lanes = t->segmented_lanes;
+    ctx->solve_split = t->solve_split;
+    return EPSEON_OK;
+}
+
+namespace {
+
+// Device time of one solve under the context's current tuning (best of
+// `reps` runs after one untimed run).
+int time_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter, int reps, double* ms_out)
+{
+    double best = 1e300;
+    for (int r = 0; r <= reps; ++r) {
+        if (const int rc = epseon_solve(ctx, rel_tol, max_iter); rc != EPSEON_OK)
+            return rc;
+        EPSEON_HIP_TRY(hipEventSynchronize(ctx->ev_solve1));
+        float ms = 0.0f;
+        EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_solve0, ctx->ev_solve1));
+        if (r > 0)
+            best = std::min(best, static_cast<double>(ms));
+    }
+    *ms_out = best;
+    return EPSEON_OK;
+}
+
+}  // namespace
+
+int epseon_autotune(epseon_ctx* ctx, double rel_tol, int32_t max_iter, int32_t flags,
+                    epseon_tuning* best_out)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (!ctx->loaded)
+        return fail(EPSEON_ERR_STATE, "epseon_autotune before epseon_load_potentials");
+    if (const int rc = check_refine_args(rel_tol, max_iter); rc != EPSEON_OK)
+        return rc;
+    epseon_tuning best;
+    (void)epseon_get_tuning(ctx, &best);
+    if (best.refine_mode == EPSEON_REFINE_AUTO)
+        best.refine_mode = choose_refine_mode(ctx);
+    double best_ms = 0.0;
+    if (const int rc = epseon_set_tuning(ctx, &best); rc != EPSEON_OK)
+        return rc;
+    if (const int rc = time_solve(ctx, rel_tol, max_iter, 2, &best_ms); rc != EPSEON_OK)
+        return rc;
+    // Coordinate search, one knob at a time, keeping a candidate only when it
+    // is at least 2% faster.
+    auto trial = [&](epseon_tuning cand) -> int {
+        if (const int rc = epseon_set_tuning(ctx, &cand); rc != EPSEON_OK)
+            return rc;
+        double ms = 0.0;
+        if (const int rc = time_solve(ctx, rel_tol, max_iter, 2, &ms); rc != EPSEON_OK)
+            return rc;
+        if (ms < 0.98 * best_ms) {
+            best = cand;
+            best_ms = ms;
+        }
+        return EPSEON_OK;
+    };
+    for (int32_t w : {2, 4}) {
+        epseon_tuning c = best;
+        c.scan_waves_per_block = w;
+        if (const int rc = trial(c); rc != EPSEON_OK)
+            return rc;
+    }
+    if (!(flags & EPSEON_TUNE_KEEP_RESULTS)) {
+        epseon_tuning c = best;
+        c.refine_mode = best.refine_mode == EPSEON_REFINE_LANE ? EPSEON_REFINE_SEGMENTED : EPSEON_REFINE_LANE;
+        if (const int rc = trial(c); rc != EPSEON_OK)
+            return rc;
+    }
+    if (best.refine_mode == EPSEON_REFINE_LANE) {
+        epseon_tuning c = best;
+        c.levels_per_lane = 3 - best.levels_per_lane;
+        if (const int rc = trial(c); rc != EPSEON_OK)
+            return rc;
+    } else if (!(flags & EPSEON_TUNE_KEEP_RESULTS)) {
+        for (int32_t lanes : {64, 128, 256}) {
+            epseon_tuning c = best;
+            c.segmented_lanes = lanes;
+            if (const int rc = trial(c); rc != EPSEON_OK)
+                return rc;
+        }
+    }
+    for (double split : {0.15, 0.3, 0.5}) {
+        epseon_tuning c = best;
+        c.solve_split = split;
+        if (const int rc = trial(c); rc != EPSEON_OK)
+            return rc;
+    }
+    if (const int rc = epseon_set_tuning(ctx, &best); rc != EPSEON_OK)
+        return rc;
+    if (best_out)
+        *best_out = best;
+    return EPSEON_OK;
 }
 
 int epseon_ctx_synchronize(epseon_ctx* ctx)
@@ -886,9 +1216,17 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
         stats->scan_point_updates = ctx->scan_point_updates;
         stats->scan_launches = ctx->n_scan_tasks > 0 ? 1 : 0;
     }
-    if (ctx->refine_timed) {
-        EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
-        stats->refine_ms = ms;
+    if (ctx->solve_timed) {
+        EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_solve0, ctx->ev_solve1));
+        stats->solve_ms = ms;
+        stats->solve_split = ctx->last_split_used;
+        stats->scan_point_updates = ctx->scan_point_updates;
+    }
+    if (ctx->refined) {
+        if (ctx->refine_timed) {
+            EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+            stats->refine_ms = ms;
+        }
         if (!ctx->host_levels_valid) {
             if (const int rc = fetch_level_meta(ctx); rc != EPSEON_OK)
                 return rc;

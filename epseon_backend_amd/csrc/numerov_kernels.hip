@@ -48,15 +48,29 @@ __device__ __forceinline__ StepConsts step_consts()
     return k;
 }
 
+struct SignCounter {
+    int32_t nodes;
+    uint32_t last;
+    __device__ __forceinline__ void sample(const double w)
+    {
+        const uint32_t sb = sign_bit(w);
+        nodes += static_cast<int32_t>(sb ^ last);
+        last = sb;
+    }
+};
+
 // Scan form: division-free homogeneous Numerov on u_i = beta V_i - 1,
 // f_i = e - u_i, 5 FP64 VALU ops per step.
 struct ScanForm {
+    using Energy = double;
+    using Counter = SignCounter;
     double wm, w, fm;
     __device__ __forceinline__ void init(const double e, const double u0)
     {
         wm = 0.0;
         w = 1.0;
         fm = e - u0;
+        rescales = 0;
     }
     __device__ __forceinline__ void step(const double u, const double e, const StepConsts& k)
     {
@@ -76,6 +90,7 @@ struct ScanForm {
         wm = __builtin_ldexp(wm, sh);
         rescales += big ? 1 : 0;
     }
+    __device__ __forceinline__ void sample(SignCounter& c) const { c.sample(w); }
     int32_t rescales;
 };
 
@@ -84,6 +99,8 @@ struct ScanForm {
 //   sig = fma(eps_i, eps_{i-1}, eps_i + eps_{i-1}),
 //   c = fma(10 eps_i, w_i, sig w_{i-1}), d -= c, w += d.
 struct DiffForm {
+    using Energy = double;
+    using Counter = SignCounter;
     double w, d, wm, em;
     __device__ __forceinline__ void init(const double e, const double u0)
     {
@@ -91,6 +108,7 @@ struct DiffForm {
         d = 1.0;
         wm = 0.0;
         em = e - u0;
+        rescales = 0;
     }
     __device__ __forceinline__ void step(const double u, const double e, const StepConsts& k)
     {
@@ -112,17 +130,40 @@ struct DiffForm {
         wm = __builtin_ldexp(wm, sh);
         rescales += big ? 1 : 0;
     }
+    __device__ __forceinline__ void sample(SignCounter& c) const { c.sample(w); }
     int32_t rescales;
 };
 
-struct SignCounter {
-    int32_t nodes;
-    uint32_t last;
-    __device__ __forceinline__ void sample(const double w)
+// Two refinement shots advanced together (two levels per lane): their
+// dependency chains interleave, which hides FP64 latency in wavefronts that
+// have a SIMD to themselves.  Each member is bit-identical to a lone shot.
+struct DiffPair {
+    struct Energy {
+        double a, b;
+    };
+    struct Counter {
+        SignCounter a, b;
+    };
+    DiffForm x, y;
+    __device__ __forceinline__ void init(const Energy e, const double u0)
     {
-        const uint32_t sb = sign_bit(w);
-        nodes += static_cast<int32_t>(sb ^ last);
-        last = sb;
+        x.init(e.a, u0);
+        y.init(e.b, u0);
+    }
+    __device__ __forceinline__ void step(const double u, const Energy e, const StepConsts& k)
+    {
+        x.step(u, e.a, k);
+        y.step(u, e.b, k);
+    }
+    __device__ __forceinline__ void rescale()
+    {
+        x.rescale();
+        y.rescale();
+    }
+    __device__ __forceinline__ void sample(Counter& c) const
+    {
+        c.a.sample(x.w);
+        c.b.sample(y.w);
     }
 };
 
@@ -134,17 +175,17 @@ struct SignCounter {
 __device__ __forceinline__ void fence_schedule() { __builtin_amdgcn_sched_barrier(0); }
 
 template <int B, int Q, class Form>
-__device__ __forceinline__ void step_at(const double u, const double e, const StepConsts& k,
-                                        Form& s, SignCounter& c)
+__device__ __forceinline__ void step_at(const double u, const typename Form::Energy e,
+                                        const StepConsts& k, Form& s, typename Form::Counter& c)
 {
     s.step(u, e, k);
     if constexpr ((Q + 1) % B == 0)
-        c.sample(s.w);
+        s.sample(c);
 }
 
 template <int B, int Q0, class Form, int... Is>
-__device__ __forceinline__ void steps_1_15(const double (&buf)[16], const double e,
-                                           const StepConsts& k, Form& s, SignCounter& c,
+__device__ __forceinline__ void steps_1_15(const double (&buf)[16], const typename Form::Energy e,
+                                           const StepConsts& k, Form& s, typename Form::Counter& c,
                                            std::integer_sequence<int, Is...>)
 {
     (step_at<B, Q0 + Is + 1>(buf[Is + 1], e, k, s, c), ...);
@@ -157,16 +198,16 @@ __device__ __forceinline__ void steps_1_15(const double (&buf)[16], const double
 // land.  The rescale test runs once per super-block, after its 17th step; the
 // sign is sampled after every B-th step and at the last point.  The table is
 // padded so the prefetch past the last super-block stays in the allocation.
+// The form holds the final state, the counter the node count(s).
 template <int B, class Form>
-__device__ __forceinline__ int32_t shoot_fixed(const double* __restrict__ u, const int32_t n,
-                                               const double e, double& w_last, int32_t& rescales)
+__device__ __forceinline__ void shoot_fixed(const double* __restrict__ u, const int32_t n,
+                                            const typename Form::Energy e, Form& s,
+                                            typename Form::Counter& c)
 {
     static_assert(32 % B == 0, "stride must divide the super-block");
     const StepConsts k = step_consts();
-    Form s;
     s.init(e, u[0]);
-    s.rescales = 0;
-    SignCounter c{0, 0u};
+    c = typename Form::Counter{};
     const int32_t steps = n - 2;
     const int32_t nsb = steps >> 5;
     const double* p = u + 1;
@@ -196,28 +237,40 @@ __device__ __forceinline__ int32_t shoot_fixed(const double* __restrict__ u, con
     for (int32_t q = 0; q < rem; ++q) {
         s.step(p[q], e, k);
         if ((q + 1) % B == 0)
-            c.sample(s.w);
+            s.sample(c);
     }
     if (rem % B != 0)
-        c.sample(s.w);
-    w_last = s.w;
-    rescales = s.rescales;
-    return c.nodes;
+        s.sample(c);
 }
 
+template <class Form>
+__device__ __forceinline__ void shoot(const double* __restrict__ u, const int32_t n,
+                                      const typename Form::Energy e, const int32_t stride, Form& s,
+                                      typename Form::Counter& c)
+{
+    switch (stride) {
+    case 32: return shoot_fixed<32, Form>(u, n, e, s, c);
+    case 16: return shoot_fixed<16, Form>(u, n, e, s, c);
+    case 8: return shoot_fixed<8, Form>(u, n, e, s, c);
+    case 4: return shoot_fixed<4, Form>(u, n, e, s, c);
+    case 2: return shoot_fixed<2, Form>(u, n, e, s, c);
+    default: return shoot_fixed<1, Form>(u, n, e, s, c);
+    }
+}
+
+// Single-form shot: node count, far-end value and rescale count.
 template <class Form>
 __device__ __forceinline__ int32_t shoot(const double* __restrict__ u, const int32_t n,
                                          const double e, const int32_t stride, double& w_last,
                                          int32_t& rescales)
 {
-    switch (stride) {
-    case 32: return shoot_fixed<32, Form>(u, n, e, w_last, rescales);
-    case 16: return shoot_fixed<16, Form>(u, n, e, w_last, rescales);
-    case 8: return shoot_fixed<8, Form>(u, n, e, w_last, rescales);
-    case 4: return shoot_fixed<4, Form>(u, n, e, w_last, rescales);
-    case 2: return shoot_fixed<2, Form>(u, n, e, w_last, rescales);
-    default: return shoot_fixed<1, Form>(u, n, e, w_last, rescales);
-    }
+    Form s;
+    s.rescales = 0;
+    SignCounter c;
+    shoot<Form>(u, n, e, stride, s, c);
+    w_last = s.w;
+    rescales = s.rescales;
+    return c.nodes;
 }
 
 __device__ __forceinline__ int32_t wave_task_index()
@@ -332,8 +385,8 @@ __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const 
 
 // Refinement state machine shared by both refine kernels.  While the
 // bracket holds more than level v (or its floor has no shot) the probe is the
-// midpoint and needs a counted shot; once it holds exactly level v the probe
-// is the Illinois regula-falsi point on F = w 2^k, kept at least tol/2
+// count-interpolated point and needs a counted shot; once it holds exactly level v the probe
+// is the regula-falsi point on F = w 2^k (Anderson-Bjorck scaling), kept at least tol/2
 // inside, and the probe's count follows from the sign of F.  Stops when the
 // width is <= rel_tol max(|E_lo|, |E_hi|) + abs_tol.
 struct Refiner {
@@ -363,7 +416,12 @@ struct Refiner {
             if (!(em <= b.eh - d))
                 em = b.eh - d;
         } else {
-            em = b.el + 0.5 * width;
+            // Several levels in the bracket (or no shot at its floor): probe
+            // where level v would sit if the ch - cl levels were evenly
+            // spaced, kept within [1/8, 7/8] of the bracket.
+            double fr = (static_cast<double>(v - b.cl) + 0.5) / static_cast<double>(b.ch - b.cl);
+            fr = fmin(fmax(fr, 0.125), 0.875);
+            em = __builtin_fma(fr, width, b.el);
             side = 0;
         }
         return em;
@@ -375,22 +433,29 @@ struct Refiner {
         if (isolated)
             c = (static_cast<int32_t>(sign_bit(w)) == (v & 1)) ? v : v + 1;
         ++it;
+        // Anderson-Bjorck: when the same end is replaced twice in a row, the
+        // kept end's value is scaled by m = 1 - F_new/F_replaced (0.5 when
+        // m <= 0 or undefined).
         if (c <= v) {
+            if (isolated && side == -1) {
+                const double m = 1.0 - __builtin_ldexp(w / b.wl, k - b.kl);
+                b.wh *= (m > 0.0) ? m : 0.5;
+            }
             b.el = em;
             b.cl = c;
             b.wl = w;
             b.kl = k;
             b.lo_ok = 1;
-            if (isolated && side == -1)
-                b.wh *= 0.5;
             side = isolated ? -1 : 0;
         } else {
+            if (isolated && side == 1) {
+                const double m = 1.0 - __builtin_ldexp(w / b.wh, k - b.kh);
+                b.wl *= (m > 0.0) ? m : 0.5;
+            }
             b.eh = em;
             b.ch = c;
             b.wh = w;
             b.kh = k;
-            if (isolated && side == 1)
-                b.wl *= 0.5;
             side = isolated ? 1 : 0;
         }
     }
@@ -410,7 +475,11 @@ __device__ __forceinline__ void write_level(double* levels, int32_t* status, int
 }  // namespace
 
 // Lane-per-level refinement: lane = level v of the wavefront's potential,
-// sequential difference-form shots over the wave-uniform uv table.
+// sequential difference-form shots over the wave-uniform uv table.  With
+// K = 2 a lane refines levels first + lane and first + lane + 64 of a
+// 128-level task together (one paired shot per iteration; a member that has
+// converged rides along on its partner's probe and its result is dropped).
+template <int K>
 __global__ __launch_bounds__(256) void numerov_refine_kernel(
     const double* __restrict__ uv, const PotentialRecord* __restrict__ pots,
     const WaveTask* __restrict__ tasks, const int32_t n_tasks, const int32_t* __restrict__ nodes,
@@ -418,42 +487,75 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
     const int32_t max_iter, double* __restrict__ levels, int32_t* __restrict__ status,
     int32_t* __restrict__ iterations)
 {
+    static_assert(K == 1 || K == 2, "one or two levels per lane");
     const int32_t tid = wave_task_index();
     if (tid >= n_tasks)
         return;
     const WaveTask task = tasks[tid];
     const PotentialRecord& P = pots[task.potential];
-    const int32_t l = task.first + static_cast<int32_t>(threadIdx.x & 63);
-    if (l >= P.level_count)
-        return;
-    const int64_t out = P.level_offset + l;
-    if (P.flags != 0) {
-        write_level(levels, status, iterations, out, __builtin_nan(""), 3, 0);
-        return;
-    }
-    Refiner r{};
-    r.v = P.level_min + l;
-    r.b = find_bracket(P, r.v, nodes, far_end, rescales);
-    if (!r.b.found) {
-        write_level(levels, status, iterations, out, __builtin_nan(""), 1, 0);
-        return;
+    const int32_t lane = static_cast<int32_t>(threadIdx.x & 63);
+    Refiner r[K];
+    bool live[K];
+    int64_t out[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const int32_t l = task.first + lane + 64 * q;
+        live[q] = false;
+        out[q] = P.level_offset + l;
+        r[q] = Refiner{};
+        if (l >= P.level_count)
+            continue;
+        if (P.flags != 0) {
+            write_level(levels, status, iterations, out[q], __builtin_nan(""), 3, 0);
+            continue;
+        }
+        r[q].v = P.level_min + l;
+        r[q].b = find_bracket(P, r[q].v, nodes, far_end, rescales);
+        if (!r[q].b.found) {
+            write_level(levels, status, iterations, out[q], __builtin_nan(""), 1, 0);
+            continue;
+        }
+        live[q] = true;
     }
     const double* u = uv + P.ut_offset;
-    int32_t st = 0;
     for (;;) {
-        if (r.converged(rel_tol, P.abs_tol))
-            break;
-        if (r.it >= max_iter) {
-            st = 2;
-            break;
+        double em[K];
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            em[q] = 0.0;
+            if (!live[q])
+                continue;
+            if (r[q].converged(rel_tol, P.abs_tol)) {
+                write_level(levels, status, iterations, out[q], r[q].level(), 0, r[q].it);
+                live[q] = false;
+            } else if (r[q].it >= max_iter) {
+                write_level(levels, status, iterations, out[q], r[q].level(), 2, r[q].it);
+                live[q] = false;
+            } else {
+                em[q] = r[q].probe(rel_tol, P.abs_tol);
+                any = true;
+            }
         }
-        const double em = r.probe(rel_tol, P.abs_tol);
-        double w;
-        int32_t k;
-        const int32_t c = shoot<DiffForm>(u, P.n_points, P.beta * em, P.stride, w, k);
-        r.update(em, w, k * kRescaleLog2, c);
+        if (!any)
+            break;
+        if constexpr (K == 1) {
+            double w;
+            int32_t k;
+            const int32_t c = shoot<DiffForm>(u, P.n_points, P.beta * em[0], P.stride, w, k);
+            r[0].update(em[0], w, k * kRescaleLog2, c);
+        } else {
+            const double ea = live[0] ? em[0] : em[1];
+            const double eb = live[1] ? em[1] : em[0];
+            DiffPair s;
+            DiffPair::Counter c;
+            shoot<DiffPair>(u, P.n_points, DiffPair::Energy{P.beta * ea, P.beta * eb}, P.stride, s, c);
+            if (live[0])
+                r[0].update(em[0], s.x.w, s.x.rescales * kRescaleLog2, c.a.nodes);
+            if (live[1])
+                r[1].update(em[1], s.y.w, s.y.rescales * kRescaleLog2, c.b.nodes);
+        }
     }
-    write_level(levels, status, iterations, out, r.level(), st, r.it);
 }
 
 namespace {
@@ -782,19 +884,24 @@ hipError_t launch_scan(hipStream_t stream, const double* ut, const PotentialReco
     return hipGetLastError();
 }
 
-hipError_t launch_refine(hipStream_t stream, const double* uv, const PotentialRecord* pots,
-                         const WaveTask* tasks, int32_t n_tasks, int32_t waves_per_block,
-                         const int32_t* nodes, const double* far_end, const int32_t* rescales,
-                         double rel_tol, int32_t max_iter, double* levels, int32_t* status,
-                         int32_t* iterations)
+hipError_t launch_refine(hipStream_t stream, int32_t levels_per_lane, const double* uv,
+                         const PotentialRecord* pots, const WaveTask* tasks, int32_t n_tasks,
+                         int32_t waves_per_block, const int32_t* nodes, const double* far_end,
+                         const int32_t* rescales, double rel_tol, int32_t max_iter, double* levels,
+                         int32_t* status, int32_t* iterations)
 {
     if (n_tasks == 0)
         return hipSuccess;
     const int32_t blocks = (n_tasks + waves_per_block - 1) / waves_per_block;
-    hipLaunchKernelGGL(numerov_refine_kernel, dim3(static_cast<unsigned>(blocks)),
-                       dim3(static_cast<unsigned>(64 * waves_per_block)), 0, stream, uv, pots,
-                       tasks, n_tasks, nodes, far_end, rescales, rel_tol, max_iter, levels, status,
-                       iterations);
+    const dim3 grid(static_cast<unsigned>(blocks)), block(static_cast<unsigned>(64 * waves_per_block));
+    if (levels_per_lane == 2)
+        hipLaunchKernelGGL(numerov_refine_kernel<2>, grid, block, 0, stream, uv, pots, tasks, n_tasks,
+                           nodes, far_end, rescales, rel_tol, max_iter, levels, status, iterations);
+    else if (levels_per_lane == 1)
+        hipLaunchKernelGGL(numerov_refine_kernel<1>, grid, block, 0, stream, uv, pots, tasks, n_tasks,
+                           nodes, far_end, rescales, rel_tol, max_iter, levels, status, iterations);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -120,6 +120,8 @@ typedef struct epseon_stats {
     int32_t refine_lanes;         /* threads per level of the last segmented refine (1: lane mode) */
     double vjp_ms;                /* device time of the last epseon_level_vjp */
     double vjp_point_updates;     /* sum over converged levels of (N - 2) per adjoint pass */
+    double solve_ms;              /* device time of the last epseon_solve (both streams) */
+    double solve_split;           /* scan-work fraction of the first pipeline group (0: sequential) */
 } epseon_stats;
 
 typedef struct epseon_ctx epseon_ctx;
@@ -128,7 +130,7 @@ typedef struct epseon_ctx epseon_ctx;
 const char* epseon_last_error(void);
 
 /* sizeof of the public structs, in this order: epseon_device_info,
- * epseon_potential_desc, epseon_potential_info, epseon_stats (at most
+ * epseon_potential_desc, epseon_potential_info, epseon_stats, epseon_tuning (at most
  * `count` values are written).  Lets a foreign-function binding verify its
  * struct layouts at load time.  No counterpart in the reference (its binding
  * is compiled against the headers). */
@@ -177,8 +179,8 @@ int epseon_potential_info_get(epseon_ctx* ctx, int32_t p, epseon_potential_info*
 int epseon_scan(epseon_ctx* ctx);
 
 /* Bracket every requested level between adjacent trial energies by node count
- * and refine it (node-count bisection until isolated, then Illinois regula
- * falsi on the far-end value) until the bracket is narrower than
+ * and refine it (count-interpolated node-count cuts until isolated, then regula
+ * falsi with Anderson-Bjorck scaling on the far-end value) until the bracket is narrower than
  * rel_tol*|E| + abs_tol, abs_tol = 1e-15 (e_top - E_lo).  Refinement shots
  * use the difference form of the recurrence (on beta V, carrying
  * beta (E - V) at full relative precision), in the mode chosen with
@@ -191,8 +193,50 @@ int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter);
  * reference, whose algorithm record has no such knob. */
 int epseon_set_refine_mode(epseon_ctx* ctx, int32_t mode);
 
-/* epseon_scan followed by epseon_refine. */
+/* Scan and refine (same results as epseon_scan followed by epseon_refine).
+ * Pipelined over two streams when the split fraction is above 0: the
+ * potentials with the longest grids, holding up to that fraction of the scan
+ * work, are scanned and refined on one stream while the rest are scanned and
+ * refined on the other, so the longest refinement shots overlap scan work.
+ * scan_ms/refine_ms of epseon_get_stats then stay 0 and solve_ms holds the
+ * whole solve.  Same dispatch site as epseon_scan. */
 int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter);
+
+/* Split fraction of the pipelined epseon_solve, in [0, 1) (default 0: scan
+ * then refine).  No counterpart in the reference
+ * (its HardwareConfig, hardware_config.hpp:11-76, sizes buffers only). */
+int epseon_set_solve_split(epseon_ctx* ctx, double fraction);
+
+/* Launch configuration of a context.  Every field only changes speed, except
+ * refine_mode and segmented_lanes, which change the levels within the
+ * refinement tolerance (lane and segmented shots round differently). */
+typedef struct epseon_tuning {
+    int32_t refine_mode;          /* EPSEON_REFINE_* */
+    int32_t levels_per_lane;      /* lane mode: 1, or 2 for potentials with more than 64 levels */
+    int32_t scan_waves_per_block; /* wavefronts per scan workgroup, 1..4 */
+    int32_t segmented_lanes;      /* segmented mode: 0 (from the batch size), 64, 128 or 256 */
+    double solve_split;           /* see epseon_set_solve_split */
+} epseon_tuning;
+
+#define EPSEON_TUNE_KEEP_RESULTS 1 /* autotune only knobs that leave results bit-identical */
+
+/* Current tuning of the context. */
+int epseon_get_tuning(epseon_ctx* ctx, epseon_tuning* tuning);
+
+/* Set all knobs at once (validated). */
+int epseon_set_tuning(epseon_ctx* ctx, const epseon_tuning* tuning);
+
+/* Time candidate tunings on the loaded batch (each a full epseon_solve, best
+ * of two after a warm-up run, device time by HIP events), keep the fastest
+ * and write it to `best` (may be NULL).  Coordinate search over scan
+ * workgroup size, refinement mode, levels per lane / segmented block size and
+ * the pipeline split.  With EPSEON_TUNE_KEEP_RESULTS the refinement mode and
+ * segmented block size stay fixed.  The levels and scan results afterwards are
+ * those of the last candidate run: call epseon_solve again.  No counterpart
+ * in the reference, whose HardwareConfig (hardware_config.hpp:11-76) is set
+ * by hand. */
+int epseon_autotune(epseon_ctx* ctx, double rel_tol, int32_t max_iter, int32_t flags,
+                    epseon_tuning* best);
 
 /* Wait for all work queued on the context's stream. */
 int epseon_ctx_synchronize(epseon_ctx* ctx);

@@ -578,8 +578,10 @@ void eo_shoot_segmented(const double* uv, int64_t n, double e, int32_t stride, i
 
 /* Refine level v inside its bracket (the bracket ends carry the scan's
  * far-end values).  While the bracket holds more than level v (or its floor
- * has no shot) it is halved with counted shots; once it holds exactly level v
- * the Illinois variant of regula falsi runs on F(E) = w_{N-1} 2^k, each new
+ * has no shot) it is cut with counted shots at the count-interpolated point
+ * (where level v would sit if the levels in it were evenly spaced, kept within
+ * [1/8, 7/8] of the bracket); once it holds exactly level v
+ * regula falsi with Anderson-Bjorck scaling runs on F(E) = w_{N-1} 2^k, each new
  * point kept at least tol/2 inside the bracket, and a probe's count follows
  * from the sign of F (count parity = sign bit of w_{N-1}).  Stops when the
  * width is <= tol = rel_tol*max(|E_lo|,|E_hi|) + abs_tol; the level is the
@@ -618,7 +620,12 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
             if (!(em <= eh - dd))
                 em = eh - dd;
         } else {
-            em = el + 0.5 * width;
+            /* Several levels in the bracket (or no shot at its floor): probe
+             * where level v would sit if the ch - cl levels were evenly
+             * spaced, kept within [1/8, 7/8] of the bracket. */
+            double fr = ((double)(v - cl) + 0.5) / (double)(ch - cl);
+            fr = fmin(fmax(fr, 0.125), 0.875);
+            em = fma(fr, width, el);
             side = 0;
         }
         double w;
@@ -632,15 +639,22 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
         if (isolated)
             c = ((int32_t)eo_signbit(w) == (v & 1)) ? v : v + 1;
         ++it;
+        /* Anderson-Bjorck: when the same end is replaced twice in a row, the
+         * kept end's value is scaled by m = 1 - F_new/F_replaced (0.5 when
+         * m <= 0 or undefined). */
         if (c <= v) {
+            if (isolated && side == -1) {
+                const double m = 1.0 - ldexp(w / wl, k - kl);
+                wh *= (m > 0.0) ? m : 0.5;
+            }
             el = em; cl = c; wl = w; kl = k; lo_ok = 1;
-            if (isolated && side == -1)
-                wh *= 0.5;
             side = isolated ? -1 : 0;
         } else {
+            if (isolated && side == 1) {
+                const double m = 1.0 - ldexp(w / wh, k - kh);
+                wl *= (m > 0.0) ? m : 0.5;
+            }
             eh = em; ch = c; wh = w; kh = k;
-            if (isolated && side == 1)
-                wl *= 0.5;
             side = isolated ? 1 : 0;
         }
     }

@@ -144,3 +144,54 @@ def test_batched_full_scale(capi):
         e0, l0 = starts_e[i], starts_l[i]
         _compare(prob, nodes[e0 : e0 + s.m], far[e0 : e0 + s.m], resc[e0 : e0 + s.m],
                  lv[l0 : l0 + counts[i]], it[l0 : l0 + counts[i]], counts[i], False, 1)
+
+
+def test_result_preserving_tunings_are_bit_identical(capi):
+    """Scan workgroup size, one or two levels per lane and the two-stream
+    pipelined solve only change the schedule: every output is bit-identical."""
+    specs = batched(4096)[:600]
+    _log("case: first 600 batched potentials, lane mode, tunings varied")
+    counts = [s.bound_levels() + 2 for s in specs]
+    outs = []
+    with capi.Context(0) as ctx:
+        ctx.load([_pot(capi, s, s.table(), c) for s, c in zip(specs, counts)])
+        for wpb, lpl, split in ((1, 1, 0.0), (2, 2, 0.0), (4, 1, 0.3), (1, 2, 0.7)):
+            t = capi.Tuning(refine_mode=capi.REFINE_LANE, levels_per_lane=lpl, scan_waves_per_block=wpb,
+                            segmented_lanes=0, solve_split=split)
+            ctx.set_tuning(t)
+            ctx.solve(1e-10, 200)
+            nodes, far, resc = ctx.scan_results()
+            lv, st, it = ctx.levels()
+            stats = ctx.stats()
+            _log(f"  wpb {wpb} levels/lane {lpl} split {split}: solve {stats.solve_ms:.2f} ms")
+            assert (stats.solve_split > 0) == (split > 0)
+            outs.append((nodes, far.view(np.int64), resc, lv.view(np.int64), st, it))
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_autotune_then_parity(capi):
+    specs = batched(4096)[:40]
+    _log("case: autotune on the first 40 batched potentials, then oracle parity in the chosen mode")
+    counts = [s.bound_levels() + 2 for s in specs]
+    tables = [s.table() for s in specs]
+    with capi.Context(0) as ctx:
+        ctx.load([_pot(capi, s, t, c) for s, t, c in zip(specs, tables, counts)])
+        best = ctx.autotune(1e-10, 200)
+        _log(f"  chosen: {best.as_dict()}")
+        assert ctx.tuning().as_dict() == best.as_dict()
+        ctx.solve(1e-10, 200)
+        nodes, far, resc = ctx.scan_results()
+        lv, st, it = ctx.levels()
+        stats = ctx.stats()
+    seg = stats.refine_mode == capi.REFINE_SEGMENTED
+    eo = lo = 0
+    for i, s in enumerate(specs):
+        prob = problem_from_spec(s)
+        prob.v = tables[i]
+        if prob.usable:
+            _compare(prob, nodes[eo : eo + s.m], far[eo : eo + s.m], resc[eo : eo + s.m],
+                     lv[lo : lo + counts[i]], it[lo : lo + counts[i]], counts[i], seg, stats.refine_lanes)
+        eo += s.m
+        lo += counts[i]

@@ -14,6 +14,11 @@ potentials with ragged grids (2k-64k points) and trial counts (256-8192); it
 also times the same solve done potential by potential (one context and one
 launch sequence per potential) on a seeded sample of 64 potentials.
 
+``--workload stress`` runs 64 double-well potentials (Morse plus a Gaussian
+barrier) on 1,000,000-point grids with 262,144 trial energies: a step is the
+solve, every normalised eigenfunction written to device memory (one launch,
+about 43 GB) and the adjoint pass with the eigenfunctions recomputed.
+
 ``--workload fit`` runs the differentiation case: 256 potentials drawn as in
 the batched case (seed + 1) on 50,000-point grids, levels v = 0..40, a
 least-squares loss against levels of +-2 % perturbed parameters plus
@@ -233,6 +238,91 @@ def run_fit(capi, args) -> int:
     return 0
 
 
+def run_stress(capi, args) -> int:
+    """Double-well stress workload: solve + eigenfunction output + adjoint."""
+    import torch
+
+    from epseon_backend_amd.workloads import STRESS_SEED, double_wells
+
+    specs = double_wells(64)
+    counts = [s.bound_levels() + 2 for s in specs]
+    pots = [
+        capi.Potential(values=s.table(), r_min=s.r_min, r_max=s.r_max, reduced_mass=s.mu, e_top=s.de,
+                       n_energies=s.m, level_min=0, level_count=c)
+        for s, c in zip(specs, counts)
+    ]
+    ctx = capi.Context(0)
+    ctx.load(pots)
+    dev = torch.device("cuda:0")
+    psi_buf = torch.empty(ctx.eigenfunction_size(), dtype=torch.float64, device=dev)
+    grad_v = torch.empty(ctx.v_span, dtype=torch.float64, device=dev)
+    g = torch.ones(sum(counts), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        a0 = time.perf_counter()
+        ctx.solve(args.rel_tol, 200)
+        _, st, _ = ctx.levels()
+        a1 = time.perf_counter()
+        ctx.all_wavefunctions_device(psi_buf.data_ptr())
+        a2 = time.perf_counter()
+        ctx.level_vjp_device(g.data_ptr(), grad_v.data_ptr())
+        a3 = time.perf_counter()
+        return a1 - a0, a2 - a1, a3 - a2, int(np.sum(st == capi.LEVEL_OK))
+
+    for _ in range(args.warmup):
+        step()
+    solve_t, psi_t, vjp_t = [], [], []
+    pu = 0.0
+    levels_ok = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a, b, c, n_ok = step()
+        solve_t.append(a)
+        psi_t.append(b)
+        vjp_t.append(c)
+        st = ctx.stats()
+        # Eigenfunction output shoots every converged level once more (same
+        # count as the normalisation pass).
+        pu += (st.scan_point_updates + st.refine_point_updates + 2 * st.normalize_point_updates
+               + st.vjp_point_updates)
+        levels_ok += n_ok
+    wall = time.perf_counter() - t0
+    ctx.close()
+    out = {
+        "metric": METRIC,
+        "value": pu / wall,
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"stress: 64 double-well potentials (Morse + B exp(-((r-r_b)/w)^2), seed {STRESS_SEED}), "
+            "1000000-point grids, 262144 trial energies, all bound levels to 1e-10 rel, every eigenfunction "
+            "written to device memory, adjoint pass with eigenfunctions recomputed",
+            "potentials": len(specs),
+            "grid_points": int(sum(s.n for s in specs)),
+            "trial_energies": int(sum(s.m for s in specs)),
+        },
+        "solve_s": float(np.mean(solve_t)),
+        "eigenfunctions_s": float(np.mean(psi_t)),
+        "adjoint_s": float(np.mean(vjp_t)),
+        "grad_fwd_ratio": float(np.mean(vjp_t) / np.mean(np.add(solve_t, psi_t))),
+        "eigenfunction_bytes_per_step": int(sum(c * s.n for s, c in zip(specs, counts)) * 8),
+        "eigenlevels_per_s": levels_ok / wall,
+        "levels_per_step": levels_ok // args.steps,
+    }
+    sys.stdout.write(json.dumps(out) + "\n")
+    sys.stdout.flush()
+    return 0
+
+
 def load_traffic(workload: str) -> float | None:
     """HBM bytes per scan launch measured by rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration), if committed."""
@@ -250,7 +340,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["single", "batched", "fit"], default="single")
+    ap.add_argument("--workload", choices=["single", "batched", "fit", "stress"], default="single")
     ap.add_argument("--rel-tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="batched: keep the default launch configuration")
@@ -290,6 +380,8 @@ def main() -> int:
 
     if args.workload == "fit":
         return run_fit(capi, args)
+    if args.workload == "stress":
+        return run_stress(capi, args)
     pots, config = single_workload(capi) if args.workload == "single" else batched_workload(capi)
     ctx = capi.Context(0)
     ctx.load(pots)

@@ -169,6 +169,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "epseon_normalize": (ctypes.c_int, [ctypes.c_void_p]),
     "epseon_get_wavefunctions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]),
     "epseon_level_vjp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
+    "epseon_get_all_wavefunctions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
+    "epseon_get_eigenfunction_size": (ctypes.c_int, [ctypes.c_void_p, _c_int64_p]),
 }
 
 _LIB: ctypes.CDLL | None = None
@@ -419,6 +421,25 @@ class Context:
         """Write potential p's eigenfunctions to device memory at out_ptr
         (level_count * n_points doubles)."""
         _check(self._lib.epseon_get_wavefunctions(self._h, p, ctypes.c_void_p(out_ptr), MEM_DEVICE))
+
+    def eigenfunction_size(self) -> int:
+        n = ctypes.c_int64()
+        _check(self._lib.epseon_get_eigenfunction_size(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+    def all_wavefunctions(self) -> list[np.ndarray]:
+        """Eigenfunctions of every potential (one launch), as (level_count, n_points) arrays."""
+        out = np.empty(max(self.eigenfunction_size(), 1), dtype=np.float64)
+        _check(self._lib.epseon_get_all_wavefunctions(self._h, out.ctypes.data, MEM_HOST))
+        res, o = [], 0
+        for lc, n in zip(self.level_counts, self.point_counts):
+            res.append(out[o : o + lc * n].reshape(lc, n))
+            o += lc * n
+        return res
+
+    def all_wavefunctions_device(self, out_ptr: int) -> None:
+        """Write every eigenfunction to device memory at out_ptr (eigenfunction_size() doubles)."""
+        _check(self._lib.epseon_get_all_wavefunctions(self._h, ctypes.c_void_p(out_ptr), MEM_DEVICE))
 
     @property
     def v_span(self) -> int:

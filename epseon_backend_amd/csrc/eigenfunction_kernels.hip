@@ -277,20 +277,23 @@ __global__ __launch_bounds__(256) void normalize_kernel(
     norms[o] = LevelNorm{a, c * a, L.s.k, R.s.k};
 }
 
-// Eigenfunctions of potential p: row l of `out` (full_points values) is
-// psi of level l, zero outside the open trimmed window and for levels that
+// Eigenfunctions: for each wavefront task (potential, first level), row l of
+// that potential (full_points values at out[psi_offset - base + l * full_points])
+// is psi of level l, zero outside the open trimmed window and for levels that
 // did not converge.  One lane per level.
 __global__ __launch_bounds__(64) void eigenfunction_kernel(
     const double* __restrict__ uv, const double* __restrict__ uvr,
-    const PotentialRecord* __restrict__ pots, const int32_t p, const double* __restrict__ levels,
-    const int32_t* __restrict__ status, const LevelNorm* __restrict__ norms, double* __restrict__ out)
+    const PotentialRecord* __restrict__ pots, const WaveTask* __restrict__ tasks,
+    const double* __restrict__ levels, const int32_t* __restrict__ status,
+    const LevelNorm* __restrict__ norms, const int64_t base, double* __restrict__ out)
 {
-    const PotentialRecord& P = pots[p];
-    const int32_t l = static_cast<int32_t>(blockIdx.x * 64 + threadIdx.x);
+    const WaveTask task = tasks[blockIdx.x];
+    const PotentialRecord& P = pots[task.potential];
+    const int32_t l = task.first + static_cast<int32_t>(threadIdx.x);
     if (l >= P.level_count)
         return;
     const int64_t N = P.full_points;
-    double* row = out + static_cast<int64_t>(l) * N;
+    double* row = out + (P.psi_offset - base) + static_cast<int64_t>(l) * N;
     const int64_t first = P.first_point;
     const int32_t n = P.n_points;
     const bool ok = P.flags == 0;
@@ -425,15 +428,14 @@ hipError_t launch_normalize(hipStream_t stream, const double* uv, const double* 
 }
 
 hipError_t launch_eigenfunctions(hipStream_t stream, const double* uv, const double* uvr,
-                                 const PotentialRecord* pots, int32_t p, int32_t level_count,
+                                 const PotentialRecord* pots, const WaveTask* tasks, int32_t n_tasks,
                                  const double* levels, const int32_t* status,
-                                 const LevelNorm* norms, double* out)
+                                 const LevelNorm* norms, int64_t base, double* out)
 {
-    if (level_count == 0)
+    if (n_tasks == 0)
         return hipSuccess;
-    const unsigned blocks = static_cast<unsigned>((level_count + 63) / 64);
-    hipLaunchKernelGGL(eigenfunction_kernel, dim3(blocks), dim3(64), 0, stream, uv, uvr, pots, p,
-                       levels, status, norms, out);
+    hipLaunchKernelGGL(eigenfunction_kernel, dim3(static_cast<unsigned>(n_tasks)), dim3(64), 0, stream,
+                       uv, uvr, pots, tasks, levels, status, norms, base, out);
     return hipGetLastError();
 }
 

@@ -36,8 +36,8 @@ hipError_t launch_refine_segmented(hipStream_t, int32_t, const double*, const Po
 hipError_t launch_normalize(hipStream_t, const double*, const double*, const PotentialRecord*,
                             const WaveTask*, int32_t, const double*, const int32_t*, LevelNorm*);
 hipError_t launch_eigenfunctions(hipStream_t, const double*, const double*, const PotentialRecord*,
-                                 int32_t, int32_t, const double*, const int32_t*, const LevelNorm*,
-                                 double*);
+                                 const WaveTask*, int32_t, const double*, const int32_t*,
+                                 const LevelNorm*, int64_t, double*);
 hipError_t launch_level_vjp(hipStream_t, int32_t, const double*, const double*,
                             const PotentialRecord*, const int32_t*, int32_t, const double*,
                             const int32_t*, const LevelNorm*, const double*, double*);
@@ -227,6 +227,9 @@ struct epseon_ctx {
     double* d_stage = nullptr;
     size_t stage_elems = 0;
     int64_t user_span = 0;
+    int64_t psi_total = 0;
+    // Position of each potential in task order.
+    std::vector<int32_t> order_pos;
 
     // Host copies of the last refine's status and iteration arrays, filled
     // by epseon_get_levels so that epseon_get_stats needs no extra copy.
@@ -406,7 +409,7 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     ctx->desc.assign(desc, desc + count);
     ctx->rec.assign(static_cast<size_t>(count), PotentialRecord{});
 
-    int64_t pts = 0, en = 0, lv = 0, user_span = 0;
+    int64_t pts = 0, en = 0, lv = 0, user_span = 0, psi_total = 0;
     int32_t max_points = 0;
     double scan_pu = 0.0;
     for (int32_t p = 0; p < count; ++p) {
@@ -497,6 +500,8 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
             std::clamp<int64_t>(imin - first, 1, static_cast<int64_t>(r.n_points) - 3));
         r.full_points = static_cast<int32_t>(d.n_points);
         r.user_offset = v_offsets[p];
+        r.psi_offset = psi_total;
+        psi_total += static_cast<int64_t>(d.level_count) * d.n_points;
         user_span = std::max<int64_t>(user_span, v_offsets[p] + d.n_points);
         r.v_offset = pts;
         r.ut_offset = pts + first;
@@ -517,6 +522,7 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     ctx->max_points = max_points;
     ctx->scan_point_updates = scan_pu;
     ctx->user_span = user_span;
+    ctx->psi_total = psi_total;
 
     // Work lists: one wavefront per 64 trial energies (scan) or 64 levels
     // (refine), longest grids first so the dispatcher drains large items early.
@@ -525,6 +531,9 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
         return ctx->rec[a].n_points > ctx->rec[b].n_points;
     });
+    ctx->order_pos.assign(static_cast<size_t>(count), 0);
+    for (int32_t k = 0; k < count; ++k)
+        ctx->order_pos[static_cast<size_t>(order[static_cast<size_t>(k)])] = k;
     // Segmented refinement uses one 64-lane block per level.
     std::vector<WaveTask> scan_tasks, refine_tasks, level_tasks, lane1_tasks, lane2_tasks;
     std::vector<int32_t> vjp_lists[4];
@@ -1056,6 +1065,38 @@ int epseon_normalize(epseon_ctx* ctx)
     return EPSEON_OK;
 }
 
+namespace {
+
+// Eigenfunctions of potentials in task-order positions [k0, k1) into `psi`
+// (host or device), whose element 0 is eigenfunction value `base`.
+int write_wavefunctions(epseon_ctx* ctx, size_t k0, size_t k1, int64_t base, size_t elems,
+                        double* psi, int32_t mem)
+{
+    if (!ctx->normalized)
+        if (const int rc = epseon_normalize(ctx); rc != EPSEON_OK)
+            return rc;
+    if (elems == 0)
+        return EPSEON_OK;
+    double* dst = psi;
+    if (mem == EPSEON_MEM_HOST) {
+        if (const int rc = ensure_stage(ctx, elems); rc != EPSEON_OK)
+            return rc;
+        dst = ctx->d_stage;
+    }
+    const int32_t t0 = ctx->cum_refine_tasks[k0];
+    const int32_t t1 = ctx->cum_refine_tasks[k1];
+    EPSEON_HIP_TRY(launch_eigenfunctions(ctx->stream, ctx->d_uv, ctx->d_uvr, ctx->d_rec,
+                                         ctx->d_refine_tasks + t0, t1 - t0, ctx->d_levels,
+                                         ctx->d_status, ctx->d_norms, base, dst));
+    if (mem == EPSEON_MEM_HOST)
+        EPSEON_HIP_TRY(hipMemcpyAsync(psi, dst, elems * sizeof(double), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return EPSEON_OK;
+}
+
+}  // namespace
+
 int epseon_get_wavefunctions(epseon_ctx* ctx, int32_t potential, double* psi, int32_t mem)
 {
     if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
@@ -1068,27 +1109,24 @@ int epseon_get_wavefunctions(epseon_ctx* ctx, int32_t potential, double* psi, in
         return fail(EPSEON_ERR_INVALID, "potential index out of range");
     if (psi == nullptr)
         return fail(EPSEON_ERR_INVALID, "null output pointer");
-    if (!ctx->normalized)
-        if (const int rc = epseon_normalize(ctx); rc != EPSEON_OK)
-            return rc;
     const PotentialRecord& r = ctx->rec[potential];
     const size_t elems = static_cast<size_t>(r.level_count) * static_cast<size_t>(r.full_points);
-    if (elems == 0)
-        return EPSEON_OK;
-    double* dst = psi;
-    if (mem == EPSEON_MEM_HOST) {
-        if (const int rc = ensure_stage(ctx, elems); rc != EPSEON_OK)
-            return rc;
-        dst = ctx->d_stage;
-    }
-    EPSEON_HIP_TRY(launch_eigenfunctions(ctx->stream, ctx->d_uv, ctx->d_uvr, ctx->d_rec, potential,
-                                         r.level_count, ctx->d_levels, ctx->d_status, ctx->d_norms,
-                                         dst));
-    if (mem == EPSEON_MEM_HOST)
-        EPSEON_HIP_TRY(hipMemcpyAsync(psi, dst, elems * sizeof(double), hipMemcpyDeviceToHost,
-                                      ctx->stream));
-    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return EPSEON_OK;
+    const size_t k = static_cast<size_t>(ctx->order_pos[static_cast<size_t>(potential)]);
+    return write_wavefunctions(ctx, k, k + 1, r.psi_offset, elems, psi, mem);
+}
+
+int epseon_get_all_wavefunctions(epseon_ctx* ctx, double* psi, int32_t mem)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (const int rc = check_mem(mem); rc != EPSEON_OK)
+        return rc;
+    if (!ctx->refined)
+        return fail(EPSEON_ERR_STATE, "epseon_get_all_wavefunctions before epseon_refine");
+    if (psi == nullptr && ctx->psi_total > 0)
+        return fail(EPSEON_ERR_INVALID, "null output pointer");
+    return write_wavefunctions(ctx, 0, n_potentials(ctx), 0, static_cast<size_t>(ctx->psi_total), psi,
+                               mem);
 }
 
 int epseon_level_vjp(epseon_ctx* ctx, const double* grad_levels, double* grad_v, int32_t mem)
@@ -1198,6 +1236,14 @@ int epseon_get_sizes(epseon_ctx* ctx, int32_t* potentials, int64_t* points, int6
         *energies = ctx->total_energies;
     if (levels)
         *levels = ctx->total_levels;
+    return EPSEON_OK;
+}
+
+int epseon_get_eigenfunction_size(epseon_ctx* ctx, int64_t* elements)
+{
+    if (ctx == nullptr || elements == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null argument");
+    *elements = ctx->psi_total;
     return EPSEON_OK;
 }
 

@@ -42,8 +42,10 @@ struct alignas(16) PotentialRecord {
     int32_t match;          // matching point of the two-sided eigenfunction shots
                             // (min V, relative to first_point, in [1, n_points - 3])
     int32_t full_points;    // points of the untrimmed grid
+    int64_t psi_offset;     // first eigenfunction value (sum over earlier potentials of
+                            // level_count * full_points)
 };
-static_assert(sizeof(PotentialRecord) == 128, "record layout");
+static_assert(sizeof(PotentialRecord) == 144, "record layout");
 
 // Normalisation of one level's eigenfunction, from two inward shots that meet
 // at PotentialRecord::match: psi_i = y_i * ldexp(a, 240 (k - k_final)) with

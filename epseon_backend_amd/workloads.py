@@ -1,7 +1,8 @@
 """Synthetic workloads of the Numerov path (shared by bench.py and the tests).
 
-All potentials are Morse curves V(r) = D_e (1 - exp(-a (r - r_e)))^2 tabulated
-in FP64 with numpy on the uniform grid r_i = r_min + i h.  Units: cm^-1, A, amu.
+Potentials are Morse curves V(r) = D_e (1 - exp(-a (r - r_e)))^2, optionally
+plus a Gaussian barrier B exp(-((r - r_b)/w)^2) (double wells), tabulated in
+FP64 with numpy on the uniform grid r_i = r_min + i h.  Units: cm^-1, A, amu.
 """
 from __future__ import annotations
 
@@ -22,12 +23,22 @@ class MorseSpec:
     r_max: float
     n: int
     m: int
+    barrier: float = 0.0  # B (cm^-1); 0 = plain Morse
+    r_b: float = 0.0
+    width: float = 1.0
+
+    def grid(self) -> np.ndarray:
+        h = (self.r_max - self.r_min) / (self.n - 1)
+        return self.r_min + np.arange(self.n, dtype=np.float64) * h
 
     def table(self) -> np.ndarray:
-        h = (self.r_max - self.r_min) / (self.n - 1)
-        r = self.r_min + np.arange(self.n, dtype=np.float64) * h
+        r = self.grid()
         x = 1.0 - np.exp(-self.a * (r - self.re))
-        return self.de * (x * x)
+        v = self.de * (x * x)
+        if self.barrier != 0.0:
+            z = (r - self.r_b) / self.width
+            v = v + self.barrier * np.exp(-(z * z))
+        return v
 
     def bound_levels(self) -> int:
         """Number of analytic Morse levels below D_e (infinite domain)."""
@@ -90,4 +101,33 @@ def perturbed(specs: list[MorseSpec], seed: int, rel: float = 0.02) -> list[Mors
     return [
         MorseSpec(s.mu, s.de * f_de[i], s.a * f_a[i], s.re * f_re[i], s.r_min, s.r_max, s.n, s.m)
         for i, s in enumerate(specs)
+    ]
+
+
+STRESS_SEED = BATCH_SEED + 2
+
+
+def double_wells(count: int = 64, seed: int = STRESS_SEED, n_points: int = 1_000_000,
+                 n_energies: int = 262_144) -> list[MorseSpec]:
+    """Stress batch: Morse parameters drawn as in ``batched`` (same draw order,
+    this seed), plus a Gaussian barrier B exp(-((r - r_b)/w)^2) with
+    B ~ U[500, 3000] cm^-1, r_b = r_e + U[0.3, 0.8] A, w ~ U[0.05, 0.2] A drawn
+    afterwards from the same generator (order B, r_b - r_e, w, each a vector);
+    every grid has ``n_points`` points on [r_e - 0.6, r_e + 6/a] and
+    ``n_energies`` trial energies."""
+    rng = np.random.default_rng(seed)
+    mu = np.exp(rng.uniform(np.log(0.5), np.log(100.0), count))
+    de = rng.uniform(5e3, 5e4, count)
+    a = rng.uniform(1.0, 2.5, count)
+    re = rng.uniform(0.7, 3.0, count)
+    _ = rng.uniform(0.0, 1.0, count)  # grid-length draw of the batched order (unused: fixed N)
+    _ = rng.uniform(0.0, 1.0, count)  # trial-count draw of the batched order (unused: fixed M)
+    b = rng.uniform(500.0, 3000.0, count)
+    off = rng.uniform(0.3, 0.8, count)
+    w = rng.uniform(0.05, 0.2, count)
+    return [
+        MorseSpec(float(mu[i]), float(de[i]), float(a[i]), float(re[i]), float(re[i] - 0.6),
+                  float(re[i] + 6.0 / a[i]), n_points, n_energies, float(b[i]), float(re[i] + off[i]),
+                  float(w[i]))
+        for i in range(count)
     ]

@@ -260,6 +260,15 @@ int epseon_normalize(epseon_ctx* ctx);
  * eigenfunction output that layout has no room for. */
 int epseon_get_wavefunctions(epseon_ctx* ctx, int32_t potential, double* psi, int32_t mem);
 
+/* Eigenfunctions of every potential in one launch: potential p's
+ * level_count x n_points block starts at element sum_{q<p} level_count_q *
+ * n_points_q (layout as epseon_get_wavefunctions, potentials in load order;
+ * total from epseon_get_eigenfunction_size).  Same counterpart as above. */
+int epseon_get_all_wavefunctions(epseon_ctx* ctx, double* psi, int32_t mem);
+
+/* Number of doubles epseon_get_all_wavefunctions writes. */
+int epseon_get_eigenfunction_size(epseon_ctx* ctx, int64_t* elements);
+
 /* Adjoint of the levels with respect to the tabulated potential (Hellmann-
  * Feynman, exact for the discrete Numerov eigenproblem):
  *   grad_v[v_offsets[p] + i] = sum_l grad_levels[level_offset(p) + l] * psi_{p,l}(i)^2 * h_p

@@ -75,6 +75,10 @@ def test_eigenfunctions_match_oracle(capi):
         lo += counts[p]
     stats = ctx.stats()
     _log(f"  normalize {stats.normalize_ms:.3f} ms over {stats.normalize_point_updates:.3g} point-updates")
+    # One launch for every potential: the same bits, in load order.
+    everything = ctx.all_wavefunctions()
+    for p in range(len(specs)):
+        np.testing.assert_array_equal(everything[p], ctx.wavefunctions(p))
     ctx.close()
 
 

@@ -234,6 +234,7 @@ struct SolverOptions {
     std::vector<double> reduced_masses;
     double rel_tol = 1e-10;
     int32_t max_iterations = 200;
+    bool eigenfunctions = false;
 };
 
 template <typename FP>
@@ -300,6 +301,9 @@ struct TaskResults {
     std::vector<int32_t> potential_flags;
     double scan_ms = 0, refine_ms = 0, total_ms = 0;
     double scan_point_updates = 0, refine_point_updates = 0;
+    bool has_eigenfunctions = false;
+    std::vector<std::vector<double>> eigenfunctions;  // per potential, level_count x n_points
+    std::vector<int64_t> eigenfunction_points;
 };
 
 class TaskCancelled : public std::exception {
@@ -446,6 +450,11 @@ class Task {
         r.rescales.resize(np);
         r.strides.assign(np, 0);
         r.potential_flags.assign(np, 0);
+        r.has_eigenfunctions = s.options.eigenfunctions;
+        if (r.has_eigenfunctions) {
+            r.eigenfunctions.resize(np);
+            r.eigenfunction_points.assign(np, 0);
+        }
 
         set_status("opening device");
         epseon_ctx* raw = nullptr;
@@ -519,6 +528,23 @@ class Task {
                            "epseon_potential_info_get");
                 r.strides[p] = info.stride;
                 r.potential_flags[p] = info.flags;
+            }
+            if (s.options.eigenfunctions) {
+                poll(st);
+                set_status("normalising eigenfunctions");
+                int64_t total_psi = 0;
+                check_capi(epseon_get_eigenfunction_size(ctx.get(), &total_psi),
+                           "epseon_get_eigenfunction_size");
+                std::vector<double> psi(static_cast<size_t>(std::max<int64_t>(total_psi, 1)));
+                check_capi(epseon_get_all_wavefunctions(ctx.get(), psi.data(), EPSEON_MEM_HOST),
+                           "epseon_get_all_wavefunctions");
+                int64_t q = 0;
+                for (size_t p = g0; p < g1; ++p) {
+                    const int64_t cnt = static_cast<int64_t>(level_count) * desc[p - g0].n_points;
+                    r.eigenfunctions[p].assign(psi.begin() + q, psi.begin() + q + cnt);
+                    r.eigenfunction_points[p] = desc[p - g0].n_points;
+                    q += cnt;
+                }
             }
             epseon_stats stats{};
             check_capi(epseon_get_stats(ctx.get(), &stats), "epseon_get_stats");
@@ -620,6 +646,25 @@ class TaskHandle {
     }
     std::vector<int32_t> get_node_strides() const { return task_->results().strides; }
     std::vector<int32_t> get_potential_flags() const { return task_->results().potential_flags; }
+
+    std::vector<py::array_t<FP>> get_wavefunctions() const
+    {
+        const TaskResults& r = task_->results();
+        if (!r.has_eigenfunctions)
+            throw std::runtime_error(
+                "Eigenfunctions were not requested: call set_eigenfunction_output(True) before submit_task.");
+        std::vector<py::array_t<FP>> out;
+        out.reserve(r.eigenfunctions.size());
+        for (size_t p = 0; p < r.eigenfunctions.size(); ++p) {
+            py::array_t<FP> a({static_cast<py::ssize_t>(r.level_count),
+                               static_cast<py::ssize_t>(r.eigenfunction_points[p])});
+            FP* d = a.mutable_data();
+            for (size_t i = 0; i < r.eigenfunctions[p].size(); ++i)
+                d[i] = static_cast<FP>(r.eigenfunctions[p][i]);
+            out.push_back(std::move(a));
+        }
+        return out;
+    }
 
     py::dict get_timings() const
     {
@@ -798,6 +843,12 @@ class TaskConfigurator {
         return *this;
     }
 
+    TaskConfigurator& set_eigenfunction_output(bool enabled)
+    {
+        state_->options.eigenfunctions = enabled;
+        return *this;
+    }
+
     bool is_configured() const
     {
         return state_->hardware.has_value() && state_->potentials.has_value() &&
@@ -940,6 +991,9 @@ void bind_configurator(py::module_& m, const char* name)
              py::return_value_policy::reference)
         .def("set_tolerance", &C::set_tolerance, py::arg("relative"), py::arg("max_iterations") = 200,
              "Relative level tolerance and refinement iteration cap.",
+             py::return_value_policy::reference)
+        .def("set_eigenfunction_output", &C::set_eigenfunction_output, py::arg("enabled") = true,
+             "Also compute the normalised eigenfunction of every level (sum psi^2 h = 1).",
              py::return_value_policy::reference);
 }
 
@@ -962,7 +1016,10 @@ void bind_handle(py::module_& m, const char* name, const char* doc)
         .def("get_rescale_counts", &H::get_rescale_counts, "Scan rescale counts.")
         .def("get_node_strides", &H::get_node_strides, "Node-sampling stride per potential.")
         .def("get_potential_flags", &H::get_potential_flags, "Per-potential flags (1 = unresolved).")
-        .def("get_timings", &H::get_timings, "Device times and point-update counts.");
+        .def("get_timings", &H::get_timings, "Device times and point-update counts.")
+        .def("get_wavefunctions", &H::get_wavefunctions,
+             "Normalised eigenfunctions, one (levels, points) array per potential; zero rows for absent "
+             "levels.");
 }
 
 }  // namespace epseon::gpu

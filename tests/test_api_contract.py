@@ -311,3 +311,31 @@ def test_worker_error_is_raised_on_wait() -> None:
         handle.wait()
     assert handle.is_done()
     assert handle.get_status_message().startswith("failed")
+
+
+@pytest.mark.gpu
+def test_eigenfunction_output_through_handle() -> None:
+    """Additive: set_eigenfunction_output returns the builder and the handle
+    then carries normalised eigenfunctions (sum psi^2 h = 1), one
+    (levels, points) array per potential; without it the getter raises."""
+    m = _gpu()
+    interface = m.EpseonComputeContext.create().get_device_interface(0)
+    c = _configure(m, interface.get_task_configurator("float64"))
+    c.set_vibwa_algorithm(mass_atom_0=87.62, mass_atom_1=87.62, integration_step=0.1,
+                          min_distance_to_asymptote=0.1, min_level=0, max_level=3)
+    assert c.set_eigenfunction_output(True) is c
+    handle = interface.submit_task(c)
+    handle.wait()
+    psi = handle.get_wavefunctions()
+    assert len(psi) == 2 and psi[0].shape == (4, 16500)
+    h = 10.0 / 16499
+    np.testing.assert_allclose((psi[0] ** 2).sum(axis=1) * h, 1.0, rtol=1e-12)
+    # Level v has v nodes inside the well.
+    for v in range(4):
+        row = psi[0][v]
+        inner = row[np.abs(row) > 1e-6 * np.abs(row).max()]
+        assert np.sum(np.diff(np.sign(inner)) != 0) == v
+    plain = interface.submit_task(_configure(m, interface.get_task_configurator("float64")))
+    plain.wait()
+    with pytest.raises(RuntimeError, match="set_eigenfunction_output"):
+        plain.get_wavefunctions()

@@ -124,6 +124,8 @@ class Tuning(ctypes.Structure):
         ("levels_per_lane", ctypes.c_int32),
         ("scan_waves_per_block", ctypes.c_int32),
         ("segmented_lanes", ctypes.c_int32),
+        ("xcd_scan", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("solve_split", ctypes.c_double),
     ]
 

@@ -26,7 +26,7 @@ namespace epseon::hip {
 hipError_t launch_shift_potential(hipStream_t, const double*, const PotentialRecord*, int32_t,
                                   int32_t, double*, double*, double*);
 hipError_t launch_scan(hipStream_t, const double*, const PotentialRecord*, const WaveTask*, int32_t,
-                       int32_t, int32_t*, double*, int32_t*);
+                       const int32_t*, int32_t, int32_t, int32_t*, double*, int32_t*);
 hipError_t launch_refine(hipStream_t, int32_t, const double*, const PotentialRecord*,
                          const WaveTask*, int32_t, int32_t, const int32_t*, const double*,
                          const int32_t*, double, int32_t, double*, int32_t*, int32_t*);
@@ -192,6 +192,12 @@ struct epseon_ctx {
     double* d_uvr = nullptr;
     PotentialRecord* d_rec = nullptr;
     WaveTask* d_scan_tasks = nullptr;
+    // Scan tasks split into one bin per XCD (whole potentials, balanced by
+    // scan work), concatenated; d_xcd_bins holds the 9 bin offsets.
+    WaveTask* d_xcd_scan_tasks = nullptr;
+    int32_t* d_xcd_bins = nullptr;
+    int32_t xcd_max_bin = 0;
+    bool xcd_scan = true;
     WaveTask* d_refine_tasks = nullptr;
     WaveTask* d_level_tasks = nullptr;
     // Lane-mode refinement tasks: 64 levels (one per lane) for potentials
@@ -569,6 +575,52 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
             (r.flags == EPSEON_POT_OK ? static_cast<double>(r.n_points) * r.n_energies : 0.0));
     }
     ctx->n_scan_tasks = static_cast<int32_t>(scan_tasks.size());
+    // Per-XCD bins.  Each potential's scan tasks are cut into chunks of at
+    // most ceil(tasks / 32) (so a single large potential still spreads over
+    // every XCD, while typical batch members stay whole); chunks go by
+    // decreasing work to the least-loaded bin, and each bin keeps the
+    // longest-grid-first order.
+    struct Chunk {
+        int32_t k, t0, t1;  // position in task order, task range
+        double work;
+    };
+    std::vector<Chunk> chunks;
+    {
+        const int32_t cmax = std::max<int32_t>(1, (ctx->n_scan_tasks + 31) / 32);
+        for (size_t k = 0; k + 1 < ctx->cum_scan_tasks.size(); ++k) {
+            const int32_t a0 = ctx->cum_scan_tasks[k], a1 = ctx->cum_scan_tasks[k + 1];
+            const double per_task = a1 > a0 ? (ctx->cum_scan_work[k + 1] - ctx->cum_scan_work[k]) / (a1 - a0) : 0.0;
+            for (int32_t t = a0; t < a1; t += cmax) {
+                const int32_t e = std::min(a1, t + cmax);
+                chunks.push_back({static_cast<int32_t>(k), t, e, per_task * (e - t)});
+            }
+        }
+    }
+    std::vector<size_t> by_work(chunks.size());
+    std::iota(by_work.begin(), by_work.end(), size_t(0));
+    std::stable_sort(by_work.begin(), by_work.end(),
+                     [&](size_t a, size_t b) { return chunks[a].work > chunks[b].work; });
+    std::vector<int32_t> chunk_bin(chunks.size(), 0);
+    {
+        double load[8] = {};
+        for (size_t c : by_work) {
+            const int x = static_cast<int>(std::min_element(load, load + 8) - load);
+            chunk_bin[c] = x;
+            load[x] += chunks[c].work;
+        }
+    }
+    std::vector<WaveTask> xcd_tasks;
+    std::vector<int32_t> xcd_bins(9, 0);
+    ctx->xcd_max_bin = 0;
+    for (int x = 0; x < 8; ++x) {
+        xcd_bins[x] = static_cast<int32_t>(xcd_tasks.size());
+        for (size_t c = 0; c < chunks.size(); ++c)  // chunks are in task order
+            if (chunk_bin[c] == x)
+                for (int32_t t = chunks[c].t0; t < chunks[c].t1; ++t)
+                    xcd_tasks.push_back(scan_tasks[static_cast<size_t>(t)]);
+        ctx->xcd_max_bin = std::max(ctx->xcd_max_bin, static_cast<int32_t>(xcd_tasks.size()) - xcd_bins[x]);
+    }
+    xcd_bins[8] = static_cast<int32_t>(xcd_tasks.size());
     ctx->n_refine_tasks = static_cast<int32_t>(refine_tasks.size());
     ctx->n_level_tasks = static_cast<int32_t>(level_tasks.size());
 
@@ -583,6 +635,8 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_uvr));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(count, 1), &ctx->d_rec));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(scan_tasks.size(), 1), &ctx->d_scan_tasks));
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(xcd_tasks.size(), 1), &ctx->d_xcd_scan_tasks));
+    EPSEON_HIP_TRY(arena_array(a, size_t(9), &ctx->d_xcd_bins));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(refine_tasks.size(), 1), &ctx->d_refine_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(level_tasks.size(), 1), &ctx->d_level_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(lane1_tasks.size(), 1), &ctx->d_lane1_tasks));
@@ -616,6 +670,12 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
         EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_scan_tasks, scan_tasks.data(),
                                       scan_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice,
                                       ctx->stream));
+    if (!xcd_tasks.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_scan_tasks, xcd_tasks.data(),
+                                      xcd_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice,
+                                      ctx->stream));
+    EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_bins, xcd_bins.data(), 9 * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, ctx->stream));
     if (!refine_tasks.empty())
         EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_refine_tasks, refine_tasks.data(),
                                       refine_tasks.size() * sizeof(WaveTask),
@@ -686,12 +746,17 @@ int32_t segmented_lanes(const epseon_ctx* ctx)
     return lanes;
 }
 
-// Scan of the potentials [k0, k1) in task order on `stream`.
+// Scan of the potentials [k0, k1) in task order on `stream`; the whole batch
+// goes through the per-XCD bins.
 hipError_t run_scan(epseon_ctx* ctx, hipStream_t stream, size_t k0, size_t k1)
 {
     const int32_t t0 = ctx->cum_scan_tasks[k0];
     const int32_t t1 = ctx->cum_scan_tasks[k1];
-    return launch_scan(stream, ctx->d_ut, ctx->d_rec, ctx->d_scan_tasks + t0, t1 - t0,
+    if (ctx->xcd_scan && k0 == 0 && k1 == ctx->cum_scan_tasks.size() - 1)
+        return launch_scan(stream, ctx->d_ut, ctx->d_rec, ctx->d_xcd_scan_tasks, t1 - t0, ctx->d_xcd_bins,
+                           ctx->xcd_max_bin, ctx->waves_per_block, ctx->d_nodes, ctx->d_far,
+                           ctx->d_resc);
+    return launch_scan(stream, ctx->d_ut, ctx->d_rec, ctx->d_scan_tasks + t0, t1 - t0, nullptr, 0,
                        ctx->waves_per_block, ctx->d_nodes, ctx->d_far, ctx->d_resc);
 }
 
@@ -883,6 +948,8 @@ int epseon_get_tuning(epseon_ctx* ctx, epseon_tuning* t)
     t->levels_per_lane = ctx->levels_per_lane;
     t->scan_waves_per_block = ctx->waves_per_block;
     t->segmented_lanes = ctx->segmented_lanes;
+    t->xcd_scan = ctx->xcd_scan ? 1 : 0;
+    t->reserved = 0;
     t->solve_split = ctx->solve_split;
     return EPSEON_OK;
 }
@@ -903,10 +970,13 @@ int epseon_set_tuning(epseon_ctx* ctx, const epseon_tuning* t)
         return fail(EPSEON_ERR_INVALID, "segmented_lanes must be 0, 64, 128 or 256");
     if (!(t->solve_split >= 0.0 && t->solve_split < 1.0))
         return fail(EPSEON_ERR_INVALID, "solve_split must be in [0, 1)");
+    if (t->xcd_scan != 0 && t->xcd_scan != 1)
+        return fail(EPSEON_ERR_INVALID, "xcd_scan must be 0 or 1");
     ctx->refine_mode = t->refine_mode;
     ctx->levels_per_lane = t->levels_per_lane;
     ctx->waves_per_block = t->scan_waves_per_block;
     ctx->segmented_lanes = t->segmented_lanes;
+    ctx->xcd_scan = t->xcd_scan != 0;
     ctx->solve_split = t->solve_split;
     return EPSEON_OK;
 }
@@ -968,6 +1038,12 @@ int epseon_autotune(epseon_ctx* ctx, double rel_tol, int32_t max_iter, int32_t f
     for (int32_t w : {2, 4}) {
         epseon_tuning c = best;
         c.scan_waves_per_block = w;
+        if (const int rc = trial(c); rc != EPSEON_OK)
+            return rc;
+    }
+    {
+        epseon_tuning c = best;
+        c.xcd_scan = 1 - best.xcd_scan;
         if (const int rc = trial(c); rc != EPSEON_OK)
             return rc;
     }

@@ -280,6 +280,31 @@ __device__ __forceinline__ int32_t wave_task_index()
     return static_cast<int32_t>(blockIdx.x) * waves_per_block + wave;
 }
 
+// Task of this wavefront.  With `bins` (9 offsets into `tasks`), the list is
+// split into one bin per XCD: workgroups are dispatched round-robin over the
+// 8 XCDs, so workgroup b runs on XCD b % 8 and takes the (b / 8)-th group of
+// that XCD's bin; all wavefronts of a potential then read its table through
+// one L2.  Without bins the list is walked in order.
+__device__ __forceinline__ bool fetch_task(const WaveTask* __restrict__ tasks, const int32_t n_tasks,
+                                           const int32_t* __restrict__ bins, WaveTask& out)
+{
+    if (bins == nullptr) {
+        const int32_t t = wave_task_index();
+        if (t >= n_tasks)
+            return false;
+        out = tasks[t];
+        return true;
+    }
+    const int32_t waves_per_block = static_cast<int32_t>(blockDim.x >> 6);
+    const int32_t wave = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(threadIdx.x >> 6));
+    const int32_t x = static_cast<int32_t>(blockIdx.x & 7u);
+    const int32_t t = bins[x] + static_cast<int32_t>(blockIdx.x >> 3) * waves_per_block + wave;
+    if (t >= bins[x + 1])
+        return false;
+    out = tasks[t];
+    return true;
+}
+
 }  // namespace
 
 // u_i = fma(beta_p, V_i, -1) (scan table), uv_i = beta_p V_i (refinement
@@ -310,14 +335,14 @@ __global__ __launch_bounds__(256) void numerov_scan_kernel(const double* __restr
                                                            const PotentialRecord* __restrict__ pots,
                                                            const WaveTask* __restrict__ tasks,
                                                            const int32_t n_tasks,
+                                                           const int32_t* __restrict__ bins,
                                                            int32_t* __restrict__ nodes,
                                                            double* __restrict__ far_end,
                                                            int32_t* __restrict__ rescales)
 {
-    const int32_t tid = wave_task_index();
-    if (tid >= n_tasks)
+    WaveTask task;
+    if (!fetch_task(tasks, n_tasks, bins, task))
         return;
-    const WaveTask task = tasks[tid];
     const PotentialRecord& P = pots[task.potential];
     const int32_t j = task.first + static_cast<int32_t>(threadIdx.x & 63);
     if (j >= P.n_energies)
@@ -871,16 +896,19 @@ hipError_t launch_shift_potential(hipStream_t stream, const double* v, const Pot
     return hipGetLastError();
 }
 
+// bins == nullptr: tasks[0, n_tasks) in order.  Otherwise `bins` (device, 9
+// offsets) splits the list per XCD and `max_bin` is the largest bin's size.
 hipError_t launch_scan(hipStream_t stream, const double* ut, const PotentialRecord* pots,
-                       const WaveTask* tasks, int32_t n_tasks, int32_t waves_per_block,
-                       int32_t* nodes, double* far_end, int32_t* rescales)
+                       const WaveTask* tasks, int32_t n_tasks, const int32_t* bins, int32_t max_bin,
+                       int32_t waves_per_block, int32_t* nodes, double* far_end, int32_t* rescales)
 {
     if (n_tasks == 0)
         return hipSuccess;
-    const int32_t blocks = (n_tasks + waves_per_block - 1) / waves_per_block;
+    const int32_t blocks = bins == nullptr ? (n_tasks + waves_per_block - 1) / waves_per_block
+                                           : 8 * ((max_bin + waves_per_block - 1) / waves_per_block);
     hipLaunchKernelGGL(numerov_scan_kernel, dim3(static_cast<unsigned>(blocks)),
                        dim3(static_cast<unsigned>(64 * waves_per_block)), 0, stream, ut, pots,
-                       tasks, n_tasks, nodes, far_end, rescales);
+                       tasks, n_tasks, bins, nodes, far_end, rescales);
     return hipGetLastError();
 }
 

@@ -215,6 +215,8 @@ typedef struct epseon_tuning {
     int32_t levels_per_lane;      /* lane mode: 1, or 2 for potentials with more than 64 levels */
     int32_t scan_waves_per_block; /* wavefronts per scan workgroup, 1..4 */
     int32_t segmented_lanes;      /* segmented mode: 0 (from the batch size), 64, 128 or 256 */
+    int32_t xcd_scan;             /* 1: whole-batch scans bin potentials per XCD (default), 0: in order */
+    int32_t reserved;             /* 0 */
     double solve_split;           /* see epseon_set_solve_split */
 } epseon_tuning;
 
@@ -229,7 +231,7 @@ int epseon_set_tuning(epseon_ctx* ctx, const epseon_tuning* tuning);
 /* Time candidate tunings on the loaded batch (each a full epseon_solve, best
  * of two after a warm-up run, device time by HIP events), keep the fastest
  * and write it to `best` (may be NULL).  Coordinate search over scan
- * workgroup size, refinement mode, levels per lane / segmented block size and
+ * workgroup size and XCD binning, refinement mode, levels per lane / segmented block size and
  * the pipeline split.  With EPSEON_TUNE_KEEP_RESULTS the refinement mode and
  * segmented block size stay fixed.  The levels and scan results afterwards are
  * those of the last candidate run: call epseon_solve again.  No counterpart

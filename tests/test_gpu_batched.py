@@ -147,23 +147,25 @@ def test_batched_full_scale(capi):
 
 
 def test_result_preserving_tunings_are_bit_identical(capi):
-    """Scan workgroup size, one or two levels per lane and the two-stream
-    pipelined solve only change the schedule: every output is bit-identical."""
+    """Scan workgroup size, XCD binning, one or two levels per lane and the
+    two-stream pipelined solve only change the schedule: every output is
+    bit-identical."""
     specs = batched(4096)[:600]
     _log("case: first 600 batched potentials, lane mode, tunings varied")
     counts = [s.bound_levels() + 2 for s in specs]
     outs = []
     with capi.Context(0) as ctx:
         ctx.load([_pot(capi, s, s.table(), c) for s, c in zip(specs, counts)])
-        for wpb, lpl, split in ((1, 1, 0.0), (2, 2, 0.0), (4, 1, 0.3), (1, 2, 0.7)):
+        for wpb, lpl, split, xcd in ((1, 1, 0.0, 1), (2, 2, 0.0, 0), (4, 1, 0.3, 1), (1, 2, 0.7, 0),
+                                     (2, 1, 0.0, 1)):
             t = capi.Tuning(refine_mode=capi.REFINE_LANE, levels_per_lane=lpl, scan_waves_per_block=wpb,
-                            segmented_lanes=0, solve_split=split)
+                            segmented_lanes=0, xcd_scan=xcd, reserved=0, solve_split=split)
             ctx.set_tuning(t)
             ctx.solve(1e-10, 200)
             nodes, far, resc = ctx.scan_results()
             lv, st, it = ctx.levels()
             stats = ctx.stats()
-            _log(f"  wpb {wpb} levels/lane {lpl} split {split}: solve {stats.solve_ms:.2f} ms")
+            _log(f"  wpb {wpb} levels/lane {lpl} split {split} xcd {xcd}: solve {stats.solve_ms:.2f} ms")
             assert (stats.solve_split > 0) == (split > 0)
             outs.append((nodes, far.view(np.int64), resc, lv.view(np.int64), st, it))
     for other in outs[1:]:

@@ -71,21 +71,31 @@ class LevelSolver:
     def close(self) -> None:
         self.ctx.close()
 
-    def _load_and_solve(self, v_host: np.ndarray, specs: Sequence[LevelSpec]) -> None:
-        pots = [
-            capi.Potential(values=v_host[i], r_min=s.r_min, r_max=s.r_max, reduced_mass=s.reduced_mass,
+    def _load_and_solve(self, v: torch.Tensor, specs: Sequence[LevelSpec]) -> None:
+        """Upload the rows of v: straight from device memory when v lives on
+        this solver's GPU, through the host otherwise."""
+        rows, n = v.shape
+        meta = [
+            capi.Potential(values=range(n), r_min=s.r_min, r_max=s.r_max, reduced_mass=s.reduced_mass,
                            e_top=s.e_top, n_energies=s.n_energies, level_min=s.level_min,
                            level_count=s.level_count)
-            for i, s in enumerate(specs)
+            for s in specs
         ]
-        self.ctx.load(pots)
+        if v.is_cuda and v.device.index == self.device:
+            vd = v.detach().to(torch.float64).contiguous()
+            torch.cuda.current_stream(vd.device).synchronize()
+            self.ctx.load_device(meta, vd.data_ptr(), [i * n for i in range(rows)])
+        else:
+            v_host = v.detach().to("cpu", torch.float64).contiguous().numpy()
+            for i, m in enumerate(meta):
+                m.values = v_host[i]
+            self.ctx.load(meta)
         self.ctx.solve(self.rel_tol, self.max_iter)
 
     def solve(self, v: torch.Tensor, specs: Sequence[LevelSpec]) -> tuple[torch.Tensor, int]:
         """Levels of the rows of v (P, N) as a (P, L) float64 tensor on v's
         device, NaN where a level did not converge; plus the generation tag."""
-        v_host = v.detach().to("cpu", torch.float64).contiguous().numpy()
-        self._load_and_solve(v_host, specs)
+        self._load_and_solve(v, specs)
         lv, st, _ = self.ctx.levels()
         lv = np.where(st == capi.LEVEL_OK, lv, np.nan)
         self.generation += 1
@@ -99,7 +109,7 @@ class LevelSolver:
             grad_levels: torch.Tensor) -> torch.Tensor:
         """sum_l grad_levels[l] dE_l/dV as a tensor shaped like v."""
         if generation != self.generation:
-            self._load_and_solve(v.detach().to("cpu", torch.float64).contiguous().numpy(), specs)
+            self._load_and_solve(v, specs)
             self.generation += 1
         g = torch.nan_to_num(grad_levels.detach().to(torch.float64), nan=0.0, posinf=0.0, neginf=0.0)
         if v.is_cuda:

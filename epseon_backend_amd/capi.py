@@ -151,6 +151,10 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(PotentialDesc), _c_double_p, _c_int64_p],
     ),
+    "epseon_load_potentials_device": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(PotentialDesc), ctypes.c_void_p, _c_int64_p],
+    ),
     "epseon_potential_info_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(PotentialInfo)]),
     "epseon_scan": (ctypes.c_int, [ctypes.c_void_p]),
     "epseon_refine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
@@ -331,6 +335,32 @@ class Context:
         self.energy_counts = [int(p.n_energies) for p in potentials]
         self.point_counts = [len(p.values) for p in potentials]
         self.v_offsets = [int(o) for o in offsets[:n]]
+
+    def load_device(self, potentials: Sequence[Potential], v_ptr: int, offsets: Sequence[int]) -> None:
+        """Upload a batch whose V tables are already in device memory at
+        v_ptr (doubles, potential p at offsets[p]; epseon_load_potentials_device).
+        ``Potential.values`` may be any sized object: only its length is used."""
+        n = len(potentials)
+        descs = (PotentialDesc * max(n, 1))()
+        offs = np.zeros(max(n, 1), dtype=np.int64)
+        for i, p in enumerate(potentials):
+            offs[i] = int(offsets[i])
+            d = descs[i]
+            d.n_points = len(p.values)
+            d.n_energies = int(p.n_energies)
+            d.r_min = float(p.r_min)
+            d.r_max = float(p.r_max)
+            d.reduced_mass = float(p.reduced_mass)
+            d.e_top = float(p.e_top)
+            d.level_min = int(p.level_min)
+            d.level_count = int(p.level_count)
+        _check(self._lib.epseon_load_potentials_device(self._h, n, descs, ctypes.c_void_p(v_ptr),
+                                                       offs.ctypes.data_as(_c_int64_p)))
+        self.n_potentials = n
+        self.level_counts = [int(p.level_count) for p in potentials]
+        self.energy_counts = [int(p.n_energies) for p in potentials]
+        self.point_counts = [len(p.values) for p in potentials]
+        self.v_offsets = [int(o) for o in offs[:n]]
 
     def potential_info(self, p: int) -> PotentialInfo:
         info = PotentialInfo()

@@ -25,6 +25,8 @@ namespace epseon::hip {
 
 hipError_t launch_shift_potential(hipStream_t, const double*, const PotentialRecord*, int32_t,
                                   int32_t, double*, double*, double*);
+hipError_t launch_potential_stats(hipStream_t, const double*, const int64_t*, const int64_t*,
+                                  const double*, int32_t, PotentialStats*);
 hipError_t launch_scan(hipStream_t, const double*, const PotentialRecord*, const WaveTask*, int32_t,
                        const int32_t*, int32_t, int32_t, int32_t*, double*, int32_t*);
 hipError_t launch_refine(hipStream_t, int32_t, const double*, const PotentialRecord*,
@@ -176,6 +178,10 @@ struct epseon_ctx {
     hipStream_t stream = nullptr;
     // Second stream of the pipelined solve (default priority, like the first).
     hipStream_t stream2 = nullptr;
+    // Third and fourth streams: the adjoint's four block-width classes run
+    // concurrently (each class is a few long, latency-bound blocks).
+    hipStream_t stream3 = nullptr, stream4 = nullptr;
+    hipEvent_t ev_join_aux[3] = {};
     hipEvent_t ev[8] = {};
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_solve0 = nullptr, ev_solve1 = nullptr;
     Arena arena;
@@ -356,6 +362,12 @@ int epseon_ctx_create(int32_t device, uint64_t block_bytes, epseon_ctx** out)
         e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&ctx->stream4, hipStreamNonBlocking);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i)
+        e = hipEventCreateWithFlags(&ctx->ev_join_aux[i], hipEventDisableTiming);
     for (int i = 0; i < 8 && e == hipSuccess; ++i)
         e = hipEventCreate(&ctx->ev[i]);
     if (e == hipSuccess)
@@ -382,29 +394,37 @@ int epseon_ctx_destroy(epseon_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream != nullptr)
         (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->stream2 != nullptr)
-        (void)hipStreamSynchronize(ctx->stream2);
+    for (hipStream_t st : {ctx->stream2, ctx->stream3, ctx->stream4})
+        if (st != nullptr)
+            (void)hipStreamSynchronize(st);
     ctx->arena.release();
     for (hipEvent_t& ev : ctx->ev)
         if (ev != nullptr)
             (void)hipEventDestroy(ev);
-    for (hipEvent_t ev : {ctx->ev_fork, ctx->ev_join, ctx->ev_solve0, ctx->ev_solve1})
+    for (hipEvent_t ev : {ctx->ev_fork, ctx->ev_join, ctx->ev_solve0, ctx->ev_solve1,
+                          ctx->ev_join_aux[0], ctx->ev_join_aux[1], ctx->ev_join_aux[2]})
         if (ev != nullptr)
             (void)hipEventDestroy(ev);
     if (ctx->stream != nullptr)
         (void)hipStreamDestroy(ctx->stream);
-    if (ctx->stream2 != nullptr)
-        (void)hipStreamDestroy(ctx->stream2);
+    for (hipStream_t st : {ctx->stream2, ctx->stream3, ctx->stream4})
+        if (st != nullptr)
+            (void)hipStreamDestroy(st);
     delete ctx;
     return EPSEON_OK;
 }
 
-int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc,
-                           const double* v, const int64_t* v_offsets)
+namespace {
+
+// Shared body of epseon_load_potentials (v_host) and
+// epseon_load_potentials_device (v_dev).
+int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc, const double* v_host,
+               const double* v_dev, const int64_t* v_offsets)
 {
     if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
         return rc;
-    if (count < 0 || (count > 0 && (desc == nullptr || v == nullptr || v_offsets == nullptr)))
+    if (count < 0 ||
+        (count > 0 && (desc == nullptr || (v_host == nullptr && v_dev == nullptr) || v_offsets == nullptr)))
         return fail(EPSEON_ERR_INVALID, "invalid potential batch");
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->loaded = ctx->scanned = ctx->refined = ctx->normalized = false;
@@ -415,12 +435,14 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     ctx->desc.assign(desc, desc + count);
     ctx->rec.assign(static_cast<size_t>(count), PotentialRecord{});
 
-    int64_t pts = 0, en = 0, lv = 0, user_span = 0, psi_total = 0;
-    int32_t max_points = 0;
-    double scan_pu = 0.0;
+    ctx->arena.reset();
+    Arena& a = ctx->arena;
+    char msg[256];
+    // Descriptor checks (no V access).
+    std::vector<double> betas(static_cast<size_t>(std::max(count, 1)));
+    int64_t user_span = 0;
     for (int32_t p = 0; p < count; ++p) {
         const epseon_potential_desc& d = desc[p];
-        char msg[256];
         if (d.n_points < 4 || d.n_points > (int64_t(1) << 30)) {
             std::snprintf(msg, sizeof msg, "potential %d: point count %lld outside [4, 2^30]", p,
                           static_cast<long long>(d.n_points));
@@ -447,19 +469,82 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
             std::snprintf(msg, sizeof msg, "potential %d: negative V offset", p);
             return fail(EPSEON_ERR_INVALID, msg);
         }
-        const double* vp = v + v_offsets[p];
-        int64_t imin = 0;
-        for (int64_t i = 0; i < d.n_points; ++i) {
-            const double x = vp[i];
-            if (!std::isfinite(x)) {
-                std::snprintf(msg, sizeof msg, "potential %d: V[%lld] is not finite", p,
-                              static_cast<long long>(i));
-                return fail(EPSEON_ERR_INVALID, msg);
+        const double h = (d.r_max - d.r_min) / static_cast<double>(d.n_points - 1);
+        betas[static_cast<size_t>(p)] = (h * h * d.reduced_mass) / (12.0 * kHbar2Over2Amu);
+        user_span = std::max<int64_t>(user_span, v_offsets[p] + d.n_points);
+    }
+
+    // V statistics: min V, its first index, the wall-trimmed Dirichlet points
+    // (the last point left of the minimum and the first point right of it
+    // where V >= V_min + 1/(2 beta); inside, 1 + beta (E - V_i) > 1/2 for
+    // every E >= V_min) and the first non-finite value.
+    std::vector<PotentialStats> vstats(static_cast<size_t>(std::max(count, 1)));
+    if (v_host != nullptr) {
+        for (int32_t p = 0; p < count; ++p) {
+            const int64_t n = desc[p].n_points;
+            const double* vp = v_host + v_offsets[p];
+            PotentialStats& st = vstats[static_cast<size_t>(p)];
+            st = PotentialStats{0.0, 0, 0, n - 1, n};
+            for (int64_t i = 0; i < n; ++i) {
+                if (!std::isfinite(vp[i])) {
+                    st.bad = i;
+                    break;
+                }
+                if (vp[i] < vp[st.imin])
+                    st.imin = i;
             }
-            if (x < vp[imin])
-                imin = i;
+            if (st.bad < n)
+                continue;
+            st.vmin = vp[st.imin];
+            const double wall = st.vmin + 0.5 / betas[static_cast<size_t>(p)];
+            for (int64_t i = st.imin; i >= 0; --i)
+                if (vp[i] >= wall) {
+                    st.first = i;
+                    break;
+                }
+            for (int64_t i = st.imin; i < n; ++i)
+                if (vp[i] >= wall) {
+                    st.last = i;
+                    break;
+                }
         }
-        const double vmin = vp[imin];
+    } else if (count > 0) {
+        std::vector<int64_t> counts(static_cast<size_t>(count));
+        for (int32_t p = 0; p < count; ++p)
+            counts[static_cast<size_t>(p)] = desc[p].n_points;
+        int64_t* d_off = nullptr;
+        int64_t* d_cnt = nullptr;
+        double* d_beta = nullptr;
+        PotentialStats* d_st = nullptr;
+        EPSEON_HIP_TRY(arena_array(a, static_cast<size_t>(count), &d_off));
+        EPSEON_HIP_TRY(arena_array(a, static_cast<size_t>(count), &d_cnt));
+        EPSEON_HIP_TRY(arena_array(a, static_cast<size_t>(count), &d_beta));
+        EPSEON_HIP_TRY(arena_array(a, static_cast<size_t>(count), &d_st));
+        EPSEON_HIP_TRY(hipMemcpyAsync(d_off, v_offsets, count * sizeof(int64_t), hipMemcpyHostToDevice,
+                                      ctx->stream));
+        EPSEON_HIP_TRY(hipMemcpyAsync(d_cnt, counts.data(), count * sizeof(int64_t),
+                                      hipMemcpyHostToDevice, ctx->stream));
+        EPSEON_HIP_TRY(hipMemcpyAsync(d_beta, betas.data(), count * sizeof(double),
+                                      hipMemcpyHostToDevice, ctx->stream));
+        EPSEON_HIP_TRY(launch_potential_stats(ctx->stream, v_dev, d_off, d_cnt, d_beta, count, d_st));
+        EPSEON_HIP_TRY(hipMemcpyAsync(vstats.data(), d_st, count * sizeof(PotentialStats),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+        EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+
+    int64_t pts = 0, en = 0, lv = 0, psi_total = 0;
+    int32_t max_points = 0;
+    double scan_pu = 0.0;
+    for (int32_t p = 0; p < count; ++p) {
+        const epseon_potential_desc& d = desc[p];
+        const PotentialStats& st = vstats[static_cast<size_t>(p)];
+        if (st.bad < d.n_points) {
+            std::snprintf(msg, sizeof msg, "potential %d: V[%lld] is not finite", p,
+                          static_cast<long long>(st.bad));
+            return fail(EPSEON_ERR_INVALID, msg);
+        }
+        const int64_t imin = st.imin;
+        const double vmin = st.vmin;
         if (!(std::isfinite(d.e_top) && d.e_top > vmin)) {
             std::snprintf(msg, sizeof msg,
                           "potential %d: energy window top %.17g is not above min V %.17g", p,
@@ -468,22 +553,8 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
         }
         PotentialRecord& r = ctx->rec[p];
         r.h = (d.r_max - d.r_min) / static_cast<double>(d.n_points - 1);
-        r.beta = (r.h * r.h * d.reduced_mass) / (12.0 * kHbar2Over2Amu);
-        // Wall trimming: the Dirichlet points move in to the last point left
-        // of the minimum and the first point right of it where V >= V_min +
-        // 1/(2 beta).  Inside, 1 + beta (E - V_i) > 1/2 for every E >= V_min.
-        const double wall = vmin + 0.5 / r.beta;
-        int64_t first = 0, last = d.n_points - 1;
-        for (int64_t i = imin; i >= 0; --i)
-            if (vp[i] >= wall) {
-                first = i;
-                break;
-            }
-        for (int64_t i = imin; i < d.n_points; ++i)
-            if (vp[i] >= wall) {
-                last = i;
-                break;
-            }
+        r.beta = betas[static_cast<size_t>(p)];
+        const int64_t first = st.first, last = st.last;
         r.first_point = first;
         r.e_lo = vmin;
         r.e_hi = d.e_top;
@@ -508,7 +579,6 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
         r.user_offset = v_offsets[p];
         r.psi_offset = psi_total;
         psi_total += static_cast<int64_t>(d.level_count) * d.n_points;
-        user_span = std::max<int64_t>(user_span, v_offsets[p] + d.n_points);
         r.v_offset = pts;
         r.ut_offset = pts + first;
         r.energy_offset = en;
@@ -624,12 +694,16 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     ctx->n_refine_tasks = static_cast<int32_t>(refine_tasks.size());
     ctx->n_level_tasks = static_cast<int32_t>(level_tasks.size());
 
-    ctx->arena.reset();
-    Arena& a = ctx->arena;
     const size_t np = static_cast<size_t>(std::max<int64_t>(pts, 1));
     const size_t ne = static_cast<size_t>(std::max<int64_t>(en, 1));
     const size_t nl = static_cast<size_t>(std::max<int64_t>(lv, 1));
-    EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_v));
+    // V in the caller's layout: one upload of the whole span from the host,
+    // or the caller's device buffer itself (read only during this call).
+    const double* v_src = v_dev;
+    if (v_host != nullptr) {
+        EPSEON_HIP_TRY(arena_array(a, static_cast<size_t>(std::max<int64_t>(user_span, 1)), &ctx->d_v));
+        v_src = ctx->d_v;
+    }
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_ut));
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_uv));
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_uvr));
@@ -658,9 +732,8 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
                                           hipMemcpyHostToDevice, ctx->stream));
     }
 
-    for (int32_t p = 0; p < count; ++p)
-        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_v + ctx->rec[p].v_offset, v + v_offsets[p],
-                                      static_cast<size_t>(desc[p].n_points) * sizeof(double),
+    if (v_host != nullptr && user_span > 0)
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_v, v_host, static_cast<size_t>(user_span) * sizeof(double),
                                       hipMemcpyHostToDevice, ctx->stream));
     if (count > 0)
         EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_rec, ctx->rec.data(),
@@ -695,13 +768,27 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_nodes, 0, ne * sizeof(int32_t), ctx->stream));
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_far, 0, ne * sizeof(double), ctx->stream));
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_resc, 0, ne * sizeof(int32_t), ctx->stream));
-    EPSEON_HIP_TRY(launch_shift_potential(ctx->stream, ctx->d_v, ctx->d_rec, count, max_points,
+    EPSEON_HIP_TRY(launch_shift_potential(ctx->stream, v_src, ctx->d_rec, count, max_points,
                                           ctx->d_ut, ctx->d_uv, ctx->d_uvr));
     // The host buffers are pageable and owned by the caller: finish the
     // copies before returning.
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->loaded = true;
     return EPSEON_OK;
+}
+
+}  // namespace
+
+int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc,
+                           const double* v, const int64_t* v_offsets)
+{
+    return load_batch(ctx, count, desc, v, nullptr, v_offsets);
+}
+
+int epseon_load_potentials_device(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc,
+                                  const double* v_device, const int64_t* v_offsets)
+{
+    return load_batch(ctx, count, desc, nullptr, v_device, v_offsets);
 }
 
 int epseon_potential_info_get(epseon_ctx* ctx, int32_t p, epseon_potential_info* info)
@@ -1231,10 +1318,19 @@ int epseon_level_vjp(epseon_ctx* ctx, const double* grad_levels, double* grad_v,
         out = ctx->d_stage;
     }
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
+    // The four block-width classes touch disjoint potentials: one stream each.
+    const hipStream_t streams[4] = {ctx->stream, ctx->stream2, ctx->stream3, ctx->stream4};
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+    for (int w = 1; w < 4; ++w)
+        EPSEON_HIP_TRY(hipStreamWaitEvent(streams[w], ctx->ev_fork, 0));
     for (int w = 0; w < 4; ++w)
-        EPSEON_HIP_TRY(launch_level_vjp(ctx->stream, w + 1, ctx->d_uv, ctx->d_uvr, ctx->d_rec,
+        EPSEON_HIP_TRY(launch_level_vjp(streams[w], w + 1, ctx->d_uv, ctx->d_uvr, ctx->d_rec,
                                         ctx->d_vjp_lists[w], ctx->n_vjp[w], ctx->d_levels,
                                         ctx->d_status, ctx->d_norms, g, out));
+    for (int w = 1; w < 4; ++w) {
+        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_join_aux[w - 1], streams[w]));
+        EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join_aux[w - 1], 0));
+    }
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[7], ctx->stream));
     ctx->vjp_timed = true;
     if (mem == EPSEON_MEM_HOST) {

@@ -307,9 +307,86 @@ __device__ __forceinline__ bool fetch_task(const WaveTask* __restrict__ tasks, c
 
 }  // namespace
 
+// Statistics of each V table in device memory (one block per potential),
+// with the host loader's semantics: first index of the minimum, first
+// non-finite index, and the wall-trimmed Dirichlet points.
+__global__ __launch_bounds__(256) void potential_stats_kernel(const double* __restrict__ v,
+                                                              const int64_t* __restrict__ offsets,
+                                                              const int64_t* __restrict__ counts,
+                                                              const double* __restrict__ betas,
+                                                              PotentialStats* __restrict__ out)
+{
+    __shared__ double s_v[256];
+    __shared__ int64_t s_i[256], s_b[256];
+    const int32_t p = static_cast<int32_t>(blockIdx.x);
+    const int32_t t = static_cast<int32_t>(threadIdx.x);
+    const double* base = v + offsets[p];
+    const int64_t n = counts[p];
+    double best = __builtin_inf();
+    int64_t bi = n, bad = n;
+    for (int64_t i = t; i < n; i += 256) {
+        const double x = base[i];
+        if (!__builtin_isfinite(x)) {
+            bad = i < bad ? i : bad;
+        } else if (x < best || (x == best && i < bi)) {
+            best = x;
+            bi = i;
+        }
+    }
+    s_v[t] = best;
+    s_i[t] = bi;
+    s_b[t] = bad;
+    __syncthreads();
+    for (int32_t d = 128; d > 0; d >>= 1) {
+        if (t < d) {
+            const double ov = s_v[t + d];
+            const int64_t oi = s_i[t + d];
+            if (ov < s_v[t] || (ov == s_v[t] && oi < s_i[t])) {
+                s_v[t] = ov;
+                s_i[t] = oi;
+            }
+            s_b[t] = s_b[t + d] < s_b[t] ? s_b[t + d] : s_b[t];
+        }
+        __syncthreads();
+    }
+    const double vmin = s_v[0];
+    const int64_t imin = s_i[0];
+    const int64_t first_bad = s_b[0];
+    __syncthreads();
+    // Trimming needs a finite table with a minimum.
+    int64_t first = 0, last = n - 1;
+    if (first_bad == n && imin < n) {
+        const double wall = vmin + 0.5 / betas[p];
+        int64_t lo = -1, hi = n;
+        for (int64_t i = t; i <= imin; i += 256)
+            if (base[i] >= wall && i > lo)
+                lo = i;
+        for (int64_t i = imin + t; i < n; i += 256)
+            if (base[i] >= wall && i < hi)
+                hi = i;
+        s_i[t] = lo;
+        s_b[t] = hi;
+        __syncthreads();
+        for (int32_t d = 128; d > 0; d >>= 1) {
+            if (t < d) {
+                s_i[t] = s_i[t + d] > s_i[t] ? s_i[t + d] : s_i[t];
+                s_b[t] = s_b[t + d] < s_b[t] ? s_b[t + d] : s_b[t];
+            }
+            __syncthreads();
+        }
+        if (s_i[0] >= 0)
+            first = s_i[0];
+        if (s_b[0] < n)
+            last = s_b[0];
+    }
+    if (t == 0)
+        out[p] = PotentialStats{vmin, imin, first, last, first_bad};
+}
+
 // u_i = fma(beta_p, V_i, -1) (scan table), uv_i = beta_p V_i (refinement
 // table) and the reversed refinement table uvr_i = uv_{n-1-i} (inward shots
-// from the right wall) for every point of the trimmed grid of every potential.
+// from the right wall) for every point of the trimmed grid of every potential,
+// read from V in the caller's layout.
 __global__ __launch_bounds__(256) void shift_potential_kernel(const double* __restrict__ v,
                                                               const PotentialRecord* __restrict__ pots,
                                                               double* __restrict__ ut,
@@ -319,9 +396,10 @@ __global__ __launch_bounds__(256) void shift_potential_kernel(const double* __re
     const PotentialRecord& P = pots[blockIdx.y];
     const double beta = P.beta;
     const int32_t n = P.n_points;
+    const double* src = v + P.user_offset + P.first_point;
     for (int32_t i = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x); i < n;
          i += static_cast<int32_t>(gridDim.x * blockDim.x)) {
-        const double x = v[P.ut_offset + i];
+        const double x = src[i];
         const double b = beta * x;
         ut[P.ut_offset + i] = __builtin_fma(beta, x, -1.0);
         uv[P.ut_offset + i] = b;
@@ -883,6 +961,17 @@ __global__ __launch_bounds__(T) void numerov_refine_segmented_kernel(
 }
 
 // ---------------------------------------------------------------- launchers
+
+hipError_t launch_potential_stats(hipStream_t stream, const double* v, const int64_t* offsets,
+                                  const int64_t* counts, const double* betas, int32_t n_pots,
+                                  PotentialStats* out)
+{
+    if (n_pots == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(potential_stats_kernel, dim3(static_cast<unsigned>(n_pots)), dim3(256), 0, stream, v,
+                       offsets, counts, betas, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_shift_potential(hipStream_t stream, const double* v, const PotentialRecord* pots,
                                   int32_t n_pots, int32_t max_points, double* ut, double* uv,

@@ -62,6 +62,15 @@ struct LevelNorm {
 constexpr double kPsiRescaleAt = 0x1p240;
 constexpr int kPsiRescaleLog2 = 240;
 
+// Per-potential facts of a V table that loading needs: min V, the first index
+// where it occurs, the Dirichlet points of the trimmed grid (last point left of
+// imin / first point right of it with V >= min V + 1/(2 beta), else the grid
+// ends) and the first non-finite index (n_points if none).
+struct PotentialStats {
+    double vmin;
+    int64_t imin, first, last, bad;
+};
+
 // Work item of one wavefront: potential index and the first trial energy
 // (scan) or first level (refine) of its 64 lanes.
 struct WaveTask {

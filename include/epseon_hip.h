@@ -168,6 +168,17 @@ int epseon_ctx_destroy(epseon_ctx* ctx);
 int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc,
                            const double* v, const int64_t* v_offsets);
 
+/* Same as epseon_load_potentials with V already in device memory of the
+ * context's device (`v_device`, e.g. a framework tensor): min V, the wall
+ * trimming and the finiteness check run on the device with the host loader's
+ * exact semantics (first index of the minimum, same threshold arithmetic), so
+ * results are identical to a host upload of the same values.  `v_offsets` is
+ * a host array.  V is read only during the call.  Same counterpart as
+ * epseon_load_potentials; the reference stages every potential through a
+ * host-visible buffer (vibwa.hpp:176-187). */
+int epseon_load_potentials_device(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc,
+                                  const double* v_device, const int64_t* v_offsets);
+
 /* Derived values of potential p after epseon_load_potentials. */
 int epseon_potential_info_get(epseon_ctx* ctx, int32_t p, epseon_potential_info* info);
 

@@ -127,3 +127,48 @@ def test_ragged_batch_parity(capi, mode):
         np.testing.assert_array_equal(it[lo : lo + 30][ok], oit[ok])
         eo += p.m
         lo += 30
+
+
+def test_device_load_matches_host_load(capi):
+    """epseon_load_potentials_device (V already in device memory, statistics
+    and wall trimming on the GPU) gives bit-identical derived values and
+    results to the host upload of the same tables, and reports a non-finite
+    value with the host loader's message."""
+    import torch
+
+    from epseon_backend_amd.workloads import batched
+
+    _log("case: 60 batched potentials + a wall-trimmed potential, host vs device upload")
+    specs = batched(4096)[:60]
+    pots = [capi.Potential(values=s.table(), r_min=s.r_min, r_max=s.r_max, reduced_mass=s.mu, e_top=s.de,
+                           n_energies=s.m, level_min=0, level_count=s.bound_levels() + 2) for s in specs]
+    # The reference test's potential (D_e 5500, a 10, [0, 10] A): V(0) ~ 9e8 cm^-1, trimmed.
+    wall = morse_problem(mu=43.81, de=5500.0, a=10.0, re=0.6, r_min=0.0, r_max=10.0, n=16500, m=512)
+    pots.append(capi.Potential(values=wall.v, r_min=0.0, r_max=10.0, reduced_mass=43.81, e_top=5500.0,
+                               n_energies=512, level_min=0, level_count=4))
+    offsets = np.cumsum([0] + [len(p.values) for p in pots])[:-1]
+    flat = np.concatenate([p.values for p in pots])
+    outs = []
+    for device in (False, True):
+        with capi.Context(0) as ctx:
+            if device:
+                v = torch.from_numpy(flat).cuda()
+                ctx.load_device(pots, v.data_ptr(), offsets)
+            else:
+                ctx.load(pots)
+            infos = [ctx.potential_info(i) for i in range(len(pots))]
+            ctx.solve(1e-10, 200)
+            nodes, far, resc = ctx.scan_results()
+            lv, st, it = ctx.levels()
+            outs.append(([(i.first_point, i.point_count, i.flags, i.stride, np.float64(i.e_lo).view(np.int64))
+                          for i in infos], nodes, far.view(np.int64), resc, lv.view(np.int64), st, it))
+    assert outs[0][0] == outs[1][0]
+    assert outs[0][0][-1][0] > 0  # the wall potential was trimmed
+    for a, b in zip(outs[0][1:], outs[1][1:]):
+        np.testing.assert_array_equal(a, b)
+    bad = flat.copy()
+    bad[offsets[3] + 10] = np.nan
+    with capi.Context(0) as ctx:
+        v = torch.from_numpy(bad).cuda()
+        with pytest.raises(capi.EpseonError, match=r"potential 3: V\[10\] is not finite"):
+            ctx.load_device(pots, v.data_ptr(), offsets)

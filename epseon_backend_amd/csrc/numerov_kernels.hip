@@ -16,8 +16,8 @@
 // with u_i = beta V_i - 1 and e = beta E, so f_i = 1 + beta (E - V_i).
 // The sign of w is sampled once per block of `stride` steps (a power of two
 // the host chose so that no two nodes fall in one block); the overflow test
-// (|w| >= 2^480 -> scale the last two values by 2^-480) runs once per
-// 32-step super-block.  Compiled with -ffp-contract=off; every fused operation is an
+// (|w| >= 2^480 -> scale the last two values by 2^-480) runs once per four
+// 32-step super-blocks (growth per step < 2^2.87, so values stay below 2^888).  Compiled with -ffp-contract=off; every fused operation is an
 // explicit fma, so results are bit-identical to the C oracle.
 #include <hip/hip_runtime.h>
 
@@ -191,11 +191,38 @@ __device__ __forceinline__ void steps_1_15(const double (&buf)[16], const typena
     (step_at<B, Q0 + Is + 1>(buf[Is + 1], e, k, s, c), ...);
 }
 
+// One 32-step super-block fed from the two 16-point SGPR buffers: each
+// buffer's scalar loads issue right after the first step of the other buffer,
+// so a load has 15 steps to land.  With R the rescale test follows the 17th
+// step.
+template <int B, bool R, class Form>
+__device__ __forceinline__ void super_block(const double*& p, double (&a)[16], double (&b)[16],
+                                            const typename Form::Energy e, const StepConsts& k, Form& s,
+                                            typename Form::Counter& c)
+{
+    step_at<B, 0>(a[0], e, k, s, c);
+    fence_schedule();
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        b[q] = p[16 + q];
+    fence_schedule();
+    steps_1_15<B, 0>(a, e, k, s, c, std::make_integer_sequence<int, 15>{});
+    step_at<B, 16>(b[0], e, k, s, c);
+    if constexpr (R)
+        s.rescale();
+    fence_schedule();
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        a[q] = p[32 + q];
+    fence_schedule();
+    steps_1_15<B, 16>(b, e, k, s, c, std::make_integer_sequence<int, 15>{});
+    p += 32;
+}
+
 // One outward shot over n points at probe e with node stride B, the grid
-// table u being wave-uniform.  The grid is consumed in super-blocks of 32
-// steps fed from two 16-point SGPR buffers: each buffer's scalar loads issue
-// right after the first step of the other buffer, so a load has 15 steps to
-// land.  The rescale test runs once per super-block, after its 17th step; the
+// table u being wave-uniform.  The grid is consumed in 32-step super-blocks,
+// four per loop iteration so that the body stays one straight-line block;
+// the rescale test runs after the 17th step of every fourth super-block.  The
 // sign is sampled after every B-th step and at the last point.  The table is
 // padded so the prefetch past the last super-block stays in the allocation.
 // The form holds the final state, the counter the node count(s).
@@ -215,24 +242,14 @@ __device__ __forceinline__ void shoot_fixed(const double* __restrict__ u, const 
 #pragma unroll
     for (int q = 0; q < 16; ++q)
         a[q] = p[q];
-    for (int32_t sb = 0; sb < nsb; ++sb) {
-        step_at<B, 0>(a[0], e, k, s, c);
-        fence_schedule();
-#pragma unroll
-        for (int q = 0; q < 16; ++q)
-            b[q] = p[16 + q];
-        fence_schedule();
-        steps_1_15<B, 0>(a, e, k, s, c, std::make_integer_sequence<int, 15>{});
-        step_at<B, 16>(b[0], e, k, s, c);
-        s.rescale();
-        fence_schedule();
-#pragma unroll
-        for (int q = 0; q < 16; ++q)
-            a[q] = p[32 + q];
-        fence_schedule();
-        steps_1_15<B, 16>(b, e, k, s, c, std::make_integer_sequence<int, 15>{});
-        p += 32;
+    for (int32_t g = 0; g < (nsb >> 2); ++g) {
+        super_block<B, false>(p, a, b, e, k, s, c);
+        super_block<B, false>(p, a, b, e, k, s, c);
+        super_block<B, false>(p, a, b, e, k, s, c);
+        super_block<B, true>(p, a, b, e, k, s, c);
     }
+    for (int32_t sb = 0; sb < (nsb & 3); ++sb)
+        super_block<B, false>(p, a, b, e, k, s, c);
     const int32_t rem = steps - (nsb << 5);
     for (int32_t q = 0; q < rem; ++q) {
         s.step(p[q], e, k);

@@ -36,10 +36,12 @@
  *               of sign changes between consecutive samples, starting from
  *               w_1 > 0.
  *   rescale     the N-2 steps form super-blocks of 32; after the 17th step of
- *               each full super-block, if max(|w_i|, |w_{i-1}|) >= 2^480 both
- *               are multiplied by 2^-480 (exact) and the rescale counter is
- *               incremented.  (Growth per step is below 16, so 47 steps
- *               between tests cannot overflow.)
+ *               every fourth full super-block (step q with q % 128 == 112),
+ *               if max(|w_i|, |w_{i-1}|) >= 2^480 both are multiplied by
+ *               2^-480 (exact) and the rescale counter is incremented.
+ *               (|t| < 7 and g < 9/4 bound the growth per step by 7.31 <
+ *               2^2.87, so the at most 142 steps between tests, or after the
+ *               last one, stay below 2^888.)
  *   trial E     E_j = fma(j+1, dE, E_lo), dE = (E_hi - E_lo)/(M+1), E_lo = min V
  *   walls       the Dirichlet points are the last point left of argmin V and the
  *               first point right of it with V >= V_min + 1/(2 beta) (grid ends
@@ -193,7 +195,7 @@ int32_t eo_shoot(const double* ut, int64_t n, double e, int32_t stride,
             nodes += (int32_t)(sb ^ sp);
             sp = sb;
         }
-        if ((q & 31) == 16 && q < (steps & ~(int64_t)31) &&
+        if ((q & 127) == 112 && q < (steps & ~(int64_t)31) &&
             fmax(fabs(w), fabs(wm)) >= EO_RESCALE_AT) {
             w *= EO_RESCALE_BY;
             wm *= EO_RESCALE_BY;
@@ -352,7 +354,7 @@ int32_t eo_shoot_diff(const double* uv, int64_t n, double e, int32_t stride, dou
             nodes += (int32_t)(sb ^ sp);
             sp = sb;
         }
-        if ((q & 31) == 16 && q < (steps & ~(int64_t)31) && fmax(fabs(w), fabs(wm)) >= EO_RESCALE_AT) {
+        if ((q & 127) == 112 && q < (steps & ~(int64_t)31) && fmax(fabs(w), fabs(wm)) >= EO_RESCALE_AT) {
             w *= EO_RESCALE_BY;
             d *= EO_RESCALE_BY;
             wm *= EO_RESCALE_BY;

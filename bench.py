@@ -100,7 +100,7 @@ def cpu_baseline(target_seconds: float = 12.0) -> dict:
 
     c = {"n": I2.n, "m": I2.m}
     prob = oracle.Problem(v=I2.table(), r_min=I2.r_min, r_max=I2.r_max, mu=I2.mu, e_top=I2.de, m=64)
-    ut, b, stride = prob.ut, prob.beta, prob.stride
+    ut, b, stride = prob.pt, prob.beta, prob.stride
     threads = oracle.num_threads()
     m = 64 * threads
     t0 = time.perf_counter()

@@ -96,6 +96,8 @@ class PotentialInfo(ctypes.Structure):
         ("flags", ctypes.c_int32),
         ("first_point", ctypes.c_int64),
         ("point_count", ctypes.c_int64),
+        ("far_exponent", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

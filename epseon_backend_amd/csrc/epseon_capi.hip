@@ -576,6 +576,15 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
         r.match = static_cast<int32_t>(
             std::clamp<int64_t>(imin - first, 1, static_cast<int64_t>(r.n_points) - 3));
         r.full_points = static_cast<int32_t>(d.n_points);
+        // Scan far-end scale: w_{N-1} = 12^(n-2) v_{N-1}, 12^(n-2) =
+        // far_scale * 2^far_exp with far_scale in [1, 2).
+        {
+            const double l2 = static_cast<double>(r.n_points - 2) * std::log2(12.0);
+            const double fl = std::floor(l2);
+            r.far_exp = static_cast<int32_t>(fl);
+            r.far_scale = std::exp2(l2 - fl);
+            r.reserved = 0;
+        }
         r.user_offset = v_offsets[p];
         r.psi_offset = psi_total;
         psi_total += static_cast<int64_t>(d.level_count) * d.n_points;
@@ -808,6 +817,8 @@ int epseon_potential_info_get(epseon_ctx* ctx, int32_t p, epseon_potential_info*
     info->flags = r.flags;
     info->first_point = r.first_point;
     info->point_count = r.n_points;
+    info->far_exponent = r.far_exp;
+    info->reserved = 0;
     return EPSEON_OK;
 }
 

@@ -10,15 +10,22 @@
 //     as the SGPR operand of v_add_f64 -- no LDS traffic and no VGPRs for the
 //     potential, and every potential point is fetched once per wavefront;
 //   * no MFMA: the work is a scalar recurrence, bound by FP64 VALU issue.
-// Recurrence (division-free homogeneous Numerov, 5 FP64 VALU ops/step):
-//   f_i = e - u_i, t = fma(-10, f_i, 12), g = f_i f_{i-1},
-//   w_{i+1} = fma(t, w_i, -(g w_{i-1})),  w_0 = 0, w_1 = 1,
-// with u_i = beta V_i - 1 and e = beta E, so f_i = 1 + beta (E - V_i).
-// The sign of w is sampled once per block of `stride` steps (a power of two
-// the host chose so that no two nodes fall in one block); the overflow test
-// (|w| >= 2^480 -> scale the last two values by 2^-480) runs once per four
-// 32-step super-blocks (growth per step < 2^2.87, so values stay below 2^888).  Compiled with -ffp-contract=off; every fused operation is an
-// explicit fma, so results are bit-identical to the C oracle.
+// Scan recurrence (division-free homogeneous Numerov scaled by 12 per step,
+// 4 FP64 VALU ops/step): with f_i = 1 + beta (E - V_i) and
+// w_i = 12^(i-1) v_i, the recurrence
+//   f_{i+1} y_{i+1} = (12 - 10 f_i) y_i - f_{i-1} y_{i-1}   (w = y f prod f)
+// becomes, with phi_i = f_i / 12 = e' - p_i (p_i = (beta V_i - 1)/12,
+// e' = beta E / 12),
+//   v_{i+1} = v_i - phi_i (10 v_i + phi_{i-1} v_{i-1}),   v_0 = 0, v_1 = 1,
+// i.e. z = phi_{i-1} v_{i-1}; s = fma(10, v_i, z); v_{i+1} = fma(-phi_i, s, v_i).
+// |w| grows by less than 7.31 per step, so v only shrinks: after the 17th
+// step of every 32-step super-block, values below 2^-480 are multiplied by
+// 2^480.  The sign of v (= sign of w) is sampled once per block of `stride`
+// steps (a power of two the host chose so that no two nodes fall in one
+// block).  Refinement shots run the difference form on uv_i = beta V_i with
+// the overflow test every fourth super-block.  Compiled with
+// -ffp-contract=off; every fused operation is an explicit fma, so results are
+// bit-identical to the C oracle.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -35,16 +42,16 @@ __device__ __forceinline__ uint32_t sign_bit(double x)
     return static_cast<uint32_t>(__double_as_longlong(x) >> 63) & 1u;
 }
 
-// Opaque loop constants.  -10, 12 and 10 are not inline constants of the
-// FP64 VALU; held in VGPRs the compiler cannot rebuild them every step.
+// Opaque loop constant.  10 is not an inline constant of the FP64 VALU; held
+// in a VGPR the compiler cannot rebuild it every step.
 struct StepConsts {
-    double m10, c12, c10;
+    double c10;
 };
 
 __device__ __forceinline__ StepConsts step_consts()
 {
-    StepConsts k{-10.0, 12.0, 10.0};
-    asm volatile("" : "+v"(k.m10), "+v"(k.c12), "+v"(k.c10));
+    StepConsts k{10.0};
+    asm volatile("" : "+v"(k.c10));
     return k;
 }
 
@@ -59,11 +66,12 @@ struct SignCounter {
     }
 };
 
-// Scan form: division-free homogeneous Numerov on u_i = beta V_i - 1,
-// f_i = e - u_i, 5 FP64 VALU ops per step.
+// Scan form on p_i = (beta V_i - 1)/12 with e' = beta E / 12: the 12-scaled
+// homogeneous recurrence, 4 FP64 VALU ops per step (w holds v).
 struct ScanForm {
     using Energy = double;
     using Counter = SignCounter;
+    static constexpr int kRescaleEvery = 1;
     double wm, w, fm;
     __device__ __forceinline__ void init(const double e, const double u0)
     {
@@ -74,21 +82,21 @@ struct ScanForm {
     }
     __device__ __forceinline__ void step(const double u, const double e, const StepConsts& k)
     {
-        const double f = e - u;
-        const double t = __builtin_fma(k.m10, f, k.c12);
-        const double g = f * fm;
-        const double wn = __builtin_fma(t, w, -(g * wm));
+        const double phi = e - u;
+        const double z = fm * wm;
+        const double s = __builtin_fma(k.c10, w, z);
+        const double wn = __builtin_fma(-phi, s, w);
         wm = w;
         w = wn;
-        fm = f;
+        fm = phi;
     }
     __device__ __forceinline__ void rescale()
     {
-        const bool big = fmax(fabs(w), fabs(wm)) >= kRescaleAt;
-        const int32_t sh = big ? -kRescaleLog2 : 0;
+        const bool small = fmax(fabs(w), fabs(wm)) < kScanRescaleBelow;
+        const int32_t sh = small ? kRescaleLog2 : 0;
         w = __builtin_ldexp(w, sh);
         wm = __builtin_ldexp(wm, sh);
-        rescales += big ? 1 : 0;
+        rescales += small ? 1 : 0;
     }
     __device__ __forceinline__ void sample(SignCounter& c) const { c.sample(w); }
     int32_t rescales;
@@ -101,6 +109,7 @@ struct ScanForm {
 struct DiffForm {
     using Energy = double;
     using Counter = SignCounter;
+    static constexpr int kRescaleEvery = 4;
     double w, d, wm, em;
     __device__ __forceinline__ void init(const double e, const double u0)
     {
@@ -138,6 +147,7 @@ struct DiffForm {
 // dependency chains interleave, which hides FP64 latency in wavefronts that
 // have a SIMD to themselves.  Each member is bit-identical to a lone shot.
 struct DiffPair {
+    static constexpr int kRescaleEvery = 4;
     struct Energy {
         double a, b;
     };
@@ -222,8 +232,9 @@ __device__ __forceinline__ void super_block(const double*& p, double (&a)[16], d
 // One outward shot over n points at probe e with node stride B, the grid
 // table u being wave-uniform.  The grid is consumed in 32-step super-blocks,
 // four per loop iteration so that the body stays one straight-line block;
-// the rescale test runs after the 17th step of every fourth super-block.  The
-// sign is sampled after every B-th step and at the last point.  The table is
+// the rescale test runs after the 17th step of every super-block
+// (Form::kRescaleEvery = 1) or of every fourth one (4).  The sign is sampled
+// after every B-th step and at the last point.  The table is
 // padded so the prefetch past the last super-block stays in the allocation.
 // The form holds the final state, the counter the node count(s).
 template <int B, class Form>
@@ -242,14 +253,15 @@ __device__ __forceinline__ void shoot_fixed(const double* __restrict__ u, const 
 #pragma unroll
     for (int q = 0; q < 16; ++q)
         a[q] = p[q];
+    constexpr bool every = Form::kRescaleEvery == 1;
     for (int32_t g = 0; g < (nsb >> 2); ++g) {
-        super_block<B, false>(p, a, b, e, k, s, c);
-        super_block<B, false>(p, a, b, e, k, s, c);
-        super_block<B, false>(p, a, b, e, k, s, c);
+        super_block<B, every>(p, a, b, e, k, s, c);
+        super_block<B, every>(p, a, b, e, k, s, c);
+        super_block<B, every>(p, a, b, e, k, s, c);
         super_block<B, true>(p, a, b, e, k, s, c);
     }
     for (int32_t sb = 0; sb < (nsb & 3); ++sb)
-        super_block<B, false>(p, a, b, e, k, s, c);
+        super_block<B, every>(p, a, b, e, k, s, c);
     const int32_t rem = steps - (nsb << 5);
     for (int32_t q = 0; q < rem; ++q) {
         s.step(p[q], e, k);
@@ -400,7 +412,7 @@ __global__ __launch_bounds__(256) void potential_stats_kernel(const double* __re
         out[p] = PotentialStats{vmin, imin, first, last, first_bad};
 }
 
-// u_i = fma(beta_p, V_i, -1) (scan table), uv_i = beta_p V_i (refinement
+// p_i = fma(beta_p, V_i, -1) / 12 (scan table), uv_i = beta_p V_i (refinement
 // table) and the reversed refinement table uvr_i = uv_{n-1-i} (inward shots
 // from the right wall) for every point of the trimmed grid of every potential,
 // read from V in the caller's layout.
@@ -418,14 +430,15 @@ __global__ __launch_bounds__(256) void shift_potential_kernel(const double* __re
          i += static_cast<int32_t>(gridDim.x * blockDim.x)) {
         const double x = src[i];
         const double b = beta * x;
-        ut[P.ut_offset + i] = __builtin_fma(beta, x, -1.0);
+        ut[P.ut_offset + i] = __builtin_fma(beta, x, -1.0) / 12.0;
         uv[P.ut_offset + i] = b;
         uvr[P.ut_offset + (n - 1 - i)] = b;
     }
 }
 
 // Scan: lane = trial energy E_j = fma(j+1, dE, E_lo); outputs node count,
-// far-end value and rescale count per trial energy.
+// far-end mantissa v_{N-1} * far_scale and scale-up count r per trial energy
+// (w_{N-1} = mantissa * 2^(far_exp - 480 r)).
 __global__ __launch_bounds__(256) void numerov_scan_kernel(const double* __restrict__ ut,
                                                            const PotentialRecord* __restrict__ pots,
                                                            const WaveTask* __restrict__ tasks,
@@ -443,13 +456,13 @@ __global__ __launch_bounds__(256) void numerov_scan_kernel(const double* __restr
     if (j >= P.n_energies)
         return;
     const double E = __builtin_fma(static_cast<double>(j + 1), P.de, P.e_lo);
-    const double e = P.beta * E;
+    const double e = (P.beta * E) / 12.0;
     double w;
     int32_t k;
     const int32_t c = shoot<ScanForm>(ut + P.ut_offset, P.n_points, e, P.stride, w, k);
     const int64_t o = P.energy_offset + j;
     nodes[o] = c;
-    far_end[o] = w;
+    far_end[o] = w * P.far_scale;
     rescales[o] = k;
 }
 
@@ -488,7 +501,7 @@ __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const 
     b.eh = __builtin_fma(static_cast<double>(j + 1), P.de, P.e_lo);
     b.ch = nd[j];
     b.wh = far_end[P.energy_offset + j];
-    b.kh = rescales[P.energy_offset + j] * kRescaleLog2;
+    b.kh = P.far_exp - rescales[P.energy_offset + j] * kRescaleLog2;
     if (j == 0 || nd[j - 1] > v) {
         b.el = P.e_lo;
         b.cl = 0;
@@ -497,7 +510,7 @@ __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const 
         b.el = __builtin_fma(static_cast<double>(j), P.de, P.e_lo);
         b.cl = nd[j - 1];
         b.wl = far_end[P.energy_offset + j - 1];
-        b.kl = rescales[P.energy_offset + j - 1] * kRescaleLog2;
+        b.kl = P.far_exp - rescales[P.energy_offset + j - 1] * kRescaleLog2;
         b.lo_ok = 1;
     }
     return b;

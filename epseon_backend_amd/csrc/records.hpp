@@ -11,10 +11,13 @@
 
 namespace epseon::hip {
 
-// Rescale threshold of the recurrence: when |w| reaches 2^480 the last two
-// values are multiplied by 2^-480 (exact) and the rescale counter grows.
+// Rescale thresholds of the recurrences: the refinement shots multiply the
+// last values by 2^-480 when |w| reaches 2^480; the scan shots (whose values
+// only shrink) multiply them by 2^480 when they fall below 2^-480.  Both are
+// exact and counted.
 constexpr double kRescaleAt = 0x1p480;
 constexpr double kRescaleBy = 0x1p-480;
+constexpr double kScanRescaleBelow = 0x1p-480;
 constexpr int kRescaleLog2 = 480;
 constexpr int kMaxStride = 32;
 // hbar^2 / (2 amu A^2) in cm^-1 (CODATA 2018 h, c and atomic mass unit).
@@ -44,8 +47,12 @@ struct alignas(16) PotentialRecord {
     int32_t full_points;    // points of the untrimmed grid
     int64_t psi_offset;     // first eigenfunction value (sum over earlier potentials of
                             // level_count * full_points)
+    double far_scale;       // 2^frac((n_points - 2) log2 12): scan far-end values are
+                            // v_{N-1} * far_scale, with w_{N-1} = that * 2^(far_exp - 480 r)
+    int32_t far_exp;        // floor((n_points - 2) log2 12)
+    int32_t reserved;
 };
-static_assert(sizeof(PotentialRecord) == 144, "record layout");
+static_assert(sizeof(PotentialRecord) == 160, "record layout");
 
 // Normalisation of one level's eigenfunction, from two inward shots that meet
 // at PotentialRecord::match: psi_i = y_i * ldexp(a, 240 (k - k_final)) with

@@ -104,6 +104,8 @@ typedef struct epseon_potential_info {
     int32_t flags;         /* EPSEON_POT_* */
     int64_t first_point;   /* left Dirichlet point of the trimmed grid (index on the full grid) */
     int64_t point_count;   /* points of the trimmed grid, both Dirichlet points included */
+    int32_t far_exponent;  /* scan far ends: w_{N-1} = far_end * 2^(far_exponent - 480 rescales) */
+    int32_t reserved;
 } epseon_potential_info;
 
 /* Counters and device times of the last calls on a context. */
@@ -183,7 +185,8 @@ int epseon_load_potentials_device(epseon_ctx* ctx, int32_t count, const epseon_p
 int epseon_potential_info_get(epseon_ctx* ctx, int32_t p, epseon_potential_info* info);
 
 /* Numerov scan: shoot every trial energy of every potential, recording node
- * count, far-end value w_{N-1} and rescale count.  Stream-ordered; returns
+ * count, far-end mantissa and scale count (w_{N-1} = far_end *
+ * 2^(far_exponent - 480 rescales), far_exponent from epseon_potential_info_get).  Stream-ordered; returns
  * after the launch (epseon_ctx_synchronize waits).  This is the dispatch that
  * would sit between descriptor update and "Thread finished" in
  * VibwaAlgorithm::run (vibwa.hpp:630-636). */

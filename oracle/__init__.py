@@ -67,6 +67,8 @@ def _lib() -> ctypes.CDLL:
         "eo_beta": (d, [d, d]),
         "eo_shifted_potential": (None, [_dp, i64, d, _dp]),
         "eo_scaled_potential": (None, [_dp, i64, d, _dp]),
+        "eo_scan_table": (None, [_dp, i64, d, _dp]),
+        "eo_far_scale": (None, [i64, _dp, _ip]),
         "eo_shoot_diff": (i32, [_dp, i64, d, i32, _dp, _ip]),
         "eo_morse": (None, [d, d, d, d, d, i64, _dp]),
         "eo_trim": (i32, [_dp, i64, d, d, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
@@ -77,7 +79,7 @@ def _lib() -> ctypes.CDLL:
         "eo_shoot_full_count": (i32, [_dp, i64, d]),
         "eo_trajectory": (None, [_dp, i64, d, _dp]),
         "eo_scan": (None, [_dp, i64, d, d, d, i64, i32, _ip, _dp, _ip]),
-        "eo_bracket_level": (None, [_ip, _dp, _ip, i64, d, d, i32, ctypes.POINTER(Bracket)]),
+        "eo_bracket_level": (None, [_ip, _dp, _ip, i64, d, d, i32, i32, ctypes.POINTER(Bracket)]),
         "eo_refine_level": (
             i32,
             [_dp, i64, d, i32, i32, ctypes.POINTER(Bracket), d, d, i32, i32, i32, _dp],
@@ -162,13 +164,29 @@ def node_stride(b: float, e_hi: float, v_min: float) -> int:
     return int(_lib().eo_node_stride(b, e_hi, v_min))
 
 
-def shoot(ut: np.ndarray, e: float, stride: int) -> tuple[int, float, int]:
-    """(node count, w_{N-1}, rescales) of one shot at probe e = beta*E."""
-    ut = _f64(ut)
+def shoot(pt: np.ndarray, e: float, stride: int) -> tuple[int, float, int]:
+    """(node count, far-end mantissa, scale-up count r) of one scan shot at
+    probe e = beta*E over the scan table pt (w_{N-1} = mantissa 2^(I - 480 r))."""
+    pt = _f64(pt)
     w = ctypes.c_double()
     k = ctypes.c_int32()
-    c = _lib().eo_shoot(ut.ctypes.data_as(_dp), len(ut), e, stride, ctypes.byref(w), ctypes.byref(k))
+    c = _lib().eo_shoot(pt.ctypes.data_as(_dp), len(pt), e, stride, ctypes.byref(w), ctypes.byref(k))
     return int(c), float(w.value), int(k.value)
+
+
+def far_scale(n: int) -> tuple[float, int]:
+    """(c, I) with 12^(n-2) = c 2^I: the far-end scale of an n-point scan shot."""
+    c = ctypes.c_double()
+    i = ctypes.c_int32()
+    _lib().eo_far_scale(n, ctypes.byref(c), ctypes.byref(i))
+    return float(c.value), int(i.value)
+
+
+def scan_far_end(pt: np.ndarray, e: float, stride: int) -> float:
+    """w_{N-1} of a scan shot as a float (small problems only: may overflow)."""
+    _, w, r = shoot(pt, e, stride)
+    _, i = far_scale(len(pt))
+    return float(np.ldexp(w, i - 480 * r))
 
 
 def shoot_diff(uv: np.ndarray, e: float, stride: int) -> tuple[int, float, int]:
@@ -192,9 +210,9 @@ def trajectory(ut: np.ndarray, e: float) -> np.ndarray:
     return out
 
 
-def scan(ut: np.ndarray, b: float, e_lo: float, e_hi: float, m: int, stride: int):
-    """(nodes, far-end values, rescales) of every trial energy."""
-    ut = _f64(ut)
+def scan(pt: np.ndarray, b: float, e_lo: float, e_hi: float, m: int, stride: int):
+    """(nodes, far-end mantissas, scale-up counts) of every trial energy."""
+    ut = _f64(pt)
     nodes = np.empty(m, dtype=np.int32)
     far = np.empty(m, dtype=np.float64)
     resc = np.empty(m, dtype=np.int32)
@@ -205,7 +223,7 @@ def scan(ut: np.ndarray, b: float, e_lo: float, e_hi: float, m: int, stride: int
     return nodes, far, resc
 
 
-def bracket_level(nodes, far, resc, e_lo: float, e_hi: float, v: int) -> Bracket:
+def bracket_level(nodes, far, resc, e_lo: float, e_hi: float, v: int, far_exp: int) -> Bracket:
     nodes = np.ascontiguousarray(nodes, dtype=np.int32)
     far = _f64(far)
     resc = np.ascontiguousarray(resc, dtype=np.int32)
@@ -213,7 +231,7 @@ def bracket_level(nodes, far, resc, e_lo: float, e_hi: float, v: int) -> Bracket
     m = len(nodes)
     _lib().eo_bracket_level(
         nodes.ctypes.data_as(_ip), far.ctypes.data_as(_dp), resc.ctypes.data_as(_ip), m,
-        e_lo, trial_spacing(e_lo, e_hi, m), v, ctypes.byref(br),
+        e_lo, trial_spacing(e_lo, e_hi, m), v, far_exp, ctypes.byref(br),
     )
     return br
 
@@ -267,9 +285,25 @@ class Problem:
 
     @property
     def ut(self) -> np.ndarray:
-        """beta V - 1 on the trimmed grid (the array every shot runs over)."""
+        """beta V - 1 on the trimmed grid (the table of the division-free w-form:
+        trajectory and full-count checks)."""
         first, last, _ = self.trimmed
         return shifted_potential(self.v[first : last + 1], self.beta)
+
+    @property
+    def pt(self) -> np.ndarray:
+        """(beta V - 1)/12 on the trimmed grid (the scan shots' table)."""
+        first, last, _ = self.trimmed
+        v = _f64(self.v[first : last + 1])
+        out = np.empty_like(v)
+        _lib().eo_scan_table(v.ctypes.data_as(_dp), len(v), self.beta, out.ctypes.data_as(_dp))
+        return out
+
+    @property
+    def far_exp(self) -> int:
+        """Binary exponent of the scan far-end scale of the trimmed grid."""
+        first, last, _ = self.trimmed
+        return far_scale(last - first + 1)[1]
 
     @property
     def uv(self) -> np.ndarray:
@@ -316,7 +350,7 @@ def solve(prob: Problem, v_min: int, n_levels: int, rel_tol: float = 1e-10, max_
     """Scan + bracket + refine, as epseon_solve does: returns
     (nodes, far, resc, levels, iterations).  ``segmented`` selects the
     refinement with segmented shots over blocks of ``lanes`` threads."""
-    ut = prob.ut
+    ut = prob.pt
     uv = prob.uv
     m = prob.m
     nodes = np.empty(m, dtype=np.int32)

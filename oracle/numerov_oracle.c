@@ -22,26 +22,36 @@
  * Numerics (cm^-1, Angstrom, amu; everything FP64, no contraction, explicit fma):
  *   grid        r_i = r_min + i*h,  h = (r_max - r_min)/(N-1)
  *   beta        = (h*h*mu) / (12*K),  K = hbar^2/(2 amu A^2) in cm^-1
- *   shifted pot ut_i = fma(beta, V_i, -1)            (= beta*V_i - 1; scan)
+ *   scan table  p_i = fma(beta, V_i, -1) / 12                (scan)
  *   scaled pot  uv_i = beta*V_i                       (refinement, see below)
- *   probe       e = beta*E
- *   Numerov f   f_i = e - ut_i                       (= 1 + beta*(E - V_i))
- *   step        t = fma(-10, f_i, 12); g = f_i*f_{i-1};
- *               w_{i+1} = fma(t, w_i, -(g*w_{i-1}))   w_0 = 0, w_1 = 1
- *     (w_i = y_i * f_i * prod_{j<i} f_j, the division-free homogeneous form of
- *      f_{i+1} y_{i+1} = (12 - 10 f_i) y_i - f_{i-1} y_{i-1})
+ *   probe       e = beta*E; scan shots use e' = e / 12
+ *   scan step   phi_i = e' - p_i                      (= f_i / 12, f_i = 1 + beta*(E - V_i))
+ *               z = phi_{i-1} * v_{i-1}; s = fma(10, v_i, z);
+ *               v_{i+1} = fma(-phi_i, s, v_i)         v_0 = 0, v_1 = 1
+ *     (v_i = w_i / 12^(i-1) with w the division-free homogeneous Numerov
+ *      solution w_{i+1} = fma(t, w_i, -(g*w_{i-1})), t = 12 - 10 f_i,
+ *      g = f_i f_{i-1}, w_i = y_i * f_i * prod_{j<i} f_j, of
+ *      f_{i+1} y_{i+1} = (12 - 10 f_i) y_i - f_{i-1} y_{i-1}; 4 operations
+ *      per step instead of 5)
  *   samples     the sign of w is sampled after every B-th step (B = node
  *               stride, a power of two dividing 32) and once more at i = N-1
  *               when N-2 is not a multiple of B.  The node count is the number
  *               of sign changes between consecutive samples, starting from
  *               w_1 > 0.
- *   rescale     the N-2 steps form super-blocks of 32; after the 17th step of
- *               every fourth full super-block (step q with q % 128 == 112),
- *               if max(|w_i|, |w_{i-1}|) >= 2^480 both are multiplied by
- *               2^-480 (exact) and the rescale counter is incremented.
- *               (|t| < 7 and g < 9/4 bound the growth per step by 7.31 <
- *               2^2.87, so the at most 142 steps between tests, or after the
- *               last one, stay below 2^888.)
+ *   rescale     the N-2 steps form super-blocks of 32.  Scan: after the 17th
+ *               step of every full super-block, if max(|v_i|, |v_{i-1}|) <
+ *               2^-480 both are multiplied by 2^480 and the scale-up count r
+ *               grows (|w| grows by less than 7.31 per step, so v only
+ *               shrinks, by at most 2^-8.5 per step in the pair norm: the <= 47
+ *               unguarded steps stay above 2^-880).  Far end: the scan reports
+ *               v_{N-1} * c and r, with 12^(N-2) = c * 2^I, c = 2^frac(L),
+ *               I = floor(L), L = (N-2) log2(12), so w_{N-1} = v_{N-1} c 2^(I - 480 r).
+ *               Refinement (difference form): after the 17th step of every
+ *               fourth full super-block (step q with q % 128 == 112), if
+ *               max(|w_i|, |w_{i-1}|) >= 2^480 both are multiplied by 2^-480
+ *               (exact) and its counter grows (|t| < 7 and g < 9/4 bound the
+ *               growth per step by 7.31 < 2^2.87, so the at most 142 steps
+ *               between tests stay below 2^888).
  *   trial E     E_j = fma(j+1, dE, E_lo), dE = (E_hi - E_lo)/(M+1), E_lo = min V
  *   walls       the Dirichlet points are the last point left of argmin V and the
  *               first point right of it with V >= V_min + 1/(2 beta) (grid ends
@@ -57,6 +67,7 @@
 #include <omp.h>
 #endif
 
+#define EO_SCAN_RESCALE_BELOW 0x1p-480
 #define EO_K 16.857629191640175        /* cm^-1 amu A^2 (CODATA 2018 h, c, u) */
 #define EO_RESCALE_AT 0x1p480
 #define EO_RESCALE_BY 0x1p-480
@@ -88,6 +99,12 @@ void eo_shifted_potential(const double* v, int64_t n, double beta, double* ut)
 {
     for (int64_t i = 0; i < n; ++i)
         ut[i] = fma(beta, v[i], -1.0);
+}
+
+void eo_scan_table(const double* v, int64_t n, double beta, double* pt)
+{
+    for (int64_t i = 0; i < n; ++i)
+        pt[i] = fma(beta, v[i], -1.0) / 12.0;
 }
 
 void eo_scaled_potential(const double* v, int64_t n, double beta, double* uv)
@@ -172,21 +189,31 @@ static inline uint32_t eo_signbit(double x)
     return (uint32_t)(bits >> 63);
 }
 
-/* One outward shot at probe e = beta*E.  Returns the sampled node count;
- * stores w_{N-1} and the number of 2^-480 rescales. */
-int32_t eo_shoot(const double* ut, int64_t n, double e, int32_t stride,
+/* Far-end scale of an n-point scan shot: 12^(n-2) = *c * 2^*i. */
+void eo_far_scale(int64_t n, double* c, int32_t* i)
+{
+    const double l2 = (double)(n - 2) * log2(12.0);
+    const double fl = floor(l2);
+    *i = (int32_t)fl;
+    *c = exp2(l2 - fl);
+}
+
+/* One outward scan shot at probe e = beta*E over p_i = (beta V_i - 1)/12.
+ * Returns the sampled node count; stores the far-end mantissa v_{N-1} * c and
+ * the scale-up count r (w_{N-1} = mantissa * 2^(I - 480 r), eo_far_scale). */
+int32_t eo_shoot(const double* pt, int64_t n, double e, int32_t stride,
                  double* w_last, int32_t* rescales)
 {
-    double wm = 0.0, w = 1.0, fm = e - ut[0];
+    const double ep = e / 12.0;
+    double wm = 0.0, w = 1.0, fm = ep - pt[0];
     int32_t nodes = 0, k = 0;
     uint32_t sp = 0;
     const int64_t steps = n - 2;
     for (int64_t q = 0; q < steps; ++q) {
-        const int64_t i = q + 1;
-        const double f = e - ut[i];
-        const double t = fma(-10.0, f, 12.0);
-        const double g = f * fm;
-        const double wn = fma(t, w, -(g * wm));
+        const double f = ep - pt[q + 1];
+        const double z = fm * wm;
+        const double sg = fma(10.0, w, z);
+        const double wn = fma(-f, sg, w);
         wm = w;
         w = wn;
         fm = f;
@@ -195,10 +222,10 @@ int32_t eo_shoot(const double* ut, int64_t n, double e, int32_t stride,
             nodes += (int32_t)(sb ^ sp);
             sp = sb;
         }
-        if ((q & 127) == 112 && q < (steps & ~(int64_t)31) &&
-            fmax(fabs(w), fabs(wm)) >= EO_RESCALE_AT) {
-            w *= EO_RESCALE_BY;
-            wm *= EO_RESCALE_BY;
+        if ((q & 31) == 16 && q < (steps & ~(int64_t)31) &&
+            fmax(fabs(w), fabs(wm)) < EO_SCAN_RESCALE_BELOW) {
+            w *= EO_RESCALE_AT;
+            wm *= EO_RESCALE_AT;
             ++k;
         }
     }
@@ -206,7 +233,10 @@ int32_t eo_shoot(const double* ut, int64_t n, double e, int32_t stride,
         const uint32_t sb = eo_signbit(w);
         nodes += (int32_t)(sb ^ sp);
     }
-    *w_last = w;
+    double c;
+    int32_t i;
+    eo_far_scale(n, &c, &i);
+    *w_last = w * c;
     *rescales = k;
     return nodes;
 }
@@ -258,14 +288,14 @@ void eo_trajectory(const double* ut, int64_t n, double e, double* w_out)
 /* ------------------------------------------------------------------ scan */
 
 /* Shoot every trial energy E_j, j < m, of the window [e_lo, e_hi]. */
-void eo_scan(const double* ut, int64_t n, double beta, double e_lo, double e_hi,
+void eo_scan(const double* pt, int64_t n, double beta, double e_lo, double e_hi,
              int64_t m, int32_t stride, int32_t* nodes, double* w_last, int32_t* rescales)
 {
     const double de = eo_trial_spacing(e_lo, e_hi, m);
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t j = 0; j < m; ++j) {
         const double e = beta * eo_trial_energy(j, e_lo, de);
-        nodes[j] = eo_shoot(ut, n, e, stride, &w_last[j], &rescales[j]);
+        nodes[j] = eo_shoot(pt, n, e, stride, &w_last[j], &rescales[j]);
     }
 }
 
@@ -273,8 +303,8 @@ void eo_scan(const double* ut, int64_t n, double beta, double e_lo, double e_hi,
 
 typedef struct eo_bracket {
     double e_lo, e_hi;   /* bracket ends */
-    double w_lo, w_hi;   /* far-end values at the ends */
-    int32_t k_lo, k_hi;  /* their rescale counts */
+    double w_lo, w_hi;   /* far-end mantissas at the ends */
+    int32_t k_lo, k_hi;  /* their binary exponents (F = w 2^k) */
     int32_t c_lo, c_hi;  /* node counts at the ends */
     int32_t lo_valid;    /* 0 when e_lo is the window floor (no shot there) */
     int32_t found;       /* 0 when no trial energy has more than v nodes */
@@ -284,7 +314,7 @@ typedef struct eo_bracket {
  * trial energy before it (or the window floor, below which no level exists
  * because every eigenvalue exceeds min V). */
 void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t* rescales,
-                      int64_t m, double e_lo, double de, int32_t v, eo_bracket* out)
+                      int64_t m, double e_lo, double de, int32_t v, int32_t far_exp, eo_bracket* out)
 {
     int64_t lo = 0, hi = m;
     while (lo < hi) {
@@ -304,7 +334,7 @@ void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t*
     out->e_hi = eo_trial_energy(j, e_lo, de);
     out->c_hi = nodes[j];
     out->w_hi = w_last[j];
-    out->k_hi = rescales[j];
+    out->k_hi = far_exp - rescales[j] * 480;
     if (j == 0 || nodes[j - 1] > v) {
         out->e_lo = e_lo;
         out->c_lo = 0;
@@ -313,7 +343,7 @@ void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t*
         out->e_lo = eo_trial_energy(j - 1, e_lo, de);
         out->c_lo = nodes[j - 1];
         out->w_lo = w_last[j - 1];
-        out->k_lo = rescales[j - 1];
+        out->k_lo = far_exp - rescales[j - 1] * 480;
         out->lo_valid = 1;
     }
 }
@@ -596,7 +626,7 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
 {
     double el = br->e_lo, eh = br->e_hi;
     double wl = br->w_lo, wh = br->w_hi;
-    int32_t kl = br->k_lo * 480, kh = br->k_hi * 480; /* binary exponents */
+    int32_t kl = br->k_lo, kh = br->k_hi; /* binary exponents */
     int32_t cl = br->c_lo, ch = br->c_hi;
     int32_t lo_ok = br->lo_valid;
     int32_t side = 0;
@@ -666,17 +696,20 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
 
 /* Scan + bracket + refine of levels [v_min, v_min + n_levels) for one
  * potential.  levels[] gets NaN for levels that are not below e_hi. */
-void eo_solve(const double* ut, const double* uv, int64_t n, double beta, double e_lo, double e_hi,
+void eo_solve(const double* pt, const double* uv, int64_t n, double beta, double e_lo, double e_hi,
               int64_t m, int32_t stride, int32_t v_min, int32_t n_levels, double rel_tol,
               double abs_tol, int32_t max_iter, int32_t segmented, int32_t lanes, int32_t* nodes,
               double* w_last, int32_t* rescales, double* levels, int32_t* iterations)
 {
-    eo_scan(ut, n, beta, e_lo, e_hi, m, stride, nodes, w_last, rescales);
+    eo_scan(pt, n, beta, e_lo, e_hi, m, stride, nodes, w_last, rescales);
     const double de = eo_trial_spacing(e_lo, e_hi, m);
+    double fc;
+    int32_t far_exp;
+    eo_far_scale(n, &fc, &far_exp);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int32_t l = 0; l < n_levels; ++l) {
         eo_bracket br;
-        eo_bracket_level(nodes, w_last, rescales, m, e_lo, de, v_min + l, &br);
+        eo_bracket_level(nodes, w_last, rescales, m, e_lo, de, v_min + l, far_exp, &br);
         if (!br.found) {
             levels[l] = NAN;
             iterations[l] = 0;

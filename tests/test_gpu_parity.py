@@ -61,7 +61,7 @@ def test_scan_parity_h2(capi):
         assert info.e_lo == prob.e_lo
         ctx.scan()
         nodes, far, resc = ctx.scan_results()
-    on, of, orr = oracle.scan(prob.ut, prob.beta, prob.e_lo, prob.e_top, prob.m, prob.stride)
+    on, of, orr = oracle.scan(prob.pt, prob.beta, prob.e_lo, prob.e_top, prob.m, prob.stride)
     np.testing.assert_array_equal(nodes, on)
     np.testing.assert_array_equal(resc, orr)
     np.testing.assert_array_equal(far.view(np.int64), of.view(np.int64))

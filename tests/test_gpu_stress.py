@@ -73,7 +73,7 @@ def test_double_wells_full_size(capi):
         assert len(found) == min(counts[p], int(nd[-1]))
         # Sampled trial energies: single oracle shots, bit-exact.
         energies = oracle.trial_energies(prob.e_lo, prob.e_top, s.m)
-        ut = prob.ut
+        ut = prob.pt
         for j in rng.choice(s.m, 24, replace=False):
             c, w, k = oracle.shoot(ut, prob.beta * energies[j], prob.stride)
             assert (c, k) == (int(nd[j]), int(rs[j])), (p, j)
@@ -81,7 +81,7 @@ def test_double_wells_full_size(capi):
         # Sampled levels: the oracle refines from the GPU's scan arrays.
         uv = prob.uv
         for v in sorted(rng.choice(len(found), 4, replace=False)):
-            br = oracle.bracket_level(nd, fr, rs, prob.e_lo, prob.e_top, int(v))
+            br = oracle.bracket_level(nd, fr, rs, prob.e_lo, prob.e_top, int(v), prob.far_exp)
             ref, its = oracle.refine_level(uv, prob.beta, prob.stride, int(v), br, 1e-10, prob.abs_tol, 200,
                                            segmented=seg, lanes=stats.refine_lanes)
             assert np.float64(ref).view(np.int64) == lev[v].view(np.int64), (p, v)

@@ -129,7 +129,7 @@ def test_sturm_counts_match_dense_eigensolve(small_morse):
     for energy in rng.uniform(prob.e_lo, prob.e_top, 200):
         expect = int(np.sum(eig < energy))
         e = prob.beta * energy
-        assert oracle.shoot(prob.ut, e, prob.stride)[0] == expect
+        assert oracle.shoot(prob.pt, e, prob.stride)[0] == expect
         assert oracle.shoot_diff(prob.uv, e, prob.stride)[0] == expect
         assert oracle.shoot_segmented(prob.uv, e, prob.stride, lanes=16)[2] == expect
 
@@ -227,7 +227,7 @@ def test_sampled_count_equals_full_count(seed):
         strides.add(p.stride)
         for energy in rng.uniform(p.e_lo, p.e_top, 60):
             e = p.beta * energy
-            assert oracle.shoot(p.ut, e, p.stride)[0] == oracle.shoot_full_count(p.ut, e)
+            assert oracle.shoot(p.pt, e, p.stride)[0] == oracle.shoot_full_count(p.ut, e)
     assert len(strides) >= 2
 
 
@@ -259,16 +259,16 @@ def test_wall_trimming_of_reference_test_potential():
 
 def test_bracket_invariants():
     prob = morse_problem(**CFG_H2)
-    nodes, far, resc = oracle.scan(prob.ut, prob.beta, prob.e_lo, prob.e_top, prob.m, prob.stride)
+    nodes, far, resc = oracle.scan(prob.pt, prob.beta, prob.e_lo, prob.e_top, prob.m, prob.stride)
     assert np.all(np.diff(nodes) >= 0)
     for v in range(17):
-        br = oracle.bracket_level(nodes, far, resc, prob.e_lo, prob.e_top, v)
+        br = oracle.bracket_level(nodes, far, resc, prob.e_lo, prob.e_top, v, prob.far_exp)
         assert br.found
         assert br.c_lo <= v < br.c_hi
         assert br.e_lo < br.e_hi
         lv, it = oracle.refine_level(prob.uv, prob.beta, prob.stride, v, br, 1e-10, prob.abs_tol)
         assert br.e_lo < lv < br.e_hi and 0 < it < 40
-    br = oracle.bracket_level(nodes, far, resc, prob.e_lo, prob.e_top, 17)
+    br = oracle.bracket_level(nodes, far, resc, prob.e_lo, prob.e_top, 17, prob.far_exp)
     assert not br.found
 
 
@@ -280,3 +280,31 @@ def test_segmented_and_sequential_refine_agree():
         ok = ~np.isnan(a)
         np.testing.assert_array_equal(np.isnan(a), np.isnan(b))
         np.testing.assert_allclose(b[ok], a[ok], rtol=1e-11)
+
+
+def test_scan_form_far_end_matches_w_form():
+    """The 12-scaled scan shot (v_i = w_i / 12^(i-1), 4 operations per step)
+    reports w_{N-1} = mantissa 2^(I - 480 r): against the unscaled
+    division-free trajectory on a short grid (no rescales in either) and
+    against the difference-form shot (which rescales the other way) on a
+    long grid where the scan form scales up many times."""
+    prob = morse_problem(mu=7.0, de=20000.0, a=2.0, re=1.2, r_min=0.7, r_max=3.5, n=420, m=64)
+    assert prob.usable
+    rng = np.random.default_rng(4)
+    for energy in rng.uniform(prob.e_lo + 50, prob.e_top, 20):
+        e = prob.beta * energy
+        ref = oracle.trajectory(prob.ut, e)[-1]
+        np.testing.assert_allclose(oracle.scan_far_end(prob.pt, e, prob.stride), ref, rtol=1e-11)
+    big = morse_problem(mu=63.452, de=12440.0, a=1.8676, re=2.666, r_min=2.0, r_max=8.0, n=200_000, m=64)
+    c, i = oracle.far_scale(len(big.pt))
+    assert 1.0 <= c < 2.0 and i > 480
+    for energy in (500.0, 3000.0, 7777.0, 12000.0):
+        e = big.beta * energy
+        cs, ws, rs = oracle.shoot(big.pt, e, big.stride)
+        cd, wd, kd = oracle.shoot_diff(big.uv, e, big.stride)
+        assert cs == cd and rs > 0
+        # log2 |F| and sign agree (F = w 2^k in both forms).
+        assert np.sign(ws) == np.sign(wd)
+        ls = np.log2(abs(ws)) + i - 480 * rs
+        ld = np.log2(abs(wd)) + 480 * kd
+        assert abs(ls - ld) < 1e-6, (energy, ls, ld)

@@ -950,14 +950,20 @@ int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     ctx->last_refine_mode = mode;
     ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
-    // Paired and single lane tasks (two levels per lane) run concurrently on
-    // the two streams.
-    EPSEON_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
-    EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
-    EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream2, mode, lanes, rel_tol, max_iter, 0,
-                              n_potentials(ctx)));
-    EPSEON_HIP_TRY(hipEventRecord(ctx->ev_join, ctx->stream2));
-    EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    if (mode == EPSEON_REFINE_LANE && ctx->levels_per_lane == 2) {
+        // Paired and single lane tasks run concurrently on the two streams.
+        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+        EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+        EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream2, mode, lanes, rel_tol, max_iter, 0,
+                                  n_potentials(ctx)));
+        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_join, ctx->stream2));
+        EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    } else {
+        // One launch on the context's stream: no cross-stream waits between
+        // the scan and the refinement.
+        EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream, mode, lanes, rel_tol, max_iter, 0,
+                                  n_potentials(ctx)));
+    }
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
     ctx->refined = true;
     ctx->refine_timed = true;
@@ -997,6 +1003,26 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
         return rc;
     const size_t np = n_potentials(ctx);
     const size_t k = split_point(ctx, ctx->solve_split);
+    const int32_t mode = choose_refine_mode(ctx);
+    const int32_t lanes = segmented_lanes(ctx);
+    if (k == 0 && !(mode == EPSEON_REFINE_LANE && ctx->levels_per_lane == 2)) {
+        // Scan and refinement back to back on one stream, timed as a whole
+        // (events between the two launches would add a barrier each).
+        ctx->last_refine_mode = mode;
+        ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
+        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve0, ctx->stream));
+        EPSEON_HIP_TRY(run_scan(ctx, ctx->stream, 0, np));
+        EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream, mode, lanes, rel_tol, max_iter, 0, np));
+        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve1, ctx->stream));
+        ctx->scanned = ctx->refined = true;
+        ctx->scan_timed = ctx->refine_timed = false;
+        ctx->normalized = false;
+        ctx->normalize_timed = ctx->vjp_timed = false;
+        ctx->host_levels_valid = false;
+        ctx->last_split_used = 0.0;
+        ctx->solve_timed = true;
+        return EPSEON_OK;
+    }
     if (k == 0) {
         EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve0, ctx->stream));
         if (const int rc = epseon_scan(ctx); rc != EPSEON_OK)
@@ -1014,8 +1040,6 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     // group overlap the scan of the second.  Every potential still runs the
     // same kernels on the same data: results are identical to epseon_scan +
     // epseon_refine.
-    const int32_t mode = choose_refine_mode(ctx);
-    const int32_t lanes = segmented_lanes(ctx);
     ctx->last_refine_mode = mode;
     ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve0, ctx->stream));

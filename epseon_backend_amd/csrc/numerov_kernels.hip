@@ -19,8 +19,8 @@
 //   v_{i+1} = v_i - phi_i (10 v_i + phi_{i-1} v_{i-1}),   v_0 = 0, v_1 = 1,
 // i.e. z = phi_{i-1} v_{i-1}; s = fma(10, v_i, z); v_{i+1} = fma(-phi_i, s, v_i).
 // |w| grows by less than 7.31 per step, so v only shrinks: after the 17th
-// step of every 32-step super-block, values below 2^-480 are multiplied by
-// 2^480.  The sign of v (= sign of w) is sampled once per block of `stride`
+// step of every second 32-step super-block, values below 2^-240 are
+// multiplied by 2^480.  The sign of v (= sign of w) is sampled once per block of `stride`
 // steps (a power of two the host chose so that no two nodes fall in one
 // block).  Refinement shots run the difference form on uv_i = beta V_i with
 // the overflow test every fourth super-block.  Compiled with
@@ -71,7 +71,7 @@ struct SignCounter {
 struct ScanForm {
     using Energy = double;
     using Counter = SignCounter;
-    static constexpr int kRescaleEvery = 1;
+    static constexpr int kRescaleEvery = 2;
     double wm, w, fm;
     __device__ __forceinline__ void init(const double e, const double u0)
     {
@@ -232,8 +232,8 @@ __device__ __forceinline__ void super_block(const double*& p, double (&a)[16], d
 // One outward shot over n points at probe e with node stride B, the grid
 // table u being wave-uniform.  The grid is consumed in 32-step super-blocks,
 // four per loop iteration so that the body stays one straight-line block;
-// the rescale test runs after the 17th step of every super-block
-// (Form::kRescaleEvery = 1) or of every fourth one (4).  The sign is sampled
+// the rescale test runs after the 17th step of every second super-block
+// (Form::kRescaleEvery = 2) or of every fourth one (4).  The sign is sampled
 // after every B-th step and at the last point.  The table is
 // padded so the prefetch past the last super-block stays in the allocation.
 // The form holds the final state, the counter the node count(s).
@@ -253,15 +253,20 @@ __device__ __forceinline__ void shoot_fixed(const double* __restrict__ u, const 
 #pragma unroll
     for (int q = 0; q < 16; ++q)
         a[q] = p[q];
-    constexpr bool every = Form::kRescaleEvery == 1;
+    constexpr bool odd = Form::kRescaleEvery == 2;
     for (int32_t g = 0; g < (nsb >> 2); ++g) {
-        super_block<B, every>(p, a, b, e, k, s, c);
-        super_block<B, every>(p, a, b, e, k, s, c);
-        super_block<B, every>(p, a, b, e, k, s, c);
+        super_block<B, false>(p, a, b, e, k, s, c);
+        super_block<B, odd>(p, a, b, e, k, s, c);
+        super_block<B, false>(p, a, b, e, k, s, c);
         super_block<B, true>(p, a, b, e, k, s, c);
     }
-    for (int32_t sb = 0; sb < (nsb & 3); ++sb)
-        super_block<B, every>(p, a, b, e, k, s, c);
+    const int32_t left = nsb & 3;
+    if (left >= 1)
+        super_block<B, false>(p, a, b, e, k, s, c);
+    if (left >= 2)
+        super_block<B, odd>(p, a, b, e, k, s, c);
+    if (left >= 3)
+        super_block<B, false>(p, a, b, e, k, s, c);
     const int32_t rem = steps - (nsb << 5);
     for (int32_t q = 0; q < rem; ++q) {
         s.step(p[q], e, k);

@@ -14,10 +14,12 @@ namespace epseon::hip {
 // Rescale thresholds of the recurrences: the refinement shots multiply the
 // last values by 2^-480 when |w| reaches 2^480; the scan shots (whose values
 // only shrink) multiply them by 2^480 when they fall below 2^-480.  Both are
-// exact and counted.
+// exact and counted.  The scan's threshold sits at 2^-240 so that values
+// still start from at least 2^-240 when the test runs only every second
+// super-block.
 constexpr double kRescaleAt = 0x1p480;
 constexpr double kRescaleBy = 0x1p-480;
-constexpr double kScanRescaleBelow = 0x1p-480;
+constexpr double kScanRescaleBelow = 0x1p-240;
 constexpr int kRescaleLog2 = 480;
 constexpr int kMaxStride = 32;
 // hbar^2 / (2 amu A^2) in cm^-1 (CODATA 2018 h, c and atomic mass unit).

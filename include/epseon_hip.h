@@ -212,8 +212,9 @@ int epseon_set_refine_mode(epseon_ctx* ctx, int32_t mode);
  * potentials with the longest grids, holding up to that fraction of the scan
  * work, are scanned and refined on one stream while the rest are scanned and
  * refined on the other, so the longest refinement shots overlap scan work.
- * scan_ms/refine_ms of epseon_get_stats then stay 0 and solve_ms holds the
- * whole solve.  Same dispatch site as epseon_scan. */
+ * epseon_get_stats reports solve_ms for the whole solve; scan_ms and
+ * refine_ms stay 0 (time the phases with epseon_scan and epseon_refine).
+ * Same dispatch site as epseon_scan. */
 int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter);
 
 /* Split fraction of the pipelined epseon_solve, in [0, 1) (default 0: scan

@@ -39,11 +39,11 @@
  *               of sign changes between consecutive samples, starting from
  *               w_1 > 0.
  *   rescale     the N-2 steps form super-blocks of 32.  Scan: after the 17th
- *               step of every full super-block, if max(|v_i|, |v_{i-1}|) <
- *               2^-480 both are multiplied by 2^480 and the scale-up count r
- *               grows (|w| grows by less than 7.31 per step, so v only
- *               shrinks, by at most 2^-8.5 per step in the pair norm: the <= 47
- *               unguarded steps stay above 2^-880).  Far end: the scan reports
+ *               step of every second full super-block (q % 64 == 48), if
+ *               max(|v_i|, |v_{i-1}|) < 2^-240 both are multiplied by 2^480
+ *               and the scale-up count r grows (|w| grows by less than 7.31
+ *               per step, so v only shrinks, by at most 2^-8.5 per step in the
+ *               pair norm: the <= 78 unguarded steps stay above 2^-903).  Far end: the scan reports
  *               v_{N-1} * c and r, with 12^(N-2) = c * 2^I, c = 2^frac(L),
  *               I = floor(L), L = (N-2) log2(12), so w_{N-1} = v_{N-1} c 2^(I - 480 r).
  *               Refinement (difference form): after the 17th step of every
@@ -67,7 +67,7 @@
 #include <omp.h>
 #endif
 
-#define EO_SCAN_RESCALE_BELOW 0x1p-480
+#define EO_SCAN_RESCALE_BELOW 0x1p-240
 #define EO_K 16.857629191640175        /* cm^-1 amu A^2 (CODATA 2018 h, c, u) */
 #define EO_RESCALE_AT 0x1p480
 #define EO_RESCALE_BY 0x1p-480
@@ -222,7 +222,7 @@ int32_t eo_shoot(const double* pt, int64_t n, double e, int32_t stride,
             nodes += (int32_t)(sb ^ sp);
             sp = sb;
         }
-        if ((q & 31) == 16 && q < (steps & ~(int64_t)31) &&
+        if ((q & 63) == 48 && q < (steps & ~(int64_t)31) &&
             fmax(fabs(w), fabs(wm)) < EO_SCAN_RESCALE_BELOW) {
             w *= EO_RESCALE_AT;
             wm *= EO_RESCALE_AT;

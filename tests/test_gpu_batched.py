@@ -93,7 +93,7 @@ def test_batched_full_scale(capi):
         lv, st, it = ctx.levels()
         stats = ctx.stats()
         flags = np.array([ctx.potential_info(i).flags for i in range(len(specs))])
-    _log(f"  refine mode {stats.refine_mode}, lanes {stats.refine_lanes}, scan {stats.scan_ms:.2f} ms, "
+    _log(f"  refine mode {stats.refine_mode}, lanes {stats.refine_lanes}, solve {stats.solve_ms:.2f} ms, "
          f"refine {stats.refine_ms:.2f} ms, levels converged {stats.levels_converged}, "
          f"unresolved potentials {int(flags.sum())}")
     assert stats.refine_mode == capi.REFINE_LANE

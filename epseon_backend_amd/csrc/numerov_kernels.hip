@@ -473,11 +473,109 @@ __global__ __launch_bounds__(256) void numerov_scan_kernel(const double* __restr
 
 namespace {
 
+// ---- deterministic exp / log / sqrt of the refinement model -------------
+// Same operations as the oracle (+, -, *, /, fma, floor, frexp, ldexp), so
+// both produce the same bits.
+
+constexpr double kLn2 = 0.69314718055994530942;
+constexpr double kLn2Hi = 6.93147180369123816490e-01;  // q * kLn2Hi exact for |q| < 2^20
+constexpr double kLn2Lo = 1.90821492927058770002e-10;
+constexpr double kSqrt1_2 = 0.70710678118654752440;
+
+// e^y = m 2^n, m in [1, 2).
+__device__ __forceinline__ void model_exp(const double y, double& m, int32_t& n)
+{
+    const double q = floor(y / kLn2);
+    const double r = (y - q * kLn2Hi) - q * kLn2Lo;
+    constexpr double c[18] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720,
+                              1.0 / 5040, 1.0 / 40320, 1.0 / 362880, 1.0 / 3628800,
+                              1.0 / 39916800, 1.0 / 479001600, 1.0 / 6227020800.0,
+                              1.0 / 87178291200.0, 1.0 / 1307674368000.0,
+                              1.0 / 20922789888000.0, 1.0 / 355687428096000.0};
+    double p = c[17];
+#pragma unroll
+    for (int i = 16; i >= 0; --i)
+        p = __builtin_fma(p, r, c[i]);
+    m = p;
+    n = static_cast<int32_t>(q);
+}
+
+__device__ __forceinline__ double model_sqrt(const double d)
+{
+    if (!(d > 0.0))
+        return 0.0;
+    int e = 0;
+    double m = __builtin_frexp(d, &e);
+    if (e & 1) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    double y = __builtin_fma(m, 0.4142135623730950, 0.5857864376269050);
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+        y = 0.5 * (y + m / y);
+    return __builtin_ldexp(y, e / 2);
+}
+
+__device__ __forceinline__ double model_log(const double x)
+{
+    int e = 0;
+    double m = __builtin_frexp(x, &e);
+    if (m < kSqrt1_2) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double p = 1.0 / 25.0;
+#pragma unroll
+    for (int i = 11; i >= 0; --i)
+        p = __builtin_fma(p, s2, 1.0 / static_cast<double>(2 * i + 1));
+    return __builtin_fma(static_cast<double>(e), kLn2, 2.0 * s * p);
+}
+
+// Rate b of F(E) = (c0 + c1 (E - E_b)) e^(b (E - E_b)) through equally spaced
+// scan points a < b < c: with x = e^(b de), F_a/F_b x + F_c/F_b / x = 2, one
+// positive root either way the third point lies.  0 when no fit.
+__device__ __forceinline__ double model_rate(const double wa, const int32_t ka, const double wb,
+                                             const int32_t kb, const double wc, const int32_t kc,
+                                             const bool above, const double de)
+{
+    if (wb == 0.0 || ka - kb > 960 || kb - ka > 960 || kc - kb > 960 || kb - kc > 960)
+        return 0.0;
+    const double ra = __builtin_ldexp(wa / wb, ka - kb);
+    const double rc = __builtin_ldexp(wc / wb, kc - kb);
+    const double disc = 1.0 - ra * rc;
+    if (!(disc >= 0.0) || ra == 0.0 || !__builtin_isfinite(ra) || !__builtin_isfinite(rc))
+        return 0.0;
+    const double sq = model_sqrt(disc);
+    const double x = above ? (1.0 - sq) / ra : (1.0 + sq) / ra;
+    if (!(x > 0.0) || !__builtin_isfinite(x))
+        return 0.0;
+    const double lx = model_log(x);
+    if (!(fabs(lx) < 600.0))
+        return 0.0;
+    return lx / de;
+}
+
+// F = w 2^k -> G = F e^(-b (e - e0)).
+__device__ __forceinline__ void model_apply(const double b, const double e0, const double e, double& w,
+                                            int32_t& k)
+{
+    double m;
+    int32_t n;
+    model_exp(-b * (e - e0), m, n);
+    w = w * m;
+    k = k + n;
+}
+
 // Bracket of level v from the scan: the first trial energy with more than v
 // nodes and the one below it (or the window floor, below which no level
-// exists since every eigenvalue exceeds min V).  Exponents are binary.
+// exists since every eigenvalue exceeds min V).  Exponents are binary.  When
+// the bracket holds exactly level v and a third scan point of the same count
+// regime exists above or below it, `rate` is the model rate b.
 struct Bracket {
-    double el, eh, wl, wh;
+    double el, eh, wl, wh, rate;
     int32_t kl, kh, cl, ch, lo_ok, found;
 };
 
@@ -518,6 +616,16 @@ __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const 
         b.kl = P.far_exp - rescales[P.energy_offset + j - 1] * kRescaleLog2;
         b.lo_ok = 1;
     }
+    b.rate = 0.0;
+    if (b.lo_ok && b.cl == v && b.ch == v + 1) {
+        const int64_t o = P.energy_offset;
+        if (j + 1 < m && nd[j + 1] == b.ch)
+            b.rate = model_rate(b.wl, b.kl, b.wh, b.kh, far_end[o + j + 1],
+                                P.far_exp - rescales[o + j + 1] * kRescaleLog2, true, P.de);
+        else if (j >= 2 && nd[j - 2] == b.cl)
+            b.rate = model_rate(far_end[o + j - 2], P.far_exp - rescales[o + j - 2] * kRescaleLog2, b.wl,
+                                b.kl, b.wh, b.kh, false, P.de);
+    }
     return b;
 }
 
@@ -531,6 +639,20 @@ struct Refiner {
     Bracket b;
     int32_t v, side, it;
     bool isolated;
+    double rate, e0;
+
+    // Regula falsi runs on G = F e^(-b (E - E_hi0)) when the scan gave a
+    // model rate (nearly linear where F grows exponentially across the
+    // bracket); sign(G) = sign(F), so counts and brackets are unchanged.
+    __device__ __forceinline__ void start_model()
+    {
+        rate = b.rate;
+        e0 = b.eh;
+        if (rate != 0.0) {
+            model_apply(rate, e0, b.el, b.wl, b.kl);
+            model_apply(rate, e0, b.eh, b.wh, b.kh);
+        }
+    }
 
     __device__ __forceinline__ bool converged(const double rel_tol, const double abs_tol) const
     {
@@ -566,10 +688,12 @@ struct Refiner {
     }
 
     // Take the probe's result; `c` is ignored when the bracket is isolated.
-    __device__ __forceinline__ void update(const double em, const double w, const int32_t k, int32_t c)
+    __device__ __forceinline__ void update(const double em, double w, int32_t k, int32_t c)
     {
         if (isolated)
             c = (static_cast<int32_t>(sign_bit(w)) == (v & 1)) ? v : v + 1;
+        if (rate != 0.0)
+            model_apply(rate, e0, em, w, k);
         ++it;
         // Anderson-Bjorck: when the same end is replaced twice in a row, the
         // kept end's value is scaled by m = 1 - F_new/F_replaced (0.5 when
@@ -649,6 +773,7 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
         }
         r[q].v = P.level_min + l;
         r[q].b = find_bracket(P, r[q].v, nodes, far_end, rescales);
+        r[q].start_model();
         if (!r[q].b.found) {
             write_level(levels, status, iterations, out[q], __builtin_nan(""), 1, 0);
             continue;
@@ -887,6 +1012,7 @@ __global__ __launch_bounds__(T) void numerov_refine_segmented_kernel(
     Refiner r{};
     r.v = P.level_min + task.first;
     r.b = find_bracket(P, r.v, nodes, far_end, rescales);
+    r.start_model();
     if (!r.b.found) {
         if (lane == 0)
             write_level(levels, status, iterations, out, __builtin_nan(""), 1, 0);

@@ -44,6 +44,7 @@ class Bracket(ctypes.Structure):
         ("c_hi", ctypes.c_int32),
         ("lo_valid", ctypes.c_int32),
         ("found", ctypes.c_int32),
+        ("rate", ctypes.c_double),
     ]
 
 
@@ -69,6 +70,9 @@ def _lib() -> ctypes.CDLL:
         "eo_scaled_potential": (None, [_dp, i64, d, _dp]),
         "eo_scan_table": (None, [_dp, i64, d, _dp]),
         "eo_far_scale": (None, [i64, _dp, _ip]),
+        "eo_model_exp_value": (d, [d]),
+        "eo_model_log_value": (d, [d]),
+        "eo_model_sqrt_value": (d, [d]),
         "eo_shoot_diff": (i32, [_dp, i64, d, i32, _dp, _ip]),
         "eo_morse": (None, [d, d, d, d, d, i64, _dp]),
         "eo_trim": (i32, [_dp, i64, d, d, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
@@ -409,3 +413,9 @@ def level_vjp(prob: Problem, levels: np.ndarray, grad: np.ndarray) -> np.ndarray
     out = np.zeros(prob.n)
     out[first : last + 1] = win
     return out
+
+
+def model_functions() -> tuple:
+    """(exp, log, sqrt) of the refinement model, for the tests."""
+    lib = _lib()
+    return lib.eo_model_exp_value, lib.eo_model_log_value, lib.eo_model_sqrt_value

@@ -308,7 +308,106 @@ typedef struct eo_bracket {
     int32_t c_lo, c_hi;  /* node counts at the ends */
     int32_t lo_valid;    /* 0 when e_lo is the window floor (no shot there) */
     int32_t found;       /* 0 when no trial energy has more than v nodes */
+    double rate;         /* b of the model F ~ (c0 + c1 E) e^(b E) (0: none) */
 } eo_bracket;
+
+/* ---- deterministic exp / log for the refinement model -----------------
+ * Written with +, -, *, /, fma, floor, frexp and ldexp only, so the oracle
+ * and the kernels produce the same bits.  Accuracy ~1e-15 relative (the model only
+ * steers the probes). */
+#define EO_LN2 0.69314718055994530942
+/* ln 2 split so that q * EO_LN2_HI is exact for |q| < 2^20 (Cody-Waite). */
+#define EO_LN2_HI 6.93147180369123816490e-01
+#define EO_LN2_LO 1.90821492927058770002e-10
+#define EO_SQRT1_2 0.70710678118654752440
+
+/* e^y = m 2^n with m in [1, 2). */
+static void eo_model_exp(double y, double* m, int32_t* n)
+{
+    const double q = floor(y / EO_LN2);
+    const double r = (y - q * EO_LN2_HI) - q * EO_LN2_LO;
+    /* Taylor coefficients 1/k!, k = 0..17 (truncation < 2e-19 on [0, ln 2)). */
+    static const double c[18] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720,
+                                 1.0 / 5040, 1.0 / 40320, 1.0 / 362880, 1.0 / 3628800,
+                                 1.0 / 39916800, 1.0 / 479001600, 1.0 / 6227020800.0,
+                                 1.0 / 87178291200.0, 1.0 / 1307674368000.0,
+                                 1.0 / 20922789888000.0, 1.0 / 355687428096000.0};
+    double p = c[17];
+    for (int i = 16; i >= 0; --i)
+        p = fma(p, r, c[i]);
+    *m = p;
+    *n = (int32_t)q;
+}
+
+/* sqrt d for d >= 0 by Newton iterations from a linear first guess. */
+static double eo_model_sqrt(double d)
+{
+    if (!(d > 0.0))
+        return 0.0;
+    int e = 0;
+    double m = frexp(d, &e);
+    if (e & 1) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    double y = fma(m, 0.4142135623730950, 0.5857864376269050);
+    for (int i = 0; i < 5; ++i)
+        y = 0.5 * (y + m / y);
+    return ldexp(y, e / 2);
+}
+
+/* ln x for x > 0. */
+static double eo_model_log(double x)
+{
+    int e = 0;
+    double m = frexp(x, &e);
+    if (m < EO_SQRT1_2) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double p = 1.0 / 25.0;
+    for (int i = 11; i >= 0; --i)
+        p = fma(p, s2, 1.0 / (double)(2 * i + 1));
+    return fma((double)e, EO_LN2, 2.0 * s * p);
+}
+
+/* Rate b of the model F(E) = (c0 + c1 (E - E_b)) e^(b (E - E_b)) through the
+ * scan points a < b < c (equally spaced by de, F = w 2^k): with
+ * x = e^(b de), F_a/F_b x + F_c/F_b / x = 2.  The bracket is [E_b, E_c] (third
+ * point below, F changes sign between b and c) or [E_a, E_b] (third point
+ * above); in either case exactly one root x is positive.  0 when the points
+ * do not allow a fit. */
+static double eo_model_rate(double wa, int32_t ka, double wb, int32_t kb, double wc, int32_t kc,
+                            int32_t above, double de)
+{
+    if (wb == 0.0 || ka - kb > 960 || kb - ka > 960 || kc - kb > 960 || kb - kc > 960)
+        return 0.0;
+    const double ra = ldexp(wa / wb, ka - kb);
+    const double rc = ldexp(wc / wb, kc - kb);
+    const double disc = 1.0 - ra * rc;
+    if (!(disc >= 0.0) || ra == 0.0 || !isfinite(ra) || !isfinite(rc))
+        return 0.0;
+    const double sq = eo_model_sqrt(disc);
+    const double x = above ? (1.0 - sq) / ra : (1.0 + sq) / ra;
+    if (!(x > 0.0) || !isfinite(x))
+        return 0.0;
+    const double lx = eo_model_log(x);
+    if (!(fabs(lx) < 600.0))
+        return 0.0;
+    return lx / de;
+}
+
+/* F -> G = F e^(-b (E - e0)) as mantissa/exponent. */
+static void eo_model_apply(double b, double e0, double e, double* w, int32_t* k)
+{
+    double m;
+    int32_t n;
+    eo_model_exp(-b * (e - e0), &m, &n);
+    *w = *w * m;
+    *k = *k + n;
+}
 
 /* Level v lies between the first trial energy with more than v nodes and the
  * trial energy before it (or the window floor, below which no level exists
@@ -335,6 +434,7 @@ void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t*
     out->c_hi = nodes[j];
     out->w_hi = w_last[j];
     out->k_hi = far_exp - rescales[j] * 480;
+    out->rate = 0.0;
     if (j == 0 || nodes[j - 1] > v) {
         out->e_lo = e_lo;
         out->c_lo = 0;
@@ -345,6 +445,16 @@ void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t*
         out->w_lo = w_last[j - 1];
         out->k_lo = far_exp - rescales[j - 1] * 480;
         out->lo_valid = 1;
+    }
+    /* Model rate from a third scan point in the same count regime: above
+     * (j + 1, same count as j) or below (j - 2, same count as j - 1). */
+    if (out->lo_valid && out->c_lo == v && out->c_hi == v + 1) {
+        if (j + 1 < m && nodes[j + 1] == out->c_hi)
+            out->rate = eo_model_rate(out->w_lo, out->k_lo, out->w_hi, out->k_hi, w_last[j + 1],
+                                      far_exp - rescales[j + 1] * 480, 1, de);
+        else if (j >= 2 && nodes[j - 2] == out->c_lo)
+            out->rate = eo_model_rate(w_last[j - 2], far_exp - rescales[j - 2] * 480, out->w_lo,
+                                      out->k_lo, out->w_hi, out->k_hi, 0, de);
     }
 }
 
@@ -613,7 +723,8 @@ void eo_shoot_segmented(const double* uv, int64_t n, double e, int32_t stride, i
  * has no shot) it is cut with counted shots at the count-interpolated point
  * (where level v would sit if the levels in it were evenly spaced, kept within
  * [1/8, 7/8] of the bracket); once it holds exactly level v
- * regula falsi with Anderson-Bjorck scaling runs on F(E) = w_{N-1} 2^k, each new
+ * regula falsi with Anderson-Bjorck scaling runs on F(E) = w_{N-1} 2^k (or on
+ * G = F e^(-b (E - E_hi)) when the scan gives a model rate b), each new
  * point kept at least tol/2 inside the bracket, and a probe's count follows
  * from the sign of F (count parity = sign bit of w_{N-1}).  Stops when the
  * width is <= tol = rel_tol*max(|E_lo|,|E_hi|) + abs_tol; the level is the
@@ -631,6 +742,14 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
     int32_t lo_ok = br->lo_valid;
     int32_t side = 0;
     int32_t it = 0;
+    /* With a model rate b, regula falsi runs on G = F e^(-b (E - E_hi0)),
+     * nearly linear in E where F grows exponentially across the bracket;
+     * sign(G) = sign(F), so counts and brackets are unchanged. */
+    const double rate = br->rate, e0 = br->e_hi;
+    if (rate != 0.0) {
+        eo_model_apply(rate, e0, el, &wl, &kl);
+        eo_model_apply(rate, e0, eh, &wh, &kh);
+    }
     for (;;) {
         const double width = eh - el;
         const double tol = fma(rel_tol, fmax(fabs(el), fabs(eh)), abs_tol);
@@ -670,6 +789,8 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
         }
         if (isolated)
             c = ((int32_t)eo_signbit(w) == (v & 1)) ? v : v + 1;
+        if (rate != 0.0)
+            eo_model_apply(rate, e0, em, &w, &k);
         ++it;
         /* Anderson-Bjorck: when the same end is replaced twice in a row, the
          * kept end's value is scaled by m = 1 - F_new/F_replaced (0.5 when
@@ -884,3 +1005,16 @@ void eo_level_vjp(const double* uv, int64_t n, const double* e, const double* g,
     }
     free(psi);
 }
+
+/* Test hooks for the model's elementary functions. */
+double eo_model_exp_value(double y)
+{
+    double m;
+    int32_t n;
+    eo_model_exp(y, &m, &n);
+    return ldexp(m, n);
+}
+
+double eo_model_log_value(double x) { return eo_model_log(x); }
+
+double eo_model_sqrt_value(double d) { return eo_model_sqrt(d); }

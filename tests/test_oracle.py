@@ -308,3 +308,17 @@ def test_scan_form_far_end_matches_w_form():
         ls = np.log2(abs(ws)) + i - 480 * rs
         ld = np.log2(abs(wd)) + 480 * kd
         assert abs(ls - ld) < 1e-6, (energy, ls, ld)
+
+
+def test_refinement_model_elementary_functions():
+    """The deterministic exp / log / sqrt that steer the refinement probes
+    (written with basic IEEE operations so the kernels match bit for bit) are
+    accurate to a few ulps over the ranges they see."""
+    mexp, mlog, msqrt = oracle.model_functions()
+    rng = np.random.default_rng(9)
+    for y in np.concatenate([rng.uniform(-700, 700, 300), rng.uniform(-1, 1, 300), [0.0]]):
+        assert abs(mexp(y) / math.exp(y) - 1) < 4e-15, y
+    for x in np.concatenate([np.exp(rng.uniform(-600, 600, 300)), rng.uniform(0.5, 2, 300)]):
+        assert abs(mlog(x) - math.log(x)) < 2e-15 * max(1.0, abs(math.log(x))), x
+    for d in np.concatenate([rng.uniform(0, 4, 300), np.exp(rng.uniform(-200, 200, 300))]):
+        assert (abs(msqrt(d) / math.sqrt(d) - 1) < 4.5e-16) if d > 0 else (msqrt(d) == 0.0), d

@@ -32,9 +32,12 @@ hipError_t launch_scan(hipStream_t, const double*, const PotentialRecord*, const
 hipError_t launch_refine(hipStream_t, int32_t, const double*, const PotentialRecord*,
                          const WaveTask*, int32_t, int32_t, const int32_t*, const double*,
                          const int32_t*, double, int32_t, double*, int32_t*, int32_t*);
-hipError_t launch_refine_segmented(hipStream_t, int32_t, const double*, const PotentialRecord*,
-                                   const WaveTask*, int32_t, const int32_t*, const double*,
-                                   const int32_t*, double, int32_t, double*, int32_t*, int32_t*);
+hipError_t launch_refine_segmented(hipStream_t, int32_t, const double*, const double*,
+                                   const PotentialRecord*, const WaveTask*, int32_t, const int32_t*,
+                                   const double*, const int32_t*, double, int32_t, double*, int32_t*,
+                                   int32_t*);
+hipError_t launch_segment_table(hipStream_t, const double*, const PotentialRecord*, int32_t, int32_t,
+                                double*);
 hipError_t launch_normalize(hipStream_t, const double*, const double*, const PotentialRecord*,
                             const WaveTask*, int32_t, const double*, const int32_t*, LevelNorm*);
 hipError_t launch_eigenfunctions(hipStream_t, const double*, const double*, const PotentialRecord*,
@@ -196,6 +199,11 @@ struct epseon_ctx {
     double* d_ut = nullptr;
     double* d_uv = nullptr;
     double* d_uvr = nullptr;
+    // Segment table of the 64-lane segmented refinement (seg_rows per
+    // potential, rows of 64 values): lane s of a level's wavefront reads
+    // column s, so its loads are coalesced.
+    double* d_segtab = nullptr;
+    bool segtab_ok = false;  // every potential's table fits one 32-bit-addressed buffer
     PotentialRecord* d_rec = nullptr;
     WaveTask* d_scan_tasks = nullptr;
     // Scan tasks split into one bin per XCD (whole potentials, balanced by
@@ -532,8 +540,8 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
         EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
 
-    int64_t pts = 0, en = 0, lv = 0, psi_total = 0;
-    int32_t max_points = 0;
+    int64_t pts = 0, en = 0, lv = 0, psi_total = 0, seg_total = 0;
+    int32_t max_points = 0, max_seg_rows = 0;
     double scan_pu = 0.0;
     for (int32_t p = 0; p < count; ++p) {
         const epseon_potential_desc& d = desc[p];
@@ -583,8 +591,12 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
             const double fl = std::floor(l2);
             r.far_exp = static_cast<int32_t>(fl);
             r.far_scale = std::exp2(l2 - fl);
-            r.reserved = 0;
         }
+        r.seg_row = static_cast<int32_t>(seg_total);
+        seg_total += seg_rows(r.n_points, r.flags, r.level_count);
+        max_seg_rows = std::max(max_seg_rows, seg_rows(r.n_points, r.flags, r.level_count));
+        if (seg_total > INT32_MAX)
+            return fail(EPSEON_ERR_INVALID, "batch too large for the segment table");
         r.user_offset = v_offsets[p];
         r.psi_offset = psi_total;
         psi_total += static_cast<int64_t>(d.level_count) * d.n_points;
@@ -716,6 +728,7 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_ut));
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_uv));
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_uvr));
+    EPSEON_HIP_TRY(arena_array(a, static_cast<size_t>(std::max<int64_t>(seg_total, 1)) * 64, &ctx->d_segtab));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(count, 1), &ctx->d_rec));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(scan_tasks.size(), 1), &ctx->d_scan_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(xcd_tasks.size(), 1), &ctx->d_xcd_scan_tasks));
@@ -779,6 +792,10 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_resc, 0, ne * sizeof(int32_t), ctx->stream));
     EPSEON_HIP_TRY(launch_shift_potential(ctx->stream, v_src, ctx->d_rec, count, max_points,
                                           ctx->d_ut, ctx->d_uv, ctx->d_uvr));
+    ctx->segtab_ok = max_seg_rows <= INT32_MAX / 512;
+    if (ctx->segtab_ok)
+        EPSEON_HIP_TRY(launch_segment_table(ctx->stream, ctx->d_uv, ctx->d_rec, count, max_seg_rows,
+                                            ctx->d_segtab));
     // The host buffers are pageable and owned by the caller: finish the
     // copies before returning.
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -888,7 +905,9 @@ hipError_t run_refine(epseon_ctx* ctx, hipStream_t stream, hipStream_t stream_b,
     }
     const int32_t t0 = ctx->cum_level_tasks[k0];
     const int32_t t1 = ctx->cum_level_tasks[k1];
-    return launch_refine_segmented(stream, lanes, ctx->d_uv, ctx->d_rec, ctx->d_level_tasks + t0,
+    return launch_refine_segmented(stream, lanes, ctx->d_uv,
+                                   lanes == 64 && ctx->segtab_ok ? ctx->d_segtab : nullptr,
+                                   ctx->d_rec, ctx->d_level_tasks + t0,
                                    t1 - t0, ctx->d_nodes, ctx->d_far, ctx->d_resc, rel_tol, max_iter,
                                    ctx->d_levels, ctx->d_status, ctx->d_iters);
 }

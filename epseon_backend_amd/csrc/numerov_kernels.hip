@@ -859,36 +859,62 @@ __device__ __forceinline__ void walk_tail(const double (&buf)[16], const int32_t
             body(buf[k], j + k);
 }
 
-template <int K>
-__device__ __forceinline__ void load16(double (&buf)[16], const double* __restrict__ p)
+// Word 3 of a buffer resource for raw 32-bit-element reads on CDNA
+// (DATA_FORMAT = 4, i.e. 32-bit elements; everything else zero).
+constexpr int32_t kBufferResourceWord3 = 0x00020000;
+
+// Grid values of one segment, value j being the j-th step's point.
+// RowSource reads the natural uv table (lane-private addresses); ColumnSource
+// reads the lane's column of the segment table through a wave-uniform buffer
+// resource (row r at byte off + r * 512), so each load instruction of a
+// wavefront covers 512 contiguous bytes and needs one offset register.
+struct RowSource {
+    const double* p;
+    __device__ __forceinline__ double at(const int32_t j) const { return p[j]; }
+};
+
+struct ColumnSource {
+    __amdgpu_buffer_rsrc_t rs;
+    int32_t off;
+    __device__ __forceinline__ double at(const int32_t j) const
+    {
+        typedef int32_t v2i __attribute__((ext_vector_type(2)));
+        const v2i x = __builtin_amdgcn_raw_buffer_load_b64(rs, off + j * 512, 0, 0);
+        return __builtin_bit_cast(double, x);
+    }
+};
+
+template <class Src>
+__device__ __forceinline__ void load16(double (&buf)[16], const Src& src, const int32_t j)
 {
 #pragma unroll
     for (int k = 0; k < 16; ++k)
-        buf[k] = p[k];
+        buf[k] = src.at(j + k);
 }
 
-// Visit the `len` grid values p[0..len) in order, body(value, index).  Three
-// 16-value VGPR buffers rotate so that every load is issued two chunks (32
-// steps) before its use; vector-memory loads return in order, so the waits
-// are partial.  Reads up to 48 values past `len` (the tables are padded).
-template <class Body>
-__device__ __forceinline__ void walk_segment(const double* __restrict__ p, const int32_t len, Body& body)
+// Visit the `len` values src.at(0 .. len - 1) in order, body(value, index).
+// Three 16-value VGPR buffers rotate so that every load is issued two chunks
+// (32 steps) before its use; vector-memory loads return in order, so the
+// waits are partial.  Also reads the 32 values that follow the segment (the
+// tables are padded).
+template <class Src, class Body>
+__device__ __forceinline__ void walk_segment(const Src& src, const int32_t len, Body& body)
 {
     double A[16], B[16], C[16];
-    load16<0>(A, p);
-    load16<0>(B, p + 16);
+    load16(A, src, 0);
+    load16(B, src, 16);
     int32_t j = 0;
     for (; j + 48 <= len; j += 48) {
-        load16<0>(C, p + j + 32);
+        load16(C, src, j + 32);
         walk_chunk(A, j, body);
-        load16<0>(A, p + j + 48);
+        load16(A, src, j + 48);
         walk_chunk(B, j + 16, body);
-        load16<0>(B, p + j + 64);
+        load16(B, src, j + 64);
         walk_chunk(C, j + 32, body);
     }
     const int32_t rem = len - j;
     if (rem >= 32) {
-        load16<0>(C, p + j + 32);
+        load16(C, src, j + 32);
         walk_chunk(A, j, body);
         walk_chunk(B, j + 16, body);
         walk_tail(C, j + 32, len, body);
@@ -975,6 +1001,29 @@ struct CountBody {
 
 }  // namespace
 
+__global__ __launch_bounds__(256) void segment_table_kernel(const double* __restrict__ uv,
+                                                            const PotentialRecord* __restrict__ pots,
+                                                            double* __restrict__ segtab)
+{
+    const PotentialRecord& P = pots[blockIdx.y];
+    const int32_t rows = seg_rows(P.n_points, P.flags, P.level_count);
+    const int32_t n = P.n_points;
+    const int32_t qn = n - 2;
+    const int32_t segs = seg_count(n);
+    const double* u = uv + P.ut_offset;
+    double* out = segtab + static_cast<int64_t>(P.seg_row) * 64;
+    for (int32_t i = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x); i < rows * 64;
+         i += static_cast<int32_t>(gridDim.x * blockDim.x)) {
+        const int32_t r = i >> 6, s = i & 63;
+        double x = 0.0;
+        if (s < segs) {
+            const int64_t q = segment_start(qn, segs, s) + r;
+            x = u[q < n - 1 ? q : n - 1];
+        }
+        out[i] = x;
+    }
+}
+
 // Segmented refinement: one T-thread block per level.  Lane s owns steps
 // [q_s, q_{s+1}) of the trimmed grid (S = clamp(Q/32, 1, T) segments) and
 // builds that segment's 2x2 difference-form transfer matrix (columns (w, d) =
@@ -983,9 +1032,9 @@ struct CountBody {
 // at d = 1, 2, 4, ...) applied to (w_1, d_1) = (1, 1).  Counted shots chain
 // the matrices sequentially (lane 0) for the segment start vectors and re-run
 // every segment.
-template <int T>
-__global__ __launch_bounds__(T) void numerov_refine_segmented_kernel(
-    const double* __restrict__ uv, const PotentialRecord* __restrict__ pots,
+template <int T, bool TABLE>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(2))) void numerov_refine_segmented_kernel(
+    const double* __restrict__ uv, const double* __restrict__ segtab, const PotentialRecord* __restrict__ pots,
     const WaveTask* __restrict__ tasks, const int32_t n_tasks, const int32_t* __restrict__ nodes,
     const double* __restrict__ far_end, const int32_t* __restrict__ rescales, const double rel_tol,
     const int32_t max_iter, double* __restrict__ levels, int32_t* __restrict__ status,
@@ -1025,6 +1074,27 @@ __global__ __launch_bounds__(T) void numerov_refine_segmented_kernel(
     const bool active = lane < segs;
     const int32_t q0 = active ? static_cast<int32_t>(segment_start(qn, segs, lane)) : 0;
     const int32_t q1 = active ? static_cast<int32_t>(segment_start(qn, segs, lane + 1)) : 0;
+    // With the segment table, lane s reads column s (row r = point q0 + r):
+    // consecutive lanes load consecutive addresses.
+    ColumnSource cs{};
+    double u_start;
+    if constexpr (TABLE) {
+        // The resource must be wave-uniform (scalar registers); the
+        // readfirstlane calls state what the compiler cannot prove.
+        const int32_t rows = seg_rows(P.n_points, P.flags, P.level_count);
+        const uint64_t base = reinterpret_cast<uint64_t>(segtab + static_cast<int64_t>(P.seg_row) * 64);
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(base));
+        const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(base >> 32));
+        const int32_t bytes = __builtin_amdgcn_readfirstlane(rows * 512);
+        cs.rs = __builtin_amdgcn_make_buffer_rsrc(
+            reinterpret_cast<double*>((static_cast<uint64_t>(hi) << 32) | lo), static_cast<int16_t>(0),
+            bytes, kBufferResourceWord3);
+        cs.off = lane * 8 + 512;
+        u_start = cs.at(-1);
+    } else {
+        u_start = u[q0];
+    }
+    const RowSource rsrc{u + q0 + 1};
     const StepConsts k = step_consts();
 
     int32_t st = 0;
@@ -1039,8 +1109,11 @@ __global__ __launch_bounds__(T) void numerov_refine_segmented_kernel(
         const double e = P.beta * em;
 
         if (active) {
-            MatrixBody mb{e, e - u[q0], 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10};
-            walk_segment(u + q0 + 1, q1 - q0, mb);
+            MatrixBody mb{e, e - u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10};
+            if constexpr (TABLE)
+                walk_segment(cs, q1 - q0, mb);
+            else
+                walk_segment(rsrc, q1 - q0, mb);
             const double mx = fmax(fmax(fabs(mb.wa), fabs(mb.da)), fmax(fabs(mb.wb), fabs(mb.db)));
             const int32_t sh = frexp_exponent(mx);
             s_m[0][lane] = __builtin_ldexp(mb.wa, -sh);
@@ -1095,8 +1168,11 @@ __global__ __launch_bounds__(T) void numerov_refine_segmented_kernel(
         if (counted) {
             if (active) {
                 const double w0 = s_start[0][lane], d0 = s_start[1][lane];
-                CountBody cb{e, e - u[q0], w0, d0, w0 - d0, q0, qn, P.stride, {0, 0, 0, 0}, k.c10};
-                walk_segment(u + q0 + 1, q1 - q0, cb);
+                CountBody cb{e, e - u_start, w0, d0, w0 - d0, q0, qn, P.stride, {0, 0, 0, 0}, k.c10};
+                if constexpr (TABLE)
+                    walk_segment(cs, q1 - q0, cb);
+                else
+                    walk_segment(rsrc, q1 - q0, cb);
                 s_cnt[lane] = cb.sc;
             }
             __syncthreads();
@@ -1183,34 +1259,56 @@ hipError_t launch_refine(hipStream_t stream, int32_t levels_per_lane, const doub
     return hipGetLastError();
 }
 
+// segtab != nullptr selects the segment table for 64-lane blocks.
 hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, const double* uv,
-                                   const PotentialRecord* pots, const WaveTask* tasks,
-                                   int32_t n_tasks, const int32_t* nodes, const double* far_end,
-                                   const int32_t* rescales, double rel_tol, int32_t max_iter,
-                                   double* levels, int32_t* status, int32_t* iterations)
+                                   const double* segtab, const PotentialRecord* pots,
+                                   const WaveTask* tasks, int32_t n_tasks, const int32_t* nodes,
+                                   const double* far_end, const int32_t* rescales, double rel_tol,
+                                   int32_t max_iter, double* levels, int32_t* status,
+                                   int32_t* iterations)
 {
     if (n_tasks == 0)
         return hipSuccess;
     const dim3 grid(static_cast<unsigned>(n_tasks));
     switch (lanes) {
     case 256:
-        hipLaunchKernelGGL(numerov_refine_segmented_kernel<256>, grid, dim3(256), 0, stream, uv, pots,
-                           tasks, n_tasks, nodes, far_end, rescales, rel_tol, max_iter, levels,
-                           status, iterations);
+        hipLaunchKernelGGL((numerov_refine_segmented_kernel<256, false>), grid, dim3(256), 0, stream, uv,
+                           segtab, pots, tasks, n_tasks, nodes, far_end, rescales, rel_tol, max_iter,
+                           levels, status, iterations);
         break;
     case 128:
-        hipLaunchKernelGGL(numerov_refine_segmented_kernel<128>, grid, dim3(128), 0, stream, uv, pots,
-                           tasks, n_tasks, nodes, far_end, rescales, rel_tol, max_iter, levels,
-                           status, iterations);
+        hipLaunchKernelGGL((numerov_refine_segmented_kernel<128, false>), grid, dim3(128), 0, stream, uv,
+                           segtab, pots, tasks, n_tasks, nodes, far_end, rescales, rel_tol, max_iter,
+                           levels, status, iterations);
         break;
     case 64:
-        hipLaunchKernelGGL(numerov_refine_segmented_kernel<64>, grid, dim3(64), 0, stream, uv, pots,
-                           tasks, n_tasks, nodes, far_end, rescales, rel_tol, max_iter, levels,
-                           status, iterations);
+        if (segtab != nullptr)
+            hipLaunchKernelGGL((numerov_refine_segmented_kernel<64, true>), grid, dim3(64), 0, stream, uv,
+                               segtab, pots, tasks, n_tasks, nodes, far_end, rescales, rel_tol,
+                               max_iter, levels, status, iterations);
+        else
+            hipLaunchKernelGGL((numerov_refine_segmented_kernel<64, false>), grid, dim3(64), 0, stream, uv,
+                               segtab, pots, tasks, n_tasks, nodes, far_end, rescales, rel_tol,
+                               max_iter, levels, status, iterations);
         break;
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// Segment table of the 64-lane segmented refinement: for every potential
+// with levels to refine, seg_rows(...) rows of 64 columns from row P.seg_row,
+// column s holding uv[min(q0_s + r, n - 1)] for the segments s < seg_count(n)
+// (zero in the unused columns).
+hipError_t launch_segment_table(hipStream_t stream, const double* uv, const PotentialRecord* pots,
+                                int32_t n_pots, int32_t max_rows, double* segtab)
+{
+    if (n_pots == 0 || max_rows == 0)
+        return hipSuccess;
+    const int32_t bx = (max_rows * 64 + 255) / 256;
+    dim3 grid(static_cast<unsigned>(bx < 1024 ? bx : 1024), static_cast<unsigned>(n_pots));
+    hipLaunchKernelGGL(segment_table_kernel, grid, dim3(256), 0, stream, uv, pots, segtab);
     return hipGetLastError();
 }
 

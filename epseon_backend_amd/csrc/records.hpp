@@ -52,9 +52,31 @@ struct alignas(16) PotentialRecord {
     double far_scale;       // 2^frac((n_points - 2) log2 12): scan far-end values are
                             // v_{N-1} * far_scale, with w_{N-1} = that * 2^(far_exp - 480 r)
     int32_t far_exp;        // floor((n_points - 2) log2 12)
-    int32_t reserved;
+    int32_t seg_row;        // first row of this potential's segment table (rows of 64 values)
 };
 static_assert(sizeof(PotentialRecord) == 160, "record layout");
+
+// Segments of the 64-lane segmented refinement of a potential with n points:
+// the q = n - 2 steps split into min(64, max(1, q / 32)) segments, segment s
+// covering steps [s q / segs, (s + 1) q / segs).
+constexpr int32_t seg_count(const int32_t n_points)
+{
+    const int32_t q = n_points - 2;
+    const int32_t s = q / 32;
+    return s < 1 ? 1 : (s > 64 ? 64 : s);
+}
+
+// Rows of a potential's segment table (row r of column s holds uv at point
+// q0_s + r): the longest segment plus the row of its start point and the 32
+// rows its walk reads ahead; none for a potential without levels to refine.
+constexpr int32_t seg_rows(const int32_t n_points, const int32_t flags, const int32_t level_count)
+{
+    if (flags != 0 || level_count <= 0)
+        return 0;
+    const int32_t q = n_points - 2;
+    const int32_t segs = seg_count(n_points);
+    return (q + segs - 1) / segs + 33;
+}
 
 // Normalisation of one level's eigenfunction, from two inward shots that meet
 // at PotentialRecord::match: psi_i = y_i * ldexp(a, 240 (k - k_final)) with

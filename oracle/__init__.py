@@ -29,6 +29,8 @@ K_CM = 16.857629191640175  # hbar^2/(2 amu A^2) in cm^-1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
+_lp = ctypes.POINTER(ctypes.c_int64)
+BANDS = 8  # energy bands of the shot windows (numerov_oracle.c, EO_BANDS)
 _LIB: ctypes.CDLL | None = None
 
 
@@ -45,6 +47,8 @@ class Bracket(ctypes.Structure):
         ("lo_valid", ctypes.c_int32),
         ("found", ctypes.c_int32),
         ("rate", ctypes.c_double),
+        ("band", ctypes.c_int32),
+        ("lo_foreign", ctypes.c_int32),
     ]
 
 
@@ -82,15 +86,21 @@ def _lib() -> ctypes.CDLL:
         "eo_shoot": (i32, [_dp, i64, d, i32, _dp, _ip]),
         "eo_shoot_full_count": (i32, [_dp, i64, d]),
         "eo_trajectory": (None, [_dp, i64, d, _dp]),
-        "eo_scan": (None, [_dp, i64, d, d, d, i64, i32, _ip, _dp, _ip]),
-        "eo_bracket_level": (None, [_ip, _dp, _ip, i64, d, d, i32, i32, ctypes.POINTER(Bracket)]),
+        "eo_scan": (None, [_dp, i64, d, d, d, i64, i32, _lp, _ip, _dp, _ip]),
+        "eo_bracket_level": (None, [_ip, _dp, _ip, i64, d, d, d, i32, _ip, ctypes.POINTER(Bracket)]),
+        "eo_window": (None, [_dp, i64, d, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+        "eo_band_windows": (None, [_dp, i64, d, d, d, _lp]),
+        "eo_full_windows": (None, [i64, _lp]),
+        "eo_band_top": (d, [i32, d, d]),
+        "eo_group_band": (i32, [i64, i64, d, d, d]),
+        "eo_solve_windows": (None, [_dp, i64, d, d, d, i32, _lp, _ip]),
         "eo_refine_level": (
             i32,
             [_dp, i64, d, i32, i32, ctypes.POINTER(Bracket), d, d, i32, i32, i32, _dp],
         ),
         "eo_solve": (
             None,
-            [_dp, _dp, i64, d, d, d, i64, i32, i32, i32, d, d, i32, i32, i32, _ip, _dp, _ip, _dp, _ip],
+            [_dp, _dp, i64, d, d, d, i64, i32, i32, i32, d, d, i32, i32, i32, i32, _ip, _dp, _ip, _dp, _ip],
         ),
         "eo_segment_count": (i32, [i64, i32]),
         "eo_segment_start": (i64, [i64, i32, i32]),
@@ -214,28 +224,70 @@ def trajectory(ut: np.ndarray, e: float) -> np.ndarray:
     return out
 
 
-def scan(pt: np.ndarray, b: float, e_lo: float, e_hi: float, m: int, stride: int):
-    """(nodes, far-end mantissas, scale-up counts) of every trial energy."""
+def window(uv: np.ndarray, e: float) -> tuple[int, int]:
+    """(lo, hi): Dirichlet points of the shot window at probe e = beta*E."""
+    uv = _f64(uv)
+    lo = ctypes.c_int64()
+    hi = ctypes.c_int64()
+    _lib().eo_window(uv.ctypes.data_as(_dp), len(uv), e, ctypes.byref(lo), ctypes.byref(hi))
+    return int(lo.value), int(hi.value)
+
+
+def band_windows(uv: np.ndarray, b: float, e_lo: float, e_hi: float, windows: bool = True) -> np.ndarray:
+    """(BANDS, 2) int64 array of the band windows (the whole grid in every
+    band when ``windows`` is False)."""
+    uv = _f64(uv)
+    win = np.empty((BANDS, 2), dtype=np.int64)
+    far = np.empty(BANDS, dtype=np.int32)
+    _lib().eo_solve_windows(uv.ctypes.data_as(_dp), len(uv), b, e_lo, e_hi, int(windows),
+                            win.ctypes.data_as(_lp), far.ctypes.data_as(_ip))
+    return win
+
+
+def band_far_exps(win: np.ndarray) -> np.ndarray:
+    """Far-end exponents of the band windows."""
+    return np.array([far_scale(int(hi - lo + 1))[1] for lo, hi in win], dtype=np.int32)
+
+
+def band_top(band: int, e_lo: float, e_hi: float) -> float:
+    return float(_lib().eo_band_top(band, e_lo, e_hi))
+
+
+def group_band(j: int, m: int, e_lo: float, e_hi: float) -> int:
+    """Band of the scan group (64 trial energies) holding trial energy j."""
+    return int(_lib().eo_group_band(j, m, e_lo, trial_spacing(e_lo, e_hi, m), e_hi))
+
+
+def scan(pt: np.ndarray, b: float, e_lo: float, e_hi: float, m: int, stride: int,
+         win: np.ndarray | None = None):
+    """(nodes, far-end mantissas, scale-up counts) of every trial energy, each
+    shot in its group's band window (``win``; the whole grid when None)."""
     ut = _f64(pt)
+    if win is None:
+        win = np.array([[0, len(ut) - 1]] * BANDS, dtype=np.int64)
+    win = np.ascontiguousarray(win, dtype=np.int64)
     nodes = np.empty(m, dtype=np.int32)
     far = np.empty(m, dtype=np.float64)
     resc = np.empty(m, dtype=np.int32)
     _lib().eo_scan(
-        ut.ctypes.data_as(_dp), len(ut), b, e_lo, e_hi, m, stride,
+        ut.ctypes.data_as(_dp), len(ut), b, e_lo, e_hi, m, stride, win.ctypes.data_as(_lp),
         nodes.ctypes.data_as(_ip), far.ctypes.data_as(_dp), resc.ctypes.data_as(_ip),
     )
     return nodes, far, resc
 
 
-def bracket_level(nodes, far, resc, e_lo: float, e_hi: float, v: int, far_exp: int) -> Bracket:
+def bracket_level(nodes, far, resc, e_lo: float, e_hi: float, v: int, far_exp) -> Bracket:
+    """Bracket of level v from scan arrays; ``far_exp`` is one far-end
+    exponent for every band (whole-grid scans) or one per band."""
     nodes = np.ascontiguousarray(nodes, dtype=np.int32)
     far = _f64(far)
     resc = np.ascontiguousarray(resc, dtype=np.int32)
+    fe = np.ascontiguousarray(np.broadcast_to(np.asarray(far_exp, dtype=np.int32), (BANDS,)))
     br = Bracket()
     m = len(nodes)
     _lib().eo_bracket_level(
         nodes.ctypes.data_as(_ip), far.ctypes.data_as(_dp), resc.ctypes.data_as(_ip), m,
-        e_lo, trial_spacing(e_lo, e_hi, m), v, far_exp, ctypes.byref(br),
+        e_lo, trial_spacing(e_lo, e_hi, m), e_hi, v, fe.ctypes.data_as(_ip), ctypes.byref(br),
     )
     return br
 
@@ -323,6 +375,36 @@ class Problem:
     def abs_tol(self) -> float:
         return 1e-15 * (self.e_top - self.e_lo)
 
+    def windows(self, on: bool = True) -> np.ndarray:
+        """(BANDS, 2) band windows (lo, hi) on the trimmed grid."""
+        return band_windows(self.uv, self.beta, self.e_lo, self.e_top, on)
+
+    def far_exps(self, on: bool = True) -> np.ndarray:
+        """Far-end exponents of the band windows."""
+        return band_far_exps(self.windows(on))
+
+    def scan(self, on: bool = True):
+        """(nodes, far, resc) of the windowed (or whole-grid) scan."""
+        return scan(self.pt, self.beta, self.e_lo, self.e_top, self.m, self.stride, self.windows(on))
+
+    def scan_shot(self, j: int, on: bool = True) -> tuple[int, float, int]:
+        """The scan's shot of trial energy j: (nodes, far-end mantissa, r)."""
+        lo, hi = self.windows(on)[group_band(j, self.m, self.e_lo, self.e_top)]
+        e = self.beta * float(trial_energies(self.e_lo, self.e_top, self.m)[j])
+        return shoot(self.pt[lo : hi + 1], e, self.stride)
+
+    def refine(self, nodes, far, resc, v: int, rel_tol: float = 1e-10, max_iter: int = 200,
+               segmented: bool = False, lanes: int = 64, on: bool = True) -> tuple[float, int]:
+        """Refine level v from scan arrays as the solve does (bracket, band
+        window, refinement): (level or NaN, iterations)."""
+        win = self.windows(on)
+        br = bracket_level(nodes, far, resc, self.e_lo, self.e_top, v, band_far_exps(win))
+        if not br.found:
+            return float("nan"), 0
+        lo, hi = win[br.band]
+        return refine_level(self.uv[lo : hi + 1], self.beta, self.stride, v, br, rel_tol, self.abs_tol,
+                            max_iter, segmented, lanes)
+
     @property
     def match(self) -> int:
         """Matching point of the two inward eigenfunction shots (min V,
@@ -350,10 +432,11 @@ def shoot_segmented(uv: np.ndarray, e: float, stride: int, count: bool = True, l
 
 
 def solve(prob: Problem, v_min: int, n_levels: int, rel_tol: float = 1e-10, max_iter: int = 200,
-          segmented: bool = False, lanes: int = 64):
+          segmented: bool = False, lanes: int = 64, windows: bool = True):
     """Scan + bracket + refine, as epseon_solve does: returns
     (nodes, far, resc, levels, iterations).  ``segmented`` selects the
-    refinement with segmented shots over blocks of ``lanes`` threads."""
+    refinement with segmented shots over blocks of ``lanes`` threads;
+    ``windows`` the band-windowed shots (else every shot spans the grid)."""
     ut = prob.pt
     uv = prob.uv
     m = prob.m
@@ -364,7 +447,7 @@ def solve(prob: Problem, v_min: int, n_levels: int, rel_tol: float = 1e-10, max_
     it = np.empty(max(n_levels, 1), dtype=np.int32)
     _lib().eo_solve(
         ut.ctypes.data_as(_dp), uv.ctypes.data_as(_dp), len(ut), prob.beta, prob.e_lo, prob.e_top, m, prob.stride,
-        v_min, n_levels, rel_tol, prob.abs_tol, max_iter, int(segmented), lanes,
+        v_min, n_levels, rel_tol, prob.abs_tol, max_iter, int(segmented), lanes, int(windows),
         nodes.ctypes.data_as(_ip), far.ctypes.data_as(_dp), resc.ctypes.data_as(_ip),
         lv.ctypes.data_as(_dp), it.ctypes.data_as(_ip),
     )

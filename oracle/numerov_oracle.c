@@ -287,15 +287,125 @@ void eo_trajectory(const double* ut, int64_t n, double e, double* w_out)
 
 /* ------------------------------------------------------------------ scan */
 
-/* Shoot every trial energy E_j, j < m, of the window [e_lo, e_hi]. */
-void eo_scan(const double* pt, int64_t n, double beta, double e_lo, double e_hi,
-             int64_t m, int32_t stride, int32_t* nodes, double* w_last, int32_t* rescales)
+/* ---------------------------------------------------------------- windows
+ *
+ * Past its outermost turning points a shot only grows (or, from the left
+ * wall, only decays towards the well) at a rate of at least
+ * acosh(1 + x/2) per step, x_i = 12 (uv_i - e) > 0 (for a locally constant x
+ * the recurrence's dominant root lambda has lambda + 1/lambda >= 2 + x).
+ * Moving a Dirichlet point into that region from depth S to depth S' > S
+ * changes the eigenvalues only by a relative e^(-2S)-order amount, so shots
+ * may stop where the guaranteed depth reaches EO_WINDOW_DEPTH = 20 e-folds
+ * (e^-40 ~ 4e-18 of a level spacing).  The depth is summed in 2^-40 fixed
+ * point from the exact lower bound g(x) = 2^(floor((k-1)/2) - 1) of
+ * acosh(1 + x/2) for x = m 2^k, m in [1/2, 1), x < 6 (which the wall trim
+ * guarantees), so the sum is exact and independent of order.
+ *
+ * Window of a probe e (= beta E) on the n-point trimmed grid: i_first and
+ * i_last are the first and last points with uv_i <= e; hi is the first
+ * i > i_last at which sum_{i_last < j <= i} g(x_j) >= 20 (else n - 1), lo the
+ * last i < i_first at which sum_{i <= j < i_first} g(x_j) >= 20 (else 0).
+ *
+ * Energy bands: the window [e_lo, e_hi] splits into EO_BANDS bands with tops
+ * E_b = fma(b + 1, (e_hi - e_lo)/EO_BANDS, e_lo); band b's window is the
+ * window at E_b (windows only grow with E).  The scan shoots trial energies in
+ * groups of 64 (j = 64 g .. 64 g + 63); a group uses the window of the band of
+ * its highest energy (the first b with E_b >= E_top, else the last band),
+ * and a level is refined in the window of the group of its bracket's upper
+ * scan point.  Windows off = every band's window is the whole grid. */
+#define EO_BANDS 8
+#define EO_WINDOW_DEPTH 20
+
+static int64_t eo_growth_term(double x)
 {
+    int k;
+    (void)frexp(x, &k);
+    int s = ((k - 1) >> 1) - 1 + 40;
+    if (s < 0)
+        return 0;
+    if (s > 48)
+        s = 48;
+    return (int64_t)1 << s;
+}
+
+void eo_window(const double* uv, int64_t n, double e, int64_t* lo, int64_t* hi)
+{
+    int64_t first = -1, last = -1;
+    for (int64_t i = 0; i < n; ++i)
+        if (uv[i] <= e) {
+            if (first < 0)
+                first = i;
+            last = i;
+        }
+    *lo = 0;
+    *hi = n - 1;
+    if (first < 0)
+        return;
+    const int64_t depth = (int64_t)EO_WINDOW_DEPTH << 40;
+    int64_t acc = 0;
+    for (int64_t i = last + 1; i < n; ++i) {
+        acc += eo_growth_term(12.0 * (uv[i] - e));
+        if (acc >= depth) {
+            *hi = i;
+            break;
+        }
+    }
+    acc = 0;
+    for (int64_t i = first - 1; i >= 0; --i) {
+        acc += eo_growth_term(12.0 * (uv[i] - e));
+        if (acc >= depth) {
+            *lo = i;
+            break;
+        }
+    }
+}
+
+double eo_band_top(int32_t b, double e_lo, double e_hi)
+{
+    return fma((double)(b + 1), (e_hi - e_lo) / EO_BANDS, e_lo);
+}
+
+/* win[2 b], win[2 b + 1] = (lo, hi) of band b. */
+void eo_band_windows(const double* uv, int64_t n, double beta, double e_lo, double e_hi, int64_t* win)
+{
+    for (int32_t b = 0; b < EO_BANDS; ++b)
+        eo_window(uv, n, beta * eo_band_top(b, e_lo, e_hi), &win[2 * b], &win[2 * b + 1]);
+}
+
+void eo_full_windows(int64_t n, int64_t* win)
+{
+    for (int32_t b = 0; b < EO_BANDS; ++b) {
+        win[2 * b] = 0;
+        win[2 * b + 1] = n - 1;
+    }
+}
+
+int32_t eo_group_band(int64_t j, int64_t m, double e_lo, double de, double e_hi)
+{
+    int64_t jt = (j / 64) * 64 + 63;
+    if (jt > m - 1)
+        jt = m - 1;
+    const double et = eo_trial_energy(jt, e_lo, de);
+    for (int32_t b = 0; b < EO_BANDS - 1; ++b)
+        if (eo_band_top(b, e_lo, e_hi) >= et)
+            return b;
+    return EO_BANDS - 1;
+}
+
+/* Shoot every trial energy E_j, j < m, of the window [e_lo, e_hi], each in
+ * its group's band window win (2 EO_BANDS entries). */
+void eo_scan(const double* pt, int64_t n, double beta, double e_lo, double e_hi,
+             int64_t m, int32_t stride, const int64_t* win, int32_t* nodes, double* w_last,
+             int32_t* rescales)
+{
+    (void)n;
     const double de = eo_trial_spacing(e_lo, e_hi, m);
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t j = 0; j < m; ++j) {
+        const int32_t b = eo_group_band(j, m, e_lo, de, e_hi);
+        const int64_t lo = win[2 * b], hi = win[2 * b + 1];
         const double e = beta * eo_trial_energy(j, e_lo, de);
-        nodes[j] = eo_shoot(pt, n, e, stride, &w_last[j], &rescales[j]);
+        nodes[j] = eo_shoot(pt + lo, hi - lo + 1, e, stride, &w_last[j], &rescales[j]);
     }
 }
 
@@ -309,6 +419,8 @@ typedef struct eo_bracket {
     int32_t lo_valid;    /* 0 when e_lo is the window floor (no shot there) */
     int32_t found;       /* 0 when no trial energy has more than v nodes */
     double rate;         /* b of the model F ~ (c0 + c1 E) e^(b E) (0: none) */
+    int32_t band;        /* band of the upper end's scan group: the level's window */
+    int32_t lo_foreign;  /* 1 when the lower end's value comes from another band's window */
 } eo_bracket;
 
 /* ---- deterministic exp / log for the refinement model -----------------
@@ -411,9 +523,14 @@ static void eo_model_apply(double b, double e0, double e, double* w, int32_t* k)
 
 /* Level v lies between the first trial energy with more than v nodes and the
  * trial energy before it (or the window floor, below which no level exists
- * because every eigenvalue exceeds min V). */
+ * because every eigenvalue exceeds min V).  far_exp[b] is the far-end
+ * exponent of band b's window.  Scan values from a group of another band
+ * than the upper end's are not comparable with the level's shots: such a
+ * lower end is flagged (lo_foreign) and a third point of the model must lie
+ * in the same band. */
 void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t* rescales,
-                      int64_t m, double e_lo, double de, int32_t v, int32_t far_exp, eo_bracket* out)
+                      int64_t m, double e_lo, double de, double e_hi, int32_t v,
+                      const int32_t* far_exp, eo_bracket* out)
 {
     int64_t lo = 0, hi = m;
     while (lo < hi) {
@@ -429,32 +546,38 @@ void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t*
         out->found = 0;
         return;
     }
+    const int32_t band = eo_group_band(j, m, e_lo, de, e_hi);
+    const int32_t fe = far_exp[band];
     out->found = 1;
+    out->band = band;
     out->e_hi = eo_trial_energy(j, e_lo, de);
     out->c_hi = nodes[j];
     out->w_hi = w_last[j];
-    out->k_hi = far_exp - rescales[j] * 480;
+    out->k_hi = fe - rescales[j] * 480;
     out->rate = 0.0;
     if (j == 0 || nodes[j - 1] > v) {
         out->e_lo = e_lo;
         out->c_lo = 0;
         out->lo_valid = 0;
     } else {
+        const int32_t bl = eo_group_band(j - 1, m, e_lo, de, e_hi);
         out->e_lo = eo_trial_energy(j - 1, e_lo, de);
         out->c_lo = nodes[j - 1];
         out->w_lo = w_last[j - 1];
-        out->k_lo = far_exp - rescales[j - 1] * 480;
+        out->k_lo = far_exp[bl] - rescales[j - 1] * 480;
         out->lo_valid = 1;
+        out->lo_foreign = bl != band;
     }
     /* Model rate from a third scan point in the same count regime: above
-     * (j + 1, same count as j) or below (j - 2, same count as j - 1). */
-    if (out->lo_valid && out->c_lo == v && out->c_hi == v + 1) {
-        if (j + 1 < m && nodes[j + 1] == out->c_hi)
+     * (j + 1, same count as j) or below (j - 2, same count as j - 1), all
+     * three in the level's band. */
+    if (out->lo_valid && !out->lo_foreign && out->c_lo == v && out->c_hi == v + 1) {
+        if (j + 1 < m && nodes[j + 1] == out->c_hi && eo_group_band(j + 1, m, e_lo, de, e_hi) == band)
             out->rate = eo_model_rate(out->w_lo, out->k_lo, out->w_hi, out->k_hi, w_last[j + 1],
-                                      far_exp - rescales[j + 1] * 480, 1, de);
-        else if (j >= 2 && nodes[j - 2] == out->c_lo)
-            out->rate = eo_model_rate(w_last[j - 2], far_exp - rescales[j - 2] * 480, out->w_lo,
-                                      out->k_lo, out->w_hi, out->k_hi, 0, de);
+                                      fe - rescales[j + 1] * 480, 1, de);
+        else if (j >= 2 && nodes[j - 2] == out->c_lo && eo_group_band(j - 2, m, e_lo, de, e_hi) == band)
+            out->rate = eo_model_rate(w_last[j - 2], fe - rescales[j - 2] * 480, out->w_lo, out->k_lo,
+                                      out->w_hi, out->k_hi, 0, de);
     }
 }
 
@@ -740,6 +863,7 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
     int32_t kl = br->k_lo, kh = br->k_hi; /* binary exponents */
     int32_t cl = br->c_lo, ch = br->c_hi;
     int32_t lo_ok = br->lo_valid;
+    int32_t lo_foreign = br->lo_foreign;
     int32_t side = 0;
     int32_t it = 0;
     /* With a model rate b, regula falsi runs on G = F e^(-b (E - E_hi0)),
@@ -761,7 +885,12 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
         }
         double em;
         const int32_t isolated = lo_ok && cl == v && ch == v + 1;
-        if (isolated) {
+        const int32_t reshoot = isolated && lo_foreign;
+        if (reshoot) {
+            /* The floor's value came from another band's window: shoot it
+             * again in this level's window (its count stays the scan's). */
+            em = el;
+        } else if (isolated) {
             const double r = ldexp(wh / wl, kh - kl);
             const double s = 1.0 / (1.0 - r);
             em = fma(s, width, el);
@@ -788,7 +917,7 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
             k *= 480;
         }
         if (isolated)
-            c = ((int32_t)eo_signbit(w) == (v & 1)) ? v : v + 1;
+            c = reshoot ? v : (((int32_t)eo_signbit(w) == (v & 1)) ? v : v + 1);
         if (rate != 0.0)
             eo_model_apply(rate, e0, em, &w, &k);
         ++it;
@@ -800,7 +929,7 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
                 const double m = 1.0 - ldexp(w / wl, k - kl);
                 wh *= (m > 0.0) ? m : 0.5;
             }
-            el = em; cl = c; wl = w; kl = k; lo_ok = 1;
+            el = em; cl = c; wl = w; kl = k; lo_ok = 1; lo_foreign = 0;
             side = isolated ? -1 : 0;
         } else {
             if (isolated && side == 1) {
@@ -815,29 +944,45 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
     return it;
 }
 
+/* Band windows and their far-end exponents: the whole grid in every band
+ * when windows == 0. */
+void eo_solve_windows(const double* uv, int64_t n, double beta, double e_lo, double e_hi,
+                      int32_t windows, int64_t* win, int32_t* far_exp)
+{
+    if (windows)
+        eo_band_windows(uv, n, beta, e_lo, e_hi, win);
+    else
+        eo_full_windows(n, win);
+    for (int32_t b = 0; b < EO_BANDS; ++b) {
+        double fc;
+        eo_far_scale(win[2 * b + 1] - win[2 * b] + 1, &fc, &far_exp[b]);
+    }
+}
+
 /* Scan + bracket + refine of levels [v_min, v_min + n_levels) for one
  * potential.  levels[] gets NaN for levels that are not below e_hi. */
 void eo_solve(const double* pt, const double* uv, int64_t n, double beta, double e_lo, double e_hi,
               int64_t m, int32_t stride, int32_t v_min, int32_t n_levels, double rel_tol,
-              double abs_tol, int32_t max_iter, int32_t segmented, int32_t lanes, int32_t* nodes,
-              double* w_last, int32_t* rescales, double* levels, int32_t* iterations)
+              double abs_tol, int32_t max_iter, int32_t segmented, int32_t lanes, int32_t windows,
+              int32_t* nodes, double* w_last, int32_t* rescales, double* levels, int32_t* iterations)
 {
-    eo_scan(pt, n, beta, e_lo, e_hi, m, stride, nodes, w_last, rescales);
+    int64_t win[2 * EO_BANDS];
+    int32_t far_exp[EO_BANDS];
+    eo_solve_windows(uv, n, beta, e_lo, e_hi, windows, win, far_exp);
+    eo_scan(pt, n, beta, e_lo, e_hi, m, stride, win, nodes, w_last, rescales);
     const double de = eo_trial_spacing(e_lo, e_hi, m);
-    double fc;
-    int32_t far_exp;
-    eo_far_scale(n, &fc, &far_exp);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int32_t l = 0; l < n_levels; ++l) {
         eo_bracket br;
-        eo_bracket_level(nodes, w_last, rescales, m, e_lo, de, v_min + l, far_exp, &br);
+        eo_bracket_level(nodes, w_last, rescales, m, e_lo, de, e_hi, v_min + l, far_exp, &br);
         if (!br.found) {
             levels[l] = NAN;
             iterations[l] = 0;
             continue;
         }
-        iterations[l] = eo_refine_level(uv, n, beta, stride, v_min + l, &br, rel_tol, abs_tol,
-                                        max_iter, segmented, lanes, &levels[l]);
+        const int64_t lo = win[2 * br.band], hi = win[2 * br.band + 1];
+        iterations[l] = eo_refine_level(uv + lo, hi - lo + 1, beta, stride, v_min + l, &br, rel_tol,
+                                        abs_tol, max_iter, segmented, lanes, &levels[l]);
     }
 }
 

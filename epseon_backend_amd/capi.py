@@ -117,6 +117,8 @@ class Stats(ctypes.Structure):
         ("vjp_point_updates", ctypes.c_double),
         ("solve_ms", ctypes.c_double),
         ("solve_split", ctypes.c_double),
+        ("scan_windows", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -127,7 +129,7 @@ class Tuning(ctypes.Structure):
         ("scan_waves_per_block", ctypes.c_int32),
         ("segmented_lanes", ctypes.c_int32),
         ("xcd_scan", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("windows", ctypes.c_int32),
         ("solve_split", ctypes.c_double),
     ]
 
@@ -136,6 +138,7 @@ class Tuning(ctypes.Structure):
 
 
 TUNE_KEEP_RESULTS = 1
+BANDS = 8  # energy bands of the shot windows (EPSEON_BANDS)
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int32_p = ctypes.POINTER(ctypes.c_int32)
@@ -146,6 +149,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "epseon_last_error": (ctypes.c_char_p, []),
     "epseon_struct_sizes": (ctypes.c_int, [_c_int64_p, ctypes.c_int32]),
     "epseon_device_count": (ctypes.c_int, [_c_int32_p]),
+    "epseon_band_windows_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _c_int32_p, _c_int32_p, _c_int32_p]),
     "epseon_device_info_get": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(DeviceInfo)]),
     "epseon_ctx_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     "epseon_ctx_destroy": (ctypes.c_int, [ctypes.c_void_p]),
@@ -368,6 +372,16 @@ class Context:
         info = PotentialInfo()
         _check(self._lib.epseon_potential_info_get(self._h, p, ctypes.byref(info)))
         return info
+
+    def band_windows(self, p: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(lo, hi, far_exponent) of potential p's BANDS shot windows (those
+        of the last scan)."""
+        lo = np.empty(BANDS, dtype=np.int32)
+        hi = np.empty(BANDS, dtype=np.int32)
+        fe = np.empty(BANDS, dtype=np.int32)
+        _check(self._lib.epseon_band_windows_get(self._h, p, lo.ctypes.data_as(_c_int32_p),
+                                                 hi.ctypes.data_as(_c_int32_p), fe.ctypes.data_as(_c_int32_p)))
+        return lo, hi, fe
 
     def scan(self) -> None:
         _check(self._lib.epseon_scan(self._h))

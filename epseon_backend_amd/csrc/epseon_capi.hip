@@ -23,21 +23,26 @@
 
 namespace epseon::hip {
 
+static_assert(kBands == EPSEON_BANDS, "band count of the C ABI");
+
 hipError_t launch_shift_potential(hipStream_t, const double*, const PotentialRecord*, int32_t,
                                   int32_t, double*, double*, double*);
 hipError_t launch_potential_stats(hipStream_t, const double*, const int64_t*, const int64_t*,
                                   const double*, int32_t, PotentialStats*);
-hipError_t launch_scan(hipStream_t, const double*, const PotentialRecord*, const WaveTask*, int32_t,
-                       const int32_t*, int32_t, int32_t, int32_t*, double*, int32_t*);
-hipError_t launch_refine(hipStream_t, int32_t, const double*, const PotentialRecord*,
+hipError_t launch_scan(hipStream_t, const double*, const PotentialRecord*, const BandWindow*,
+                       const WaveTask*, int32_t, const int32_t*, int32_t, int32_t, int32_t*, double*,
+                       int32_t*);
+hipError_t launch_refine(hipStream_t, int32_t, const double*, const PotentialRecord*, const BandWindow*,
                          const WaveTask*, int32_t, int32_t, const int32_t*, const double*,
-                         const int32_t*, double, int32_t, double*, int32_t*, int32_t*);
+                         const int32_t*, double, int32_t, double*, int32_t*, int32_t*, int32_t*);
 hipError_t launch_refine_segmented(hipStream_t, int32_t, const double*, const double*,
-                                   const PotentialRecord*, const WaveTask*, int32_t, const int32_t*,
-                                   const double*, const int32_t*, double, int32_t, double*, int32_t*,
+                                   const PotentialRecord*, const BandWindow*, const WaveTask*, int32_t,
+                                   const int32_t*, int32_t, const int32_t*, const double*,
+                                   const int32_t*, double, int32_t, double*, int32_t*, int32_t*,
                                    int32_t*);
-hipError_t launch_segment_table(hipStream_t, const double*, const PotentialRecord*, int32_t, int32_t,
-                                double*);
+hipError_t launch_segment_tables(hipStream_t, const double*, const PotentialRecord*, const TableJob*,
+                                 int32_t, int32_t, double*);
+hipError_t launch_band_windows(hipStream_t, const double*, const PotentialRecord*, int32_t, int2*);
 hipError_t launch_normalize(hipStream_t, const double*, const double*, const PotentialRecord*,
                             const WaveTask*, int32_t, const double*, const int32_t*, LevelNorm*);
 hipError_t launch_eigenfunctions(hipStream_t, const double*, const double*, const PotentialRecord*,
@@ -152,6 +157,9 @@ hipError_t arena_array(Arena& a, size_t count, T** out)
 // per level.
 constexpr int64_t kLanePerLevelMinLevels = int64_t(64) * 2048;
 
+// Automatic windows need at least this many scan wavefronts per SIMD.
+constexpr int64_t kWindowsMinScanWavesPerSimd = 8;
+
 // Node-sampling stride: the largest power of two B <= 32 with B * theta_max
 // <= pi/2, theta_max being the discrete phase advance per step at the
 // highest kinetic energy of the window.
@@ -199,11 +207,22 @@ struct epseon_ctx {
     double* d_ut = nullptr;
     double* d_uv = nullptr;
     double* d_uvr = nullptr;
-    // Segment table of the 64-lane segmented refinement (seg_rows per
-    // potential, rows of 64 values): lane s of a level's wavefront reads
-    // column s, so its loads are coalesced.
-    double* d_segtab = nullptr;
-    bool segtab_ok = false;  // every potential's table fits one 32-bit-addressed buffer
+    // Band windows (kBands per potential, in potential order): the
+    // whole grid in every band (d_win_full) and the windowed ones
+    // (d_win_band, computed on the device at load).  A scan and the
+    // refinement that follows it use the same set.
+    BandWindow* d_win_full = nullptr;
+    BandWindow* d_win_band = nullptr;
+    int32_t windows = 2;       // tuning: 1 windowed shots, 0 whole grid, 2 auto
+    int compute_units = 0;
+    bool scan_windows = true;  // the set the last scan used
+    // Segment tables of the 64-lane segmented refinement, one per distinct
+    // window of a potential with levels to refine (rows of 64 values): lane
+    // s of a level's wavefront reads column s, so its loads are coalesced.
+    double* d_segtab_full = nullptr;
+    double* d_segtab_band = nullptr;
+    bool segtab_ok = false;  // every table fits one 32-bit-addressed buffer
+    int32_t* d_steps = nullptr;  // recurrence steps of each level's shots (its window's n - 2)
     PotentialRecord* d_rec = nullptr;
     WaveTask* d_scan_tasks = nullptr;
     // Scan tasks split into one bin per XCD (whole potentials, balanced by
@@ -211,6 +230,12 @@ struct epseon_ctx {
     WaveTask* d_xcd_scan_tasks = nullptr;
     int32_t* d_xcd_bins = nullptr;
     int32_t xcd_max_bin = 0;
+    // Segmented-refinement level tasks binned per XCD the same way (weight
+    // n_points per level), so the levels of a potential read its tables
+    // through one L2.
+    WaveTask* d_xcd_level_tasks = nullptr;
+    int32_t* d_xcd_level_bins = nullptr;
+    int32_t xcd_max_level_bin = 0;
     bool xcd_scan = true;
     WaveTask* d_refine_tasks = nullptr;
     WaveTask* d_level_tasks = nullptr;
@@ -258,7 +283,10 @@ struct epseon_ctx {
 
     bool loaded = false, scanned = false, refined = false, normalized = false;
     bool scan_timed = false, refine_timed = false, normalize_timed = false, vjp_timed = false;
-    double scan_point_updates = 0.0;
+    double scan_point_updates = 0.0;       // of the last scan
+    double scan_pu_full = 0.0, scan_pu_band = 0.0;
+    std::vector<int32_t> h_steps;
+    std::vector<BandWindow> h_win_full, h_win_band;
 
     explicit epseon_ctx(size_t block_bytes) : arena(block_bytes) {}
 };
@@ -386,6 +414,8 @@ int epseon_ctx_create(int32_t device, uint64_t block_bytes, epseon_ctx** out)
         e = hipEventCreate(&ctx->ev_solve0);
     if (e == hipSuccess)
         e = hipEventCreate(&ctx->ev_solve1);
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&ctx->compute_units, hipDeviceAttributeMultiprocessorCount, device);
     if (e != hipSuccess) {
         const int rc = hip_fail(e, "context creation");
         epseon_ctx_destroy(ctx);
@@ -426,6 +456,54 @@ namespace {
 
 // Shared body of epseon_load_potentials (v_host) and
 // epseon_load_potentials_device (v_dev).
+// Per-XCD bins of a task list (potentials in task order, potential k's
+// tasks at [cum[k], cum[k + 1]), each worth work[k]).  Each potential's tasks
+// are cut into chunks of at most ceil(tasks / 32) (so a single large
+// potential still spreads over every XCD, while typical batch members stay
+// whole); chunks go by decreasing work to the least-loaded bin, and each bin
+// keeps the task order.  Returns the concatenated bins; `bins` gets the 9
+// offsets and `max_bin` the largest bin's size.
+std::vector<WaveTask> xcd_binned(const std::vector<WaveTask>& tasks, const std::vector<int32_t>& cum,
+                                 const std::vector<double>& work, std::vector<int32_t>& bins, int32_t& max_bin)
+{
+    struct Chunk {
+        int32_t t0, t1;
+        double work;
+    };
+    std::vector<Chunk> chunks;
+    const int32_t cmax = std::max<int32_t>(1, (static_cast<int32_t>(tasks.size()) + 31) / 32);
+    for (size_t k = 0; k + 1 < cum.size(); ++k)
+        for (int32_t t = cum[k]; t < cum[k + 1]; t += cmax) {
+            const int32_t e = std::min(cum[k + 1], t + cmax);
+            chunks.push_back({t, e, work[k] * (e - t)});
+        }
+    std::vector<size_t> by_work(chunks.size());
+    std::iota(by_work.begin(), by_work.end(), size_t(0));
+    std::stable_sort(by_work.begin(), by_work.end(),
+                     [&](size_t a, size_t b) { return chunks[a].work > chunks[b].work; });
+    std::vector<int32_t> chunk_bin(chunks.size(), 0);
+    double load[8] = {};
+    for (size_t c : by_work) {
+        const int x = static_cast<int>(std::min_element(load, load + 8) - load);
+        chunk_bin[c] = x;
+        load[x] += chunks[c].work;
+    }
+    std::vector<WaveTask> out;
+    out.reserve(tasks.size());
+    bins.assign(9, 0);
+    max_bin = 0;
+    for (int x = 0; x < 8; ++x) {
+        bins[static_cast<size_t>(x)] = static_cast<int32_t>(out.size());
+        for (size_t c = 0; c < chunks.size(); ++c)  // chunks are in task order
+            if (chunk_bin[c] == x)
+                for (int32_t t = chunks[c].t0; t < chunks[c].t1; ++t)
+                    out.push_back(tasks[static_cast<size_t>(t)]);
+        max_bin = std::max(max_bin, static_cast<int32_t>(out.size()) - bins[static_cast<size_t>(x)]);
+    }
+    bins[8] = static_cast<int32_t>(out.size());
+    return out;
+}
+
 int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc, const double* v_host,
                const double* v_dev, const int64_t* v_offsets)
 {
@@ -617,7 +695,7 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     ctx->total_energies = en;
     ctx->total_levels = lv;
     ctx->max_points = max_points;
-    ctx->scan_point_updates = scan_pu;
+    ctx->scan_pu_full = scan_pu;
     ctx->user_span = user_span;
     ctx->psi_total = psi_total;
 
@@ -666,52 +744,17 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
             (r.flags == EPSEON_POT_OK ? static_cast<double>(r.n_points) * r.n_energies : 0.0));
     }
     ctx->n_scan_tasks = static_cast<int32_t>(scan_tasks.size());
-    // Per-XCD bins.  Each potential's scan tasks are cut into chunks of at
-    // most ceil(tasks / 32) (so a single large potential still spreads over
-    // every XCD, while typical batch members stay whole); chunks go by
-    // decreasing work to the least-loaded bin, and each bin keeps the
-    // longest-grid-first order.
-    struct Chunk {
-        int32_t k, t0, t1;  // position in task order, task range
-        double work;
-    };
-    std::vector<Chunk> chunks;
-    {
-        const int32_t cmax = std::max<int32_t>(1, (ctx->n_scan_tasks + 31) / 32);
-        for (size_t k = 0; k + 1 < ctx->cum_scan_tasks.size(); ++k) {
-            const int32_t a0 = ctx->cum_scan_tasks[k], a1 = ctx->cum_scan_tasks[k + 1];
-            const double per_task = a1 > a0 ? (ctx->cum_scan_work[k + 1] - ctx->cum_scan_work[k]) / (a1 - a0) : 0.0;
-            for (int32_t t = a0; t < a1; t += cmax) {
-                const int32_t e = std::min(a1, t + cmax);
-                chunks.push_back({static_cast<int32_t>(k), t, e, per_task * (e - t)});
-            }
-        }
+    std::vector<double> scan_task_work(ctx->cum_scan_tasks.size() - 1), level_task_work(scan_task_work.size());
+    for (size_t k = 0; k < scan_task_work.size(); ++k) {
+        const int32_t a0 = ctx->cum_scan_tasks[k], a1 = ctx->cum_scan_tasks[k + 1];
+        scan_task_work[k] = a1 > a0 ? (ctx->cum_scan_work[k + 1] - ctx->cum_scan_work[k]) / (a1 - a0) : 0.0;
+        level_task_work[k] = static_cast<double>(ctx->rec[static_cast<size_t>(order[k])].n_points);
     }
-    std::vector<size_t> by_work(chunks.size());
-    std::iota(by_work.begin(), by_work.end(), size_t(0));
-    std::stable_sort(by_work.begin(), by_work.end(),
-                     [&](size_t a, size_t b) { return chunks[a].work > chunks[b].work; });
-    std::vector<int32_t> chunk_bin(chunks.size(), 0);
-    {
-        double load[8] = {};
-        for (size_t c : by_work) {
-            const int x = static_cast<int>(std::min_element(load, load + 8) - load);
-            chunk_bin[c] = x;
-            load[x] += chunks[c].work;
-        }
-    }
-    std::vector<WaveTask> xcd_tasks;
-    std::vector<int32_t> xcd_bins(9, 0);
-    ctx->xcd_max_bin = 0;
-    for (int x = 0; x < 8; ++x) {
-        xcd_bins[x] = static_cast<int32_t>(xcd_tasks.size());
-        for (size_t c = 0; c < chunks.size(); ++c)  // chunks are in task order
-            if (chunk_bin[c] == x)
-                for (int32_t t = chunks[c].t0; t < chunks[c].t1; ++t)
-                    xcd_tasks.push_back(scan_tasks[static_cast<size_t>(t)]);
-        ctx->xcd_max_bin = std::max(ctx->xcd_max_bin, static_cast<int32_t>(xcd_tasks.size()) - xcd_bins[x]);
-    }
-    xcd_bins[8] = static_cast<int32_t>(xcd_tasks.size());
+    std::vector<int32_t> xcd_bins, xcd_level_bins;
+    const std::vector<WaveTask> xcd_tasks =
+        xcd_binned(scan_tasks, ctx->cum_scan_tasks, scan_task_work, xcd_bins, ctx->xcd_max_bin);
+    const std::vector<WaveTask> xcd_level_tasks =
+        xcd_binned(level_tasks, ctx->cum_level_tasks, level_task_work, xcd_level_bins, ctx->xcd_max_level_bin);
     ctx->n_refine_tasks = static_cast<int32_t>(refine_tasks.size());
     ctx->n_level_tasks = static_cast<int32_t>(level_tasks.size());
 
@@ -728,11 +771,19 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_ut));
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_uv));
     EPSEON_HIP_TRY(arena_array(a, np, &ctx->d_uvr));
-    EPSEON_HIP_TRY(arena_array(a, static_cast<size_t>(std::max<int64_t>(seg_total, 1)) * 64, &ctx->d_segtab));
+    EPSEON_HIP_TRY(arena_array(a, static_cast<size_t>(std::max<int64_t>(seg_total, 1)) * 64, &ctx->d_segtab_full));
+    const size_t nw = std::max<size_t>(static_cast<size_t>(count) * kBands, 1);
+    int2* d_raw_windows = nullptr;
+    EPSEON_HIP_TRY(arena_array(a, nw, &d_raw_windows));
+    EPSEON_HIP_TRY(arena_array(a, nw, &ctx->d_win_full));
+    EPSEON_HIP_TRY(arena_array(a, nw, &ctx->d_win_band));
+    EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_steps));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(count, 1), &ctx->d_rec));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(scan_tasks.size(), 1), &ctx->d_scan_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(xcd_tasks.size(), 1), &ctx->d_xcd_scan_tasks));
     EPSEON_HIP_TRY(arena_array(a, size_t(9), &ctx->d_xcd_bins));
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(xcd_level_tasks.size(), 1), &ctx->d_xcd_level_tasks));
+    EPSEON_HIP_TRY(arena_array(a, size_t(9), &ctx->d_xcd_level_bins));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(refine_tasks.size(), 1), &ctx->d_refine_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(level_tasks.size(), 1), &ctx->d_level_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(lane1_tasks.size(), 1), &ctx->d_lane1_tasks));
@@ -769,6 +820,12 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
         EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_scan_tasks, xcd_tasks.data(),
                                       xcd_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice,
                                       ctx->stream));
+    if (!xcd_level_tasks.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_level_tasks, xcd_level_tasks.data(),
+                                      xcd_level_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice,
+                                      ctx->stream));
+    EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_level_bins, xcd_level_bins.data(), 9 * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, ctx->stream));
     EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_bins, xcd_bins.data(), 9 * sizeof(int32_t),
                                   hipMemcpyHostToDevice, ctx->stream));
     if (!refine_tasks.empty())
@@ -792,10 +849,91 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     EPSEON_HIP_TRY(hipMemsetAsync(ctx->d_resc, 0, ne * sizeof(int32_t), ctx->stream));
     EPSEON_HIP_TRY(launch_shift_potential(ctx->stream, v_src, ctx->d_rec, count, max_points,
                                           ctx->d_ut, ctx->d_uv, ctx->d_uvr));
-    ctx->segtab_ok = max_seg_rows <= INT32_MAX / 512;
-    if (ctx->segtab_ok)
-        EPSEON_HIP_TRY(launch_segment_table(ctx->stream, ctx->d_uv, ctx->d_rec, count, max_seg_rows,
-                                            ctx->d_segtab));
+    // Band windows: computed on the device, read back to size the segment
+    // tables and to give each window its far-end scale.
+    std::vector<int2> raw(static_cast<size_t>(count) * kBands);
+    EPSEON_HIP_TRY(launch_band_windows(ctx->stream, ctx->d_uv, ctx->d_rec, count, d_raw_windows));
+    if (count > 0)
+        EPSEON_HIP_TRY(hipMemcpyAsync(raw.data(), d_raw_windows, raw.size() * sizeof(int2),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::vector<BandWindow> win_full(raw.size()), win_band(raw.size());
+    std::vector<TableJob> jobs_full, jobs_band;
+    int64_t band_total = 0;
+    int32_t max_rows = max_seg_rows;
+    double scan_pu_band = 0.0;
+    for (int32_t p = 0; p < count; ++p) {
+        const PotentialRecord& r = ctx->rec[static_cast<size_t>(p)];
+        const bool tables = seg_rows(r.n_points, r.flags, r.level_count) > 0;
+        if (tables)
+            jobs_full.push_back({p, 0, r.n_points, r.seg_row});
+        for (int32_t b = 0; b < kBands; ++b) {
+            const size_t i = static_cast<size_t>(p) * kBands + static_cast<size_t>(b);
+            win_full[i] = BandWindow{0, r.n_points - 1, r.far_exp, r.seg_row, r.far_scale};
+            BandWindow w{raw[i].x, raw[i].y, 0, 0, 1.0};
+            const int32_t n = w.hi - w.lo + 1;
+            const double l2 = static_cast<double>(n - 2) * std::log2(12.0);
+            const double fl = std::floor(l2);
+            w.far_exp = static_cast<int32_t>(fl);
+            w.far_scale = std::exp2(l2 - fl);
+            if (b > 0 && win_band[i - 1].lo == w.lo && win_band[i - 1].hi == w.hi) {
+                w.seg_row = win_band[i - 1].seg_row;
+            } else if (tables) {
+                w.seg_row = static_cast<int32_t>(band_total);
+                jobs_band.push_back({p, w.lo, n, w.seg_row});
+                band_total += seg_rows(n, 0, 1);
+                max_rows = std::max(max_rows, seg_rows(n, 0, 1));
+                if (band_total > INT32_MAX)
+                    return fail(EPSEON_ERR_INVALID, "batch too large for the segment tables");
+            }
+            win_band[i] = w;
+        }
+        if (r.flags == EPSEON_POT_OK) {
+            // Performed point updates of the windowed scan: each group of 64
+            // trial energies over its band window.
+            const double step = (r.e_hi - r.e_lo) / kBands;
+            for (int32_t j0 = 0; j0 < r.n_energies; j0 += 64) {
+                const int32_t jt = std::min(j0 + 63, r.n_energies - 1);
+                const double et = std::fma(static_cast<double>(jt + 1), r.de, r.e_lo);
+                int32_t b = kBands - 1;
+                for (int32_t c = 0; c < kBands - 1; ++c)
+                    if (std::fma(static_cast<double>(c + 1), step, r.e_lo) >= et) {
+                        b = c;
+                        break;
+                    }
+                const BandWindow& w = win_band[static_cast<size_t>(p) * kBands + static_cast<size_t>(b)];
+                scan_pu_band += static_cast<double>(w.hi - w.lo - 1) * static_cast<double>(jt - j0 + 1);
+            }
+        }
+    }
+    ctx->scan_pu_band = scan_pu_band;
+    ctx->h_win_full = win_full;
+    ctx->h_win_band = win_band;
+    ctx->segtab_ok = max_rows <= INT32_MAX / 512;
+    EPSEON_HIP_TRY(arena_array(a, static_cast<size_t>(std::max<int64_t>(band_total, 1)) * 64, &ctx->d_segtab_band));
+    TableJob* d_jobs = nullptr;
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(jobs_full.size() + jobs_band.size(), 1), &d_jobs));
+    if (count > 0) {
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_win_full, win_full.data(), win_full.size() * sizeof(BandWindow),
+                                      hipMemcpyHostToDevice, ctx->stream));
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_win_band, win_band.data(), win_band.size() * sizeof(BandWindow),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (!jobs_full.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(d_jobs, jobs_full.data(), jobs_full.size() * sizeof(TableJob),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    if (!jobs_band.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(d_jobs + jobs_full.size(), jobs_band.data(),
+                                      jobs_band.size() * sizeof(TableJob), hipMemcpyHostToDevice,
+                                      ctx->stream));
+    if (ctx->segtab_ok) {
+        EPSEON_HIP_TRY(launch_segment_tables(ctx->stream, ctx->d_uv, ctx->d_rec, d_jobs,
+                                             static_cast<int32_t>(jobs_full.size()), max_rows,
+                                             ctx->d_segtab_full));
+        EPSEON_HIP_TRY(launch_segment_tables(ctx->stream, ctx->d_uv, ctx->d_rec, d_jobs + jobs_full.size(),
+                                             static_cast<int32_t>(jobs_band.size()), max_rows,
+                                             ctx->d_segtab_band));
+    }
     // The host buffers are pageable and owned by the caller: finish the
     // copies before returning.
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -841,11 +979,22 @@ int epseon_potential_info_get(epseon_ctx* ctx, int32_t p, epseon_potential_info*
 
 namespace {
 
+// Windows pay when the scan has several wavefronts per SIMD: a SIMD then
+// takes new wavefronts as short-window ones finish.  With about one per SIMD
+// (a single large potential) the full-window wavefronts set the time anyway.
+bool windows_enabled(const epseon_ctx* ctx)
+{
+    if (ctx->windows != 2)
+        return ctx->windows == 1;
+    return static_cast<int64_t>(ctx->n_scan_tasks) >= kWindowsMinScanWavesPerSimd * 4 * ctx->compute_units;
+}
+
 int32_t choose_refine_mode(const epseon_ctx* ctx)
 {
     int32_t mode = ctx->refine_mode;
     if (mode == EPSEON_REFINE_AUTO)
-        mode = ctx->total_levels >= kLanePerLevelMinLevels ? EPSEON_REFINE_LANE : EPSEON_REFINE_SEGMENTED;
+        mode = !windows_enabled(ctx) && ctx->total_levels >= kLanePerLevelMinLevels ? EPSEON_REFINE_LANE
+                                                                                     : EPSEON_REFINE_SEGMENTED;
     return mode;
 }
 
@@ -861,55 +1010,81 @@ int32_t segmented_lanes(const epseon_ctx* ctx)
     return lanes;
 }
 
+// Window set of a scan followed by a refinement in `mode`: the band windows
+// when enabled, except for lane mode (whose wavefronts walk one table for
+// all their levels), which shoots over the whole grid.
+bool windows_for(const epseon_ctx* ctx, int32_t mode) { return windows_enabled(ctx) && mode != EPSEON_REFINE_LANE; }
+
+// Start of a scan: fix the window set it (and the refinement after it) uses.
+void begin_scan(epseon_ctx* ctx, int32_t mode)
+{
+    ctx->scan_windows = windows_for(ctx, mode);
+    ctx->scan_point_updates = ctx->scan_windows ? ctx->scan_pu_band : ctx->scan_pu_full;
+}
+
+const BandWindow* scan_window_set(const epseon_ctx* ctx)
+{
+    return ctx->scan_windows ? ctx->d_win_band : ctx->d_win_full;
+}
+
 // Scan of the potentials [k0, k1) in task order on `stream`; the whole batch
 // goes through the per-XCD bins.
 hipError_t run_scan(epseon_ctx* ctx, hipStream_t stream, size_t k0, size_t k1)
 {
     const int32_t t0 = ctx->cum_scan_tasks[k0];
     const int32_t t1 = ctx->cum_scan_tasks[k1];
+    const BandWindow* win = scan_window_set(ctx);
     if (ctx->xcd_scan && k0 == 0 && k1 == ctx->cum_scan_tasks.size() - 1)
-        return launch_scan(stream, ctx->d_ut, ctx->d_rec, ctx->d_xcd_scan_tasks, t1 - t0, ctx->d_xcd_bins,
-                           ctx->xcd_max_bin, ctx->waves_per_block, ctx->d_nodes, ctx->d_far,
-                           ctx->d_resc);
-    return launch_scan(stream, ctx->d_ut, ctx->d_rec, ctx->d_scan_tasks + t0, t1 - t0, nullptr, 0,
+        return launch_scan(stream, ctx->d_ut, ctx->d_rec, win, ctx->d_xcd_scan_tasks, t1 - t0,
+                           ctx->d_xcd_bins, ctx->xcd_max_bin, ctx->waves_per_block, ctx->d_nodes,
+                           ctx->d_far, ctx->d_resc);
+    return launch_scan(stream, ctx->d_ut, ctx->d_rec, win, ctx->d_scan_tasks + t0, t1 - t0, nullptr, 0,
                        ctx->waves_per_block, ctx->d_nodes, ctx->d_far, ctx->d_resc);
 }
 
-// Refinement of the levels of potentials [k0, k1) in task order.  Lane mode
-// with one level per lane runs the 64-level tasks on `stream`; with two, the
-// paired tasks (potentials with more than 64 levels) run on `stream` and the
-// single ones on `stream_b`, which may be the same stream.
+// Refinement of the levels of potentials [k0, k1) in task order, in the
+// windows of the last scan.  Lane mode with one level per lane runs the
+// 64-level tasks on `stream`; with two, the paired tasks (potentials with
+// more than 64 levels) run on `stream` and the single ones on `stream_b`,
+// which may be the same stream.
 hipError_t run_refine(epseon_ctx* ctx, hipStream_t stream, hipStream_t stream_b, int32_t mode,
                       int32_t lanes, double rel_tol, int32_t max_iter, size_t k0, size_t k1)
 {
+    const BandWindow* win = scan_window_set(ctx);
     if (mode == EPSEON_REFINE_LANE && ctx->levels_per_lane == 1) {
         const int32_t t0 = ctx->cum_refine_tasks[k0];
         const int32_t t1 = ctx->cum_refine_tasks[k1];
-        return launch_refine(stream, 1, ctx->d_uv, ctx->d_rec, ctx->d_refine_tasks + t0, t1 - t0,
+        return launch_refine(stream, 1, ctx->d_uv, ctx->d_rec, win, ctx->d_refine_tasks + t0, t1 - t0,
                              ctx->waves_per_block, ctx->d_nodes, ctx->d_far, ctx->d_resc, rel_tol,
-                             max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters);
+                             max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters, ctx->d_steps);
     }
     if (mode == EPSEON_REFINE_LANE) {
         const int32_t a0 = ctx->cum_lane2_tasks[k0];
         const int32_t a1 = ctx->cum_lane2_tasks[k1];
         const int32_t b0 = ctx->cum_lane1_tasks[k0];
         const int32_t b1 = ctx->cum_lane1_tasks[k1];
-        hipError_t e = launch_refine(stream, 2, ctx->d_uv, ctx->d_rec, ctx->d_lane2_tasks + a0, a1 - a0,
-                                     ctx->waves_per_block, ctx->d_nodes, ctx->d_far, ctx->d_resc,
-                                     rel_tol, max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters);
+        hipError_t e = launch_refine(stream, 2, ctx->d_uv, ctx->d_rec, win, ctx->d_lane2_tasks + a0,
+                                     a1 - a0, ctx->waves_per_block, ctx->d_nodes, ctx->d_far,
+                                     ctx->d_resc, rel_tol, max_iter, ctx->d_levels, ctx->d_status,
+                                     ctx->d_iters, ctx->d_steps);
         if (e != hipSuccess)
             return e;
-        return launch_refine(stream_b, 1, ctx->d_uv, ctx->d_rec, ctx->d_lane1_tasks + b0, b1 - b0,
+        return launch_refine(stream_b, 1, ctx->d_uv, ctx->d_rec, win, ctx->d_lane1_tasks + b0, b1 - b0,
                              ctx->waves_per_block, ctx->d_nodes, ctx->d_far, ctx->d_resc, rel_tol,
-                             max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters);
+                             max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters, ctx->d_steps);
     }
     const int32_t t0 = ctx->cum_level_tasks[k0];
     const int32_t t1 = ctx->cum_level_tasks[k1];
-    return launch_refine_segmented(stream, lanes, ctx->d_uv,
-                                   lanes == 64 && ctx->segtab_ok ? ctx->d_segtab : nullptr,
-                                   ctx->d_rec, ctx->d_level_tasks + t0,
-                                   t1 - t0, ctx->d_nodes, ctx->d_far, ctx->d_resc, rel_tol, max_iter,
-                                   ctx->d_levels, ctx->d_status, ctx->d_iters);
+    const double* tab = lanes == 64 && ctx->segtab_ok ? (ctx->scan_windows ? ctx->d_segtab_band : ctx->d_segtab_full)
+                                                     : nullptr;
+    if (ctx->xcd_scan && k0 == 0 && k1 == ctx->cum_level_tasks.size() - 1)
+        return launch_refine_segmented(stream, lanes, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_xcd_level_tasks,
+                                       t1 - t0, ctx->d_xcd_level_bins, ctx->xcd_max_level_bin, ctx->d_nodes,
+                                       ctx->d_far, ctx->d_resc, rel_tol, max_iter, ctx->d_levels,
+                                       ctx->d_status, ctx->d_iters, ctx->d_steps);
+    return launch_refine_segmented(stream, lanes, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_level_tasks + t0,
+                                   t1 - t0, nullptr, 0, ctx->d_nodes, ctx->d_far, ctx->d_resc, rel_tol,
+                                   max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters, ctx->d_steps);
 }
 
 int check_refine_args(double rel_tol, int32_t max_iter)
@@ -946,6 +1121,7 @@ int epseon_scan(epseon_ctx* ctx)
         return rc;
     if (!ctx->loaded)
         return fail(EPSEON_ERR_STATE, "epseon_scan before epseon_load_potentials");
+    begin_scan(ctx, choose_refine_mode(ctx));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
     EPSEON_HIP_TRY(run_scan(ctx, ctx->stream, 0, n_potentials(ctx)));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -964,7 +1140,10 @@ int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
         return fail(EPSEON_ERR_STATE, "epseon_refine before epseon_scan");
     if (const int rc = check_refine_args(rel_tol, max_iter); rc != EPSEON_OK)
         return rc;
-    const int32_t mode = choose_refine_mode(ctx);
+    // A windowed scan is refined in segmented mode whatever the mode setting.
+    int32_t mode = choose_refine_mode(ctx);
+    if (ctx->scan_windows && mode == EPSEON_REFINE_LANE)
+        mode = EPSEON_REFINE_SEGMENTED;
     const int32_t lanes = segmented_lanes(ctx);
     ctx->last_refine_mode = mode;
     ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
@@ -1029,6 +1208,7 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
         // (events between the two launches would add a barrier each).
         ctx->last_refine_mode = mode;
         ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
+        begin_scan(ctx, mode);
         EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve0, ctx->stream));
         EPSEON_HIP_TRY(run_scan(ctx, ctx->stream, 0, np));
         EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream, mode, lanes, rel_tol, max_iter, 0, np));
@@ -1061,6 +1241,7 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     // epseon_refine.
     ctx->last_refine_mode = mode;
     ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
+    begin_scan(ctx, mode);
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve0, ctx->stream));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
     EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
@@ -1081,6 +1262,25 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     return EPSEON_OK;
 }
 
+int epseon_band_windows_get(epseon_ctx* ctx, int32_t p, int32_t* lo, int32_t* hi, int32_t* far_exponent)
+{
+    if (ctx == nullptr || lo == nullptr || hi == nullptr || far_exponent == nullptr)
+        return fail(EPSEON_ERR_INVALID, "null argument");
+    if (!ctx->loaded)
+        return fail(EPSEON_ERR_STATE, "no batch loaded");
+    if (p < 0 || static_cast<size_t>(p) >= ctx->rec.size())
+        return fail(EPSEON_ERR_INVALID, "potential index out of range");
+    const bool band = ctx->scanned ? ctx->scan_windows : windows_for(ctx, choose_refine_mode(ctx));
+    const std::vector<BandWindow>& w = band ? ctx->h_win_band : ctx->h_win_full;
+    for (int32_t b = 0; b < kBands; ++b) {
+        const BandWindow& x = w[static_cast<size_t>(p) * kBands + static_cast<size_t>(b)];
+        lo[b] = x.lo;
+        hi[b] = x.hi;
+        far_exponent[b] = x.far_exp;
+    }
+    return EPSEON_OK;
+}
+
 int epseon_get_tuning(epseon_ctx* ctx, epseon_tuning* t)
 {
     if (ctx == nullptr || t == nullptr)
@@ -1090,7 +1290,7 @@ int epseon_get_tuning(epseon_ctx* ctx, epseon_tuning* t)
     t->scan_waves_per_block = ctx->waves_per_block;
     t->segmented_lanes = ctx->segmented_lanes;
     t->xcd_scan = ctx->xcd_scan ? 1 : 0;
-    t->reserved = 0;
+    t->windows = ctx->windows;
     t->solve_split = ctx->solve_split;
     return EPSEON_OK;
 }
@@ -1113,9 +1313,12 @@ int epseon_set_tuning(epseon_ctx* ctx, const epseon_tuning* t)
         return fail(EPSEON_ERR_INVALID, "solve_split must be in [0, 1)");
     if (t->xcd_scan != 0 && t->xcd_scan != 1)
         return fail(EPSEON_ERR_INVALID, "xcd_scan must be 0 or 1");
+    if (t->windows < 0 || t->windows > 2)
+        return fail(EPSEON_ERR_INVALID, "windows must be 0, 1 or 2");
     ctx->refine_mode = t->refine_mode;
     ctx->levels_per_lane = t->levels_per_lane;
     ctx->waves_per_block = t->scan_waves_per_block;
+    ctx->windows = t->windows;
     ctx->segmented_lanes = t->segmented_lanes;
     ctx->xcd_scan = t->xcd_scan != 0;
     ctx->solve_split = t->solve_split;
@@ -1235,10 +1438,13 @@ int fetch_level_meta(epseon_ctx* ctx)
     const size_t n = static_cast<size_t>(ctx->total_levels);
     ctx->h_status.resize(n);
     ctx->h_iters.resize(n);
+    ctx->h_steps.resize(n);
     if (n > 0) {
         EPSEON_HIP_TRY(hipMemcpyAsync(ctx->h_status.data(), ctx->d_status, n * sizeof(int32_t),
                                       hipMemcpyDeviceToHost, ctx->stream));
         EPSEON_HIP_TRY(hipMemcpyAsync(ctx->h_iters.data(), ctx->d_iters, n * sizeof(int32_t),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->h_steps.data(), ctx->d_steps, n * sizeof(int32_t),
                                       hipMemcpyDeviceToHost, ctx->stream));
     }
     return EPSEON_OK;
@@ -1480,6 +1686,7 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
     if (stats == nullptr)
         return fail(EPSEON_ERR_INVALID, "null stats pointer");
     std::memset(stats, 0, sizeof *stats);
+    stats->scan_windows = ctx->scanned && ctx->scan_windows ? 1 : 0;
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;
     if (ctx->scan_timed) {
@@ -1513,7 +1720,8 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
             const PotentialRecord& r = ctx->rec[p];
             for (int32_t l = 0; l < r.level_count; ++l) {
                 const size_t o = static_cast<size_t>(r.level_offset + l);
-                pu += static_cast<double>(it[o]) * static_cast<double>(r.n_points - 2);
+                if (it[o] > 0)
+                    pu += static_cast<double>(it[o]) * static_cast<double>(ctx->h_steps[o]);
                 ok += st[o] == EPSEON_LEVEL_OK;
             }
         }

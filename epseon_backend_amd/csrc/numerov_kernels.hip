@@ -339,6 +339,24 @@ __device__ __forceinline__ bool fetch_task(const WaveTask* __restrict__ tasks, c
     return true;
 }
 
+// Task of this workgroup (one task per workgroup), binned per XCD like
+// fetch_task when `bins` is given.
+__device__ __forceinline__ bool fetch_block_task(const WaveTask* __restrict__ tasks, const int32_t n_tasks,
+                                                 const int32_t* __restrict__ bins, WaveTask& out)
+{
+    int32_t t = static_cast<int32_t>(blockIdx.x);
+    if (bins != nullptr) {
+        const int32_t x = static_cast<int32_t>(blockIdx.x & 7u);
+        t = bins[x] + static_cast<int32_t>(blockIdx.x >> 3);
+        if (t >= bins[x + 1])
+            return false;
+    } else if (t >= n_tasks) {
+        return false;
+    }
+    out = tasks[t];
+    return true;
+}
+
 }  // namespace
 
 // Statistics of each V table in device memory (one block per potential),
@@ -441,11 +459,131 @@ __global__ __launch_bounds__(256) void shift_potential_kernel(const double* __re
     }
 }
 
-// Scan: lane = trial energy E_j = fma(j+1, dE, E_lo); outputs node count,
-// far-end mantissa v_{N-1} * far_scale and scale-up count r per trial energy
-// (w_{N-1} = mantissa * 2^(far_exp - 480 r)).
+namespace {
+
+// Band of the scan group (64 trial energies from a multiple of 64) holding
+// trial energy j: the first band whose top is at or above the group's
+// highest energy, else the last band.
+__device__ __forceinline__ int32_t group_band(const PotentialRecord& P, const int32_t j)
+{
+    const int32_t jt = min((j / 64) * 64 + 63, P.n_energies - 1);
+    const double et = __builtin_fma(static_cast<double>(jt + 1), P.de, P.e_lo);
+    const double step = (P.e_hi - P.e_lo) / kBands;
+    for (int32_t b = 0; b < kBands - 1; ++b)
+        if (__builtin_fma(static_cast<double>(b + 1), step, P.e_lo) >= et)
+            return b;
+    return kBands - 1;
+}
+
+__device__ __forceinline__ bool same_window(const BandWindow* W, const int32_t a, const int32_t b)
+{
+    return W[a].lo == W[b].lo && W[a].hi == W[b].hi;
+}
+
+// Lower bound g(x) of the per-step growth in 2^-40 fixed point.
+__device__ __forceinline__ int64_t growth_term(const double x)
+{
+    int k = 0;
+    (void)__builtin_frexp(x, &k);
+    const int32_t s = ((k - 1) >> 1) - 1 + 40;
+    if (s < 0)
+        return 0;
+    return int64_t(1) << (s > 48 ? 48 : s);
+}
+
+__device__ __forceinline__ int64_t wave_inclusive_sum(int64_t x)
+{
+    const int32_t lane = static_cast<int32_t>(threadIdx.x & 63);
+#pragma unroll
+    for (int32_t d = 1; d < 64; d *= 2) {
+        const int64_t y = __shfl_up(x, d, 64);
+        if (lane >= d)
+            x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int32_t wave_min(int32_t x)
+{
+#pragma unroll
+    for (int32_t d = 32; d >= 1; d /= 2)
+        x = min(x, __shfl_xor(x, d, 64));
+    return x;
+}
+
+__device__ __forceinline__ int32_t wave_max(int32_t x)
+{
+#pragma unroll
+    for (int32_t d = 32; d >= 1; d /= 2)
+        x = max(x, __shfl_xor(x, d, 64));
+    return x;
+}
+
+}  // namespace
+
+// Band windows: one wavefront per (potential, band) writes (lo, hi) of the
+// window at the band's top energy (whole grid for rejected potentials).
+__global__ __launch_bounds__(256) void band_window_kernel(const double* __restrict__ uv,
+                                                          const PotentialRecord* __restrict__ pots,
+                                                          const int32_t n_pots, int2* __restrict__ out)
+{
+    const int32_t wave = static_cast<int32_t>(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (wave >= n_pots * kBands)
+        return;
+    const int32_t p = wave / kBands, b = wave % kBands;
+    const PotentialRecord& P = pots[p];
+    const int32_t lane = static_cast<int32_t>(threadIdx.x & 63);
+    const int32_t n = P.n_points;
+    int32_t lo = 0, hi = n - 1;
+    if (P.flags == 0) {
+        const double top = __builtin_fma(static_cast<double>(b + 1), (P.e_hi - P.e_lo) / kBands, P.e_lo);
+        const double e = P.beta * top;
+        const double* u = uv + P.ut_offset;
+        int32_t first = n, last = -1;
+        for (int32_t i = lane; i < n; i += 64)
+            if (u[i] <= e) {
+                first = min(first, i);
+                last = i;
+            }
+        first = wave_min(first);
+        last = wave_max(last);
+        if (last >= 0) {
+            const int64_t depth = int64_t(kWindowDepth) << 40;
+            int64_t acc = 0;
+            for (int32_t base = last + 1; base < n; base += 64) {
+                const int32_t i = base + lane;
+                const int64_t inc = wave_inclusive_sum(i < n ? growth_term(12.0 * (u[i] - e)) : 0);
+                const uint64_t hit = __ballot(acc + inc >= depth);
+                if (hit != 0) {
+                    hi = base + __builtin_ctzll(hit);
+                    break;
+                }
+                acc += __shfl(inc, 63, 64);
+            }
+            acc = 0;
+            for (int32_t base = first - 1; base >= 0; base -= 64) {
+                const int32_t i = base - lane;
+                const int64_t inc = wave_inclusive_sum(i >= 0 ? growth_term(12.0 * (u[i] - e)) : 0);
+                const uint64_t hit = __ballot(acc + inc >= depth);
+                if (hit != 0) {
+                    lo = base - __builtin_ctzll(hit);
+                    break;
+                }
+                acc += __shfl(inc, 63, 64);
+            }
+        }
+    }
+    if (lane == 0)
+        out[wave] = make_int2(lo, hi);
+}
+
+// Scan: lane = trial energy E_j = fma(j+1, dE, E_lo) shot over the band
+// window of its group; outputs node count, far-end mantissa v_{N-1} *
+// far_scale and scale-up count r per trial energy (w_{N-1} = mantissa *
+// 2^(far_exp - 480 r), N and the scale those of the window).
 __global__ __launch_bounds__(256) void numerov_scan_kernel(const double* __restrict__ ut,
                                                            const PotentialRecord* __restrict__ pots,
+                                                           const BandWindow* __restrict__ windows,
                                                            const WaveTask* __restrict__ tasks,
                                                            const int32_t n_tasks,
                                                            const int32_t* __restrict__ bins,
@@ -460,14 +598,15 @@ __global__ __launch_bounds__(256) void numerov_scan_kernel(const double* __restr
     const int32_t j = task.first + static_cast<int32_t>(threadIdx.x & 63);
     if (j >= P.n_energies)
         return;
+    const BandWindow& W = windows[task.potential * kBands + group_band(P, task.first)];
     const double E = __builtin_fma(static_cast<double>(j + 1), P.de, P.e_lo);
     const double e = (P.beta * E) / 12.0;
     double w;
     int32_t k;
-    const int32_t c = shoot<ScanForm>(ut + P.ut_offset, P.n_points, e, P.stride, w, k);
+    const int32_t c = shoot<ScanForm>(ut + P.ut_offset + W.lo, W.hi - W.lo + 1, e, P.stride, w, k);
     const int64_t o = P.energy_offset + j;
     nodes[o] = c;
-    far_end[o] = w * P.far_scale;
+    far_end[o] = w * W.far_scale;
     rescales[o] = k;
 }
 
@@ -577,10 +716,13 @@ __device__ __forceinline__ void model_apply(const double b, const double e0, con
 struct Bracket {
     double el, eh, wl, wh, rate;
     int32_t kl, kh, cl, ch, lo_ok, found;
+    int32_t band;        // band of the upper end's scan group: the level's window
+    int32_t lo_foreign;  // lower end's value from a different band window
 };
 
-__device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const int32_t v,
-                                                const int32_t* __restrict__ nodes,
+// W: the potential's kBands band windows (those the scan used).
+__device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const BandWindow* W,
+                                                const int32_t v, const int32_t* __restrict__ nodes,
                                                 const double* __restrict__ far_end,
                                                 const int32_t* __restrict__ rescales)
 {
@@ -600,31 +742,36 @@ __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const 
         b.found = 0;
         return b;
     }
+    const int32_t band = group_band(P, j);
+    const int32_t fe = W[band].far_exp;
     b.found = 1;
+    b.band = band;
     b.eh = __builtin_fma(static_cast<double>(j + 1), P.de, P.e_lo);
     b.ch = nd[j];
     b.wh = far_end[P.energy_offset + j];
-    b.kh = P.far_exp - rescales[P.energy_offset + j] * kRescaleLog2;
+    b.kh = fe - rescales[P.energy_offset + j] * kRescaleLog2;
     if (j == 0 || nd[j - 1] > v) {
         b.el = P.e_lo;
         b.cl = 0;
         b.lo_ok = 0;
     } else {
+        const int32_t bl = group_band(P, j - 1);
         b.el = __builtin_fma(static_cast<double>(j), P.de, P.e_lo);
         b.cl = nd[j - 1];
         b.wl = far_end[P.energy_offset + j - 1];
-        b.kl = P.far_exp - rescales[P.energy_offset + j - 1] * kRescaleLog2;
+        b.kl = W[bl].far_exp - rescales[P.energy_offset + j - 1] * kRescaleLog2;
         b.lo_ok = 1;
+        b.lo_foreign = same_window(W, bl, band) ? 0 : 1;
     }
     b.rate = 0.0;
-    if (b.lo_ok && b.cl == v && b.ch == v + 1) {
+    if (b.lo_ok && !b.lo_foreign && b.cl == v && b.ch == v + 1) {
         const int64_t o = P.energy_offset;
-        if (j + 1 < m && nd[j + 1] == b.ch)
+        if (j + 1 < m && nd[j + 1] == b.ch && same_window(W, group_band(P, j + 1), band))
             b.rate = model_rate(b.wl, b.kl, b.wh, b.kh, far_end[o + j + 1],
-                                P.far_exp - rescales[o + j + 1] * kRescaleLog2, true, P.de);
-        else if (j >= 2 && nd[j - 2] == b.cl)
-            b.rate = model_rate(far_end[o + j - 2], P.far_exp - rescales[o + j - 2] * kRescaleLog2, b.wl,
-                                b.kl, b.wh, b.kh, false, P.de);
+                                fe - rescales[o + j + 1] * kRescaleLog2, true, P.de);
+        else if (j >= 2 && nd[j - 2] == b.cl && same_window(W, group_band(P, j - 2), band))
+            b.rate = model_rate(far_end[o + j - 2], fe - rescales[o + j - 2] * kRescaleLog2, b.wl, b.kl,
+                                b.wh, b.kh, false, P.de);
     }
     return b;
 }
@@ -638,7 +785,7 @@ __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const 
 struct Refiner {
     Bracket b;
     int32_t v, side, it;
-    bool isolated;
+    bool isolated, reshoot;
     double rate, e0;
 
     // Regula falsi runs on G = F e^(-b (E - E_hi0)) when the scan gave a
@@ -665,8 +812,13 @@ struct Refiner {
         const double width = b.eh - b.el;
         const double tol = __builtin_fma(rel_tol, fmax(fabs(b.el), fabs(b.eh)), abs_tol);
         isolated = b.lo_ok && b.cl == v && b.ch == v + 1;
+        reshoot = isolated && b.lo_foreign;
         double em;
-        if (isolated) {
+        if (reshoot) {
+            // The floor's value came from another band window: shoot it again
+            // in this level's window (its count stays the scan's).
+            em = b.el;
+        } else if (isolated) {
             const double r = __builtin_ldexp(b.wh / b.wl, b.kh - b.kl);
             const double s = 1.0 / (1.0 - r);
             em = __builtin_fma(s, width, b.el);
@@ -691,7 +843,7 @@ struct Refiner {
     __device__ __forceinline__ void update(const double em, double w, int32_t k, int32_t c)
     {
         if (isolated)
-            c = (static_cast<int32_t>(sign_bit(w)) == (v & 1)) ? v : v + 1;
+            c = reshoot ? v : ((static_cast<int32_t>(sign_bit(w)) == (v & 1)) ? v : v + 1);
         if (rate != 0.0)
             model_apply(rate, e0, em, w, k);
         ++it;
@@ -708,6 +860,7 @@ struct Refiner {
             b.wl = w;
             b.kl = k;
             b.lo_ok = 1;
+            b.lo_foreign = 0;
             side = isolated ? -1 : 0;
         } else {
             if (isolated && side == 1) {
@@ -741,13 +894,16 @@ __device__ __forceinline__ void write_level(double* levels, int32_t* status, int
 // K = 2 a lane refines levels first + lane and first + lane + 64 of a
 // 128-level task together (one paired shot per iteration; a member that has
 // converged rides along on its partner's probe and its result is dropped).
+// Every lane walks the same table, so the scan must have used one window
+// (the whole grid) in every band.
 template <int K>
 __global__ __launch_bounds__(256) void numerov_refine_kernel(
     const double* __restrict__ uv, const PotentialRecord* __restrict__ pots,
-    const WaveTask* __restrict__ tasks, const int32_t n_tasks, const int32_t* __restrict__ nodes,
-    const double* __restrict__ far_end, const int32_t* __restrict__ rescales, const double rel_tol,
-    const int32_t max_iter, double* __restrict__ levels, int32_t* __restrict__ status,
-    int32_t* __restrict__ iterations)
+    const BandWindow* __restrict__ windows, const WaveTask* __restrict__ tasks, const int32_t n_tasks,
+    const int32_t* __restrict__ nodes, const double* __restrict__ far_end,
+    const int32_t* __restrict__ rescales, const double rel_tol, const int32_t max_iter,
+    double* __restrict__ levels, int32_t* __restrict__ status, int32_t* __restrict__ iterations,
+    int32_t* __restrict__ steps)
 {
     static_assert(K == 1 || K == 2, "one or two levels per lane");
     const int32_t tid = wave_task_index();
@@ -755,6 +911,7 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
         return;
     const WaveTask task = tasks[tid];
     const PotentialRecord& P = pots[task.potential];
+    const BandWindow* W = windows + task.potential * kBands;
     const int32_t lane = static_cast<int32_t>(threadIdx.x & 63);
     Refiner r[K];
     bool live[K];
@@ -772,7 +929,8 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
             continue;
         }
         r[q].v = P.level_min + l;
-        r[q].b = find_bracket(P, r[q].v, nodes, far_end, rescales);
+        r[q].b = find_bracket(P, W, r[q].v, nodes, far_end, rescales);
+        steps[out[q]] = W[0].hi - W[0].lo - 1;
         r[q].start_model();
         if (!r[q].b.found) {
             write_level(levels, status, iterations, out[q], __builtin_nan(""), 1, 0);
@@ -780,7 +938,8 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
         }
         live[q] = true;
     }
-    const double* u = uv + P.ut_offset;
+    const double* u = uv + P.ut_offset + W[0].lo;
+    const int32_t n = W[0].hi - W[0].lo + 1;
     for (;;) {
         double em[K];
         bool any = false;
@@ -805,14 +964,14 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
         if constexpr (K == 1) {
             double w;
             int32_t k;
-            const int32_t c = shoot<DiffForm>(u, P.n_points, P.beta * em[0], P.stride, w, k);
+            const int32_t c = shoot<DiffForm>(u, n, P.beta * em[0], P.stride, w, k);
             r[0].update(em[0], w, k * kRescaleLog2, c);
         } else {
             const double ea = live[0] ? em[0] : em[1];
             const double eb = live[1] ? em[1] : em[0];
             DiffPair s;
             DiffPair::Counter c;
-            shoot<DiffPair>(u, P.n_points, DiffPair::Energy{P.beta * ea, P.beta * eb}, P.stride, s, c);
+            shoot<DiffPair>(u, n, DiffPair::Energy{P.beta * ea, P.beta * eb}, P.stride, s, c);
             if (live[0])
                 r[0].update(em[0], s.x.w, s.x.rescales * kRescaleLog2, c.a.nodes);
             if (live[1])
@@ -1001,26 +1160,33 @@ struct CountBody {
 
 }  // namespace
 
+// Segment tables of the 64-lane segmented refinement: for job J, rows
+// r < seg_rows(J.n) of 64 columns from row J.seg_row, column s holding the
+// window's uv at point min(q0_s + r, n - 1) for the segments s < seg_count(n)
+// (zero in the unused columns).
 __global__ __launch_bounds__(256) void segment_table_kernel(const double* __restrict__ uv,
                                                             const PotentialRecord* __restrict__ pots,
-                                                            double* __restrict__ segtab)
+                                                            const TableJob* __restrict__ jobs,
+                                                            const int32_t n_jobs, double* __restrict__ segtab)
 {
-    const PotentialRecord& P = pots[blockIdx.y];
-    const int32_t rows = seg_rows(P.n_points, P.flags, P.level_count);
-    const int32_t n = P.n_points;
-    const int32_t qn = n - 2;
-    const int32_t segs = seg_count(n);
-    const double* u = uv + P.ut_offset;
-    double* out = segtab + static_cast<int64_t>(P.seg_row) * 64;
-    for (int32_t i = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x); i < rows * 64;
-         i += static_cast<int32_t>(gridDim.x * blockDim.x)) {
-        const int32_t r = i >> 6, s = i & 63;
-        double x = 0.0;
-        if (s < segs) {
-            const int64_t q = segment_start(qn, segs, s) + r;
-            x = u[q < n - 1 ? q : n - 1];
+    for (int32_t jb = static_cast<int32_t>(blockIdx.y); jb < n_jobs; jb += static_cast<int32_t>(gridDim.y)) {
+        const TableJob J = jobs[jb];
+        const int32_t n = J.n;
+        const int32_t rows = seg_rows(n, 0, 1);
+        const int32_t qn = n - 2;
+        const int32_t segs = seg_count(n);
+        const double* u = uv + pots[J.potential].ut_offset + J.lo;
+        double* out = segtab + static_cast<int64_t>(J.seg_row) * 64;
+        for (int32_t i = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x); i < rows * 64;
+             i += static_cast<int32_t>(gridDim.x * blockDim.x)) {
+            const int32_t r = i >> 6, sg = i & 63;
+            double x = 0.0;
+            if (sg < segs) {
+                const int64_t q = segment_start(qn, segs, sg) + r;
+                x = u[q < n - 1 ? q : n - 1];
+            }
+            out[i] = x;
         }
-        out[i] = x;
     }
 }
 
@@ -1035,10 +1201,11 @@ __global__ __launch_bounds__(256) void segment_table_kernel(const double* __rest
 template <int T, bool TABLE>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(2))) void numerov_refine_segmented_kernel(
     const double* __restrict__ uv, const double* __restrict__ segtab, const PotentialRecord* __restrict__ pots,
-    const WaveTask* __restrict__ tasks, const int32_t n_tasks, const int32_t* __restrict__ nodes,
-    const double* __restrict__ far_end, const int32_t* __restrict__ rescales, const double rel_tol,
-    const int32_t max_iter, double* __restrict__ levels, int32_t* __restrict__ status,
-    int32_t* __restrict__ iterations)
+    const BandWindow* __restrict__ windows, const WaveTask* __restrict__ tasks, const int32_t n_tasks,
+    const int32_t* __restrict__ bins, const int32_t* __restrict__ nodes, const double* __restrict__ far_end,
+    const int32_t* __restrict__ rescales, const double rel_tol, const int32_t max_iter,
+    double* __restrict__ levels, int32_t* __restrict__ status, int32_t* __restrict__ iterations,
+    int32_t* __restrict__ steps)
 {
     __shared__ double s_m[4][T];
     __shared__ int32_t s_e[T];
@@ -1046,10 +1213,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(2))) void num
     __shared__ SegCount s_cnt[T];
     __shared__ int32_t s_c;
 
-    const int32_t tid = static_cast<int32_t>(blockIdx.x);
-    if (tid >= n_tasks)
+    WaveTask task;
+    if (!fetch_block_task(tasks, n_tasks, bins, task))
         return;
-    const WaveTask task = tasks[tid];
     const PotentialRecord& P = pots[task.potential];
     const int32_t lane = static_cast<int32_t>(threadIdx.x);
     const int64_t out = P.level_offset + task.first;
@@ -1058,17 +1224,22 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(2))) void num
             write_level(levels, status, iterations, out, __builtin_nan(""), 3, 0);
         return;
     }
+    const BandWindow* W = windows + task.potential * kBands;
     Refiner r{};
     r.v = P.level_min + task.first;
-    r.b = find_bracket(P, r.v, nodes, far_end, rescales);
+    r.b = find_bracket(P, W, r.v, nodes, far_end, rescales);
     r.start_model();
     if (!r.b.found) {
         if (lane == 0)
             write_level(levels, status, iterations, out, __builtin_nan(""), 1, 0);
         return;
     }
-    const double* u = uv + P.ut_offset;
-    const int32_t n = P.n_points;
+    // The level's band window: Dirichlet points lo and hi of the trimmed grid.
+    const BandWindow& B = W[r.b.band];
+    const double* u = uv + P.ut_offset + B.lo;
+    const int32_t n = B.hi - B.lo + 1;
+    if (lane == 0)
+        steps[out] = n - 2;
     const int32_t qn = n - 2;
     const int32_t segs = min(T, max(1, qn / 32));
     const bool active = lane < segs;
@@ -1081,8 +1252,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(2))) void num
     if constexpr (TABLE) {
         // The resource must be wave-uniform (scalar registers); the
         // readfirstlane calls state what the compiler cannot prove.
-        const int32_t rows = seg_rows(P.n_points, P.flags, P.level_count);
-        const uint64_t base = reinterpret_cast<uint64_t>(segtab + static_cast<int64_t>(P.seg_row) * 64);
+        const int32_t rows = seg_rows(n, 0, 1);
+        const uint64_t base = reinterpret_cast<uint64_t>(segtab + static_cast<int64_t>(B.seg_row) * 64);
         const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(base));
         const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(base >> 32));
         const int32_t bytes = __builtin_amdgcn_readfirstlane(rows * 512);
@@ -1225,90 +1396,101 @@ hipError_t launch_shift_potential(hipStream_t stream, const double* v, const Pot
 // bins == nullptr: tasks[0, n_tasks) in order.  Otherwise `bins` (device, 9
 // offsets) splits the list per XCD and `max_bin` is the largest bin's size.
 hipError_t launch_scan(hipStream_t stream, const double* ut, const PotentialRecord* pots,
-                       const WaveTask* tasks, int32_t n_tasks, const int32_t* bins, int32_t max_bin,
-                       int32_t waves_per_block, int32_t* nodes, double* far_end, int32_t* rescales)
+                       const BandWindow* windows, const WaveTask* tasks, int32_t n_tasks,
+                       const int32_t* bins, int32_t max_bin, int32_t waves_per_block, int32_t* nodes,
+                       double* far_end, int32_t* rescales)
 {
     if (n_tasks == 0)
         return hipSuccess;
     const int32_t blocks = bins == nullptr ? (n_tasks + waves_per_block - 1) / waves_per_block
                                            : 8 * ((max_bin + waves_per_block - 1) / waves_per_block);
     hipLaunchKernelGGL(numerov_scan_kernel, dim3(static_cast<unsigned>(blocks)),
-                       dim3(static_cast<unsigned>(64 * waves_per_block)), 0, stream, ut, pots,
+                       dim3(static_cast<unsigned>(64 * waves_per_block)), 0, stream, ut, pots, windows,
                        tasks, n_tasks, bins, nodes, far_end, rescales);
     return hipGetLastError();
 }
 
 hipError_t launch_refine(hipStream_t stream, int32_t levels_per_lane, const double* uv,
-                         const PotentialRecord* pots, const WaveTask* tasks, int32_t n_tasks,
-                         int32_t waves_per_block, const int32_t* nodes, const double* far_end,
-                         const int32_t* rescales, double rel_tol, int32_t max_iter, double* levels,
-                         int32_t* status, int32_t* iterations)
+                         const PotentialRecord* pots, const BandWindow* windows, const WaveTask* tasks,
+                         int32_t n_tasks, int32_t waves_per_block, const int32_t* nodes,
+                         const double* far_end, const int32_t* rescales, double rel_tol,
+                         int32_t max_iter, double* levels, int32_t* status, int32_t* iterations,
+                         int32_t* steps)
 {
     if (n_tasks == 0)
         return hipSuccess;
     const int32_t blocks = (n_tasks + waves_per_block - 1) / waves_per_block;
     const dim3 grid(static_cast<unsigned>(blocks)), block(static_cast<unsigned>(64 * waves_per_block));
     if (levels_per_lane == 2)
-        hipLaunchKernelGGL(numerov_refine_kernel<2>, grid, block, 0, stream, uv, pots, tasks, n_tasks,
-                           nodes, far_end, rescales, rel_tol, max_iter, levels, status, iterations);
+        hipLaunchKernelGGL(numerov_refine_kernel<2>, grid, block, 0, stream, uv, pots, windows, tasks,
+                           n_tasks, nodes, far_end, rescales, rel_tol, max_iter, levels, status,
+                           iterations, steps);
     else if (levels_per_lane == 1)
-        hipLaunchKernelGGL(numerov_refine_kernel<1>, grid, block, 0, stream, uv, pots, tasks, n_tasks,
-                           nodes, far_end, rescales, rel_tol, max_iter, levels, status, iterations);
+        hipLaunchKernelGGL(numerov_refine_kernel<1>, grid, block, 0, stream, uv, pots, windows, tasks,
+                           n_tasks, nodes, far_end, rescales, rel_tol, max_iter, levels, status,
+                           iterations, steps);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-// segtab != nullptr selects the segment table for 64-lane blocks.
+// segtab != nullptr selects the segment tables for 64-lane blocks.
+// bins == nullptr: tasks[0, n_tasks) in order; else `bins` (device, 9
+// offsets) splits the list per XCD and `max_bin` is the largest bin's size.
 hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, const double* uv,
                                    const double* segtab, const PotentialRecord* pots,
-                                   const WaveTask* tasks, int32_t n_tasks, const int32_t* nodes,
+                                   const BandWindow* windows, const WaveTask* tasks, int32_t n_tasks,
+                                   const int32_t* bins, int32_t max_bin, const int32_t* nodes,
                                    const double* far_end, const int32_t* rescales, double rel_tol,
                                    int32_t max_iter, double* levels, int32_t* status,
-                                   int32_t* iterations)
+                                   int32_t* iterations, int32_t* steps)
 {
     if (n_tasks == 0)
         return hipSuccess;
-    const dim3 grid(static_cast<unsigned>(n_tasks));
+    const dim3 grid(static_cast<unsigned>(bins == nullptr ? n_tasks : 8 * max_bin));
+#define EPSEON_SEGMENTED(T, TAB)                                                                   \
+    hipLaunchKernelGGL((numerov_refine_segmented_kernel<T, TAB>), grid, dim3(T), 0, stream, uv,   \
+                       segtab, pots, windows, tasks, n_tasks, bins, nodes, far_end, rescales,      \
+                       rel_tol, max_iter, levels, status, iterations, steps)
     switch (lanes) {
     case 256:
-        hipLaunchKernelGGL((numerov_refine_segmented_kernel<256, false>), grid, dim3(256), 0, stream, uv,
-                           segtab, pots, tasks, n_tasks, nodes, far_end, rescales, rel_tol, max_iter,
-                           levels, status, iterations);
+        EPSEON_SEGMENTED(256, false);
         break;
     case 128:
-        hipLaunchKernelGGL((numerov_refine_segmented_kernel<128, false>), grid, dim3(128), 0, stream, uv,
-                           segtab, pots, tasks, n_tasks, nodes, far_end, rescales, rel_tol, max_iter,
-                           levels, status, iterations);
+        EPSEON_SEGMENTED(128, false);
         break;
     case 64:
         if (segtab != nullptr)
-            hipLaunchKernelGGL((numerov_refine_segmented_kernel<64, true>), grid, dim3(64), 0, stream, uv,
-                               segtab, pots, tasks, n_tasks, nodes, far_end, rescales, rel_tol,
-                               max_iter, levels, status, iterations);
+            EPSEON_SEGMENTED(64, true);
         else
-            hipLaunchKernelGGL((numerov_refine_segmented_kernel<64, false>), grid, dim3(64), 0, stream, uv,
-                               segtab, pots, tasks, n_tasks, nodes, far_end, rescales, rel_tol,
-                               max_iter, levels, status, iterations);
+            EPSEON_SEGMENTED(64, false);
         break;
     default:
         return hipErrorInvalidValue;
     }
+#undef EPSEON_SEGMENTED
     return hipGetLastError();
 }
 
-// Segment table of the 64-lane segmented refinement: for every potential
-// with levels to refine, seg_rows(...) rows of 64 columns from row P.seg_row,
-// column s holding uv[min(q0_s + r, n - 1)] for the segments s < seg_count(n)
-// (zero in the unused columns).
-hipError_t launch_segment_table(hipStream_t stream, const double* uv, const PotentialRecord* pots,
-                                int32_t n_pots, int32_t max_rows, double* segtab)
+hipError_t launch_segment_tables(hipStream_t stream, const double* uv, const PotentialRecord* pots,
+                                 const TableJob* jobs, int32_t n_jobs, int32_t max_rows, double* segtab)
 {
-    if (n_pots == 0 || max_rows == 0)
+    if (n_jobs == 0 || max_rows == 0)
         return hipSuccess;
     const int32_t bx = (max_rows * 64 + 255) / 256;
-    dim3 grid(static_cast<unsigned>(bx < 1024 ? bx : 1024), static_cast<unsigned>(n_pots));
-    hipLaunchKernelGGL(segment_table_kernel, grid, dim3(256), 0, stream, uv, pots, segtab);
+    dim3 grid(static_cast<unsigned>(bx < 1024 ? bx : 1024), static_cast<unsigned>(n_jobs < 65535 ? n_jobs : 65535));
+    hipLaunchKernelGGL(segment_table_kernel, grid, dim3(256), 0, stream, uv, pots, jobs, n_jobs, segtab);
+    return hipGetLastError();
+}
+
+hipError_t launch_band_windows(hipStream_t stream, const double* uv, const PotentialRecord* pots,
+                               int32_t n_pots, int2* out)
+{
+    if (n_pots == 0)
+        return hipSuccess;
+    const int32_t waves = n_pots * kBands;
+    hipLaunchKernelGGL(band_window_kernel, dim3(static_cast<unsigned>((waves + 3) / 4)), dim3(256), 0, stream,
+                       uv, pots, n_pots, out);
     return hipGetLastError();
 }
 

@@ -102,6 +102,42 @@ struct PotentialStats {
     int64_t imin, first, last, bad;
 };
 
+// Shot windows.  Past its outermost turning points a shot only grows (from
+// the left wall: only decays towards the well) by at least acosh(1 + x/2)
+// per step, x_i = 12 (uv_i - e) > 0, bounded below exactly by
+// g(x) = 2^(floor((k-1)/2) - 1) for x = m 2^k, m in [1/2, 1).  A window's
+// Dirichlet points sit where the sum of g from the outermost allowed point
+// (uv_i <= e) first reaches kWindowDepth, or at the grid ends; moving a wall
+// from there to the grid end changes the levels by an e^-40-order fraction
+// of a spacing.  The sum is kept in 2^-40 fixed point (exact, any order).
+// The energy window [e_lo, e_hi] splits into kBands bands with tops
+// E_b = fma(b + 1, (e_hi - e_lo) / kBands, e_lo); a band's window is the
+// window at its top.  The scan shoots a group of 64 trial energies in the
+// window of the band holding the group's highest energy; a level is refined
+// in the window of its bracket's upper scan point.
+constexpr int kBands = 8;
+constexpr int kWindowDepth = 20;
+
+// One band window of a potential: Dirichlet points lo, hi (on the trimmed
+// grid) and the far-end scale of its n = hi - lo + 1 point scan shots
+// (12^(n-2) = far_scale 2^far_exp); seg_row is the first row of its segment
+// table.
+struct BandWindow {
+    int32_t lo, hi;
+    int32_t far_exp;
+    int32_t seg_row;
+    double far_scale;
+};
+static_assert(sizeof(BandWindow) == 24, "window layout");
+
+// One segment table to build: the n-point window from point lo of a
+// potential's trimmed grid, stored from row seg_row.
+struct TableJob {
+    int32_t potential;
+    int32_t lo, n;
+    int32_t seg_row;
+};
+
 // Work item of one wavefront: potential index and the first trial energy
 // (scan) or first level (refine) of its 64 lanes.
 struct WaveTask {

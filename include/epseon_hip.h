@@ -42,7 +42,7 @@ extern "C" {
 #define EPSEON_LEVEL_UNRESOLVED 3 /* potential rejected: grid too coarse for the window */
 
 /* ---- refinement modes (epseon_set_refine_mode) ------------------------- */
-#define EPSEON_REFINE_AUTO 0       /* LANE for batches of >= 131072 levels, else SEGMENTED */
+#define EPSEON_REFINE_AUTO 0       /* SEGMENTED with windows in use; else LANE for >= 131072 levels */
 #define EPSEON_REFINE_LANE 1       /* one lane per level, sequential shots */
 #define EPSEON_REFINE_SEGMENTED 2  /* one wavefront per level, shots split over 64 grid segments */
 
@@ -113,8 +113,9 @@ typedef struct epseon_stats {
     double scan_ms;               /* device time of the last epseon_scan (HIP events) */
     double refine_ms;             /* device time of the last epseon_refine */
     double normalize_ms;          /* device time of the last epseon_normalize */
-    double scan_point_updates;    /* sum over potentials of (N - 2) * M */
-    double refine_point_updates;  /* sum over levels of shots * (N - 2) */
+    double scan_point_updates;    /* recurrence steps of the last scan: sum over trial energies of the
+                                     points of their shot's window - 2 (N - 2 without windows) */
+    double refine_point_updates;  /* sum over levels of shots * (window points - 2) */
     double normalize_point_updates; /* sum over converged levels of (N - 2) */
     int64_t scan_launches;        /* kernel launches of the last scan */
     int64_t levels_converged;     /* levels with EPSEON_LEVEL_OK after the last refine */
@@ -124,6 +125,8 @@ typedef struct epseon_stats {
     double vjp_point_updates;     /* sum over converged levels of (N - 2) per adjoint pass */
     double solve_ms;              /* device time of the last epseon_solve (both streams) */
     double solve_split;           /* scan-work fraction of the first pipeline group (0: sequential) */
+    int32_t scan_windows;         /* 1 when the last scan (and its refinement) used band windows */
+    int32_t reserved;
 } epseon_stats;
 
 typedef struct epseon_ctx epseon_ctx;
@@ -181,8 +184,25 @@ int epseon_load_potentials(epseon_ctx* ctx, int32_t count, const epseon_potentia
 int epseon_load_potentials_device(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc,
                                   const double* v_device, const int64_t* v_offsets);
 
-/* Derived values of potential p after epseon_load_potentials. */
+/* Derived values of potential p after epseon_load_potentials.  far_exponent
+ * is that of whole-grid scans; windowed scans: epseon_band_windows_get. */
 int epseon_potential_info_get(epseon_ctx* ctx, int32_t p, epseon_potential_info* info);
+
+/* Energy bands per potential: the window [e_lo, e_hi] splits into bands with
+ * tops E_b = fma(b + 1, (e_hi - e_lo) / EPSEON_BANDS, e_lo). */
+#define EPSEON_BANDS 8
+
+/* Shot windows of potential p for the last scan (or, before any scan, for
+ * the current tuning): for each band b, the Dirichlet points lo[b] and hi[b]
+ * on the trimmed grid and the exponent of the scan's far ends in it (a trial
+ * energy E_j of the scan group j / 64 whose highest energy lies in band b has
+ * w = far_end * 2^(far_exponent[b] - 480 rescales)).  Windowed: hi[b] is the
+ * first point past the last one with V <= E_b where the sum of the growth
+ * bound g(x) = 2^(floor((k-1)/2) - 1), x = 12 beta (V_i - E_b) = m 2^k with m
+ * in [1/2, 1), reaches 20 (else the grid end); lo[b] likewise on the left.
+ * Whole grid: lo = 0, hi = points - 1.  Each array holds EPSEON_BANDS
+ * entries.  No counterpart in the reference, whose shots span the grid. */
+int epseon_band_windows_get(epseon_ctx* ctx, int32_t p, int32_t* lo, int32_t* hi, int32_t* far_exponent);
 
 /* Numerov scan: shoot every trial energy of every potential, recording node
  * count, far-end mantissa and scale count (w_{N-1} = far_end *
@@ -223,15 +243,24 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter);
 int epseon_set_solve_split(epseon_ctx* ctx, double fraction);
 
 /* Launch configuration of a context.  Every field only changes speed, except
- * refine_mode and segmented_lanes, which change the levels within the
- * refinement tolerance (lane and segmented shots round differently). */
+ * refine_mode, segmented_lanes and windows, which change the levels within
+ * the refinement tolerance (lane and segmented shots round differently;
+ * windowed shots stop at other Dirichlet points) and, for windows, the scan's
+ * far-end values (node counts stay the same). */
 typedef struct epseon_tuning {
     int32_t refine_mode;          /* EPSEON_REFINE_* */
     int32_t levels_per_lane;      /* lane mode: 1, or 2 for potentials with more than 64 levels */
     int32_t scan_waves_per_block; /* wavefronts per scan workgroup, 1..4 */
     int32_t segmented_lanes;      /* segmented mode: 0 (from the batch size), 64, 128 or 256 */
-    int32_t xcd_scan;             /* 1: whole-batch scans bin potentials per XCD (default), 0: in order */
-    int32_t reserved;             /* 0 */
+    int32_t xcd_scan;             /* 1: whole-batch scans and segmented refinements bin potentials per
+                                     XCD (default), 0: in order */
+    int32_t windows;              /* 1: shots stop 20 guaranteed e-folds past the outermost turning
+                                     points of their energy band's top (EPSEON_BANDS bands per
+                                     potential); 0: every shot spans the trimmed grid; 2 (default):
+                                     1 when the scan has at least 8 wavefronts (64 trial energies
+                                     each) per SIMD of the device, else 0.  Lane-mode refinement
+                                     always shoots the whole grid, so with windows in use AUTO
+                                     selects SEGMENTED and a windowed scan is refined segmented. */
     double solve_split;           /* see epseon_set_solve_split */
 } epseon_tuning;
 

@@ -87,7 +87,7 @@ def _lib() -> ctypes.CDLL:
         "eo_shoot_full_count": (i32, [_dp, i64, d]),
         "eo_trajectory": (None, [_dp, i64, d, _dp]),
         "eo_scan": (None, [_dp, i64, d, d, d, i64, i32, _lp, _ip, _dp, _ip]),
-        "eo_bracket_level": (None, [_ip, _dp, _ip, i64, d, d, d, i32, _ip, ctypes.POINTER(Bracket)]),
+        "eo_bracket_level": (None, [_ip, _dp, _ip, i64, d, d, d, i32, _lp, _ip, ctypes.POINTER(Bracket)]),
         "eo_window": (None, [_dp, i64, d, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "eo_band_windows": (None, [_dp, i64, d, d, d, _lp]),
         "eo_full_windows": (None, [i64, _lp]),
@@ -276,18 +276,24 @@ def scan(pt: np.ndarray, b: float, e_lo: float, e_hi: float, m: int, stride: int
     return nodes, far, resc
 
 
-def bracket_level(nodes, far, resc, e_lo: float, e_hi: float, v: int, far_exp) -> Bracket:
-    """Bracket of level v from scan arrays; ``far_exp`` is one far-end
-    exponent for every band (whole-grid scans) or one per band."""
+def bracket_level(nodes, far, resc, e_lo: float, e_hi: float, v: int, far_exp,
+                  win: np.ndarray | None = None) -> Bracket:
+    """Bracket of level v from scan arrays.  ``win`` holds the band windows
+    of the scan (None: one window, the whole grid, for every band) and
+    ``far_exp`` their far-end exponents (one for all bands, or one each)."""
     nodes = np.ascontiguousarray(nodes, dtype=np.int32)
     far = _f64(far)
     resc = np.ascontiguousarray(resc, dtype=np.int32)
     fe = np.ascontiguousarray(np.broadcast_to(np.asarray(far_exp, dtype=np.int32), (BANDS,)))
+    if win is None:
+        win = np.zeros((BANDS, 2), dtype=np.int64)
+    win = np.ascontiguousarray(win, dtype=np.int64)
     br = Bracket()
     m = len(nodes)
     _lib().eo_bracket_level(
         nodes.ctypes.data_as(_ip), far.ctypes.data_as(_dp), resc.ctypes.data_as(_ip), m,
-        e_lo, trial_spacing(e_lo, e_hi, m), e_hi, v, fe.ctypes.data_as(_ip), ctypes.byref(br),
+        e_lo, trial_spacing(e_lo, e_hi, m), e_hi, v, win.ctypes.data_as(_lp), fe.ctypes.data_as(_ip),
+        ctypes.byref(br),
     )
     return br
 
@@ -398,7 +404,7 @@ class Problem:
         """Refine level v from scan arrays as the solve does (bracket, band
         window, refinement): (level or NaN, iterations)."""
         win = self.windows(on)
-        br = bracket_level(nodes, far, resc, self.e_lo, self.e_top, v, band_far_exps(win))
+        br = bracket_level(nodes, far, resc, self.e_lo, self.e_top, v, band_far_exps(win), win)
         if not br.found:
             return float("nan"), 0
         lo, hi = win[br.band]

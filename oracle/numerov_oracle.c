@@ -523,14 +523,19 @@ static void eo_model_apply(double b, double e0, double e, double* w, int32_t* k)
 
 /* Level v lies between the first trial energy with more than v nodes and the
  * trial energy before it (or the window floor, below which no level exists
- * because every eigenvalue exceeds min V).  far_exp[b] is the far-end
- * exponent of band b's window.  Scan values from a group of another band
- * than the upper end's are not comparable with the level's shots: such a
- * lower end is flagged (lo_foreign) and a third point of the model must lie
- * in the same band. */
+ * because every eigenvalue exceeds min V).  win and far_exp hold band b's
+ * window (win[2 b], win[2 b + 1]) and far-end exponent.  Scan values from a
+ * group whose band window differs from the upper end's are not comparable
+ * with the level's shots: such a lower end is flagged (lo_foreign) and a
+ * third point of the model must share the window. */
+static int32_t eo_same_window(const int64_t* win, int32_t a, int32_t b)
+{
+    return win[2 * a] == win[2 * b] && win[2 * a + 1] == win[2 * b + 1];
+}
+
 void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t* rescales,
                       int64_t m, double e_lo, double de, double e_hi, int32_t v,
-                      const int32_t* far_exp, eo_bracket* out)
+                      const int64_t* win, const int32_t* far_exp, eo_bracket* out)
 {
     int64_t lo = 0, hi = m;
     while (lo < hi) {
@@ -566,16 +571,18 @@ void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t*
         out->w_lo = w_last[j - 1];
         out->k_lo = far_exp[bl] - rescales[j - 1] * 480;
         out->lo_valid = 1;
-        out->lo_foreign = bl != band;
+        out->lo_foreign = !eo_same_window(win, bl, band);
     }
     /* Model rate from a third scan point in the same count regime: above
      * (j + 1, same count as j) or below (j - 2, same count as j - 1), all
      * three in the level's band. */
     if (out->lo_valid && !out->lo_foreign && out->c_lo == v && out->c_hi == v + 1) {
-        if (j + 1 < m && nodes[j + 1] == out->c_hi && eo_group_band(j + 1, m, e_lo, de, e_hi) == band)
+        if (j + 1 < m && nodes[j + 1] == out->c_hi &&
+            eo_same_window(win, eo_group_band(j + 1, m, e_lo, de, e_hi), band))
             out->rate = eo_model_rate(out->w_lo, out->k_lo, out->w_hi, out->k_hi, w_last[j + 1],
                                       fe - rescales[j + 1] * 480, 1, de);
-        else if (j >= 2 && nodes[j - 2] == out->c_lo && eo_group_band(j - 2, m, e_lo, de, e_hi) == band)
+        else if (j >= 2 && nodes[j - 2] == out->c_lo &&
+                 eo_same_window(win, eo_group_band(j - 2, m, e_lo, de, e_hi), band))
             out->rate = eo_model_rate(w_last[j - 2], fe - rescales[j - 2] * 480, out->w_lo, out->k_lo,
                                       out->w_hi, out->k_hi, 0, de);
     }
@@ -974,7 +981,7 @@ void eo_solve(const double* pt, const double* uv, int64_t n, double beta, double
 #pragma omp parallel for schedule(dynamic, 1)
     for (int32_t l = 0; l < n_levels; ++l) {
         eo_bracket br;
-        eo_bracket_level(nodes, w_last, rescales, m, e_lo, de, e_hi, v_min + l, far_exp, &br);
+        eo_bracket_level(nodes, w_last, rescales, m, e_lo, de, e_hi, v_min + l, win, far_exp, &br);
         if (!br.found) {
             levels[l] = NAN;
             iterations[l] = 0;

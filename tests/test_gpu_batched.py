@@ -41,8 +41,11 @@ def _pot(capi, spec, table, levels):
                           e_top=spec.de, n_energies=spec.m, level_min=0, level_count=levels)
 
 
-def _compare(prob, nodes, far, resc, lv, it, levels, seg, lanes):
-    on, of, orr, olv, oit = oracle.solve(prob, 0, levels, 1e-10, 200, segmented=seg, lanes=lanes)
+def _compare(prob, nodes, far, resc, lv, it, levels, seg, lanes, windows=None):
+    # Lane mode shoots the whole grid; segmented mode the band windows.
+    windows = seg if windows is None else windows
+    on, of, orr, olv, oit = oracle.solve(prob, 0, levels, 1e-10, 200, segmented=seg, lanes=lanes,
+                                         windows=windows)
     np.testing.assert_array_equal(nodes, on)
     np.testing.assert_array_equal(resc, orr)
     np.testing.assert_array_equal(far.view(np.int64), of.view(np.int64))
@@ -60,12 +63,18 @@ def test_batched_subset_parity(capi, mode):
     tables = [s.table() for s in specs]
     counts = [s.bound_levels() + 2 for s in specs]
     with capi.Context(0) as ctx:
-        ctx.set_refine_mode(capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE)
+        # Segmented with band windows (forced: 40 potentials are below the
+        # automatic threshold), lane over the whole grid.
+        t = ctx.tuning()
+        t.refine_mode = capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE
+        t.windows = int(seg)
+        ctx.set_tuning(t)
         ctx.load([_pot(capi, s, t, c) for s, t, c in zip(specs, tables, counts)])
         ctx.solve(1e-10, 200)
         nodes, far, resc = ctx.scan_results()
         lv, st, it = ctx.levels()
         lanes = ctx.stats().refine_lanes
+        assert ctx.stats().scan_windows == int(seg)
         flags = [ctx.potential_info(i).flags for i in range(len(specs))]
     eo = lo = 0
     for i, s in enumerate(specs):
@@ -96,7 +105,7 @@ def test_batched_full_scale(capi):
     _log(f"  refine mode {stats.refine_mode}, lanes {stats.refine_lanes}, solve {stats.solve_ms:.2f} ms, "
          f"refine {stats.refine_ms:.2f} ms, levels converged {stats.levels_converged}, "
          f"unresolved potentials {int(flags.sum())}")
-    assert stats.refine_mode == capi.REFINE_LANE
+    assert stats.refine_mode == capi.REFINE_SEGMENTED and stats.refine_lanes == 64 and stats.scan_windows == 1
     assert np.all(st != 2), "a level hit the iteration cap"
     eo = lo = 0
     resolved_checked = 0
@@ -143,7 +152,7 @@ def test_batched_full_scale(capi):
         prob.v = tables[i]
         e0, l0 = starts_e[i], starts_l[i]
         _compare(prob, nodes[e0 : e0 + s.m], far[e0 : e0 + s.m], resc[e0 : e0 + s.m],
-                 lv[l0 : l0 + counts[i]], it[l0 : l0 + counts[i]], counts[i], False, 1)
+                 lv[l0 : l0 + counts[i]], it[l0 : l0 + counts[i]], counts[i], True, 64)
 
 
 def test_result_preserving_tunings_are_bit_identical(capi):
@@ -159,7 +168,7 @@ def test_result_preserving_tunings_are_bit_identical(capi):
         for wpb, lpl, split, xcd in ((1, 1, 0.0, 1), (2, 2, 0.0, 0), (4, 1, 0.3, 1), (1, 2, 0.7, 0),
                                      (2, 1, 0.0, 1)):
             t = capi.Tuning(refine_mode=capi.REFINE_LANE, levels_per_lane=lpl, scan_waves_per_block=wpb,
-                            segmented_lanes=0, xcd_scan=xcd, reserved=0, solve_split=split)
+                            segmented_lanes=0, xcd_scan=xcd, windows=0, solve_split=split)
             ctx.set_tuning(t)
             ctx.solve(1e-10, 200)
             nodes, far, resc = ctx.scan_results()
@@ -171,6 +180,42 @@ def test_result_preserving_tunings_are_bit_identical(capi):
     for other in outs[1:]:
         for a, b in zip(outs[0], other):
             np.testing.assert_array_equal(a, b)
+
+
+def test_result_preserving_tunings_windowed(capi):
+    """The same schedule knobs with windowed shots and segmented refinement:
+    bit-identical outputs, equal to the oracle's windowed solve."""
+    specs = batched(4096)[:300]
+    _log("case: first 300 batched potentials, segmented refine with band windows, tunings varied")
+    counts = [s.bound_levels() + 2 for s in specs]
+    tables = [s.table() for s in specs]
+    outs = []
+    with capi.Context(0) as ctx:
+        ctx.load([_pot(capi, s, t, c) for s, t, c in zip(specs, tables, counts)])
+        for wpb, split, xcd in ((1, 0.0, 1), (2, 0.3, 0), (4, 0.0, 0)):
+            t = capi.Tuning(refine_mode=capi.REFINE_SEGMENTED, levels_per_lane=1, scan_waves_per_block=wpb,
+                            segmented_lanes=64, xcd_scan=xcd, windows=1, solve_split=split)
+            ctx.set_tuning(t)
+            ctx.solve(1e-10, 200)
+            nodes, far, resc = ctx.scan_results()
+            lv, st, it = ctx.levels()
+            stats = ctx.stats()
+            _log(f"  wpb {wpb} split {split} xcd {xcd}: solve {stats.solve_ms:.2f} ms")
+            outs.append((nodes, far.view(np.int64), resc, lv.view(np.int64), st, it))
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            np.testing.assert_array_equal(a, b)
+    nodes, far, resc, lvi, st, it = outs[0]
+    lv = lvi.view(np.float64)
+    eo = lo = 0
+    for i, s in enumerate(specs[:40]):
+        prob = problem_from_spec(s)
+        prob.v = tables[i]
+        if prob.usable:
+            _compare(prob, nodes[eo : eo + s.m], far.view(np.float64)[eo : eo + s.m], resc[eo : eo + s.m],
+                     lv[lo : lo + counts[i]], it[lo : lo + counts[i]], counts[i], True, 64)
+        eo += s.m
+        lo += counts[i]
 
 
 def test_autotune_then_parity(capi):
@@ -194,6 +239,7 @@ def test_autotune_then_parity(capi):
         prob.v = tables[i]
         if prob.usable:
             _compare(prob, nodes[eo : eo + s.m], far[eo : eo + s.m], resc[eo : eo + s.m],
-                     lv[lo : lo + counts[i]], it[lo : lo + counts[i]], counts[i], seg, stats.refine_lanes)
+                     lv[lo : lo + counts[i]], it[lo : lo + counts[i]], counts[i], seg, stats.refine_lanes,
+                     windows=bool(stats.scan_windows))
         eo += s.m
         lo += counts[i]

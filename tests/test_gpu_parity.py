@@ -12,7 +12,8 @@ import numpy as np
 import pytest
 
 import oracle
-from tests.helpers import CFG_H2, morse_problem, random_morse_batch
+from epseon_backend_amd.workloads import batched, double_wells
+from tests.helpers import CFG_H2, morse_problem, problem_from_spec, random_morse_batch
 
 pytestmark = pytest.mark.gpu
 
@@ -44,16 +45,29 @@ def _to_capi(capi, prob: oracle.Problem, level_min: int = 0, level_count: int = 
     )
 
 
+def _set_mode(ctx, capi, seg: bool) -> None:
+    """Lane mode over the whole grid, or segmented mode with band windows
+    (forced on: these batches are too small for the automatic choice)."""
+    t = ctx.tuning()
+    t.refine_mode = capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE
+    t.windows = int(seg)
+    ctx.set_tuning(t)
+
+
 def test_device_is_gfx950(capi):
     info = capi.device_info(0)
     _log(f"device: {info.name.decode()} {info.arch.decode()} CUs={info.compute_units}")
     assert info.arch.decode().startswith("gfx950")
 
 
-def test_scan_parity_h2(capi):
-    _log("case: scan parity, H2 Morse N=10000 M=2048")
+@pytest.mark.parametrize("windows", [True, False])
+def test_scan_parity_h2(capi, windows):
+    _log(f"case: scan parity, H2 Morse N=10000 M=2048, windows {windows}")
     prob = morse_problem(**CFG_H2)
     with capi.Context(0) as ctx:
+        t = ctx.tuning()
+        t.windows = int(windows)
+        ctx.set_tuning(t)
         ctx.load([_to_capi(capi, prob)])
         info = ctx.potential_info(0)
         assert info.beta == prob.beta
@@ -61,11 +75,43 @@ def test_scan_parity_h2(capi):
         assert info.e_lo == prob.e_lo
         ctx.scan()
         nodes, far, resc = ctx.scan_results()
-    on, of, orr = oracle.scan(prob.pt, prob.beta, prob.e_lo, prob.e_top, prob.m, prob.stride)
+        lo, hi, fe = ctx.band_windows(0)
+    win = prob.windows(windows)
+    np.testing.assert_array_equal(lo, win[:, 0])
+    np.testing.assert_array_equal(hi, win[:, 1])
+    np.testing.assert_array_equal(fe, oracle.band_far_exps(win))
+    on, of, orr = prob.scan(windows)
     np.testing.assert_array_equal(nodes, on)
     np.testing.assert_array_equal(resc, orr)
     np.testing.assert_array_equal(far.view(np.int64), of.view(np.int64))
     assert nodes.max() == 17
+
+
+def test_band_windows_match_oracle(capi):
+    """The device's band windows (fixed-point depth sums in wavefront prefix
+    sums) equal the oracle's sequential ones, for a batch of uneven grids."""
+    _log("case: band windows of 40 batched potentials and 2 double wells vs the oracle")
+    specs = batched(4096)[:40] + double_wells(64, n_points=200000, n_energies=4096)[:2]
+    probs = [problem_from_spec(s) for s in specs]
+    with capi.Context(0) as ctx:
+        t = ctx.tuning()
+        t.windows = 1
+        ctx.set_tuning(t)
+        ctx.load([_to_capi(capi, p, 0, 4) for p in probs])
+        got = [ctx.band_windows(i) for i in range(len(probs))]
+        t.windows = 0
+        ctx.set_tuning(t)
+        whole = [ctx.band_windows(i) for i in range(len(probs))]
+    for p, (lo, hi, fe) in zip(probs, whole):
+        np.testing.assert_array_equal(lo, 0)
+        np.testing.assert_array_equal(hi, p.trimmed[1] - p.trimmed[0])
+    for p, (lo, hi, fe) in zip(probs, got):
+        if not p.usable:
+            continue
+        win = p.windows()
+        np.testing.assert_array_equal(lo, win[:, 0])
+        np.testing.assert_array_equal(hi, win[:, 1])
+        np.testing.assert_array_equal(fe, oracle.band_far_exps(win))
 
 
 @pytest.mark.parametrize("mode", ["lane", "segmented"])
@@ -74,14 +120,17 @@ def test_solve_parity_h2(capi, mode):
     prob = morse_problem(**CFG_H2)
     seg = mode == "segmented"
     with capi.Context(0) as ctx:
-        ctx.set_refine_mode(capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE)
+        _set_mode(ctx, capi, seg)
         ctx.load([_to_capi(capi, prob, 0, 20)])
         ctx.solve(1e-10, 200)
         lv, st, it = ctx.levels()
         stats = ctx.stats()
         assert stats.refine_mode == (capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE)
+        assert stats.scan_windows == int(seg)
     _log(f"  refine lanes per level: {stats.refine_lanes}")
-    _, _, _, olv, oit = oracle.solve(prob, 0, 20, 1e-10, 200, segmented=seg, lanes=stats.refine_lanes)
+    # Lane mode shoots the whole grid; segmented mode the band windows.
+    _, _, _, olv, oit = oracle.solve(prob, 0, 20, 1e-10, 200, segmented=seg, lanes=stats.refine_lanes,
+                                     windows=seg)
     np.testing.assert_array_equal(st[:17], 0)
     np.testing.assert_array_equal(st[17:], 1)
     np.testing.assert_array_equal(lv[:17].view(np.int64), olv[:17].view(np.int64))
@@ -99,7 +148,7 @@ def test_ragged_batch_parity(capi, mode):
     probs = random_morse_batch(12, seed=7, n_range=(300, 6000), m_range=(40, 700))
     seg = mode == "segmented"
     with capi.Context(0) as ctx:
-        ctx.set_refine_mode(capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE)
+        _set_mode(ctx, capi, seg)
         ctx.load([_to_capi(capi, p, 0, 30) for p in probs])
         ctx.solve(1e-10, 200)
         nodes, far, resc = ctx.scan_results()
@@ -116,7 +165,7 @@ def test_ragged_batch_parity(capi, mode):
             eo += p.m
             lo += 30
             continue
-        on, of, orr, olv, oit = oracle.solve(p, 0, 30, 1e-10, 200, segmented=seg, lanes=lanes)
+        on, of, orr, olv, oit = oracle.solve(p, 0, 30, 1e-10, 200, segmented=seg, lanes=lanes, windows=seg)
         np.testing.assert_array_equal(nodes[eo : eo + p.m], on)
         np.testing.assert_array_equal(resc[eo : eo + p.m], orr)
         np.testing.assert_array_equal(far[eo : eo + p.m].view(np.int64), of.view(np.int64))

@@ -71,19 +71,17 @@ def test_double_wells_full_size(capi):
         assert np.all(np.diff(nd) >= 0)
         assert np.all(np.diff(found) > 0)
         assert len(found) == min(counts[p], int(nd[-1]))
-        # Sampled trial energies: single oracle shots, bit-exact.
-        energies = oracle.trial_energies(prob.e_lo, prob.e_top, s.m)
-        ut = prob.pt
+        # Sampled trial energies: single oracle shots in the scan group's
+        # band window, bit-exact.
+        windows = bool(stats.scan_windows)
         for j in rng.choice(s.m, 24, replace=False):
-            c, w, k = oracle.shoot(ut, prob.beta * energies[j], prob.stride)
+            c, w, k = prob.scan_shot(int(j), windows)
             assert (c, k) == (int(nd[j]), int(rs[j])), (p, j)
             assert np.float64(w).view(np.int64) == fr[j].view(np.int64), (p, j)
         # Sampled levels: the oracle refines from the GPU's scan arrays.
-        uv = prob.uv
         for v in sorted(rng.choice(len(found), 4, replace=False)):
-            br = oracle.bracket_level(nd, fr, rs, prob.e_lo, prob.e_top, int(v), prob.far_exp)
-            ref, its = oracle.refine_level(uv, prob.beta, prob.stride, int(v), br, 1e-10, prob.abs_tol, 200,
-                                           segmented=seg, lanes=stats.refine_lanes)
+            ref, its = prob.refine(nd, fr, rs, int(v), 1e-10, 200, segmented=seg, lanes=stats.refine_lanes,
+                                   on=windows)
             assert np.float64(ref).view(np.int64) == lev[v].view(np.int64), (p, v)
             assert its == int(it[lo + v]), (p, v)
         eo += s.m

@@ -234,6 +234,11 @@ struct epseon_ctx {
     // n_points per level), so the levels of a potential read its tables
     // through one L2.
     WaveTask* d_xcd_level_tasks = nullptr;
+    // Windowed scans: the same tasks binned by their windowed work, longest
+    // first within each bin.
+    WaveTask* d_xcd_scan_tasks_band = nullptr;
+    int32_t* d_xcd_bins_band = nullptr;
+    int32_t xcd_max_bin_band = 0;
     int32_t* d_xcd_level_bins = nullptr;
     int32_t xcd_max_level_bin = 0;
     bool xcd_scan = true;
@@ -456,15 +461,31 @@ namespace {
 
 // Shared body of epseon_load_potentials (v_host) and
 // epseon_load_potentials_device (v_dev).
+// Band of the scan group starting at trial energy j0 (as the kernels
+// compute it).
+int32_t host_group_band(const PotentialRecord& r, int32_t j0)
+{
+    const int32_t jt = std::min(j0 + 63, r.n_energies - 1);
+    const double et = std::fma(static_cast<double>(jt + 1), r.de, r.e_lo);
+    const double step = (r.e_hi - r.e_lo) / kBands;
+    for (int32_t b = 0; b < kBands - 1; ++b)
+        if (std::fma(static_cast<double>(b + 1), step, r.e_lo) >= et)
+            return b;
+    return kBands - 1;
+}
+
 // Per-XCD bins of a task list (potentials in task order, potential k's
-// tasks at [cum[k], cum[k + 1]), each worth work[k]).  Each potential's tasks
-// are cut into chunks of at most ceil(tasks / 32) (so a single large
+// tasks at [cum[k], cum[k + 1]), task t worth work[t]).  Each potential's
+// tasks are cut into chunks of at most ceil(tasks / 32) (so a single large
 // potential still spreads over every XCD, while typical batch members stay
-// whole); chunks go by decreasing work to the least-loaded bin, and each bin
-// keeps the task order.  Returns the concatenated bins; `bins` gets the 9
-// offsets and `max_bin` the largest bin's size.
+// whole); chunks go by decreasing work to the least-loaded bin.  Each bin
+// keeps the chunks in task order (a potential's tasks stay together, so its
+// table stays in that XCD's L2); with `longest_first` the tasks of each
+// chunk go by decreasing work (stable).  Returns the concatenated bins;
+// `bins` gets the 9 offsets and `max_bin` the largest bin's size.
 std::vector<WaveTask> xcd_binned(const std::vector<WaveTask>& tasks, const std::vector<int32_t>& cum,
-                                 const std::vector<double>& work, std::vector<int32_t>& bins, int32_t& max_bin)
+                                 const std::vector<double>& work, bool longest_first, std::vector<int32_t>& bins,
+                                 int32_t& max_bin)
 {
     struct Chunk {
         int32_t t0, t1;
@@ -475,7 +496,10 @@ std::vector<WaveTask> xcd_binned(const std::vector<WaveTask>& tasks, const std::
     for (size_t k = 0; k + 1 < cum.size(); ++k)
         for (int32_t t = cum[k]; t < cum[k + 1]; t += cmax) {
             const int32_t e = std::min(cum[k + 1], t + cmax);
-            chunks.push_back({t, e, work[k] * (e - t)});
+            double w = 0.0;
+            for (int32_t i = t; i < e; ++i)
+                w += work[static_cast<size_t>(i)];
+            chunks.push_back({t, e, w});
         }
     std::vector<size_t> by_work(chunks.size());
     std::iota(by_work.begin(), by_work.end(), size_t(0));
@@ -494,10 +518,19 @@ std::vector<WaveTask> xcd_binned(const std::vector<WaveTask>& tasks, const std::
     max_bin = 0;
     for (int x = 0; x < 8; ++x) {
         bins[static_cast<size_t>(x)] = static_cast<int32_t>(out.size());
-        for (size_t c = 0; c < chunks.size(); ++c)  // chunks are in task order
-            if (chunk_bin[c] == x)
-                for (int32_t t = chunks[c].t0; t < chunks[c].t1; ++t)
-                    out.push_back(tasks[static_cast<size_t>(t)]);
+        for (size_t c = 0; c < chunks.size(); ++c) {  // chunks are in task order
+            if (chunk_bin[c] != x)
+                continue;
+            std::vector<int32_t> ids;
+            for (int32_t t = chunks[c].t0; t < chunks[c].t1; ++t)
+                ids.push_back(t);
+            if (longest_first)
+                std::stable_sort(ids.begin(), ids.end(), [&](int32_t a, int32_t b) {
+                    return work[static_cast<size_t>(a)] > work[static_cast<size_t>(b)];
+                });
+            for (int32_t t : ids)
+                out.push_back(tasks[static_cast<size_t>(t)]);
+        }
         max_bin = std::max(max_bin, static_cast<int32_t>(out.size()) - bins[static_cast<size_t>(x)]);
     }
     bins[8] = static_cast<int32_t>(out.size());
@@ -744,17 +777,21 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
             (r.flags == EPSEON_POT_OK ? static_cast<double>(r.n_points) * r.n_energies : 0.0));
     }
     ctx->n_scan_tasks = static_cast<int32_t>(scan_tasks.size());
-    std::vector<double> scan_task_work(ctx->cum_scan_tasks.size() - 1), level_task_work(scan_task_work.size());
-    for (size_t k = 0; k < scan_task_work.size(); ++k) {
-        const int32_t a0 = ctx->cum_scan_tasks[k], a1 = ctx->cum_scan_tasks[k + 1];
-        scan_task_work[k] = a1 > a0 ? (ctx->cum_scan_work[k + 1] - ctx->cum_scan_work[k]) / (a1 - a0) : 0.0;
-        level_task_work[k] = static_cast<double>(ctx->rec[static_cast<size_t>(order[k])].n_points);
+    // Work per task: trial energies times recurrence steps (whole grid) for
+    // the scan, grid points for a level's refinement.
+    std::vector<double> scan_task_work(scan_tasks.size()), level_task_work(level_tasks.size());
+    for (size_t t = 0; t < scan_tasks.size(); ++t) {
+        const PotentialRecord& r = ctx->rec[static_cast<size_t>(scan_tasks[t].potential)];
+        scan_task_work[t] = static_cast<double>(std::min(64, r.n_energies - scan_tasks[t].first)) *
+                            static_cast<double>(r.n_points - 2);
     }
+    for (size_t t = 0; t < level_tasks.size(); ++t)
+        level_task_work[t] = static_cast<double>(ctx->rec[static_cast<size_t>(level_tasks[t].potential)].n_points);
     std::vector<int32_t> xcd_bins, xcd_level_bins;
     const std::vector<WaveTask> xcd_tasks =
-        xcd_binned(scan_tasks, ctx->cum_scan_tasks, scan_task_work, xcd_bins, ctx->xcd_max_bin);
-    const std::vector<WaveTask> xcd_level_tasks =
-        xcd_binned(level_tasks, ctx->cum_level_tasks, level_task_work, xcd_level_bins, ctx->xcd_max_level_bin);
+        xcd_binned(scan_tasks, ctx->cum_scan_tasks, scan_task_work, false, xcd_bins, ctx->xcd_max_bin);
+    const std::vector<WaveTask> xcd_level_tasks = xcd_binned(level_tasks, ctx->cum_level_tasks, level_task_work,
+                                                             false, xcd_level_bins, ctx->xcd_max_level_bin);
     ctx->n_refine_tasks = static_cast<int32_t>(refine_tasks.size());
     ctx->n_level_tasks = static_cast<int32_t>(level_tasks.size());
 
@@ -888,24 +925,29 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
             }
             win_band[i] = w;
         }
-        if (r.flags == EPSEON_POT_OK) {
-            // Performed point updates of the windowed scan: each group of 64
-            // trial energies over its band window.
-            const double step = (r.e_hi - r.e_lo) / kBands;
-            for (int32_t j0 = 0; j0 < r.n_energies; j0 += 64) {
-                const int32_t jt = std::min(j0 + 63, r.n_energies - 1);
-                const double et = std::fma(static_cast<double>(jt + 1), r.de, r.e_lo);
-                int32_t b = kBands - 1;
-                for (int32_t c = 0; c < kBands - 1; ++c)
-                    if (std::fma(static_cast<double>(c + 1), step, r.e_lo) >= et) {
-                        b = c;
-                        break;
-                    }
-                const BandWindow& w = win_band[static_cast<size_t>(p) * kBands + static_cast<size_t>(b)];
-                scan_pu_band += static_cast<double>(w.hi - w.lo - 1) * static_cast<double>(jt - j0 + 1);
-            }
-        }
     }
+    // Windowed scan: work per task (trial energies times its band window's
+    // steps), its total, and its own XCD bins with the longest tasks first.
+    std::vector<double> band_task_work(scan_tasks.size());
+    for (size_t t = 0; t < scan_tasks.size(); ++t) {
+        const WaveTask& task = scan_tasks[t];
+        const PotentialRecord& r = ctx->rec[static_cast<size_t>(task.potential)];
+        const BandWindow& w = win_band[static_cast<size_t>(task.potential) * kBands +
+                                       static_cast<size_t>(host_group_band(r, task.first))];
+        band_task_work[t] = static_cast<double>(std::min(64, r.n_energies - task.first)) *
+                            static_cast<double>(w.hi - w.lo - 1);
+        scan_pu_band += band_task_work[t];
+    }
+    std::vector<int32_t> band_bins;
+    const std::vector<WaveTask> band_tasks =
+        xcd_binned(scan_tasks, ctx->cum_scan_tasks, band_task_work, true, band_bins, ctx->xcd_max_bin_band);
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(band_tasks.size(), 1), &ctx->d_xcd_scan_tasks_band));
+    EPSEON_HIP_TRY(arena_array(a, size_t(9), &ctx->d_xcd_bins_band));
+    if (!band_tasks.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_scan_tasks_band, band_tasks.data(),
+                                      band_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice, ctx->stream));
+    EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_bins_band, band_bins.data(), 9 * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, ctx->stream));
     ctx->scan_pu_band = scan_pu_band;
     ctx->h_win_full = win_full;
     ctx->h_win_band = win_band;
@@ -1034,10 +1076,15 @@ hipError_t run_scan(epseon_ctx* ctx, hipStream_t stream, size_t k0, size_t k1)
     const int32_t t0 = ctx->cum_scan_tasks[k0];
     const int32_t t1 = ctx->cum_scan_tasks[k1];
     const BandWindow* win = scan_window_set(ctx);
-    if (ctx->xcd_scan && k0 == 0 && k1 == ctx->cum_scan_tasks.size() - 1)
+    if (ctx->xcd_scan && k0 == 0 && k1 == ctx->cum_scan_tasks.size() - 1) {
+        if (ctx->scan_windows)
+            return launch_scan(stream, ctx->d_ut, ctx->d_rec, win, ctx->d_xcd_scan_tasks_band, t1 - t0,
+                               ctx->d_xcd_bins_band, ctx->xcd_max_bin_band, ctx->waves_per_block,
+                               ctx->d_nodes, ctx->d_far, ctx->d_resc);
         return launch_scan(stream, ctx->d_ut, ctx->d_rec, win, ctx->d_xcd_scan_tasks, t1 - t0,
                            ctx->d_xcd_bins, ctx->xcd_max_bin, ctx->waves_per_block, ctx->d_nodes,
                            ctx->d_far, ctx->d_resc);
+    }
     return launch_scan(stream, ctx->d_ut, ctx->d_rec, win, ctx->d_scan_tasks + t0, t1 - t0, nullptr, 0,
                        ctx->waves_per_block, ctx->d_nodes, ctx->d_far, ctx->d_resc);
 }
@@ -1360,6 +1407,8 @@ int epseon_autotune(epseon_ctx* ctx, double rel_tol, int32_t max_iter, int32_t f
     (void)epseon_get_tuning(ctx, &best);
     if (best.refine_mode == EPSEON_REFINE_AUTO)
         best.refine_mode = choose_refine_mode(ctx);
+    if (best.windows == 2)
+        best.windows = windows_enabled(ctx) ? 1 : 0;
     double best_ms = 0.0;
     if (const int rc = epseon_set_tuning(ctx, &best); rc != EPSEON_OK)
         return rc;
@@ -1382,6 +1431,12 @@ int epseon_autotune(epseon_ctx* ctx, double rel_tol, int32_t max_iter, int32_t f
     for (int32_t w : {2, 4}) {
         epseon_tuning c = best;
         c.scan_waves_per_block = w;
+        if (const int rc = trial(c); rc != EPSEON_OK)
+            return rc;
+    }
+    if (!(flags & EPSEON_TUNE_KEEP_RESULTS) && best.refine_mode != EPSEON_REFINE_LANE) {
+        epseon_tuning c = best;
+        c.windows = 1 - best.windows;
         if (const int rc = trial(c); rc != EPSEON_OK)
             return rc;
     }

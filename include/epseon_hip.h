@@ -275,9 +275,10 @@ int epseon_set_tuning(epseon_ctx* ctx, const epseon_tuning* tuning);
 /* Time candidate tunings on the loaded batch (each a full epseon_solve, best
  * of two after a warm-up run, device time by HIP events), keep the fastest
  * and write it to `best` (may be NULL).  Coordinate search over scan
- * workgroup size and XCD binning, refinement mode, levels per lane / segmented block size and
- * the pipeline split.  With EPSEON_TUNE_KEEP_RESULTS the refinement mode and
- * segmented block size stay fixed.  The levels and scan results afterwards are
+ * workgroup size and XCD binning, shot windows, refinement mode, levels per lane / segmented
+ * block size and the pipeline split (an automatic choice is first resolved to
+ * its current value).  With EPSEON_TUNE_KEEP_RESULTS the windows, the
+ * refinement mode and the segmented block size stay fixed.  The levels and scan results afterwards are
  * those of the last candidate run: call epseon_solve again.  No counterpart
  * in the reference, whose HardwareConfig (hardware_config.hpp:11-76) is set
  * by hand. */

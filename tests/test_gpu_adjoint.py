@@ -193,7 +193,7 @@ def test_parameter_gradient_vs_finite_differences(capi):
             # Compare sensitivities to a relative change of each parameter.
             scale = max(abs(grads[q][j] * float(th[q][j].detach())) for q in range(3))
             assert abs(grads[k][j] - fd) * float(th[k][j].detach()) <= 1e-6 * scale, (k, j, grads[k][j], fd)
-    _log(f"  loss {float(loss):.6g}; adjoint pass {solver.last_stats.vjp_ms:.3f} ms")
+    _log(f"  loss {float(loss.detach()):.6g}; adjoint pass {solver.last_stats.vjp_ms:.3f} ms")
     solver.close()
 
 

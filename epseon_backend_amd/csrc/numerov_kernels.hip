@@ -521,60 +521,97 @@ __device__ __forceinline__ int32_t wave_max(int32_t x)
 
 }  // namespace
 
-// Band windows: one wavefront per (potential, band) writes (lo, hi) of the
-// window at the band's top energy (whole grid for rejected potentials).
+namespace {
+
+// Walk a fringe of the window at e from `start` by `dir` (+1: right, -1:
+// left), 64 points per step, until the depth sum reaches kWindowDepth;
+// returns the point where it does, or the grid end.  One wavefront.
+__device__ __forceinline__ int32_t fringe_end(const double* __restrict__ u, const int32_t n, const double e,
+                                              const int32_t start, const int32_t dir)
+{
+    const int32_t lane = static_cast<int32_t>(threadIdx.x & 63);
+    const int64_t depth = int64_t(kWindowDepth) << 40;
+    int64_t acc = 0;
+    for (int32_t base = start; base >= 0 && base < n; base += 64 * dir) {
+        const int32_t i = base + dir * lane;
+        const int64_t inc = wave_inclusive_sum(i >= 0 && i < n ? growth_term(12.0 * (u[i] - e)) : 0);
+        const uint64_t hit = __ballot(acc + inc >= depth);
+        if (hit != 0)
+            return base + dir * static_cast<int32_t>(__builtin_ctzll(hit));
+        acc += __shfl(inc, 63, 64);
+    }
+    return dir > 0 ? n - 1 : 0;
+}
+
+}  // namespace
+
+// Band windows: one 256-thread workgroup per potential.  One pass over the
+// table finds the first and last allowed point (uv_i <= e_b) of all
+// kBands band tops at once; then its four wavefronts walk the 2 kBands
+// fringes.  Writes (lo, hi) per band (whole grid for rejected potentials).
 __global__ __launch_bounds__(256) void band_window_kernel(const double* __restrict__ uv,
                                                           const PotentialRecord* __restrict__ pots,
                                                           const int32_t n_pots, int2* __restrict__ out)
 {
-    const int32_t wave = static_cast<int32_t>(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (wave >= n_pots * kBands)
+    __shared__ int32_t s_first[4][kBands], s_last[4][kBands];
+    __shared__ int32_t s_win[2][kBands];
+    const int32_t p = static_cast<int32_t>(blockIdx.x);
+    if (p >= n_pots)
         return;
-    const int32_t p = wave / kBands, b = wave % kBands;
     const PotentialRecord& P = pots[p];
-    const int32_t lane = static_cast<int32_t>(threadIdx.x & 63);
+    const int32_t tid = static_cast<int32_t>(threadIdx.x);
+    const int32_t lane = tid & 63, wave = tid >> 6;
     const int32_t n = P.n_points;
-    int32_t lo = 0, hi = n - 1;
-    if (P.flags == 0) {
-        const double top = __builtin_fma(static_cast<double>(b + 1), (P.e_hi - P.e_lo) / kBands, P.e_lo);
-        const double e = P.beta * top;
-        const double* u = uv + P.ut_offset;
-        int32_t first = n, last = -1;
-        for (int32_t i = lane; i < n; i += 64)
-            if (u[i] <= e) {
-                first = min(first, i);
-                last = i;
+    if (P.flags != 0) {
+        if (tid < kBands)
+            out[p * kBands + tid] = make_int2(0, n - 1);
+        return;
+    }
+    const double* u = uv + P.ut_offset;
+    double eb[kBands];
+    int32_t first[kBands], last[kBands];
+#pragma unroll
+    for (int32_t b = 0; b < kBands; ++b) {
+        eb[b] = P.beta * __builtin_fma(static_cast<double>(b + 1), (P.e_hi - P.e_lo) / kBands, P.e_lo);
+        first[b] = n;
+        last[b] = -1;
+    }
+    for (int32_t i = tid; i < n; i += 256) {
+        const double x = u[i];
+#pragma unroll
+        for (int32_t b = 0; b < kBands; ++b)
+            if (x <= eb[b]) {
+                first[b] = min(first[b], i);
+                last[b] = i;
             }
-        first = wave_min(first);
-        last = wave_max(last);
-        if (last >= 0) {
-            const int64_t depth = int64_t(kWindowDepth) << 40;
-            int64_t acc = 0;
-            for (int32_t base = last + 1; base < n; base += 64) {
-                const int32_t i = base + lane;
-                const int64_t inc = wave_inclusive_sum(i < n ? growth_term(12.0 * (u[i] - e)) : 0);
-                const uint64_t hit = __ballot(acc + inc >= depth);
-                if (hit != 0) {
-                    hi = base + __builtin_ctzll(hit);
-                    break;
-                }
-                acc += __shfl(inc, 63, 64);
-            }
-            acc = 0;
-            for (int32_t base = first - 1; base >= 0; base -= 64) {
-                const int32_t i = base - lane;
-                const int64_t inc = wave_inclusive_sum(i >= 0 ? growth_term(12.0 * (u[i] - e)) : 0);
-                const uint64_t hit = __ballot(acc + inc >= depth);
-                if (hit != 0) {
-                    lo = base - __builtin_ctzll(hit);
-                    break;
-                }
-                acc += __shfl(inc, 63, 64);
-            }
+    }
+#pragma unroll
+    for (int32_t b = 0; b < kBands; ++b) {
+        first[b] = wave_min(first[b]);
+        last[b] = wave_max(last[b]);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int32_t b = 0; b < kBands; ++b) {
+            s_first[wave][b] = first[b];
+            s_last[wave][b] = last[b];
         }
     }
-    if (lane == 0)
-        out[wave] = make_int2(lo, hi);
+    __syncthreads();
+    // Fringe walks: task t = 2 b + side, wavefront w takes t = w, w + 4, ...
+    for (int32_t t = wave; t < 2 * kBands; t += 4) {
+        const int32_t b = t >> 1, side = t & 1;
+        const int32_t f = min(min(s_first[0][b], s_first[1][b]), min(s_first[2][b], s_first[3][b]));
+        const int32_t l = max(max(s_last[0][b], s_last[1][b]), max(s_last[2][b], s_last[3][b]));
+        int32_t r = side ? n - 1 : 0;
+        if (l >= 0)
+            r = side ? fringe_end(u, n, eb[b], l + 1, 1) : fringe_end(u, n, eb[b], f - 1, -1);
+        if (lane == 0)
+            s_win[side][b] = r;
+    }
+    __syncthreads();
+    if (tid < kBands)
+        out[p * kBands + tid] = make_int2(s_win[0][tid], s_win[1][tid]);
 }
 
 // Scan: lane = trial energy E_j = fma(j+1, dE, E_lo) shot over the band
@@ -1488,9 +1525,8 @@ hipError_t launch_band_windows(hipStream_t stream, const double* uv, const Poten
 {
     if (n_pots == 0)
         return hipSuccess;
-    const int32_t waves = n_pots * kBands;
-    hipLaunchKernelGGL(band_window_kernel, dim3(static_cast<unsigned>((waves + 3) / 4)), dim3(256), 0, stream,
-                       uv, pots, n_pots, out);
+    hipLaunchKernelGGL(band_window_kernel, dim3(static_cast<unsigned>(n_pots)), dim3(256), 0, stream, uv, pots,
+                       n_pots, out);
     return hipGetLastError();
 }
 

@@ -750,13 +750,6 @@ __device__ __forceinline__ void model_apply(const double b, const double e0, con
 // exists since every eigenvalue exceeds min V).  Exponents are binary.  When
 // the bracket holds exactly level v and a third scan point of the same count
 // regime exists above or below it, `rate` is the model rate b.
-struct Bracket {
-    double el, eh, wl, wh, rate;
-    int32_t kl, kh, cl, ch, lo_ok, found;
-    int32_t band;        // band of the upper end's scan group: the level's window
-    int32_t lo_foreign;  // lower end's value from a different band window
-};
-
 // W: the potential's kBands band windows (those the scan used).
 __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const BandWindow* W,
                                                 const int32_t v, const int32_t* __restrict__ nodes,
@@ -937,8 +930,7 @@ template <int K>
 __global__ __launch_bounds__(256) void numerov_refine_kernel(
     const double* __restrict__ uv, const PotentialRecord* __restrict__ pots,
     const BandWindow* __restrict__ windows, const WaveTask* __restrict__ tasks, const int32_t n_tasks,
-    const int32_t* __restrict__ nodes, const double* __restrict__ far_end,
-    const int32_t* __restrict__ rescales, const double rel_tol, const int32_t max_iter,
+    const Bracket* __restrict__ brackets, const double rel_tol, const int32_t max_iter,
     double* __restrict__ levels, int32_t* __restrict__ status, int32_t* __restrict__ iterations,
     int32_t* __restrict__ steps)
 {
@@ -966,7 +958,7 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
             continue;
         }
         r[q].v = P.level_min + l;
-        r[q].b = find_bracket(P, W, r[q].v, nodes, far_end, rescales);
+        r[q].b = brackets[out[q]];
         steps[out[q]] = W[0].hi - W[0].lo - 1;
         r[q].start_model();
         if (!r[q].b.found) {
@@ -1239,10 +1231,9 @@ template <int T, bool TABLE>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(2))) void numerov_refine_segmented_kernel(
     const double* __restrict__ uv, const double* __restrict__ segtab, const PotentialRecord* __restrict__ pots,
     const BandWindow* __restrict__ windows, const WaveTask* __restrict__ tasks, const int32_t n_tasks,
-    const int32_t* __restrict__ bins, const int32_t* __restrict__ nodes, const double* __restrict__ far_end,
-    const int32_t* __restrict__ rescales, const double rel_tol, const int32_t max_iter,
-    double* __restrict__ levels, int32_t* __restrict__ status, int32_t* __restrict__ iterations,
-    int32_t* __restrict__ steps)
+    const int32_t* __restrict__ bins, const Bracket* __restrict__ brackets, const double rel_tol,
+    const int32_t max_iter, double* __restrict__ levels, int32_t* __restrict__ status,
+    int32_t* __restrict__ iterations, int32_t* __restrict__ steps)
 {
     __shared__ double s_m[4][T];
     __shared__ int32_t s_e[T];
@@ -1264,7 +1255,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(2))) void num
     const BandWindow* W = windows + task.potential * kBands;
     Refiner r{};
     r.v = P.level_min + task.first;
-    r.b = find_bracket(P, W, r.v, nodes, far_end, rescales);
+    r.b = brackets[out];
     r.start_model();
     if (!r.b.found) {
         if (lane == 0)
@@ -1405,6 +1396,29 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(2))) void num
         write_level(levels, status, iterations, out, r.level(), st, r.it);
 }
 
+// Bracket pass: one thread per level task (potential, level index) finds
+// the level's bracket in the scan arrays (binary search over its
+// potential's node counts, model rate from three scan points), so the
+// refinement's wavefronts start shooting at once.
+__global__ __launch_bounds__(256) void bracket_kernel(const PotentialRecord* __restrict__ pots,
+                                                      const BandWindow* __restrict__ windows,
+                                                      const WaveTask* __restrict__ tasks, const int32_t n_tasks,
+                                                      const int32_t* __restrict__ nodes,
+                                                      const double* __restrict__ far_end,
+                                                      const int32_t* __restrict__ rescales,
+                                                      Bracket* __restrict__ brackets)
+{
+    const int32_t t = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= n_tasks)
+        return;
+    const WaveTask task = tasks[t];
+    const PotentialRecord& P = pots[task.potential];
+    if (P.flags != 0 || task.first >= P.level_count)
+        return;
+    brackets[P.level_offset + task.first] =
+        find_bracket(P, windows + task.potential * kBands, P.level_min + task.first, nodes, far_end, rescales);
+}
+
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_potential_stats(hipStream_t stream, const double* v, const int64_t* offsets,
@@ -1447,10 +1461,20 @@ hipError_t launch_scan(hipStream_t stream, const double* ut, const PotentialReco
     return hipGetLastError();
 }
 
+hipError_t launch_brackets(hipStream_t stream, const PotentialRecord* pots, const BandWindow* windows,
+                           const WaveTask* level_tasks, int32_t n_tasks, const int32_t* nodes,
+                           const double* far_end, const int32_t* rescales, Bracket* brackets)
+{
+    if (n_tasks == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(bracket_kernel, dim3(static_cast<unsigned>((n_tasks + 255) / 256)), dim3(256), 0, stream,
+                       pots, windows, level_tasks, n_tasks, nodes, far_end, rescales, brackets);
+    return hipGetLastError();
+}
+
 hipError_t launch_refine(hipStream_t stream, int32_t levels_per_lane, const double* uv,
                          const PotentialRecord* pots, const BandWindow* windows, const WaveTask* tasks,
-                         int32_t n_tasks, int32_t waves_per_block, const int32_t* nodes,
-                         const double* far_end, const int32_t* rescales, double rel_tol,
+                         int32_t n_tasks, int32_t waves_per_block, const Bracket* brackets, double rel_tol,
                          int32_t max_iter, double* levels, int32_t* status, int32_t* iterations,
                          int32_t* steps)
 {
@@ -1460,12 +1484,10 @@ hipError_t launch_refine(hipStream_t stream, int32_t levels_per_lane, const doub
     const dim3 grid(static_cast<unsigned>(blocks)), block(static_cast<unsigned>(64 * waves_per_block));
     if (levels_per_lane == 2)
         hipLaunchKernelGGL(numerov_refine_kernel<2>, grid, block, 0, stream, uv, pots, windows, tasks,
-                           n_tasks, nodes, far_end, rescales, rel_tol, max_iter, levels, status,
-                           iterations, steps);
+                           n_tasks, brackets, rel_tol, max_iter, levels, status, iterations, steps);
     else if (levels_per_lane == 1)
         hipLaunchKernelGGL(numerov_refine_kernel<1>, grid, block, 0, stream, uv, pots, windows, tasks,
-                           n_tasks, nodes, far_end, rescales, rel_tol, max_iter, levels, status,
-                           iterations, steps);
+                           n_tasks, brackets, rel_tol, max_iter, levels, status, iterations, steps);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1477,9 +1499,8 @@ hipError_t launch_refine(hipStream_t stream, int32_t levels_per_lane, const doub
 hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, const double* uv,
                                    const double* segtab, const PotentialRecord* pots,
                                    const BandWindow* windows, const WaveTask* tasks, int32_t n_tasks,
-                                   const int32_t* bins, int32_t max_bin, const int32_t* nodes,
-                                   const double* far_end, const int32_t* rescales, double rel_tol,
-                                   int32_t max_iter, double* levels, int32_t* status,
+                                   const int32_t* bins, int32_t max_bin, const Bracket* brackets,
+                                   double rel_tol, int32_t max_iter, double* levels, int32_t* status,
                                    int32_t* iterations, int32_t* steps)
 {
     if (n_tasks == 0)
@@ -1487,8 +1508,8 @@ hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, const doub
     const dim3 grid(static_cast<unsigned>(bins == nullptr ? n_tasks : 8 * max_bin));
 #define EPSEON_SEGMENTED(T, TAB)                                                                   \
     hipLaunchKernelGGL((numerov_refine_segmented_kernel<T, TAB>), grid, dim3(T), 0, stream, uv,   \
-                       segtab, pots, windows, tasks, n_tasks, bins, nodes, far_end, rescales,      \
-                       rel_tol, max_iter, levels, status, iterations, steps)
+                       segtab, pots, windows, tasks, n_tasks, bins, brackets, rel_tol, max_iter,   \
+                       levels, status, iterations, steps)
     switch (lanes) {
     case 256:
         EPSEON_SEGMENTED(256, false);

@@ -138,6 +138,21 @@ struct TableJob {
     int32_t seg_row;
 };
 
+// Bracket of one level from the scan arrays (written by the bracket pass,
+// read by the refinement): ends el < eh with far-end values F = w 2^k (of
+// the level's band window) and node counts c, lo_ok = 0 when el is the
+// window floor (no shot there), found = 0 when no trial energy has more
+// than v nodes, rate = b of the refinement model (0: none), band = the
+// upper end's band, lo_foreign = 1 when the lower end's value comes from a
+// different band window.
+struct Bracket {
+    double el, eh, wl, wh, rate;
+    int32_t kl, kh, cl, ch, lo_ok, found;
+    int32_t band;
+    int32_t lo_foreign;
+};
+static_assert(sizeof(Bracket) == 72, "bracket layout");
+
 // Work item of one wavefront: potential index and the first trial energy
 // (scan) or first level (refine) of its 64 lanes.
 struct WaveTask {

@@ -17,6 +17,11 @@ Outputs:
   numerov_trajectories.json  y_i for small grids at a few energies
   morse_levels.json          analytic Morse levels of the two single-potential
                              workloads (H2-like and I2-like) in 30 digits
+  potential_tables.json      V(r_i) and the Morse parameter derivatives at ~20
+                             grid points of the two single-potential grids and
+                             of one double well, r_i the FP64 grid values
+                             r_min + i h (h = (r_max - r_min)/(N - 1) in FP64)
+                             taken exactly
 """
 from __future__ import annotations
 
@@ -89,5 +94,43 @@ def main() -> None:
     (HERE / "morse_levels.json").write_text(json.dumps(levels, indent=1))
 
 
+def potential_tables() -> None:
+    """Golden V(r_i) and dV/d(D_e, a, r_e) at FP64 grid points, 30 digits."""
+    # (name, D_e, r_e, a, r_min, r_max, N, barrier B, r_b, w) -- the double
+    # well is the first member of the stress batch (its float parameters).
+    specs = [
+        ("h2", 38297.0, 0.7414, 1.9426, 0.2, 5.0, 10000, 0.0, 0.0, 1.0),
+        ("i2", 12440.0, 2.666, 1.8676, 2.0, 8.0, 200000, 0.0, 0.0, 1.0),
+        ("double_well", 8193.433993653385, 2.0423502771804785, 1.616745917580468, 1.4423502771804784,
+         5.753508557993871, 1000000, 2013.4310468218557, 2.4928887652620624, 0.10133013237059119),
+    ]
+    out = {}
+    for name, de, re, a, r_min, r_max, n, bb, rb, w in specs:
+        h = (r_max - r_min) / (n - 1)  # FP64, as the tabulating code computes it
+        idx = sorted({0, 1, 2, 3, 17, 100, 1000, n // 7, n // 5, n // 3, n // 2, 2 * n // 3,
+                      3 * n // 4, 9 * n // 10, n - 100, n - 3, n - 2, n - 1})
+        rows = []
+        for i in idx:
+            r = r_min + i * h  # FP64 grid value
+            R, DE, RE, A = mp.mpf(r), mp.mpf(de), mp.mpf(re), mp.mpf(a)
+            ex = mp.e ** (-A * (R - RE))
+            x = 1 - ex
+            v = DE * x * x
+            if bb != 0.0:
+                z = (R - mp.mpf(rb)) / mp.mpf(w)
+                v += mp.mpf(bb) * mp.e ** (-(z * z))
+            rows.append({
+                "i": i, "r": repr(r), "v": mp.nstr(v, 30),
+                "dv_dde": mp.nstr(x * x, 30),
+                "dv_da": mp.nstr(2 * DE * x * ex * (R - RE), 30),
+                "dv_dre": mp.nstr(-2 * DE * A * x * ex, 30),
+            })
+        out[name] = {"de": repr(de), "re": repr(re), "a": repr(a), "r_min": repr(r_min),
+                     "r_max": repr(r_max), "n": n, "barrier": repr(bb), "r_b": repr(rb),
+                     "width": repr(w), "points": rows}
+    (HERE / "potential_tables.json").write_text(json.dumps(out, indent=1))
+
+
 if __name__ == "__main__":
     main()
+    potential_tables()

@@ -2,7 +2,8 @@
 torch.autograd front end.
 
 * eigenfunctions (epseon_get_wavefunctions) against the oracle's eo_psi at
-  the GPU's own levels (which are bit-identical to the oracle's);
+  the GPU's own levels (which are bit-identical to the oracle's), and their
+  orthonormality;
 * the adjoint (epseon_level_vjp) against the oracle's eo_level_vjp, host
   and device-pointer forms;
 * Hellmann-Feynman against central finite differences of GPU solves, both
@@ -73,6 +74,16 @@ def test_eigenfunctions_match_oracle(capi):
                                        err_msg=f"potential {p} level {l}")
             assert psi[l][prob.trimmed[0] + 1] > 0.0
         lo += counts[p]
+    # Orthonormality (the psi_v are eigenvectors of one symmetric matrix),
+    # independent of the oracle: levels at 1e-10 relative put the Gram matrix
+    # within ~1e-9 of the identity.
+    for p in (0, 1):
+        psi = ctx.wavefunctions(p)
+        h = (specs[p].r_max - specs[p].r_min) / (specs[p].n - 1)
+        gram = psi @ psi.T * h
+        err = np.abs(gram - np.eye(len(psi))).max()
+        _log(f"  potential {p}: max |Gram - I| = {err:.2e} over {len(psi)} levels")
+        assert err < 1e-8
     stats = ctx.stats()
     _log(f"  normalize {stats.normalize_ms:.3f} ms over {stats.normalize_point_updates:.3g} point-updates")
     # One launch for every potential: the same bits, in load order.

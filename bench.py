@@ -47,7 +47,14 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
 
 FP64_VECTOR_PEAK_TFLOPS = 78.6  # MI355X: 256 CU x 128 FP64 FLOP/clk x 2.4 GHz
-FLOP_PER_POINT_UPDATE = 7  # sub + FMA + mul + mul + FMA (FMA = 2)
+# Algorithmic FLOP of one Numerov step (the count the workload's roofline is
+# quoted in); the scan kernel issues 4 FP64 instructions for it (sub, mul,
+# 2 FMA).
+FLOP_PER_POINT_UPDATE = 7
+# Refinement shots: a segmented shot carries two columns of the transfer
+# matrix (12 FP64 instructions, 4 of them FMA, per grid point), a lane-mode
+# shot one difference-form column (8, 2 of them FMA).
+REFINE_FLOP_PER_POINT_UPDATE = {"segmented": 16, "lane": 10}
 METRIC = "Numerov point-updates/s & eigenlevels/s at 1e-9 rel err; grad/fwd time ratio"
 
 def single_workload(capi):
@@ -459,6 +466,18 @@ def main() -> int:
             "traffic": load_traffic(args.workload),
         },
     }
+    mode = "segmented" if s.refine_mode == capi.REFINE_SEGMENTED else "lane"
+    refine_achieved = (REFINE_FLOP_PER_POINT_UPDATE[mode] * (refine_pu / args.steps)
+                       / (float(np.mean(refine_ms)) * 1e-3) / 1e12)
+    out["refine_roofline"] = {
+        "kernel": f"numerov_refine ({mode})",
+        "bound": "valu",
+        "achieved": refine_achieved,
+        "peak": FP64_VECTOR_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": refine_achieved / FP64_VECTOR_PEAK_TFLOPS,
+    }
+    out["scan_windows"] = int(s.scan_windows)
     if not args.no_cpu_baseline and args.workload == "single":
         out["cpu_baseline"] = cpu_baseline()
     if tuning is not None:

@@ -402,6 +402,7 @@ def main() -> int:
 
     solve_ms: list[float] = []
     solve_split = 0.0
+    solve_graph = 0
     scan_pu = refine_pu = 0.0
     levels_ok = 0
     ctx.synchronize()
@@ -412,6 +413,7 @@ def main() -> int:
         s = ctx.stats()
         solve_ms.append(s.solve_ms)
         solve_split = s.solve_split
+        solve_graph = int(s.solve_graph)
         scan_pu += s.scan_point_updates
         refine_pu += s.refine_point_updates
         levels_ok += int(s.levels_converged)
@@ -453,6 +455,7 @@ def main() -> int:
         "levels_per_step": levels_ok // args.steps,
         "solve_ms": float(np.mean(solve_ms)),
         "solve_split": solve_split,
+        "solve_graph": solve_graph,
         "scan_ms": float(np.mean(scan_ms)),
         "refine_ms": float(np.mean(refine_ms)),
         "scan_point_updates_per_step": scan_pu / args.steps,

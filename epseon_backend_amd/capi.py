@@ -118,7 +118,7 @@ class Stats(ctypes.Structure):
         ("solve_ms", ctypes.c_double),
         ("solve_split", ctypes.c_double),
         ("scan_windows", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("solve_graph", ctypes.c_int32),
     ]
 
 
@@ -130,6 +130,7 @@ class Tuning(ctypes.Structure):
         ("segmented_lanes", ctypes.c_int32),
         ("xcd_scan", ctypes.c_int32),
         ("windows", ctypes.c_int32),
+        ("graph", ctypes.c_int32),
         ("solve_split", ctypes.c_double),
     ]
 

@@ -185,6 +185,26 @@ int32_t node_stride(double beta, double e_hi, double v_min)
 
 using namespace epseon::hip;
 
+namespace {
+
+// Everything the launches of a captured solve depend on: device pointers
+// and task lists change only with a load; the rest is the tuning and the
+// refinement arguments passed by value.
+struct SolveKey {
+    uint64_t load = 0;
+    int32_t refine_mode = 0, lanes = 0, waves_per_block = 0, xcd_scan = 0, scan_windows = 0, max_iter = 0;
+    double rel_tol = 0.0;
+
+    bool operator==(const SolveKey& o) const
+    {
+        return load == o.load && refine_mode == o.refine_mode && lanes == o.lanes &&
+               waves_per_block == o.waves_per_block && xcd_scan == o.xcd_scan && scan_windows == o.scan_windows &&
+               max_iter == o.max_iter && rel_tol == o.rel_tol;
+    }
+};
+
+}  // namespace
+
 struct epseon_ctx {
     int32_t device = 0;
     hipStream_t stream = nullptr;
@@ -257,6 +277,13 @@ struct epseon_ctx {
         cum_lane2_tasks;
     std::vector<double> cum_scan_work;
     double solve_split = 0.0;
+    // The single-stream solve's launches recorded as a HIP graph (tuning
+    // `graph`), replayed while its key (batch, tuning, tolerance) holds.
+    int32_t graph = 1;
+    hipGraphExec_t solve_exec = nullptr;
+    SolveKey graph_key{};
+    uint64_t loads = 0;  // batches loaded so far: part of the graph key
+    bool last_solve_graph = false;
     int32_t levels_per_lane = 1;
     int32_t segmented_lanes = 0;  // 0: chosen from the batch size
     bool solve_timed = false;
@@ -442,6 +469,8 @@ int epseon_ctx_destroy(epseon_ctx* ctx)
     for (hipStream_t st : {ctx->stream2, ctx->stream3, ctx->stream4})
         if (st != nullptr)
             (void)hipStreamSynchronize(st);
+    if (ctx->solve_exec != nullptr)
+        (void)hipGraphExecDestroy(ctx->solve_exec);
     ctx->arena.release();
     for (hipEvent_t& ev : ctx->ev)
         if (ev != nullptr)
@@ -548,6 +577,12 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
         (count > 0 && (desc == nullptr || (v_host == nullptr && v_dev == nullptr) || v_offsets == nullptr)))
         return fail(EPSEON_ERR_INVALID, "invalid potential batch");
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    // The captured solve points into the arena that this load reuses.
+    ++ctx->loads;
+    if (ctx->solve_exec != nullptr) {
+        EPSEON_HIP_TRY(hipGraphExecDestroy(ctx->solve_exec));
+        ctx->solve_exec = nullptr;
+    }
     ctx->loaded = ctx->scanned = ctx->refined = ctx->normalized = false;
     ctx->scan_timed = ctx->refine_timed = ctx->normalize_timed = ctx->vjp_timed = false;
     ctx->solve_timed = false;
@@ -1173,6 +1208,36 @@ size_t split_point(const epseon_ctx* ctx, double fraction)
     return k;
 }
 
+// Record the single-stream solve (scan, brackets, refinement of the whole
+// batch) on the context's stream as a graph and instantiate it.  The
+// kernels take only device pointers of this load and by-value arguments;
+// none of the recorded work is a memset or a copy.
+int capture_solve(epseon_ctx* ctx, int32_t mode, int32_t lanes, double rel_tol, int32_t max_iter)
+{
+    if (ctx->solve_exec != nullptr) {
+        EPSEON_HIP_TRY(hipGraphExecDestroy(ctx->solve_exec));
+        ctx->solve_exec = nullptr;
+    }
+    const size_t np = n_potentials(ctx);
+    EPSEON_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    hipError_t e = run_scan(ctx, ctx->stream, 0, np);
+    if (e == hipSuccess)
+        e = run_refine(ctx, ctx->stream, ctx->stream, mode, lanes, rel_tol, max_iter, 0, np);
+    hipGraph_t graph = nullptr;
+    const hipError_t e_end = hipStreamEndCapture(ctx->stream, &graph);
+    if (e == hipSuccess)
+        e = e_end;
+    if (e == hipSuccess)
+        e = hipGraphInstantiate(&ctx->solve_exec, graph, nullptr, nullptr, 0);
+    if (graph != nullptr)
+        (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        ctx->solve_exec = nullptr;
+        return hip_fail(e, "capturing the solve graph");
+    }
+    return EPSEON_OK;
+}
+
 }  // namespace
 
 int epseon_scan(epseon_ctx* ctx)
@@ -1263,16 +1328,31 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     const size_t k = split_point(ctx, ctx->solve_split);
     const int32_t mode = choose_refine_mode(ctx);
     const int32_t lanes = segmented_lanes(ctx);
+    ctx->last_solve_graph = false;
     if (k == 0 && !(mode == EPSEON_REFINE_LANE && ctx->levels_per_lane == 2)) {
         // Scan and refinement back to back on one stream, timed as a whole
         // (events between the two launches would add a barrier each).
         ctx->last_refine_mode = mode;
         ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
         begin_scan(ctx, mode);
-        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve0, ctx->stream));
-        EPSEON_HIP_TRY(run_scan(ctx, ctx->stream, 0, np));
-        EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream, mode, lanes, rel_tol, max_iter, 0, np));
-        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve1, ctx->stream));
+        if (ctx->graph && ctx->n_scan_tasks + ctx->n_level_tasks > 0) {
+            const SolveKey key{ctx->loads, mode, lanes, ctx->waves_per_block, ctx->xcd_scan ? 1 : 0,
+                               ctx->scan_windows ? 1 : 0, max_iter, rel_tol};
+            if (ctx->solve_exec == nullptr || !(ctx->graph_key == key)) {
+                if (const int rc = capture_solve(ctx, mode, lanes, rel_tol, max_iter); rc != EPSEON_OK)
+                    return rc;
+                ctx->graph_key = key;
+            }
+            EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve0, ctx->stream));
+            EPSEON_HIP_TRY(hipGraphLaunch(ctx->solve_exec, ctx->stream));
+            EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve1, ctx->stream));
+            ctx->last_solve_graph = true;
+        } else {
+            EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve0, ctx->stream));
+            EPSEON_HIP_TRY(run_scan(ctx, ctx->stream, 0, np));
+            EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream, mode, lanes, rel_tol, max_iter, 0, np));
+            EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve1, ctx->stream));
+        }
         ctx->scanned = ctx->refined = true;
         ctx->scan_timed = ctx->refine_timed = false;
         ctx->normalized = false;
@@ -1351,6 +1431,7 @@ int epseon_get_tuning(epseon_ctx* ctx, epseon_tuning* t)
     t->segmented_lanes = ctx->segmented_lanes;
     t->xcd_scan = ctx->xcd_scan ? 1 : 0;
     t->windows = ctx->windows;
+    t->graph = ctx->graph;
     t->solve_split = ctx->solve_split;
     return EPSEON_OK;
 }
@@ -1375,12 +1456,15 @@ int epseon_set_tuning(epseon_ctx* ctx, const epseon_tuning* t)
         return fail(EPSEON_ERR_INVALID, "xcd_scan must be 0 or 1");
     if (t->windows < 0 || t->windows > 2)
         return fail(EPSEON_ERR_INVALID, "windows must be 0, 1 or 2");
+    if (t->graph != 0 && t->graph != 1)
+        return fail(EPSEON_ERR_INVALID, "graph must be 0 or 1");
     ctx->refine_mode = t->refine_mode;
     ctx->levels_per_lane = t->levels_per_lane;
     ctx->waves_per_block = t->scan_waves_per_block;
     ctx->windows = t->windows;
     ctx->segmented_lanes = t->segmented_lanes;
     ctx->xcd_scan = t->xcd_scan != 0;
+    ctx->graph = t->graph;
     ctx->solve_split = t->solve_split;
     return EPSEON_OK;
 }
@@ -1481,6 +1565,12 @@ int epseon_autotune(epseon_ctx* ctx, double rel_tol, int32_t max_iter, int32_t f
     for (double split : {0.15, 0.3, 0.5}) {
         epseon_tuning c = best;
         c.solve_split = split;
+        if (const int rc = trial(c); rc != EPSEON_OK)
+            return rc;
+    }
+    {
+        epseon_tuning c = best;
+        c.graph = 1 - best.graph;
         if (const int rc = trial(c); rc != EPSEON_OK)
             return rc;
     }
@@ -1768,6 +1858,7 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
         stats->solve_ms = ms;
         stats->solve_split = ctx->last_split_used;
         stats->scan_point_updates = ctx->scan_point_updates;
+        stats->solve_graph = ctx->last_solve_graph ? 1 : 0;
     }
     if (ctx->refined) {
         if (ctx->refine_timed) {

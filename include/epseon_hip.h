@@ -126,7 +126,7 @@ typedef struct epseon_stats {
     double solve_ms;              /* device time of the last epseon_solve (both streams) */
     double solve_split;           /* scan-work fraction of the first pipeline group (0: sequential) */
     int32_t scan_windows;         /* 1 when the last scan (and its refinement) used band windows */
-    int32_t reserved;
+    int32_t solve_graph;          /* 1 when the last epseon_solve replayed a captured HIP graph */
 } epseon_stats;
 
 typedef struct epseon_ctx epseon_ctx;
@@ -261,6 +261,12 @@ typedef struct epseon_tuning {
                                      each) per SIMD of the device, else 0.  Lane-mode refinement
                                      always shoots the whole grid, so with windows in use AUTO
                                      selects SEGMENTED and a windowed scan is refined segmented. */
+    int32_t graph;                /* 1 (default): epseon_solve records its launches (scan, brackets,
+                                     refinement) once as a HIP graph and replays the graph while
+                                     the batch, the tuning, rel_tol and max_iter stay the same;
+                                     0: every call launches the kernels one by one.  Pipelined
+                                     (solve_split > 0) and paired-lane solves always launch
+                                     directly.  Results are bit-identical either way. */
     double solve_split;           /* see epseon_set_solve_split */
 } epseon_tuning;
 
@@ -276,7 +282,7 @@ int epseon_set_tuning(epseon_ctx* ctx, const epseon_tuning* tuning);
  * of two after a warm-up run, device time by HIP events), keep the fastest
  * and write it to `best` (may be NULL).  Coordinate search over scan
  * workgroup size and XCD binning, shot windows, refinement mode, levels per lane / segmented
- * block size and the pipeline split (an automatic choice is first resolved to
+ * block size, the pipeline split and graph replay (an automatic choice is first resolved to
  * its current value).  With EPSEON_TUNE_KEEP_RESULTS the windows, the
  * refinement mode and the segmented block size stay fixed.  The levels and scan results afterwards are
  * those of the last candidate run: call epseon_solve again.  No counterpart

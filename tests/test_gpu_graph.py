@@ -1,0 +1,139 @@
+"""Graph replay of the solve (tuning `graph`): the scan, bracket and
+refinement launches are recorded once as a HIP graph and replayed.
+
+* every replay equals the eager solve bit for bit, with eager work (a plain
+  scan and refinement at another tolerance) and host synchronisation between
+  the replays;
+* a new tolerance, a new tuning or a new batch records a new graph, whose
+  replays again equal the eager solves;
+* the first replays are checked against the oracle on a few potentials.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from epseon_backend_amd.workloads import batched
+from tests.helpers import problem_from_spec
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg: str) -> None:
+    sys.stdout.write(msg + "\n")
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from epseon_backend_amd import capi as c
+
+    if c.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return c
+
+
+def _pots(capi, specs, counts):
+    return [capi.Potential(values=s.table(), r_min=s.r_min, r_max=s.r_max, reduced_mass=s.mu, e_top=s.de,
+                           n_energies=s.m, level_min=0, level_count=c) for s, c in zip(specs, counts)]
+
+
+def _results(ctx):
+    nodes, far, resc = ctx.scan_results()
+    lv, st, it = ctx.levels()
+    return nodes, far.view(np.int64), resc, lv.view(np.int64), st, it
+
+
+def _equal(a, b):
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def _tune(ctx, capi, seg: bool, windows: int, graph: int) -> None:
+    t = ctx.tuning()
+    t.refine_mode = capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE
+    t.levels_per_lane = 1
+    t.windows = windows
+    t.solve_split = 0.0
+    t.graph = graph
+    ctx.set_tuning(t)
+
+
+@pytest.mark.parametrize("mode,windows", [("segmented", 1), ("lane", 0)])
+def test_graph_replays_match_eager(capi, mode, windows):
+    seg = mode == "segmented"
+    specs = batched(4096)[:200]
+    counts = [s.bound_levels() + 2 for s in specs]
+    _log(f"case: first 200 batched potentials, {mode} refine, windows {windows}: graph replays vs eager")
+    with capi.Context(0) as ctx:
+        _tune(ctx, capi, seg, windows, graph=0)
+        ctx.load(_pots(capi, specs, counts))
+        ctx.solve(1e-10, 200)
+        assert ctx.stats().solve_graph == 0
+        eager = _results(ctx)
+        ctx.solve(1e-12, 200)
+        eager_tight = _results(ctx)
+        _tune(ctx, capi, seg, windows, graph=1)
+        for rep in range(4):
+            ctx.solve(1e-10, 200)
+            stats = ctx.stats()
+            assert stats.solve_graph == 1 and stats.scan_windows == windows
+            _equal(_results(ctx), eager)
+            _log(f"  replay {rep}: solve {stats.solve_ms:.3f} ms, identical")
+            # Eager work on the same buffers between the replays.
+            ctx.scan()
+            ctx.refine(1e-12, 200)
+            _equal(_results(ctx), eager_tight)
+        # Another tolerance records another graph; going back records again.
+        for tol, want in ((1e-12, eager_tight), (1e-10, eager), (1e-12, eager_tight)):
+            ctx.solve(tol, 200)
+            assert ctx.stats().solve_graph == 1
+            _equal(_results(ctx), want)
+        # Another scan workgroup size: a new graph, the same results.
+        t = ctx.tuning()
+        t.scan_waves_per_block = 2
+        ctx.set_tuning(t)
+        ctx.solve(1e-10, 200)
+        assert ctx.stats().solve_graph == 1
+        _equal(_results(ctx), eager)
+        lanes = ctx.stats().refine_lanes
+    # The oracle agrees on the first potentials.
+    nodes, far, resc, lvi, st, it = eager
+    lv = lvi.view(np.float64)
+    eo = lo = 0
+    for s, c in zip(specs[:12], counts[:12]):
+        prob = problem_from_spec(s)
+        if prob.usable:
+            on, of, orr, olv, oit = oracle.solve(prob, 0, c, 1e-10, 200, segmented=seg, lanes=lanes,
+                                                 windows=bool(windows))
+            np.testing.assert_array_equal(nodes[eo : eo + s.m], on)
+            np.testing.assert_array_equal(far[eo : eo + s.m], of.view(np.int64))
+            ok = ~np.isnan(olv)
+            np.testing.assert_array_equal(lv[lo : lo + c][ok].view(np.int64), olv[ok].view(np.int64))
+            np.testing.assert_array_equal(it[lo : lo + c][ok], oit[ok])
+        eo += s.m
+        lo += c
+
+
+def test_graph_follows_reloads(capi):
+    """A load drops the recorded graph (its buffers are reused): replays after
+    each reload equal an eager solve of the same batch on another context."""
+    specs = batched(4096)
+    batches = [specs[200:260], specs[1000:1100], specs[200:260]]
+    _log("case: three batches loaded in turn on one context with graph replay")
+    with capi.Context(0) as ctx, capi.Context(0) as ref:
+        _tune(ctx, capi, True, 1, graph=1)
+        _tune(ref, capi, True, 1, graph=0)
+        for b in batches:
+            counts = [s.bound_levels() + 2 for s in b]
+            ctx.load(_pots(capi, b, counts))
+            ref.load(_pots(capi, b, counts))
+            ref.solve(1e-10, 200)
+            want = _results(ref)
+            for _ in range(2):
+                ctx.solve(1e-10, 200)
+                assert ctx.stats().solve_graph == 1
+                _equal(_results(ctx), want)

@@ -52,8 +52,8 @@ FP64_VECTOR_PEAK_TFLOPS = 78.6  # MI355X: 256 CU x 128 FP64 FLOP/clk x 2.4 GHz
 # 2 FMA).
 FLOP_PER_POINT_UPDATE = 7
 # Refinement shots: a segmented shot carries two columns of the transfer
-# matrix (12 FP64 instructions, 4 of them FMA, per grid point), a lane-mode
-# shot one difference-form column (8, 2 of them FMA).
+# matrix (10 FP64 instructions, 6 of them FMA, per grid point), a lane-mode
+# shot one difference-form column (7, 3 of them FMA).
 REFINE_FLOP_PER_POINT_UPDATE = {"segmented": 16, "lane": 10}
 METRIC = "Numerov point-updates/s & eigenlevels/s at 1e-9 rel err; grad/fwd time ratio"
 

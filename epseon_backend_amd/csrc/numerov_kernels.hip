@@ -103,9 +103,9 @@ struct ScanForm {
 };
 
 // Difference form on uv_i = beta V_i (refinement): eps_i = e - uv_i carried
-// at full relative precision, 8 FP64 VALU ops per step:
+// at full relative precision, 7 FP64 VALU ops per step:
 //   sig = fma(eps_i, eps_{i-1}, eps_i + eps_{i-1}),
-//   c = fma(10 eps_i, w_i, sig w_{i-1}), d -= c, w += d.
+//   d = fma(-10 eps_i, w_i, fma(-sig, w_{i-1}, d)), w += d.
 struct DiffForm {
     using Energy = double;
     using Counter = SignCounter;
@@ -124,8 +124,7 @@ struct DiffForm {
         const double ep = e - u;
         const double e10 = ep * k.c10;
         const double sg = __builtin_fma(ep, em, ep + em);
-        const double c = __builtin_fma(e10, w, sg * wm);
-        d = d - c;
+        d = __builtin_fma(-e10, w, __builtin_fma(-sg, wm, d));
         wm = w;
         w = w + d;
         em = ep;
@@ -1127,10 +1126,8 @@ struct MatrixBody {
         const double ep = e - u;
         const double e10 = ep * c10;
         const double sg = __builtin_fma(ep, em, ep + em);
-        const double ca = __builtin_fma(e10, wa, sg * pa);
-        const double cb = __builtin_fma(e10, wb, sg * pb);
-        da = da - ca;
-        db = db - cb;
+        da = __builtin_fma(-e10, wa, __builtin_fma(-sg, pa, da));
+        db = __builtin_fma(-e10, wb, __builtin_fma(-sg, pb, db));
         pa = wa;
         pb = wb;
         wa = wa + da;
@@ -1163,8 +1160,7 @@ struct CountBody {
         const double ep = e - u;
         const double e10 = ep * c10;
         const double sg = __builtin_fma(ep, em, ep + em);
-        const double cc = __builtin_fma(e10, w, sg * wm);
-        d = d - cc;
+        d = __builtin_fma(-e10, w, __builtin_fma(-sg, wm, d));
         wm = w;
         w = w + d;
         em = ep;

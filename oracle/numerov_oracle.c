@@ -594,8 +594,8 @@ void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t*
  * w_{i-1}) with the small quantities kept at full relative precision:
  *   eps_i = e - uv_i  (uv_i = beta V_i, so eps_i = beta (E - V_i))
  *   sig_i = fma(eps_i, eps_{i-1}, eps_i + eps_{i-1})      (g_i = 1 + sig_i)
- *   c     = fma(10 eps_i, w_i, sig_i w_{i-1})
- *   d_{i+1} = d_i - c,  w_{i+1} = w_i + d_{i+1}
+ *   d_{i+1} = fma(-10 eps_i, w_i, fma(-sig_i, w_{i-1}, d_i))
+ *   w_{i+1} = w_i + d_{i+1}
  * (from w_{i+1} = (2 - 10 eps_i) w_i - (1 + sig_i) w_{i-1}).  With f = 1 + eps
  * held near 1, FP64 quantises E in steps of 2^-52/beta; here the root of
  * w_{N-1}(E) is resolved to ~1e-15 relative.  w_0 = 0, w_1 = d_1 = 1; the
@@ -614,8 +614,7 @@ int32_t eo_shoot_diff(const double* uv, int64_t n, double e, int32_t stride, dou
         const double ep = e - uv[q + 1];
         const double e10 = ep * 10.0;
         const double sg = fma(ep, em, ep + em);
-        const double c = fma(e10, w, sg * wm);
-        d = d - c;
+        d = fma(-e10, w, fma(-sg, wm, d));
         wm = w;
         w = w + d;
         em = ep;
@@ -709,8 +708,7 @@ void eo_segment_matrix(const double* uv, int64_t q0, int64_t q1, double e, eo_ma
         const double e10 = ep * 10.0;
         const double sg = fma(ep, em, ep + em);
         for (int c = 0; c < 6; c += 3) {
-            const double cc = fma(e10, x[c], sg * x[c + 2]);
-            x[c + 1] = x[c + 1] - cc;
+            x[c + 1] = fma(-e10, x[c], fma(-sg, x[c + 2], x[c + 1]));
             x[c + 2] = x[c];
             x[c] = x[c] + x[c + 1];
         }
@@ -769,8 +767,7 @@ void eo_segment_count_nodes(const double* uv, int64_t n, int64_t q0, int64_t q1,
         const double ep = e - uv[q + 1];
         const double e10 = ep * 10.0;
         const double sg = fma(ep, em, ep + em);
-        const double cc = fma(e10, x[0], sg * x[2]);
-        x[1] = x[1] - cc;
+        x[1] = fma(-e10, x[0], fma(-sg, x[2], x[1]));
         x[2] = x[0];
         x[0] = x[0] + x[1];
         em = ep;

@@ -73,6 +73,28 @@ struct YShot {
         f = fn;
     }
 
+    // Continue a shot from its checkpoint after j steps: u_prev and u_cur
+    // are points j and j + 1 of the side's table.
+    __device__ __forceinline__ void resume(const PsiCheckpoint& c, const double e, const double u_prev,
+                                           const double u_cur)
+    {
+        y = c.y;
+        ym = c.ym;
+        d = c.d;
+        k = c.k;
+        fm = e - u_prev;
+        f = e - u_cur;
+    }
+
+    // A level without a refined energy: every value stays zero.
+    __device__ __forceinline__ void clear(const double e, const double u_prev, const double u_cur)
+    {
+        y = ym = d = 0.0;
+        k = 0;
+        fm = e - u_prev;
+        f = e - u_cur;
+    }
+
     // Returns the applied binary shift (0 or -240).
     __device__ __forceinline__ int32_t rescale()
     {
@@ -123,10 +145,16 @@ __device__ __forceinline__ void psi_walk(const double* __restrict__ p, const int
     }
 }
 
-// Pass 1: sum of y_i^2 over the visited points (in the shot's current scale).
+// Pass 1: sum of y_i^2 over the visited points (in the shot's current scale),
+// and the shot's state at the start of each of its segments after the first
+// (ck[s * ck_stride] for segment s), from which the later passes re-run the
+// segments independently.
 struct NormBody {
     YShot s;
     double e, c10, sum;
+    PsiCheckpoint* ck;
+    int64_t ck_stride;
+    int32_t blocks_per_seg, blk, seg, segs;
     __device__ __forceinline__ void step(const double u, int)
     {
         sum = __builtin_fma(s.y, s.y, sum);
@@ -137,6 +165,11 @@ struct NormBody {
         if (full) {
             const int32_t sh = s.rescale();
             sum = __builtin_ldexp(sum, 2 * sh);
+            if (++blk == blocks_per_seg) {
+                blk = 0;
+                if (++seg < segs)
+                    ck[seg * ck_stride] = PsiCheckpoint{s.y, s.ym, s.d, s.k, 0};
+            }
         }
     }
 };
@@ -228,7 +261,7 @@ __global__ __launch_bounds__(256) void normalize_kernel(
     const double* __restrict__ uv, const double* __restrict__ uvr,
     const PotentialRecord* __restrict__ pots, const WaveTask* __restrict__ tasks,
     const int32_t n_tasks, const double* __restrict__ levels, const int32_t* __restrict__ status,
-    LevelNorm* __restrict__ norms)
+    LevelNorm* __restrict__ norms, PsiCheckpoint* __restrict__ ckpt)
 {
     const int32_t waves = static_cast<int32_t>(blockDim.x >> 6);
     const int32_t tid = static_cast<int32_t>(blockIdx.x) * waves +
@@ -252,10 +285,17 @@ __global__ __launch_bounds__(256) void normalize_kernel(
     const double* u = uv + P.ut_offset;
     const double* ur = uvr + P.ut_offset;
 
+    PsiCheckpoint* ck = ckpt + P.psi_ckpt + l;
+    const int64_t stride = P.level_count;
+
     NormBody L{};
     L.e = e;
     L.c10 = c10;
     L.sum = 0.0;
+    L.ck = ck;
+    L.ck_stride = stride;
+    L.blocks_per_seg = P.psi_seg_len / 32;
+    L.segs = P.psi_segs_left;
     L.s.init(e, u[0], u[1]);
     psi_walk(u + 2, m, L);
 
@@ -263,6 +303,10 @@ __global__ __launch_bounds__(256) void normalize_kernel(
     R.e = e;
     R.c10 = c10;
     R.sum = 0.0;
+    R.ck = ck + P.psi_segs_left * stride;
+    R.ck_stride = stride;
+    R.blocks_per_seg = P.psi_seg_len / 32;
+    R.segs = P.psi_segs_right;
     R.s.init(e, ur[0], ur[1]);
     psi_walk(ur + 2, n - 2 - m, R);
 
@@ -277,17 +321,52 @@ __global__ __launch_bounds__(256) void normalize_kernel(
     norms[o] = LevelNorm{a, c * a, L.s.k, R.s.k};
 }
 
-// Eigenfunctions: for each wavefront task (potential, first level), row l of
-// that potential (full_points values at out[psi_offset - base + l * full_points])
-// is psi of level l, zero outside the open trimmed window and for levels that
-// did not converge.  One lane per level.
+// Shot segment of a PsiTask: the side's table, the first step j0 and the
+// number of steps.
+struct PsiSegment {
+    const double* p;
+    int32_t j0, steps;
+};
+
+__device__ __forceinline__ PsiSegment psi_segment(const PotentialRecord& P, const double* __restrict__ uv,
+                                                  const double* __restrict__ uvr, const PsiTask& t)
+{
+    const int32_t total = t.side == 0 ? P.match : P.n_points - 2 - P.match;
+    const int32_t j0 = t.segment * P.psi_seg_len;
+    return PsiSegment{(t.side == 0 ? uv : uvr) + P.ut_offset, j0, min(P.psi_seg_len, total - j0)};
+}
+
+// Start a segment's shot: from the wall (segment 0), from the normalisation
+// pass's checkpoint, or all zero for a level that did not converge.
+__device__ __forceinline__ void psi_start(YShot& s, const PotentialRecord& P, const PsiTask& t,
+                                          const PsiSegment& g, const PsiCheckpoint* __restrict__ ckpt,
+                                          const int32_t l, const bool active, const double e)
+{
+    if (!active)
+        s.clear(e, g.p[g.j0], g.p[g.j0 + 1]);
+    else if (t.segment == 0)
+        s.init(e, g.p[0], g.p[1]);
+    else
+        s.resume(ckpt[P.psi_ckpt + static_cast<int64_t>((t.side == 0 ? 0 : P.psi_segs_left) + t.segment) *
+                                        P.level_count + l],
+                 e, g.p[g.j0], g.p[g.j0 + 1]);
+}
+
+// Eigenfunctions: for each task (potential, first level, side, segment), the
+// segment's points of rows first.. of that potential (full_points values per
+// row at out[psi_offset - base + l * full_points]), re-run from the
+// normalisation pass's checkpoints; the segment-0 tasks also zero the row
+// outside the open trimmed window on their side (the whole row of an
+// unresolved potential, whose only task is side 0, segment 0).  Zero for
+// levels that did not converge.  One lane per level.
 __global__ __launch_bounds__(64) void eigenfunction_kernel(
     const double* __restrict__ uv, const double* __restrict__ uvr,
-    const PotentialRecord* __restrict__ pots, const WaveTask* __restrict__ tasks,
+    const PotentialRecord* __restrict__ pots, const PsiTask* __restrict__ tasks,
     const double* __restrict__ levels, const int32_t* __restrict__ status,
-    const LevelNorm* __restrict__ norms, const int64_t base, double* __restrict__ out)
+    const LevelNorm* __restrict__ norms, const PsiCheckpoint* __restrict__ ckpt, const int64_t base,
+    double* __restrict__ out)
 {
-    const WaveTask task = tasks[blockIdx.x];
+    const PsiTask task = tasks[blockIdx.x];
     const PotentialRecord& P = pots[task.potential];
     const int32_t l = task.first + static_cast<int32_t>(threadIdx.x);
     if (l >= P.level_count)
@@ -297,120 +376,116 @@ __global__ __launch_bounds__(64) void eigenfunction_kernel(
     const int64_t first = P.first_point;
     const int32_t n = P.n_points;
     const bool ok = P.flags == 0;
-    const int64_t lo_end = ok ? first + 1 : N;
-    for (int64_t i = 0; i < lo_end; ++i)
-        row[i] = 0.0;
+    if (task.segment == 0) {
+        if (!ok) {
+            for (int64_t i = 0; i < N; ++i)
+                row[i] = 0.0;
+            return;
+        }
+        if (task.side == 0)
+            for (int64_t i = 0; i <= first; ++i)
+                row[i] = 0.0;
+        else
+            for (int64_t i = first + n - 1; i < N; ++i)
+                row[i] = 0.0;
+    }
     if (!ok)
         return;
-    for (int64_t i = first + n - 1; i < N; ++i)
-        row[i] = 0.0;
     const int64_t o = P.level_offset + l;
     const bool active = level_active(status, o);
     const LevelNorm nm = active ? norms[o] : LevelNorm{0.0, 0.0, 0, 0};
     const double e = P.beta * (active ? levels[o] : P.e_lo);
-    const double c10 = opaque(10.0);
-    const int32_t m = P.match;
-    const double* u = uv + P.ut_offset;
-    const double* ur = uvr + P.ut_offset;
-    double* win = row + first;
+    const PsiSegment g = psi_segment(P, uv, uvr, task);
 
-    PsiBody L{};
-    L.e = e;
-    L.c10 = c10;
-    L.a = nm.a_left;
-    L.kf = nm.k_left;
-    L.mult = __builtin_ldexp(nm.a_left, -kPsiRescaleLog2 * nm.k_left);
-    L.i0 = 1;
-    L.dir = 1;
-    L.base = 0;
-    L.row = win;
-    L.s.init(e, u[0], u[1]);
-    psi_walk(u + 2, m, L);
-
-    PsiBody R{};
-    R.e = e;
-    R.c10 = c10;
-    R.a = nm.a_right;
-    R.kf = nm.k_right;
-    R.mult = __builtin_ldexp(nm.a_right, -kPsiRescaleLog2 * nm.k_right);
-    R.i0 = n - 2;
-    R.dir = -1;
-    R.base = 0;
-    R.row = win;
-    R.s.init(e, ur[0], ur[1]);
-    psi_walk(ur + 2, n - 2 - m, R);
+    PsiBody B{};
+    B.e = e;
+    B.c10 = opaque(10.0);
+    B.a = task.side == 0 ? nm.a_left : nm.a_right;
+    B.kf = task.side == 0 ? nm.k_left : nm.k_right;
+    B.i0 = task.side == 0 ? 1 : n - 2;
+    B.dir = task.side == 0 ? 1 : -1;
+    B.base = g.j0;
+    B.row = row + first;
+    psi_start(B.s, P, task, g, ckpt, l, active, e);
+    B.mult = __builtin_ldexp(B.a, kPsiRescaleLog2 * (B.s.k - B.kf));
+    psi_walk(g.p + 2 + g.j0, g.steps, B);
 }
 
 // Adjoint: grad_v[user_offset + i] = sum_l grad_levels[l] psi_l(i)^2 h over
-// the converged levels of the block's potential, zero elsewhere on the
-// potential's full grid.  One block of W wavefronts per potential; levels in
-// groups of 64 W, the first group storing and later ones adding, so the sum
-// order is fixed.
+// the converged levels of the task's potential, zero elsewhere on the
+// potential's full grid.  One block of W wavefronts per task (potential,
+// side, segment): the segment's points, levels in groups of 64 W, the first
+// group storing and later ones adding, so the sum order is fixed.  The
+// segment-0 tasks also zero the points outside the open trimmed window on
+// their side (all points of an unresolved potential, whose only task is side
+// 0, segment 0).
 template <int W>
 __global__ __launch_bounds__(64 * W) void level_vjp_kernel(
     const double* __restrict__ uv, const double* __restrict__ uvr,
-    const PotentialRecord* __restrict__ pots, const int32_t* __restrict__ pot_list,
+    const PotentialRecord* __restrict__ pots, const PsiTask* __restrict__ tasks,
     const double* __restrict__ levels, const int32_t* __restrict__ status,
-    const LevelNorm* __restrict__ norms, const double* __restrict__ grad_levels,
-    double* __restrict__ grad_v)
+    const LevelNorm* __restrict__ norms, const PsiCheckpoint* __restrict__ ckpt,
+    const double* __restrict__ grad_levels, double* __restrict__ grad_v)
 {
     __shared__ double s_c[W][32][65];
     __shared__ double s_w[W][32];
-    const PotentialRecord& P = pots[pot_list[blockIdx.x]];
+    const PsiTask task = tasks[blockIdx.x];
+    const PotentialRecord& P = pots[task.potential];
     double* out = grad_v + P.user_offset;
     const int32_t N = P.full_points;
     const int32_t n = P.n_points;
     const int32_t first = static_cast<int32_t>(P.first_point);
     const bool ok = P.flags == 0;
-    for (int32_t i = static_cast<int32_t>(threadIdx.x); i < N; i += 64 * W)
-        if (!ok || i <= first || i >= first + n - 1)
-            out[i] = 0.0;
+    const int32_t t = static_cast<int32_t>(threadIdx.x);
+    if (task.segment == 0) {
+        if (!ok) {
+            for (int32_t i = t; i < N; i += 64 * W)
+                out[i] = 0.0;
+            return;
+        }
+        if (task.side == 0)
+            for (int32_t i = t; i <= first; i += 64 * W)
+                out[i] = 0.0;
+        else
+            for (int32_t i = first + n - 1 + t; i < N; i += 64 * W)
+                out[i] = 0.0;
+    }
     if (!ok)
         return;
-    const double c10 = opaque(10.0);
-    const int32_t m = P.match;
-    const double* u = uv + P.ut_offset;
-    const double* ur = uvr + P.ut_offset;
+    const PsiSegment g = psi_segment(P, uv, uvr, task);
+    const int32_t i0 = task.side == 0 ? 1 : n - 2;
+    const int32_t dir = task.side == 0 ? 1 : -1;
     double* win = out + first;
-    for (int32_t g = 0; g < P.level_count; g += 64 * W) {
-        const int32_t l = g + static_cast<int32_t>(threadIdx.x);
+    if (P.level_count == 0) {
+        for (int32_t q = t; q < g.steps; q += 64 * W)
+            win[i0 + dir * (g.j0 + q)] = 0.0;
+        return;
+    }
+    const double c10 = opaque(10.0);
+    for (int32_t g0 = 0; g0 < P.level_count; g0 += 64 * W) {
+        const int32_t l = g0 + t;
         const int64_t o = P.level_offset + l;
         const bool active = l < P.level_count && level_active(status, o);
         const LevelNorm nm = active ? norms[o] : LevelNorm{0.0, 0.0, 0, 0};
         const double gl = active ? grad_levels[o] : 0.0;
         const double e = P.beta * (active ? levels[o] : P.e_lo);
+        const double a = task.side == 0 ? nm.a_left : nm.a_right;
 
-        VjpBody<W> L{};
-        L.e = e;
-        L.c10 = c10;
-        L.g0 = gl * P.h * nm.a_left * nm.a_left;
-        L.kf = nm.k_left;
-        L.G = __builtin_ldexp(L.g0, -2 * kPsiRescaleLog2 * nm.k_left);
-        L.i0 = 1;
-        L.dir = 1;
-        L.base = 0;
-        L.accumulate = g > 0;
-        L.sc = s_c;
-        L.sw = s_w;
-        L.out = win;
-        L.s.init(e, u[0], u[1]);
-        psi_walk(u + 2, m, L);
-
-        VjpBody<W> R{};
-        R.e = e;
-        R.c10 = c10;
-        R.g0 = gl * P.h * nm.a_right * nm.a_right;
-        R.kf = nm.k_right;
-        R.G = __builtin_ldexp(R.g0, -2 * kPsiRescaleLog2 * nm.k_right);
-        R.i0 = n - 2;
-        R.dir = -1;
-        R.base = 0;
-        R.accumulate = g > 0;
-        R.sc = s_c;
-        R.sw = s_w;
-        R.out = win;
-        R.s.init(e, ur[0], ur[1]);
-        psi_walk(ur + 2, n - 2 - m, R);
+        VjpBody<W> B{};
+        B.e = e;
+        B.c10 = c10;
+        B.g0 = gl * P.h * a * a;
+        B.kf = task.side == 0 ? nm.k_left : nm.k_right;
+        B.i0 = i0;
+        B.dir = dir;
+        B.base = g.j0;
+        B.accumulate = g0 > 0;
+        B.sc = s_c;
+        B.sw = s_w;
+        B.out = win;
+        psi_start(B.s, P, task, g, ckpt, l, active, e);
+        B.G = __builtin_ldexp(B.g0, 2 * kPsiRescaleLog2 * (B.s.k - B.kf));
+        psi_walk(g.p + 2 + g.j0, g.steps, B);
     }
 }
 
@@ -418,55 +493,56 @@ __global__ __launch_bounds__(64 * W) void level_vjp_kernel(
 
 hipError_t launch_normalize(hipStream_t stream, const double* uv, const double* uvr,
                             const PotentialRecord* pots, const WaveTask* tasks, int32_t n_tasks,
-                            const double* levels, const int32_t* status, LevelNorm* norms)
+                            const double* levels, const int32_t* status, LevelNorm* norms,
+                            PsiCheckpoint* ckpt)
 {
     if (n_tasks == 0)
         return hipSuccess;
     hipLaunchKernelGGL(normalize_kernel, dim3(static_cast<unsigned>(n_tasks)), dim3(64), 0, stream,
-                       uv, uvr, pots, tasks, n_tasks, levels, status, norms);
+                       uv, uvr, pots, tasks, n_tasks, levels, status, norms, ckpt);
     return hipGetLastError();
 }
 
 hipError_t launch_eigenfunctions(hipStream_t stream, const double* uv, const double* uvr,
-                                 const PotentialRecord* pots, const WaveTask* tasks, int32_t n_tasks,
-                                 const double* levels, const int32_t* status,
-                                 const LevelNorm* norms, int64_t base, double* out)
+                                 const PotentialRecord* pots, const PsiTask* tasks, int32_t n_tasks,
+                                 const double* levels, const int32_t* status, const LevelNorm* norms,
+                                 const PsiCheckpoint* ckpt, int64_t base, double* out)
 {
     if (n_tasks == 0)
         return hipSuccess;
     hipLaunchKernelGGL(eigenfunction_kernel, dim3(static_cast<unsigned>(n_tasks)), dim3(64), 0, stream,
-                       uv, uvr, pots, tasks, levels, status, norms, base, out);
+                       uv, uvr, pots, tasks, levels, status, norms, ckpt, base, out);
     return hipGetLastError();
 }
 
 hipError_t launch_level_vjp(hipStream_t stream, int32_t waves, const double* uv, const double* uvr,
-                            const PotentialRecord* pots, const int32_t* pot_list, int32_t n_list,
+                            const PotentialRecord* pots, const PsiTask* tasks, int32_t n_tasks,
                             const double* levels, const int32_t* status, const LevelNorm* norms,
-                            const double* grad_levels, double* grad_v)
+                            const PsiCheckpoint* ckpt, const double* grad_levels, double* grad_v)
 {
-    if (n_list == 0)
+    if (n_tasks == 0)
         return hipSuccess;
-    const dim3 grid(static_cast<unsigned>(n_list));
+    const dim3 grid(static_cast<unsigned>(n_tasks));
+#define EPSEON_VJP(Wv)                                                                               \
+    hipLaunchKernelGGL(level_vjp_kernel<Wv>, grid, dim3(64 * (Wv)), 0, stream, uv, uvr, pots, tasks, \
+                       levels, status, norms, ckpt, grad_levels, grad_v)
     switch (waves) {
     case 1:
-        hipLaunchKernelGGL(level_vjp_kernel<1>, grid, dim3(64), 0, stream, uv, uvr, pots, pot_list,
-                           levels, status, norms, grad_levels, grad_v);
+        EPSEON_VJP(1);
         break;
     case 2:
-        hipLaunchKernelGGL(level_vjp_kernel<2>, grid, dim3(128), 0, stream, uv, uvr, pots, pot_list,
-                           levels, status, norms, grad_levels, grad_v);
+        EPSEON_VJP(2);
         break;
     case 3:
-        hipLaunchKernelGGL(level_vjp_kernel<3>, grid, dim3(192), 0, stream, uv, uvr, pots, pot_list,
-                           levels, status, norms, grad_levels, grad_v);
+        EPSEON_VJP(3);
         break;
     case 4:
-        hipLaunchKernelGGL(level_vjp_kernel<4>, grid, dim3(256), 0, stream, uv, uvr, pots, pot_list,
-                           levels, status, norms, grad_levels, grad_v);
+        EPSEON_VJP(4);
         break;
     default:
         return hipErrorInvalidValue;
     }
+#undef EPSEON_VJP
     return hipGetLastError();
 }
 

@@ -45,13 +45,15 @@ hipError_t launch_segment_tables(hipStream_t, const double*, const PotentialReco
                                  int32_t, int32_t, double*);
 hipError_t launch_band_windows(hipStream_t, const double*, const PotentialRecord*, int32_t, int2*);
 hipError_t launch_normalize(hipStream_t, const double*, const double*, const PotentialRecord*,
-                            const WaveTask*, int32_t, const double*, const int32_t*, LevelNorm*);
+                            const WaveTask*, int32_t, const double*, const int32_t*, LevelNorm*,
+                            PsiCheckpoint*);
 hipError_t launch_eigenfunctions(hipStream_t, const double*, const double*, const PotentialRecord*,
-                                 const WaveTask*, int32_t, const double*, const int32_t*,
-                                 const LevelNorm*, int64_t, double*);
+                                 const PsiTask*, int32_t, const double*, const int32_t*,
+                                 const LevelNorm*, const PsiCheckpoint*, int64_t, double*);
 hipError_t launch_level_vjp(hipStream_t, int32_t, const double*, const double*,
-                            const PotentialRecord*, const int32_t*, int32_t, const double*,
-                            const int32_t*, const LevelNorm*, const double*, double*);
+                            const PotentialRecord*, const PsiTask*, int32_t, const double*,
+                            const int32_t*, const LevelNorm*, const PsiCheckpoint*, const double*,
+                            double*);
 
 namespace {
 
@@ -299,9 +301,17 @@ struct epseon_ctx {
     int32_t* d_iters = nullptr;
     LevelNorm* d_norms = nullptr;
     double* d_grad_levels = nullptr;
-    // Potentials grouped by adjoint block width (1..4 wavefronts).
-    int32_t* d_vjp_lists[4] = {};
+    // Adjoint tasks (potential, side, segment) grouped by block width (1..4
+    // wavefronts), and the eigenfunction tasks (potential, 64 levels, side,
+    // segment) in potential task order with cumulative counts.
+    PsiTask* d_vjp_lists[4] = {};
     int32_t n_vjp[4] = {};
+    PsiTask* d_psi_tasks = nullptr;
+    std::vector<int32_t> cum_psi_tasks;
+    // Eigenfunction-shot checkpoints written by the normalisation pass
+    // (carved from the arena on first use).
+    PsiCheckpoint* d_ckpt = nullptr;
+    int64_t ckpt_total = 0;
     // Scratch for host-side outputs, carved from the arena on first use.
     double* d_stage = nullptr;
     size_t stage_elems = 0;
@@ -588,6 +598,7 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     ctx->solve_timed = false;
     ctx->d_stage = nullptr;
     ctx->stage_elems = 0;
+    ctx->d_ckpt = nullptr;
     ctx->desc.assign(desc, desc + count);
     ctx->rec.assign(static_cast<size_t>(count), PotentialRecord{});
 
@@ -688,7 +699,7 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
         EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
 
-    int64_t pts = 0, en = 0, lv = 0, psi_total = 0, seg_total = 0;
+    int64_t pts = 0, en = 0, lv = 0, psi_total = 0, seg_total = 0, ckpt_total = 0;
     int32_t max_points = 0, max_seg_rows = 0;
     double scan_pu = 0.0;
     for (int32_t p = 0; p < count; ++p) {
@@ -745,6 +756,15 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
         max_seg_rows = std::max(max_seg_rows, seg_rows(r.n_points, r.flags, r.level_count));
         if (seg_total > INT32_MAX)
             return fail(EPSEON_ERR_INVALID, "batch too large for the segment table");
+        // Eigenfunction shots run in segments of psi_seg_len steps; the
+        // normalisation pass leaves a checkpoint per level at each segment
+        // start (slot 0 of each side is unused).
+        r.psi_seg_len = psi_seg_len(r.n_points);
+        r.psi_segs_left = (r.match + r.psi_seg_len - 1) / r.psi_seg_len;
+        r.psi_segs_right = (r.n_points - 2 - r.match + r.psi_seg_len - 1) / r.psi_seg_len;
+        r.psi_ckpt = ckpt_total;
+        if (r.flags == EPSEON_POT_OK)
+            ckpt_total += static_cast<int64_t>(d.level_count) * (r.psi_segs_left + r.psi_segs_right);
         r.user_offset = v_offsets[p];
         r.psi_offset = psi_total;
         psi_total += static_cast<int64_t>(d.level_count) * d.n_points;
@@ -768,6 +788,7 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     ctx->scan_pu_full = scan_pu;
     ctx->user_span = user_span;
     ctx->psi_total = psi_total;
+    ctx->ckpt_total = ckpt_total;
 
     // Work lists: one wavefront per 64 trial energies (scan) or 64 levels
     // (refine), longest grids first so the dispatcher drains large items early.
@@ -781,7 +802,8 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
         ctx->order_pos[static_cast<size_t>(order[static_cast<size_t>(k)])] = k;
     // Segmented refinement uses one 64-lane block per level.
     std::vector<WaveTask> scan_tasks, refine_tasks, level_tasks, lane1_tasks, lane2_tasks;
-    std::vector<int32_t> vjp_lists[4];
+    std::vector<PsiTask> vjp_lists[4], psi_tasks;
+    ctx->cum_psi_tasks.assign(1, 0);
     ctx->cum_lane1_tasks.assign(1, 0);
     ctx->cum_lane2_tasks.assign(1, 0);
     ctx->cum_scan_tasks.assign(1, 0);
@@ -790,8 +812,22 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     ctx->cum_scan_work.assign(1, 0.0);
     for (int32_t p : order) {
         const int32_t groups = (ctx->rec[p].level_count + 63) / 64;
-        vjp_lists[std::clamp(groups, 1, 4) - 1].push_back(p);
         const PotentialRecord& r = ctx->rec[p];
+        {
+            // Unresolved potentials get one task (side 0, segment 0), which
+            // zeroes their rows and gradient.
+            std::vector<PsiTask>& vl = vjp_lists[std::clamp(groups, 1, 4) - 1];
+            const bool ok = r.flags == EPSEON_POT_OK;
+            for (int32_t side = 0; side < (ok ? 2 : 1); ++side) {
+                const int32_t segs = !ok ? 1 : (side == 0 ? r.psi_segs_left : r.psi_segs_right);
+                for (int32_t sg = 0; sg < segs; ++sg) {
+                    vl.push_back({p, 0, side, sg});
+                    for (int32_t l = 0; l < r.level_count; l += 64)
+                        psi_tasks.push_back({p, l, side, sg});
+                }
+            }
+            ctx->cum_psi_tasks.push_back(static_cast<int32_t>(psi_tasks.size()));
+        }
         if (r.flags == EPSEON_POT_OK)
             for (int32_t j = 0; j < r.n_energies; j += 64)
                 scan_tasks.push_back({p, j});
@@ -876,9 +912,13 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
         EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(vjp_lists[w].size(), 1), &ctx->d_vjp_lists[w]));
         if (!vjp_lists[w].empty())
             EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_vjp_lists[w], vjp_lists[w].data(),
-                                          vjp_lists[w].size() * sizeof(int32_t),
+                                          vjp_lists[w].size() * sizeof(PsiTask),
                                           hipMemcpyHostToDevice, ctx->stream));
     }
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(psi_tasks.size(), 1), &ctx->d_psi_tasks));
+    if (!psi_tasks.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_psi_tasks, psi_tasks.data(), psi_tasks.size() * sizeof(PsiTask),
+                                      hipMemcpyHostToDevice, ctx->stream));
 
     if (v_host != nullptr && user_span > 0)
         EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_v, v_host, static_cast<size_t>(user_span) * sizeof(double),
@@ -1636,10 +1676,13 @@ int epseon_normalize(epseon_ctx* ctx)
         return rc;
     if (!ctx->refined)
         return fail(EPSEON_ERR_STATE, "epseon_normalize before epseon_refine");
+    if (ctx->d_ckpt == nullptr)
+        EPSEON_HIP_TRY(arena_array(ctx->arena, static_cast<size_t>(std::max<int64_t>(ctx->ckpt_total, 1)),
+                                   &ctx->d_ckpt));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
     EPSEON_HIP_TRY(launch_normalize(ctx->stream, ctx->d_uv, ctx->d_uvr, ctx->d_rec,
                                     ctx->d_refine_tasks, ctx->n_refine_tasks, ctx->d_levels,
-                                    ctx->d_status, ctx->d_norms));
+                                    ctx->d_status, ctx->d_norms, ctx->d_ckpt));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
     ctx->normalized = true;
     ctx->normalize_timed = true;
@@ -1664,11 +1707,11 @@ int write_wavefunctions(epseon_ctx* ctx, size_t k0, size_t k1, int64_t base, siz
             return rc;
         dst = ctx->d_stage;
     }
-    const int32_t t0 = ctx->cum_refine_tasks[k0];
-    const int32_t t1 = ctx->cum_refine_tasks[k1];
+    const int32_t t0 = ctx->cum_psi_tasks[k0];
+    const int32_t t1 = ctx->cum_psi_tasks[k1];
     EPSEON_HIP_TRY(launch_eigenfunctions(ctx->stream, ctx->d_uv, ctx->d_uvr, ctx->d_rec,
-                                         ctx->d_refine_tasks + t0, t1 - t0, ctx->d_levels,
-                                         ctx->d_status, ctx->d_norms, base, dst));
+                                         ctx->d_psi_tasks + t0, t1 - t0, ctx->d_levels, ctx->d_status,
+                                         ctx->d_norms, ctx->d_ckpt, base, dst));
     if (mem == EPSEON_MEM_HOST)
         EPSEON_HIP_TRY(hipMemcpyAsync(psi, dst, elems * sizeof(double), hipMemcpyDeviceToHost,
                                       ctx->stream));
@@ -1744,7 +1787,7 @@ int epseon_level_vjp(epseon_ctx* ctx, const double* grad_levels, double* grad_v,
     for (int w = 0; w < 4; ++w)
         EPSEON_HIP_TRY(launch_level_vjp(streams[w], w + 1, ctx->d_uv, ctx->d_uvr, ctx->d_rec,
                                         ctx->d_vjp_lists[w], ctx->n_vjp[w], ctx->d_levels,
-                                        ctx->d_status, ctx->d_norms, g, out));
+                                        ctx->d_status, ctx->d_norms, ctx->d_ckpt, g, out));
     for (int w = 1; w < 4; ++w) {
         EPSEON_HIP_TRY(hipEventRecord(ctx->ev_join_aux[w - 1], streams[w]));
         EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join_aux[w - 1], 0));

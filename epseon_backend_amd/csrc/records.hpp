@@ -53,8 +53,13 @@ struct alignas(16) PotentialRecord {
                             // v_{N-1} * far_scale, with w_{N-1} = that * 2^(far_exp - 480 r)
     int32_t far_exp;        // floor((n_points - 2) log2 12)
     int32_t seg_row;        // first row of this potential's segment table (rows of 64 values)
+    int64_t psi_ckpt;       // first eigenfunction-shot checkpoint slot of this potential
+    int32_t psi_seg_len;    // steps per segment of the eigenfunction shots (a multiple of 32)
+    int32_t psi_segs_left;  // segments of the left shot (match steps)
+    int32_t psi_segs_right; // segments of the right shot (n_points - 2 - match steps)
+    int32_t psi_pad;
 };
-static_assert(sizeof(PotentialRecord) == 160, "record layout");
+static_assert(sizeof(PotentialRecord) == 176, "record layout");
 
 // Segments of the 64-lane segmented refinement of a potential with n points:
 // the q = n - 2 steps split into min(64, max(1, q / 32)) segments, segment s
@@ -77,6 +82,35 @@ constexpr int32_t seg_rows(const int32_t n_points, const int32_t flags, const in
     const int32_t segs = seg_count(n_points);
     return (q + segs - 1) / segs + 33;
 }
+
+// Segment length of the eigenfunction shots of a potential with n points: a
+// multiple of 32 (the rescale block), at least 512 steps, and long enough
+// that the n - 2 steps make at most about 256 segments.
+constexpr int32_t psi_seg_len(const int32_t n_points)
+{
+    const int32_t q = n_points - 2;
+    const int32_t blocks = (q + 32 * 256 - 1) / (32 * 256);
+    return 32 * (blocks < 16 ? 16 : blocks);
+}
+
+// State of an eigenfunction shot at the start of one of its segments (after
+// j = segment * psi_seg_len steps): y_{j+1}, y_j, their difference and the
+// rescale count.  f and f_prev are recomputed from the table, so a shot
+// resumed from here repeats the uninterrupted shot's operations exactly.
+struct alignas(32) PsiCheckpoint {
+    double y, ym, d;
+    int32_t k, pad;
+};
+
+// One wavefront (or workgroup) of the segmented eigenfunction passes: up to
+// 64 levels from `first` of `potential`, one segment of one side (0: the
+// shot from the left wall, 1: from the right wall).
+struct PsiTask {
+    int32_t potential;
+    int32_t first;
+    int32_t side;
+    int32_t segment;
+};
 
 // Normalisation of one level's eigenfunction, from two inward shots that meet
 // at PotentialRecord::match: psi_i = y_i * ldexp(a, 240 (k - k_final)) with

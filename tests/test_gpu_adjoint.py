@@ -124,6 +124,41 @@ def test_vjp_matches_oracle_host_and_device(capi):
     ctx.close()
 
 
+def test_vjp_covers_every_point(capi):
+    """Device-pointer adjoint into a NaN-filled buffer: potentials without
+    levels and unresolved potentials get zeros everywhere, the others match
+    the oracle; no point is left unwritten."""
+    import torch
+
+    from epseon_backend_amd.workloads import MorseSpec
+
+    _log("case: adjoint over a batch with a zero-level and an unresolved potential, NaN-filled output")
+    specs = [H2, MorseSpec(2.0, 8000.0, 1.2, 1.2, 0.65, 7.0, 900, 64),
+             MorseSpec(50.0, 40000.0, 2.5, 2.0, 0.2, 9.0, 60, 64)] + _batch_specs()[:4]
+    counts = [17, 0, 3] + [min(41, s.bound_levels()) for s in specs[3:]]
+    ctx = _solve(capi, specs, counts)
+    assert ctx.potential_info(2).flags == 1
+    lv, st, _ = ctx.levels()
+    g = np.random.default_rng(11).normal(size=lv.shape)
+    gd = torch.from_numpy(g).cuda()
+    out = torch.full((ctx.v_span,), float("nan"), dtype=torch.float64, device="cuda")
+    ctx.level_vjp_device(gd.data_ptr(), out.data_ptr())
+    gv = out.cpu().numpy()
+    assert np.all(np.isfinite(gv))
+    lo = 0
+    for p, s in enumerate(specs):
+        seg = gv[ctx.v_offsets[p] : ctx.v_offsets[p] + s.n]
+        if p in (1, 2):
+            assert np.all(seg == 0.0), p
+        else:
+            prob = problem_from_spec(s)
+            lev = np.where(st[lo : lo + counts[p]] == 0, lv[lo : lo + counts[p]], np.nan)
+            ref = oracle.level_vjp(prob, lev, g[lo : lo + counts[p]])
+            np.testing.assert_allclose(seg, ref, rtol=0, atol=1e-11 * np.abs(ref).max(), err_msg=f"potential {p}")
+        lo += counts[p]
+    ctx.close()
+
+
 def test_pointwise_hellmann_feynman_vs_finite_differences(capi):
     _log("case: dE/dV_i vs central differences of GPU solves, H2, levels 0, 5, 12")
     ctx = _solve(capi, [H2], [17], rel_tol=1e-14)

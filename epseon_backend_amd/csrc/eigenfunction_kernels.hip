@@ -103,7 +103,7 @@ struct YShot {
         y = __builtin_ldexp(y, sh);
         ym = __builtin_ldexp(ym, sh);
         d = __builtin_ldexp(d, sh);
-        k += big ? 1 : 0;
+        k += big ? kPsiRescaleLog2 : 0;
         return sh;
     }
 };
@@ -145,32 +145,65 @@ __device__ __forceinline__ void psi_walk(const double* __restrict__ p, const int
     }
 }
 
-// Pass 1: sum of y_i^2 over the visited points (in the shot's current scale),
-// and the shot's state at the start of each of its segments after the first
-// (ck[s * ck_stride] for segment s), from which the later passes re-run the
-// segments independently.
-struct NormBody {
-    YShot s;
-    double e, c10, sum;
-    PsiCheckpoint* ck;
-    int64_t ck_stride;
-    int32_t blocks_per_seg, blk, seg, segs;
+// frexp exponent of m (0 for m = 0).
+__device__ __forceinline__ int32_t binary_exponent(const double m)
+{
+    int e = 0;
+    (void)__builtin_frexp(m, &e);
+    return e;
+}
+
+// Pass 1, per segment: the shot's two basis solutions through the segment,
+// started from (y, y_prev, d) = (1, 1, 0) and (0, -1, 1), so that a shot
+// entering with (y, d) leaves with y times column a plus d times column b;
+// and the Gram sums of their values at the visited points, from which the
+// segment's sum of y^2 follows as a quadratic form in (y, d).  All values
+// are renormalised by a power of two after every 32 steps and at the end;
+// e is the accumulated exponent (the Gram sums carry it twice).
+struct MatBody {
+    double ay, aym, ad, by, bym, bd;
+    double saa, sab, sbb;
+    double f, fm, e, c10;
+    int32_t ex;
     __device__ __forceinline__ void step(const double u, int)
     {
-        sum = __builtin_fma(s.y, s.y, sum);
-        s.step(u, e, c10);
+        saa = __builtin_fma(ay, ay, saa);
+        sab = __builtin_fma(ay, by, sab);
+        sbb = __builtin_fma(by, by, sbb);
+        const double fn = e - u;
+        const double r = 1.0 / (1.0 + fn);
+        const double sg = __builtin_fma(c10, f, fn);
+        const double ca = __builtin_fma(sg, ay, fm * aym);
+        const double cb = __builtin_fma(sg, by, fm * bym);
+        ad = (ad - ca) * r;
+        bd = (bd - cb) * r;
+        aym = ay;
+        bym = by;
+        ay = ay + ad;
+        by = by + bd;
+        fm = f;
+        f = fn;
+    }
+    __device__ __forceinline__ void renormalize()
+    {
+        const double mx = fmax(fmax(fmax(fabs(ay), fabs(aym)), fmax(fabs(ad), fabs(by))),
+                               fmax(fabs(bym), fabs(bd)));
+        const int32_t x = binary_exponent(mx);
+        ay = __builtin_ldexp(ay, -x);
+        aym = __builtin_ldexp(aym, -x);
+        ad = __builtin_ldexp(ad, -x);
+        by = __builtin_ldexp(by, -x);
+        bym = __builtin_ldexp(bym, -x);
+        bd = __builtin_ldexp(bd, -x);
+        saa = __builtin_ldexp(saa, -2 * x);
+        sab = __builtin_ldexp(sab, -2 * x);
+        sbb = __builtin_ldexp(sbb, -2 * x);
+        ex += x;
     }
     __device__ __forceinline__ void block_end(int32_t, const bool full)
     {
-        if (full) {
-            const int32_t sh = s.rescale();
-            sum = __builtin_ldexp(sum, 2 * sh);
-            if (++blk == blocks_per_seg) {
-                blk = 0;
-                if (++seg < segs)
-                    ck[seg * ck_stride] = PsiCheckpoint{s.y, s.ym, s.d, s.k, 0};
-            }
-        }
+        if (full)
+            renormalize();
     }
 };
 
@@ -190,7 +223,7 @@ struct PsiBody {
         base += count;
         if (full) {
             (void)s.rescale();
-            mult = __builtin_ldexp(a, kPsiRescaleLog2 * (s.k - kf));
+            mult = __builtin_ldexp(a, s.k - kf);
         }
     }
 };
@@ -242,7 +275,7 @@ struct VjpBody {
         base += count;
         if (full) {
             (void)s.rescale();
-            G = __builtin_ldexp(g0, 2 * kPsiRescaleLog2 * (s.k - kf));
+            G = __builtin_ldexp(g0, 2 * (s.k - kf));
         }
     }
 };
@@ -253,73 +286,6 @@ __device__ __forceinline__ bool level_active(const int32_t* __restrict__ status,
 }
 
 }  // namespace
-
-// Pass 1: one lane per level (wavefront = up to 64 levels of one potential).
-// Shoots inward from both walls at the refined E, matches at m and writes the
-// level's LevelNorm.
-__global__ __launch_bounds__(256) void normalize_kernel(
-    const double* __restrict__ uv, const double* __restrict__ uvr,
-    const PotentialRecord* __restrict__ pots, const WaveTask* __restrict__ tasks,
-    const int32_t n_tasks, const double* __restrict__ levels, const int32_t* __restrict__ status,
-    LevelNorm* __restrict__ norms, PsiCheckpoint* __restrict__ ckpt)
-{
-    const int32_t waves = static_cast<int32_t>(blockDim.x >> 6);
-    const int32_t tid = static_cast<int32_t>(blockIdx.x) * waves +
-                        __builtin_amdgcn_readfirstlane(static_cast<int32_t>(threadIdx.x >> 6));
-    if (tid >= n_tasks)
-        return;
-    const WaveTask task = tasks[tid];
-    const PotentialRecord& P = pots[task.potential];
-    const int32_t l = task.first + static_cast<int32_t>(threadIdx.x & 63);
-    if (l >= P.level_count)
-        return;
-    const int64_t o = P.level_offset + l;
-    if (P.flags != 0 || !level_active(status, o)) {
-        norms[o] = LevelNorm{0.0, 0.0, 0, 0};
-        return;
-    }
-    const double e = P.beta * levels[o];
-    const double c10 = opaque(10.0);
-    const int32_t n = P.n_points;
-    const int32_t m = P.match;
-    const double* u = uv + P.ut_offset;
-    const double* ur = uvr + P.ut_offset;
-
-    PsiCheckpoint* ck = ckpt + P.psi_ckpt + l;
-    const int64_t stride = P.level_count;
-
-    NormBody L{};
-    L.e = e;
-    L.c10 = c10;
-    L.sum = 0.0;
-    L.ck = ck;
-    L.ck_stride = stride;
-    L.blocks_per_seg = P.psi_seg_len / 32;
-    L.segs = P.psi_segs_left;
-    L.s.init(e, u[0], u[1]);
-    psi_walk(u + 2, m, L);
-
-    NormBody R{};
-    R.e = e;
-    R.c10 = c10;
-    R.sum = 0.0;
-    R.ck = ck + P.psi_segs_left * stride;
-    R.ck_stride = stride;
-    R.blocks_per_seg = P.psi_seg_len / 32;
-    R.segs = P.psi_segs_right;
-    R.s.init(e, ur[0], ur[1]);
-    psi_walk(ur + 2, n - 2 - m, R);
-
-    // Left: (y_m, y_{m+1}) = (ym, y); right (reversed): (y_{m+1}, y_m) = (ym, y).
-    const double yl0 = L.s.ym, yl1 = L.s.y;
-    const double yr1 = R.s.ym, yr0 = R.s.y;
-    const double num = __builtin_fma(yl0, yr0, yl1 * yr1);
-    const double den = __builtin_fma(yr0, yr0, yr1 * yr1);
-    const double c = num / den;
-    const double s = __builtin_fma(c * c, R.sum, L.sum);
-    const double a = 1.0 / sqrt(s * P.h);
-    norms[o] = LevelNorm{a, c * a, L.s.k, R.s.k};
-}
 
 // Shot segment of a PsiTask: the side's table, the first step j0 and the
 // number of steps.
@@ -336,6 +302,12 @@ __device__ __forceinline__ PsiSegment psi_segment(const PotentialRecord& P, cons
     return PsiSegment{(t.side == 0 ? uv : uvr) + P.ut_offset, j0, min(P.psi_seg_len, total - j0)};
 }
 
+__device__ __forceinline__ int64_t psi_slot(const PotentialRecord& P, const int32_t side, const int32_t segment,
+                                            const int32_t l)
+{
+    return P.psi_ckpt + static_cast<int64_t>((side == 0 ? 0 : P.psi_segs_left) + segment) * P.level_count + l;
+}
+
 // Start a segment's shot: from the wall (segment 0), from the normalisation
 // pass's checkpoint, or all zero for a level that did not converge.
 __device__ __forceinline__ void psi_start(YShot& s, const PotentialRecord& P, const PsiTask& t,
@@ -347,9 +319,107 @@ __device__ __forceinline__ void psi_start(YShot& s, const PotentialRecord& P, co
     else if (t.segment == 0)
         s.init(e, g.p[0], g.p[1]);
     else
-        s.resume(ckpt[P.psi_ckpt + static_cast<int64_t>((t.side == 0 ? 0 : P.psi_segs_left) + t.segment) *
-                                        P.level_count + l],
-                 e, g.p[g.j0], g.p[g.j0 + 1]);
+        s.resume(ckpt[psi_slot(P, t.side, t.segment, l)], e, g.p[g.j0], g.p[g.j0 + 1]);
+}
+
+// Pass 1a: one wavefront per eigenfunction task (potential, 64 levels, side,
+// segment): each converged level's segment matrix and Gram sums.
+__global__ __launch_bounds__(64) void psi_segment_kernel(
+    const double* __restrict__ uv, const double* __restrict__ uvr,
+    const PotentialRecord* __restrict__ pots, const PsiTask* __restrict__ tasks,
+    const double* __restrict__ levels, const int32_t* __restrict__ status,
+    PsiSegMatrix* __restrict__ mats)
+{
+    const PsiTask task = tasks[blockIdx.x];
+    const PotentialRecord& P = pots[task.potential];
+    const int32_t l = task.first + static_cast<int32_t>(threadIdx.x);
+    if (P.flags != 0 || l >= P.level_count)
+        return;
+    const int64_t o = P.level_offset + l;
+    if (!level_active(status, o))
+        return;
+    const double e = P.beta * levels[o];
+    const PsiSegment g = psi_segment(P, uv, uvr, task);
+    MatBody B{};
+    B.ay = 1.0;
+    B.aym = 1.0;
+    B.ad = 0.0;
+    B.by = 0.0;
+    B.bym = -1.0;
+    B.bd = 1.0;
+    B.e = e;
+    B.c10 = opaque(10.0);
+    B.fm = e - g.p[g.j0];
+    B.f = e - g.p[g.j0 + 1];
+    psi_walk(g.p + 2 + g.j0, g.steps, B);
+    B.renormalize();
+    mats[psi_slot(P, task.side, task.segment, l)] =
+        PsiSegMatrix{B.ay, B.aym, B.ad, B.by, B.bym, B.bd, B.saa, B.sab, B.sbb, B.ex, 0};
+}
+
+// Pass 1b: one lane per level (wavefront = up to 64 levels of one potential).
+// Chains the segment matrices of each side from its wall, (y, y_prev, d) =
+// (1, 0, 1), renormalising by a power of two after each product; writes the
+// checkpoint (state and binary exponent) at each later segment start, sums
+// y^2 as the segments' quadratic forms, matches the sides at m and writes the
+// level's LevelNorm (exponents in bits).
+__global__ __launch_bounds__(64) void normalize_kernel(
+    const PotentialRecord* __restrict__ pots, const WaveTask* __restrict__ tasks, const int32_t n_tasks,
+    const int32_t* __restrict__ status, const PsiSegMatrix* __restrict__ mats,
+    PsiCheckpoint* __restrict__ ckpt, LevelNorm* __restrict__ norms)
+{
+    const int32_t tid = static_cast<int32_t>(blockIdx.x);
+    if (tid >= n_tasks)
+        return;
+    const WaveTask task = tasks[tid];
+    const PotentialRecord& P = pots[task.potential];
+    const int32_t l = task.first + static_cast<int32_t>(threadIdx.x);
+    if (l >= P.level_count)
+        return;
+    const int64_t o = P.level_offset + l;
+    if (P.flags != 0 || !level_active(status, o)) {
+        norms[o] = LevelNorm{0.0, 0.0, 0, 0};
+        return;
+    }
+    double ye[2], yme[2], srel[2];
+    int32_t ke[2];
+    for (int32_t side = 0; side < 2; ++side) {
+        const int32_t segs = side == 0 ? P.psi_segs_left : P.psi_segs_right;
+        double y = 1.0, ym = 0.0, d = 1.0, sr = 0.0;
+        int32_t k = 0;
+        for (int32_t sg = 0; sg < segs; ++sg) {
+            const int64_t slot = psi_slot(P, side, sg, l);
+            if (sg > 0)
+                ckpt[slot] = PsiCheckpoint{y, ym, d, k, 0};
+            const PsiSegMatrix M = mats[slot];
+            const double q = __builtin_fma(y * y, M.saa, __builtin_fma(2.0 * y * d, M.sab, d * d * M.sbb));
+            const double yn = __builtin_fma(M.ay, y, M.by * d);
+            const double ymn = __builtin_fma(M.aym, y, M.bym * d);
+            const double dn = __builtin_fma(M.ad, y, M.bd * d);
+            const double mx = fmax(fmax(fabs(yn), fabs(ymn)), fabs(dn));
+            const int32_t x = binary_exponent(mx);
+            y = __builtin_ldexp(yn, -x);
+            ym = __builtin_ldexp(ymn, -x);
+            d = __builtin_ldexp(dn, -x);
+            // y^2 sum relative to 2^(2k) of the new exponent k + e + x.
+            sr = __builtin_ldexp(sr, -2 * (M.e + x)) + __builtin_ldexp(q, -2 * x);
+            k += M.e + x;
+        }
+        ye[side] = y;
+        yme[side] = ym;
+        srel[side] = sr;
+        ke[side] = k;
+    }
+    // Left: (y_m, y_{m+1}) = (ym, y); right (reversed): (y_{m+1}, y_m) = (ym, y);
+    // the right shot scaled by c = cm 2^(k_left - k_right).
+    const double yl0 = yme[0], yl1 = ye[0];
+    const double yr1 = yme[1], yr0 = ye[1];
+    const double num = __builtin_fma(yl0, yr0, yl1 * yr1);
+    const double den = __builtin_fma(yr0, yr0, yr1 * yr1);
+    const double cm = num / den;
+    const double s = __builtin_fma(cm * cm, srel[1], srel[0]);
+    const double a = 1.0 / sqrt(s * P.h);
+    norms[o] = LevelNorm{a, cm * a, ke[0], ke[1]};
 }
 
 // Eigenfunctions: for each task (potential, first level, side, segment), the
@@ -407,7 +477,7 @@ __global__ __launch_bounds__(64) void eigenfunction_kernel(
     B.base = g.j0;
     B.row = row + first;
     psi_start(B.s, P, task, g, ckpt, l, active, e);
-    B.mult = __builtin_ldexp(B.a, kPsiRescaleLog2 * (B.s.k - B.kf));
+    B.mult = __builtin_ldexp(B.a, B.s.k - B.kf);
     psi_walk(g.p + 2 + g.j0, g.steps, B);
 }
 
@@ -484,7 +554,7 @@ __global__ __launch_bounds__(64 * W) void level_vjp_kernel(
         B.sw = s_w;
         B.out = win;
         psi_start(B.s, P, task, g, ckpt, l, active, e);
-        B.G = __builtin_ldexp(B.g0, 2 * kPsiRescaleLog2 * (B.s.k - B.kf));
+        B.G = __builtin_ldexp(B.g0, 2 * (B.s.k - B.kf));
         psi_walk(g.p + 2 + g.j0, g.steps, B);
     }
 }
@@ -492,14 +562,20 @@ __global__ __launch_bounds__(64 * W) void level_vjp_kernel(
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_normalize(hipStream_t stream, const double* uv, const double* uvr,
-                            const PotentialRecord* pots, const WaveTask* tasks, int32_t n_tasks,
-                            const double* levels, const int32_t* status, LevelNorm* norms,
-                            PsiCheckpoint* ckpt)
+                            const PotentialRecord* pots, const PsiTask* psi_tasks, int32_t n_psi_tasks,
+                            const WaveTask* tasks, int32_t n_tasks, const double* levels,
+                            const int32_t* status, PsiSegMatrix* mats, PsiCheckpoint* ckpt, LevelNorm* norms)
 {
+    if (n_psi_tasks > 0) {
+        hipLaunchKernelGGL(psi_segment_kernel, dim3(static_cast<unsigned>(n_psi_tasks)), dim3(64), 0, stream,
+                           uv, uvr, pots, psi_tasks, levels, status, mats);
+        if (const hipError_t e = hipGetLastError(); e != hipSuccess)
+            return e;
+    }
     if (n_tasks == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(normalize_kernel, dim3(static_cast<unsigned>(n_tasks)), dim3(64), 0, stream,
-                       uv, uvr, pots, tasks, n_tasks, levels, status, norms, ckpt);
+    hipLaunchKernelGGL(normalize_kernel, dim3(static_cast<unsigned>(n_tasks)), dim3(64), 0, stream, pots, tasks,
+                       n_tasks, status, mats, ckpt, norms);
     return hipGetLastError();
 }
 

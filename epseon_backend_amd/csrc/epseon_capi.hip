@@ -45,8 +45,8 @@ hipError_t launch_segment_tables(hipStream_t, const double*, const PotentialReco
                                  int32_t, int32_t, double*);
 hipError_t launch_band_windows(hipStream_t, const double*, const PotentialRecord*, int32_t, int2*);
 hipError_t launch_normalize(hipStream_t, const double*, const double*, const PotentialRecord*,
-                            const WaveTask*, int32_t, const double*, const int32_t*, LevelNorm*,
-                            PsiCheckpoint*);
+                            const PsiTask*, int32_t, const WaveTask*, int32_t, const double*, const int32_t*,
+                            PsiSegMatrix*, PsiCheckpoint*, LevelNorm*);
 hipError_t launch_eigenfunctions(hipStream_t, const double*, const double*, const PotentialRecord*,
                                  const PsiTask*, int32_t, const double*, const int32_t*,
                                  const LevelNorm*, const PsiCheckpoint*, int64_t, double*);
@@ -311,6 +311,7 @@ struct epseon_ctx {
     // Eigenfunction-shot checkpoints written by the normalisation pass
     // (carved from the arena on first use).
     PsiCheckpoint* d_ckpt = nullptr;
+    PsiSegMatrix* d_psi_mats = nullptr;
     int64_t ckpt_total = 0;
     // Scratch for host-side outputs, carved from the arena on first use.
     double* d_stage = nullptr;
@@ -599,6 +600,7 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     ctx->d_stage = nullptr;
     ctx->stage_elems = 0;
     ctx->d_ckpt = nullptr;
+    ctx->d_psi_mats = nullptr;
     ctx->desc.assign(desc, desc + count);
     ctx->rec.assign(static_cast<size_t>(count), PotentialRecord{});
 
@@ -1676,13 +1678,16 @@ int epseon_normalize(epseon_ctx* ctx)
         return rc;
     if (!ctx->refined)
         return fail(EPSEON_ERR_STATE, "epseon_normalize before epseon_refine");
-    if (ctx->d_ckpt == nullptr)
-        EPSEON_HIP_TRY(arena_array(ctx->arena, static_cast<size_t>(std::max<int64_t>(ctx->ckpt_total, 1)),
-                                   &ctx->d_ckpt));
+    if (ctx->d_ckpt == nullptr) {
+        const size_t slots = static_cast<size_t>(std::max<int64_t>(ctx->ckpt_total, 1));
+        EPSEON_HIP_TRY(arena_array(ctx->arena, slots, &ctx->d_ckpt));
+        EPSEON_HIP_TRY(arena_array(ctx->arena, slots, &ctx->d_psi_mats));
+    }
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
-    EPSEON_HIP_TRY(launch_normalize(ctx->stream, ctx->d_uv, ctx->d_uvr, ctx->d_rec,
-                                    ctx->d_refine_tasks, ctx->n_refine_tasks, ctx->d_levels,
-                                    ctx->d_status, ctx->d_norms, ctx->d_ckpt));
+    EPSEON_HIP_TRY(launch_normalize(ctx->stream, ctx->d_uv, ctx->d_uvr, ctx->d_rec, ctx->d_psi_tasks,
+                                    ctx->cum_psi_tasks.back(), ctx->d_refine_tasks, ctx->n_refine_tasks,
+                                    ctx->d_levels, ctx->d_status, ctx->d_psi_mats, ctx->d_ckpt,
+                                    ctx->d_norms));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
     ctx->normalized = true;
     ctx->normalize_timed = true;

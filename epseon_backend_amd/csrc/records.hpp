@@ -94,12 +94,23 @@ constexpr int32_t psi_seg_len(const int32_t n_points)
 }
 
 // State of an eigenfunction shot at the start of one of its segments (after
-// j = segment * psi_seg_len steps): y_{j+1}, y_j, their difference and the
-// rescale count.  f and f_prev are recomputed from the table, so a shot
-// resumed from here repeats the uninterrupted shot's operations exactly.
+// j = segment * psi_seg_len steps): y_{j+1}, y_j and their difference as
+// mantissas of the binary exponent k (the true values are these times 2^k).
+// f and f_prev are recomputed from the table.
 struct alignas(32) PsiCheckpoint {
     double y, ym, d;
     int32_t k, pad;
+};
+
+// Eigenfunction shot through one segment, from the two basis states (y,
+// y_prev, d) = (1, 1, 0) (column a) and (0, -1, 1) (column b): the end state
+// of a shot entering with (y, d) is y * a + d * b, times 2^e; s_aa, s_ab,
+// s_bb are the Gram sums of the columns' y values over the segment's visited
+// points, times 2^(2e).
+struct PsiSegMatrix {
+    double ay, aym, ad, by, bym, bd;
+    double saa, sab, sbb;
+    int32_t e, pad;
 };
 
 // One wavefront (or workgroup) of the segmented eigenfunction passes: up to
@@ -113,10 +124,10 @@ struct PsiTask {
 };
 
 // Normalisation of one level's eigenfunction, from two inward shots that meet
-// at PotentialRecord::match: psi_i = y_i * ldexp(a, 240 (k - k_final)) with
+// at PotentialRecord::match: psi_i = y_i * ldexp(a, k - k_final) with
 // (a, k_final) = (a_left, k_left) left of the match and (a_right, k_right)
-// right of it, k being the shot's rescale count at point i.  Zero for levels
-// that did not converge.
+// right of it, y_i * 2^k being the shot's value at point i (k in bits).
+// Zero for levels that did not converge.
 struct LevelNorm {
     double a_left, a_right;
     int32_t k_left, k_right;

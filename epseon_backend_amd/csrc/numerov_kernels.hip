@@ -528,15 +528,39 @@ namespace {
 __device__ __forceinline__ int32_t fringe_end(const double* __restrict__ u, const int32_t n, const double e,
                                               const int32_t start, const int32_t dir)
 {
+    // Chunks of 64 x kPer points, kPer consecutive ones (in walk order) per
+    // lane: each lane sums its terms, one wave prefix sum orders the lanes,
+    // and the first lane reaching the depth finds the crossing among its own.
+    constexpr int32_t kPer = 8;
     const int32_t lane = static_cast<int32_t>(threadIdx.x & 63);
     const int64_t depth = int64_t(kWindowDepth) << 40;
     int64_t acc = 0;
-    for (int32_t base = start; base >= 0 && base < n; base += 64 * dir) {
-        const int32_t i = base + dir * lane;
-        const int64_t inc = wave_inclusive_sum(i >= 0 && i < n ? growth_term(12.0 * (u[i] - e)) : 0);
+    for (int32_t base = start; base >= 0 && base < n; base += 64 * kPer * dir) {
+        int64_t g[kPer];
+        int64_t own = 0;
+#pragma unroll
+        for (int32_t q = 0; q < kPer; ++q) {
+            const int32_t i = base + dir * (kPer * lane + q);
+            g[q] = (i >= 0 && i < n) ? growth_term(12.0 * (u[i] - e)) : 0;
+            own += g[q];
+        }
+        const int64_t inc = wave_inclusive_sum(own);
         const uint64_t hit = __ballot(acc + inc >= depth);
-        if (hit != 0)
-            return base + dir * static_cast<int32_t>(__builtin_ctzll(hit));
+        if (hit != 0) {
+            const int32_t h = static_cast<int32_t>(__builtin_ctzll(hit));
+            int64_t run = acc + inc - own;
+            int32_t at = kPer - 1;
+            bool found = false;
+#pragma unroll
+            for (int32_t q = 0; q < kPer; ++q) {
+                run += g[q];
+                if (!found && run >= depth) {
+                    at = q;
+                    found = true;
+                }
+            }
+            return base + dir * (kPer * h + __shfl(at, h, 64));
+        }
         acc += __shfl(inc, 63, 64);
     }
     return dir > 0 ? n - 1 : 0;
@@ -544,15 +568,16 @@ __device__ __forceinline__ int32_t fringe_end(const double* __restrict__ u, cons
 
 }  // namespace
 
-// Band windows: one 256-thread workgroup per potential.  One pass over the
-// table finds the first and last allowed point (uv_i <= e_b) of all
-// kBands band tops at once; then its four wavefronts walk the 2 kBands
+// Band windows: one workgroup of kBandWaves wavefronts per potential.  One
+// pass over the table finds the first and last allowed point (uv_i <= e_b)
+// of all kBands band tops at once; then the wavefronts walk the 2 kBands
 // fringes.  Writes (lo, hi) per band (whole grid for rejected potentials).
-__global__ __launch_bounds__(256) void band_window_kernel(const double* __restrict__ uv,
+constexpr int32_t kBandWaves = 8;
+__global__ __launch_bounds__(64 * kBandWaves) void band_window_kernel(const double* __restrict__ uv,
                                                           const PotentialRecord* __restrict__ pots,
                                                           const int32_t n_pots, int2* __restrict__ out)
 {
-    __shared__ int32_t s_first[4][kBands], s_last[4][kBands];
+    __shared__ int32_t s_first[kBandWaves][kBands], s_last[kBandWaves][kBands];
     __shared__ int32_t s_win[2][kBands];
     const int32_t p = static_cast<int32_t>(blockIdx.x);
     if (p >= n_pots)
@@ -575,14 +600,21 @@ __global__ __launch_bounds__(256) void band_window_kernel(const double* __restri
         first[b] = n;
         last[b] = -1;
     }
-    for (int32_t i = tid; i < n; i += 256) {
-        const double x = u[i];
+    // Four loads in flight per thread.
+    constexpr int32_t kT = 64 * kBandWaves;
+    for (int32_t i0 = tid; i0 < n; i0 += 4 * kT) {
+        double x[4];
 #pragma unroll
-        for (int32_t b = 0; b < kBands; ++b)
-            if (x <= eb[b]) {
-                first[b] = min(first[b], i);
-                last[b] = i;
-            }
+        for (int32_t q = 0; q < 4; ++q)
+            x[q] = i0 + q * kT < n ? u[i0 + q * kT] : __builtin_inf();
+#pragma unroll
+        for (int32_t q = 0; q < 4; ++q)
+#pragma unroll
+            for (int32_t b = 0; b < kBands; ++b)
+                if (x[q] <= eb[b]) {
+                    first[b] = min(first[b], i0 + q * kT);
+                    last[b] = i0 + q * kT;
+                }
     }
 #pragma unroll
     for (int32_t b = 0; b < kBands; ++b) {
@@ -597,11 +629,15 @@ __global__ __launch_bounds__(256) void band_window_kernel(const double* __restri
         }
     }
     __syncthreads();
-    // Fringe walks: task t = 2 b + side, wavefront w takes t = w, w + 4, ...
-    for (int32_t t = wave; t < 2 * kBands; t += 4) {
+    // Fringe walks: task t = 2 b + side, wavefront w takes t = w, w + kBandWaves, ...
+    for (int32_t t = wave; t < 2 * kBands; t += kBandWaves) {
         const int32_t b = t >> 1, side = t & 1;
-        const int32_t f = min(min(s_first[0][b], s_first[1][b]), min(s_first[2][b], s_first[3][b]));
-        const int32_t l = max(max(s_last[0][b], s_last[1][b]), max(s_last[2][b], s_last[3][b]));
+        int32_t f = s_first[0][b], l = s_last[0][b];
+#pragma unroll
+        for (int32_t w = 1; w < kBandWaves; ++w) {
+            f = min(f, s_first[w][b]);
+            l = max(l, s_last[w][b]);
+        }
         int32_t r = side ? n - 1 : 0;
         if (l >= 0)
             r = side ? fringe_end(u, n, eb[b], l + 1, 1) : fringe_end(u, n, eb[b], f - 1, -1);
@@ -1542,8 +1578,8 @@ hipError_t launch_band_windows(hipStream_t stream, const double* uv, const Poten
 {
     if (n_pots == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(band_window_kernel, dim3(static_cast<unsigned>(n_pots)), dim3(256), 0, stream, uv, pots,
-                       n_pots, out);
+    hipLaunchKernelGGL(band_window_kernel, dim3(static_cast<unsigned>(n_pots)), dim3(64 * kBandWaves), 0, stream,
+                       uv, pots, n_pots, out);
     return hipGetLastError();
 }
 

@@ -1064,20 +1064,24 @@ struct SegCount {
     int32_t internal, first, last, has;
 };
 
+// Values per VGPR buffer of the segment walks: three rotating buffers of 8
+// keep the walk's registers low enough for three wavefronts per SIMD.
+constexpr int kWalkChunk = 8;
+
 template <class Body>
-__device__ __forceinline__ void walk_chunk(const double (&buf)[16], const int32_t j, Body& body)
+__device__ __forceinline__ void walk_chunk(const double (&buf)[kWalkChunk], const int32_t j, Body& body)
 {
 #pragma unroll
-    for (int k = 0; k < 16; ++k)
+    for (int k = 0; k < kWalkChunk; ++k)
         body(buf[k], j + k);
 }
 
 template <class Body>
-__device__ __forceinline__ void walk_tail(const double (&buf)[16], const int32_t j, const int32_t len,
+__device__ __forceinline__ void walk_tail(const double (&buf)[kWalkChunk], const int32_t j, const int32_t len,
                                           Body& body)
 {
 #pragma unroll
-    for (int k = 0; k < 16; ++k)
+    for (int k = 0; k < kWalkChunk; ++k)
         if (j + k < len)
             body(buf[k], j + k);
 }
@@ -1108,42 +1112,43 @@ struct ColumnSource {
 };
 
 template <class Src>
-__device__ __forceinline__ void load16(double (&buf)[16], const Src& src, const int32_t j)
+__device__ __forceinline__ void load_chunk(double (&buf)[kWalkChunk], const Src& src, const int32_t j)
 {
 #pragma unroll
-    for (int k = 0; k < 16; ++k)
+    for (int k = 0; k < kWalkChunk; ++k)
         buf[k] = src.at(j + k);
 }
 
 // Visit the `len` values src.at(0 .. len - 1) in order, body(value, index).
-// Three 16-value VGPR buffers rotate so that every load is issued two chunks
-// (32 steps) before its use; vector-memory loads return in order, so the
-// waits are partial.  Also reads the 32 values that follow the segment (the
-// tables are padded).
+// Three kWalkChunk-value VGPR buffers rotate so that every load is issued two
+// chunks before its use; vector-memory loads return in order, so the waits
+// are partial.  Also reads the 2 kWalkChunk values that follow the segment
+// (the tables are padded).
 template <class Src, class Body>
 __device__ __forceinline__ void walk_segment(const Src& src, const int32_t len, Body& body)
 {
-    double A[16], B[16], C[16];
-    load16(A, src, 0);
-    load16(B, src, 16);
+    constexpr int32_t C = kWalkChunk;
+    double A[C], B[C], D[C];
+    load_chunk(A, src, 0);
+    load_chunk(B, src, C);
     int32_t j = 0;
-    for (; j + 48 <= len; j += 48) {
-        load16(C, src, j + 32);
+    for (; j + 3 * C <= len; j += 3 * C) {
+        load_chunk(D, src, j + 2 * C);
         walk_chunk(A, j, body);
-        load16(A, src, j + 48);
-        walk_chunk(B, j + 16, body);
-        load16(B, src, j + 64);
-        walk_chunk(C, j + 32, body);
+        load_chunk(A, src, j + 3 * C);
+        walk_chunk(B, j + C, body);
+        load_chunk(B, src, j + 4 * C);
+        walk_chunk(D, j + 2 * C, body);
     }
     const int32_t rem = len - j;
-    if (rem >= 32) {
-        load16(C, src, j + 32);
+    if (rem >= 2 * C) {
+        load_chunk(D, src, j + 2 * C);
         walk_chunk(A, j, body);
-        walk_chunk(B, j + 16, body);
-        walk_tail(C, j + 32, len, body);
-    } else if (rem >= 16) {
+        walk_chunk(B, j + C, body);
+        walk_tail(D, j + 2 * C, len, body);
+    } else if (rem >= C) {
         walk_chunk(A, j, body);
-        walk_tail(B, j + 16, len, body);
+        walk_tail(B, j + C, len, body);
     } else {
         walk_tail(A, j, len, body);
     }
@@ -1260,7 +1265,7 @@ __global__ __launch_bounds__(256) void segment_table_kernel(const double* __rest
 // the matrices sequentially (lane 0) for the segment start vectors and re-run
 // every segment.
 template <int T, bool TABLE>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(2))) void numerov_refine_segmented_kernel(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void numerov_refine_segmented_kernel(
     const double* __restrict__ uv, const double* __restrict__ segtab, const PotentialRecord* __restrict__ pots,
     const BandWindow* __restrict__ windows, const WaveTask* __restrict__ tasks, const int32_t n_tasks,
     const int32_t* __restrict__ bins, const Bracket* __restrict__ brackets, const double rel_tol,

@@ -346,7 +346,9 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps before timing (default: 20 single, 10 batched and fit, 0 stress; "
+                         "the clock settles over the first ~40 ms of back-to-back solves)")
     ap.add_argument("--workload", choices=["single", "batched", "fit", "stress"], default="single")
     ap.add_argument("--rel-tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -358,6 +360,8 @@ def main() -> int:
         "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic of the scan)",
     )
     args = ap.parse_args()
+    if args.warmup is None:
+        args.warmup = {"single": 20, "batched": 10, "fit": 10, "stress": 0}[args.workload]
 
     if args.copy_calibration:
         import torch

@@ -284,6 +284,8 @@ struct epseon_ctx {
     int32_t graph = 1;
     hipGraphExec_t solve_exec = nullptr;
     SolveKey graph_key{};
+    SolveKey last_key{};  // key of the last single-stream solve
+    bool has_last_key = false;
     uint64_t loads = 0;  // batches loaded so far: part of the graph key
     bool last_solve_graph = false;
     int32_t levels_per_lane = 1;
@@ -1377,10 +1379,17 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
         ctx->last_refine_mode = mode;
         ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
         begin_scan(ctx, mode);
-        if (ctx->graph && ctx->n_scan_tasks + ctx->n_level_tasks > 0) {
-            const SolveKey key{ctx->loads, mode, lanes, ctx->waves_per_block, ctx->xcd_scan ? 1 : 0,
-                               ctx->scan_windows ? 1 : 0, max_iter, rel_tol};
-            if (ctx->solve_exec == nullptr || !(ctx->graph_key == key)) {
+        // A graph is recorded when a solve repeats the previous one's key (so
+        // a load-then-solve loop, where every solve follows a new load,
+        // launches directly) and replayed while the key holds.
+        const SolveKey key{ctx->loads, mode, lanes, ctx->waves_per_block, ctx->xcd_scan ? 1 : 0,
+                           ctx->scan_windows ? 1 : 0, max_iter, rel_tol};
+        const bool have = ctx->solve_exec != nullptr && ctx->graph_key == key;
+        const bool repeat = ctx->has_last_key && ctx->last_key == key;
+        ctx->last_key = key;
+        ctx->has_last_key = true;
+        if (ctx->graph && ctx->n_scan_tasks + ctx->n_level_tasks > 0 && (have || repeat)) {
+            if (!have) {
                 if (const int rc = capture_solve(ctx, mode, lanes, rel_tol, max_iter); rc != EPSEON_OK)
                     return rc;
                 ctx->graph_key = key;

@@ -261,12 +261,14 @@ typedef struct epseon_tuning {
                                      each) per SIMD of the device, else 0.  Lane-mode refinement
                                      always shoots the whole grid, so with windows in use AUTO
                                      selects SEGMENTED and a windowed scan is refined segmented. */
-    int32_t graph;                /* 1 (default): epseon_solve records its launches (scan, brackets,
-                                     refinement) once as a HIP graph and replays the graph while
-                                     the batch, the tuning, rel_tol and max_iter stay the same;
-                                     0: every call launches the kernels one by one.  Pipelined
-                                     (solve_split > 0) and paired-lane solves always launch
-                                     directly.  Results are bit-identical either way. */
+    int32_t graph;                /* 1 (default): when a solve repeats the previous solve's batch,
+                                     tuning, rel_tol and max_iter, epseon_solve records its launches
+                                     (scan, brackets, refinement) as a HIP graph, and replays the
+                                     graph while they stay the same (the first solve after a load
+                                     launches directly); 0: every call launches the kernels one by
+                                     one.  Pipelined (solve_split > 0) and paired-lane solves
+                                     always launch directly.  Results are bit-identical either
+                                     way. */
     double solve_split;           /* see epseon_set_solve_split */
 } epseon_tuning;
 

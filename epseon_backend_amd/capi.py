@@ -122,6 +122,24 @@ class Stats(ctypes.Structure):
     ]
 
 
+class Bracket(ctypes.Structure):
+    _fields_ = [
+        ("e_lo", ctypes.c_double),
+        ("e_hi", ctypes.c_double),
+        ("nodes_lo", ctypes.c_int32),
+        ("nodes_hi", ctypes.c_int32),
+        ("lo_valid", ctypes.c_int32),
+        ("found", ctypes.c_int32),
+        ("band", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+BRACKET_DTYPE = np.dtype([("e_lo", np.float64), ("e_hi", np.float64), ("nodes_lo", np.int32),
+                          ("nodes_hi", np.int32), ("lo_valid", np.int32), ("found", np.int32),
+                          ("band", np.int32), ("reserved", np.int32)])
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [
         ("refine_mode", ctypes.c_int32),
@@ -177,6 +195,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "epseon_ctx_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "epseon_get_levels": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_int32_p, _c_int32_p]),
     "epseon_get_scan": (ctypes.c_int, [ctypes.c_void_p, _c_int32_p, _c_double_p, _c_int32_p]),
+    "epseon_get_brackets": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "epseon_get_sizes": (ctypes.c_int, [ctypes.c_void_p, _c_int32_p, _c_int64_p, _c_int64_p, _c_int64_p]),
     "epseon_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     "epseon_normalize": (ctypes.c_int, [ctypes.c_void_p]),
@@ -234,7 +253,7 @@ def load_library() -> ctypes.CDLL:
 
 
 # Public structs in the order epseon_struct_sizes reports them.
-ABI_STRUCTS = (DeviceInfo, PotentialDesc, PotentialInfo, Stats, Tuning)
+ABI_STRUCTS = (DeviceInfo, PotentialDesc, PotentialInfo, Stats, Tuning, Bracket)
 
 
 def _check(rc: int) -> None:
@@ -450,6 +469,14 @@ class Context:
             )
         )
         return nodes[:n], far[:n], resc[:n]
+
+    def brackets(self) -> np.ndarray:
+        """Brackets of the last refinement, one record per level in level order
+        (fields e_lo, e_hi, nodes_lo, nodes_hi, lo_valid, found, band)."""
+        n = sum(self.level_counts)
+        out = np.zeros(max(n, 1), dtype=BRACKET_DTYPE)
+        _check(self._lib.epseon_get_brackets(self._h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out[:n]
 
     def stats(self) -> Stats:
         st = Stats()

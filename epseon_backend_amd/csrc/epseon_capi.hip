@@ -365,8 +365,9 @@ int epseon_struct_sizes(int64_t* sizes, int32_t count)
         return fail(EPSEON_ERR_INVALID, "invalid size buffer");
     const int64_t all[] = {sizeof(epseon_device_info), sizeof(epseon_potential_desc),
                            sizeof(epseon_potential_info), sizeof(epseon_stats),
-                           sizeof(epseon_tuning)};
-    for (int32_t i = 0; i < count && i < 5; ++i)
+                           sizeof(epseon_tuning), sizeof(epseon_bracket)};
+    constexpr int32_t n_all = static_cast<int32_t>(sizeof all / sizeof all[0]);
+    for (int32_t i = 0; i < count && i < n_all; ++i)
         sizes[i] = all[i];
     return EPSEON_OK;
 }
@@ -1863,6 +1864,40 @@ int epseon_get_scan(epseon_ctx* ctx, int32_t* nodes, double* far_end, int32_t* r
                                           hipMemcpyDeviceToHost, ctx->stream));
     }
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return EPSEON_OK;
+}
+
+int epseon_get_brackets(epseon_ctx* ctx, epseon_bracket* out)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (!ctx->refined)
+        return fail(EPSEON_ERR_STATE, "epseon_get_brackets before epseon_refine");
+    if (out == nullptr && ctx->total_levels > 0)
+        return fail(EPSEON_ERR_INVALID, "null output pointer");
+    std::vector<Bracket> br(static_cast<size_t>(ctx->total_levels));
+    if (!br.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(br.data(), ctx->d_brackets, br.size() * sizeof(Bracket),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (const PotentialRecord& r : ctx->rec) {
+        for (int32_t l = 0; l < r.level_count; ++l) {
+            const Bracket& b = br[static_cast<size_t>(r.level_offset + l)];
+            epseon_bracket& o = out[r.level_offset + l];
+            o = epseon_bracket{};
+            if (r.flags != 0 || !b.found) {
+                o.e_lo = o.e_hi = std::nan("");
+                continue;
+            }
+            o.e_lo = b.el;
+            o.e_hi = b.eh;
+            o.nodes_lo = b.cl;
+            o.nodes_hi = b.ch;
+            o.lo_valid = b.lo_ok;
+            o.found = 1;
+            o.band = b.band;
+        }
+    }
     return EPSEON_OK;
 }
 

@@ -135,7 +135,8 @@ typedef struct epseon_ctx epseon_ctx;
 const char* epseon_last_error(void);
 
 /* sizeof of the public structs, in this order: epseon_device_info,
- * epseon_potential_desc, epseon_potential_info, epseon_stats, epseon_tuning (at most
+ * epseon_potential_desc, epseon_potential_info, epseon_stats, epseon_tuning,
+ * epseon_bracket (at most
  * `count` values are written).  Lets a foreign-function binding verify its
  * struct layouts at load time.  No counterpart in the reference (its binding
  * is compiled against the headers). */
@@ -344,6 +345,25 @@ int epseon_get_levels(epseon_ctx* ctx, double* levels, int32_t* status, int32_t*
 /* Copy the scan results out, in potential order, n_energies per potential.
  * Any of the pointers may be NULL.  Synchronises the stream. */
 int epseon_get_scan(epseon_ctx* ctx, int32_t* nodes, double* far_end, int32_t* rescales);
+
+/* Bracket of one level as the refinement started from it. */
+typedef struct epseon_bracket {
+    double e_lo, e_hi;     /* ends (cm^-1): the last trial energy with at most v nodes (or the
+                              window floor min V when lo_valid is 0) and the first with more */
+    int32_t nodes_lo;      /* node count of the scan shot at e_lo (0 at the floor) */
+    int32_t nodes_hi;      /* node count of the scan shot at e_hi */
+    int32_t lo_valid;      /* 1 when e_lo is a trial energy */
+    int32_t found;         /* 0: no trial energy has more than v nodes (the level lies above the
+                              window top) or the potential is unresolved; the rest is then unset */
+    int32_t band;          /* energy band whose shot window the refinement used */
+    int32_t reserved;
+} epseon_bracket;
+
+/* Brackets of the last refinement (its bracket pass), in level order, the
+ * potentials' level_count each.  The bracketing stage has no code in the
+ * reference; it would sit in VibwaAlgorithm<FP>::run (vibwa.hpp:605-637).
+ * Synchronises the stream. */
+int epseon_get_brackets(epseon_ctx* ctx, epseon_bracket* out);
 
 /* Totals: number of potentials, grid points, trial energies and levels of the
  * loaded batch (any pointer may be NULL). */

@@ -243,3 +243,52 @@ def test_autotune_then_parity(capi):
                      windows=bool(stats.scan_windows))
         eo += s.m
         lo += counts[i]
+
+
+@pytest.mark.parametrize("windows", [0, 1])
+def test_brackets_match_oracle(capi, windows):
+    """The bracket pass (epseon_get_brackets) against the oracle's
+    bracket_level from the same scan arrays: ends, node counts, floor flag
+    and band bit-exact; the ends straddle each found level's node count."""
+    specs = batched(4096)[:40]
+    _log(f"case: brackets of the first 40 batched potentials, windows {windows}")
+    counts = [s.bound_levels() + 2 for s in specs]
+    tables = [s.table() for s in specs]
+    with capi.Context(0) as ctx:
+        t = ctx.tuning()
+        t.refine_mode = capi.REFINE_SEGMENTED
+        t.windows = windows
+        ctx.set_tuning(t)
+        ctx.load([_pot(capi, s, tb, c) for s, tb, c in zip(specs, tables, counts)])
+        ctx.solve(1e-10, 200)
+        nodes, far, resc = ctx.scan_results()
+        br = ctx.brackets()
+        flags = [ctx.potential_info(i).flags for i in range(len(specs))]
+    assert len(br) == sum(counts)
+    eo = lo = 0
+    n_found = 0
+    for i, s in enumerate(specs):
+        mine = br[lo : lo + counts[i]]
+        if flags[i] != 0:
+            assert np.all(mine["found"] == 0)
+        else:
+            prob = problem_from_spec(s)
+            prob.v = tables[i]
+            win = prob.windows(bool(windows))
+            fe = oracle.band_far_exps(win)
+            nd, fr, rs = nodes[eo : eo + s.m], far[eo : eo + s.m], resc[eo : eo + s.m]
+            for v in range(counts[i]):
+                ref = oracle.bracket_level(nd, fr, rs, prob.e_lo, prob.e_top, v, fe, win)
+                got = mine[v]
+                assert int(got["found"]) == ref.found, (i, v)
+                if not ref.found:
+                    continue
+                n_found += 1
+                assert np.float64(got["e_hi"]).view(np.int64) == np.float64(ref.e_hi).view(np.int64), (i, v)
+                assert np.float64(got["e_lo"]).view(np.int64) == np.float64(ref.e_lo).view(np.int64), (i, v)
+                assert (int(got["nodes_lo"]), int(got["nodes_hi"]), int(got["lo_valid"]), int(got["band"])) == (
+                    ref.c_lo, ref.c_hi, ref.lo_valid, ref.band), (i, v)
+                assert got["nodes_lo"] <= v < got["nodes_hi"] and got["e_lo"] < got["e_hi"]
+        eo += s.m
+        lo += counts[i]
+    assert n_found > 1000

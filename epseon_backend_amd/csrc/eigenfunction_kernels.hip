@@ -28,6 +28,14 @@
 // which |y| reached 2^240.  The right shot is scaled by the least-squares
 // factor c that fits it to the left one at points m and m+1.
 //
+// Segments.  Each side's shot is cut into segments of psi_seg_len steps.
+// The normalisation runs every segment at once from two basis states
+// (psi_segment_kernel: the segment's transfer map and the Gram sums of its
+// columns), then chains the maps per level from the wall (normalize_kernel),
+// which gives each segment's start state (a checkpoint), sum y^2 and the
+// match.  The eigenfunction and adjoint passes resume every segment from its
+// checkpoint in parallel.
+//
 // Lanes are levels of one potential; the potential table is wave-uniform and
 // read with scalar loads, as in the scan.
 #include <hip/hip_runtime.h>

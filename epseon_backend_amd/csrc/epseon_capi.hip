@@ -218,6 +218,10 @@ struct epseon_ctx {
     hipEvent_t ev_join_aux[3] = {};
     hipEvent_t ev[8] = {};
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_solve0 = nullptr, ev_solve1 = nullptr;
+    // The segment tables are built on stream2 during the first scan after a
+    // load; refinements wait for ev_tables once (tables_pending).
+    hipEvent_t ev_tables = nullptr;
+    bool tables_pending = false;
     Arena arena;
     int32_t waves_per_block = 1;
 
@@ -463,6 +467,8 @@ int epseon_ctx_create(int32_t device, uint64_t block_bytes, epseon_ctx** out)
     if (e == hipSuccess)
         e = hipEventCreate(&ctx->ev_solve1);
     if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&ctx->ev_tables, hipEventDisableTiming);
+    if (e == hipSuccess)
         e = hipDeviceGetAttribute(&ctx->compute_units, hipDeviceAttributeMultiprocessorCount, device);
     if (e != hipSuccess) {
         const int rc = hip_fail(e, "context creation");
@@ -489,7 +495,7 @@ int epseon_ctx_destroy(epseon_ctx* ctx)
     for (hipEvent_t& ev : ctx->ev)
         if (ev != nullptr)
             (void)hipEventDestroy(ev);
-    for (hipEvent_t ev : {ctx->ev_fork, ctx->ev_join, ctx->ev_solve0, ctx->ev_solve1,
+    for (hipEvent_t ev : {ctx->ev_fork, ctx->ev_join, ctx->ev_solve0, ctx->ev_solve1, ctx->ev_tables,
                           ctx->ev_join_aux[0], ctx->ev_join_aux[1], ctx->ev_join_aux[2]})
         if (ev != nullptr)
             (void)hipEventDestroy(ev);
@@ -591,6 +597,8 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
         (count > 0 && (desc == nullptr || (v_host == nullptr && v_dev == nullptr) || v_offsets == nullptr)))
         return fail(EPSEON_ERR_INVALID, "invalid potential batch");
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream2));  // tables of the previous load
+    ctx->tables_pending = false;
     // The captured solve points into the arena that this load reuses.
     ++ctx->loads;
     if (ctx->solve_exec != nullptr) {
@@ -1051,13 +1059,18 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
         EPSEON_HIP_TRY(hipMemcpyAsync(d_jobs + jobs_full.size(), jobs_band.data(),
                                       jobs_band.size() * sizeof(TableJob), hipMemcpyHostToDevice,
                                       ctx->stream));
-    if (ctx->segtab_ok) {
-        EPSEON_HIP_TRY(launch_segment_tables(ctx->stream, ctx->d_uv, ctx->d_rec, d_jobs,
+    if (ctx->segtab_ok && jobs_full.size() + jobs_band.size() > 0) {
+        // Built on stream2, overlapping the next scan; the refinement waits.
+        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+        EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+        EPSEON_HIP_TRY(launch_segment_tables(ctx->stream2, ctx->d_uv, ctx->d_rec, d_jobs,
                                              static_cast<int32_t>(jobs_full.size()), max_rows,
                                              ctx->d_segtab_full));
-        EPSEON_HIP_TRY(launch_segment_tables(ctx->stream, ctx->d_uv, ctx->d_rec, d_jobs + jobs_full.size(),
+        EPSEON_HIP_TRY(launch_segment_tables(ctx->stream2, ctx->d_uv, ctx->d_rec, d_jobs + jobs_full.size(),
                                              static_cast<int32_t>(jobs_band.size()), max_rows,
                                              ctx->d_segtab_band));
+        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_tables, ctx->stream2));
+        ctx->tables_pending = true;
     }
     // The host buffers are pageable and owned by the caller: finish the
     // copies before returning.
@@ -1172,6 +1185,15 @@ hipError_t run_scan(epseon_ctx* ctx, hipStream_t stream, size_t k0, size_t k1)
                        ctx->waves_per_block, ctx->d_nodes, ctx->d_far, ctx->d_resc);
 }
 
+// Make `stream` wait for the segment tables of the last load (once).
+hipError_t wait_tables(epseon_ctx* ctx, hipStream_t stream)
+{
+    if (!ctx->tables_pending)
+        return hipSuccess;
+    ctx->tables_pending = false;
+    return hipStreamWaitEvent(stream, ctx->ev_tables, 0);
+}
+
 // Refinement of the levels of potentials [k0, k1) in task order, in the
 // windows of the last scan.  Lane mode with one level per lane runs the
 // 64-level tasks on `stream`; with two, the paired tasks (potentials with
@@ -1264,6 +1286,8 @@ int capture_solve(epseon_ctx* ctx, int32_t mode, int32_t lanes, double rel_tol, 
         ctx->solve_exec = nullptr;
     }
     const size_t np = n_potentials(ctx);
+    // No wait on an event recorded outside the capture inside it.
+    EPSEON_HIP_TRY(wait_tables(ctx, ctx->stream));
     EPSEON_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
     hipError_t e = run_scan(ctx, ctx->stream, 0, np);
     if (e == hipSuccess)
@@ -1317,6 +1341,7 @@ int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     const int32_t lanes = segmented_lanes(ctx);
     ctx->last_refine_mode = mode;
     ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
+    EPSEON_HIP_TRY(wait_tables(ctx, ctx->stream));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
     if (mode == EPSEON_REFINE_LANE && ctx->levels_per_lane == 2) {
         // Paired and single lane tasks run concurrently on the two streams.
@@ -1402,6 +1427,7 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
         } else {
             EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve0, ctx->stream));
             EPSEON_HIP_TRY(run_scan(ctx, ctx->stream, 0, np));
+            EPSEON_HIP_TRY(wait_tables(ctx, ctx->stream));
             EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream, mode, lanes, rel_tol, max_iter, 0, np));
             EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve1, ctx->stream));
         }
@@ -1439,6 +1465,8 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
     EPSEON_HIP_TRY(run_scan(ctx, ctx->stream, 0, k));
     EPSEON_HIP_TRY(run_scan(ctx, ctx->stream2, k, np));
+    // stream2 built the tables before its scan; the first stream waits.
+    EPSEON_HIP_TRY(wait_tables(ctx, ctx->stream));
     EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream, mode, lanes, rel_tol, max_iter, 0, k));
     EPSEON_HIP_TRY(run_refine(ctx, ctx->stream2, ctx->stream2, mode, lanes, rel_tol, max_iter, k, np));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev_join, ctx->stream2));

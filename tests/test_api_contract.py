@@ -339,3 +339,50 @@ def test_eigenfunction_output_through_handle() -> None:
     plain.wait()
     with pytest.raises(RuntimeError, match="set_eigenfunction_output"):
         plain.get_wavefunctions()
+
+
+@pytest.mark.gpu
+def test_drop_in_path_matches_oracle() -> None:
+    """The H2 workload through the pybind11 drop-in path (configurator,
+    submit, wait, handle getters) equals the oracle run on the same Morse
+    tabulation: node counts, far-end values, levels and iterations
+    bit-exact."""
+    import oracle
+
+    m = _gpu()
+    interface = m.EpseonComputeContext.create().get_device_interface(0)
+    de, re, a, r_min, r_max, n = 38297.0, 0.7414, 1.9426, 0.2, 5.0, 10000
+    mass = 2 * 0.50391
+    c = (
+        interface.get_task_configurator("float64")
+        .set_hardware_config(potential_buffer_size=n, group_size=1, allocation_block_size=16 << 20)
+        .set_morse_potential([m.MorsePotentialConfig(dissociation_energy=de, equilibrium_bond_distance=re,
+                                                     well_width=a, min_r=r_min, max_r=r_max, point_count=n)])
+        .set_vibwa_algorithm(mass_atom_0=mass, mass_atom_1=mass, integration_step=0.1,
+                             min_distance_to_asymptote=1.0, min_level=0, max_level=16)
+    )
+    handle = interface.submit_task(c)
+    handle.wait()
+    levels = handle.get_levels()[0]
+    nodes = handle.get_node_counts()[0]
+    far = handle.get_far_end_values()[0]
+    its = handle.get_iterations()[0]
+    status = handle.get_level_status()[0]
+    # The worker's defaults: 320 trial energies for 17 levels (16 per level,
+    # whole 64-energy groups, at least 256), whole-grid shots (one small
+    # potential), segmented refinement in 256-lane blocks.
+    mu = mass * mass / (mass + mass)
+    v = oracle.morse(de, re, a, r_min, oracle.grid_step(r_min, r_max, n), n)
+    prob = oracle.Problem(v=v, r_min=r_min, r_max=r_max, mu=mu, e_top=de - 1.0, m=320)
+    on, of, orr, olv, oit = oracle.solve(prob, 0, 17, 1e-10, 200, segmented=True, lanes=256, windows=False)
+    np.testing.assert_array_equal(nodes, on)
+    np.testing.assert_array_equal(far.view(np.int64), of.view(np.int64))
+    # Levels the window does not bracket (v = 16 lies within 1 cm^-1 of D_e)
+    # are NaN with status UNBOUND on both sides.
+    ok = ~np.isnan(olv)
+    assert ok.sum() >= 16
+    np.testing.assert_array_equal(status[ok], 0)
+    np.testing.assert_array_equal(status[~ok], 1)
+    np.testing.assert_array_equal(np.isnan(levels), ~ok)
+    np.testing.assert_array_equal(levels[ok].view(np.int64), olv[ok].view(np.int64))
+    np.testing.assert_array_equal(its[ok], oit[ok])

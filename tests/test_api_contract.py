@@ -386,3 +386,19 @@ def test_drop_in_path_matches_oracle() -> None:
     np.testing.assert_array_equal(np.isnan(levels), ~ok)
     np.testing.assert_array_equal(levels[ok].view(np.int64), olv[ok].view(np.int64))
     np.testing.assert_array_equal(its[ok], oit[ok])
+
+
+@pytest.mark.gpu
+def test_float32_handle_rounds_the_float64_levels() -> None:
+    """TaskConfiguratorFloat32 computes in FP64 like the float64 one and
+    returns the levels rounded to float32 (the reference's FP template
+    parameter sets its buffer type)."""
+    m = _gpu()
+    interface = m.EpseonComputeContext.create().get_device_interface(0)
+    out = {}
+    for precision in ("float32", "float64"):
+        handle = interface.submit_task(_configure(m, interface.get_task_configurator(precision)))
+        handle.wait()
+        out[precision] = handle.get_levels()
+    assert out["float32"].dtype == np.float32
+    np.testing.assert_array_equal(out["float32"], out["float64"].astype(np.float32))

@@ -220,14 +220,26 @@ struct PsiBody {
     YShot s;
     double e, c10, a, mult;
     int32_t kf, i0, dir, base;
-    double* row;
+    // Staging: lane l's 32 values of a block go to tile[l], then the wave
+    // writes them out two rows per store (32 contiguous values each).
+    double (*tile)[33];
+    double* rows;  // row of the wave's first level, at the trimmed window's first point
+    int64_t stride;
+    int32_t nrows;
     __device__ __forceinline__ void step(const double u, const int q)
     {
-        row[i0 + dir * (base + q)] = s.y * mult;
+        tile[threadIdx.x][q] = s.y * mult;
         s.step(u, e, c10);
     }
     __device__ __forceinline__ void block_end(const int32_t count, const bool full)
     {
+        __syncthreads();
+        const int32_t c = static_cast<int32_t>(threadIdx.x & 31);
+        const int32_t h = static_cast<int32_t>(threadIdx.x >> 5);
+        for (int32_t r = h; r < nrows; r += 2)
+            if (c < count)
+                rows[r * stride + i0 + dir * (base + c)] = tile[r][c];
+        __syncthreads();
         base += count;
         if (full) {
             (void)s.rescale();
@@ -433,10 +445,11 @@ __global__ __launch_bounds__(64) void normalize_kernel(
 // Eigenfunctions: for each task (potential, first level, side, segment), the
 // segment's points of rows first.. of that potential (full_points values per
 // row at out[psi_offset - base + l * full_points]), re-run from the
-// normalisation pass's checkpoints; the segment-0 tasks also zero the row
-// outside the open trimmed window on their side (the whole row of an
-// unresolved potential, whose only task is side 0, segment 0).  Zero for
-// levels that did not converge.  One lane per level.
+// normalisation pass's checkpoints, staged in LDS and stored 32 contiguous
+// values at a time; the segment-0 tasks also zero the row outside the open
+// trimmed window on their side (the whole row of an unresolved potential,
+// whose only task is side 0, segment 0).  Zero for levels that did not
+// converge.  One lane per level.
 __global__ __launch_bounds__(64) void eigenfunction_kernel(
     const double* __restrict__ uv, const double* __restrict__ uvr,
     const PotentialRecord* __restrict__ pots, const PsiTask* __restrict__ tasks,
@@ -444,33 +457,34 @@ __global__ __launch_bounds__(64) void eigenfunction_kernel(
     const LevelNorm* __restrict__ norms, const PsiCheckpoint* __restrict__ ckpt, const int64_t base,
     double* __restrict__ out)
 {
+    __shared__ double s_tile[64][33];
     const PsiTask task = tasks[blockIdx.x];
     const PotentialRecord& P = pots[task.potential];
     const int32_t l = task.first + static_cast<int32_t>(threadIdx.x);
-    if (l >= P.level_count)
-        return;
+    const int32_t nrows = min(64, P.level_count - task.first);
+    const bool valid = l < P.level_count;
     const int64_t N = P.full_points;
-    double* row = out + (P.psi_offset - base) + static_cast<int64_t>(l) * N;
+    double* row0 = out + (P.psi_offset - base) + static_cast<int64_t>(task.first) * N;
+    double* row = row0 + static_cast<int64_t>(threadIdx.x) * N;
     const int64_t first = P.first_point;
     const int32_t n = P.n_points;
     const bool ok = P.flags == 0;
-    if (task.segment == 0) {
+    if (task.segment == 0 && valid) {
         if (!ok) {
             for (int64_t i = 0; i < N; ++i)
                 row[i] = 0.0;
-            return;
-        }
-        if (task.side == 0)
+        } else if (task.side == 0) {
             for (int64_t i = 0; i <= first; ++i)
                 row[i] = 0.0;
-        else
+        } else {
             for (int64_t i = first + n - 1; i < N; ++i)
                 row[i] = 0.0;
+        }
     }
     if (!ok)
         return;
     const int64_t o = P.level_offset + l;
-    const bool active = level_active(status, o);
+    const bool active = valid && level_active(status, o);
     const LevelNorm nm = active ? norms[o] : LevelNorm{0.0, 0.0, 0, 0};
     const double e = P.beta * (active ? levels[o] : P.e_lo);
     const PsiSegment g = psi_segment(P, uv, uvr, task);
@@ -483,7 +497,10 @@ __global__ __launch_bounds__(64) void eigenfunction_kernel(
     B.i0 = task.side == 0 ? 1 : n - 2;
     B.dir = task.side == 0 ? 1 : -1;
     B.base = g.j0;
-    B.row = row + first;
+    B.tile = s_tile;
+    B.rows = row0 + first;
+    B.stride = N;
+    B.nrows = nrows;
     psi_start(B.s, P, task, g, ckpt, l, active, e);
     B.mult = __builtin_ldexp(B.a, B.s.k - B.kf);
     psi_walk(g.p + 2 + g.j0, g.steps, B);

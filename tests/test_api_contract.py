@@ -402,3 +402,43 @@ def test_float32_handle_rounds_the_float64_levels() -> None:
         out[precision] = handle.get_levels()
     assert out["float32"].dtype == np.float32
     np.testing.assert_array_equal(out["float32"], out["float64"].astype(np.float32))
+
+
+@pytest.mark.gpu
+def test_drop_in_tabulated_double_wells_match_oracle() -> None:
+    """Two ragged tabulated double wells through set_tabulated_potential
+    with per-potential reduced masses and trial-energy counts: the handle's
+    node counts and levels equal the oracle's bit for bit."""
+    import oracle
+    from epseon_backend_amd.workloads import double_wells
+
+    m = _gpu()
+    specs = [double_wells(64, n_points=n, n_energies=k)[i] for i, n, k in ((0, 20000, 2048), (1, 12000, 1536))]
+    tables = [s.table() for s in specs]
+    levels = 20
+    interface = m.EpseonComputeContext.create().get_device_interface(0)
+    c = (
+        interface.get_task_configurator("float64")
+        .set_hardware_config(potential_buffer_size=20000, group_size=2, allocation_block_size=16 << 20)
+        .set_tabulated_potential([m.TabulatedPotentialConfig(values=t.tolist(), min_r=s.r_min, max_r=s.r_max,
+                                                             asymptote=s.de) for s, t in zip(specs, tables)])
+        .set_vibwa_algorithm(mass_atom_0=1.0, mass_atom_1=1.0, integration_step=0.1,
+                             min_distance_to_asymptote=0.0, min_level=0, max_level=levels - 1)
+        .set_reduced_masses([s.mu for s in specs])
+        .set_trial_energy_counts([s.m for s in specs])
+    )
+    handle = interface.submit_task(c)
+    handle.wait()
+    got_levels = handle.get_levels()
+    got_nodes = handle.get_node_counts()
+    status = handle.get_level_status()
+    # Whole-grid shots (few trial energies), segmented refinement in
+    # 256-lane blocks (40 levels in the batch).
+    for p, (s, t) in enumerate(zip(specs, tables)):
+        prob = oracle.Problem(v=t, r_min=s.r_min, r_max=s.r_max, mu=s.mu, e_top=s.de, m=s.m)
+        on, of, orr, olv, oit = oracle.solve(prob, 0, levels, 1e-10, 200, segmented=True, lanes=256, windows=False)
+        np.testing.assert_array_equal(got_nodes[p], on)
+        ok = ~np.isnan(olv)
+        assert ok.sum() >= 10
+        np.testing.assert_array_equal(status[p][ok], 0)
+        np.testing.assert_array_equal(got_levels[p][ok].view(np.int64), olv[ok].view(np.int64))

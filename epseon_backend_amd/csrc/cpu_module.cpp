@@ -2,8 +2,8 @@
 //
 // Same surface as the reference's CPU module (cpp/cpu/source/libcpu.cpp:8-36):
 // one function, greet(), returning a greeting string.  The reference has no
-// CPU numerics for the Numerov path; the FP64 CPU restatement this project
-// uses as its checker lives under oracle/ and is test infrastructure, not a
+// CPU numerics for the Numerov path; the FP64 CPU restatement the tests
+// compare against lives under oracle/ and is test infrastructure, not a
 // device path.
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>

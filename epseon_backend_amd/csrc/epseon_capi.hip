@@ -331,6 +331,14 @@ struct epseon_ctx {
     // by epseon_get_levels so that epseon_get_stats needs no extra copy.
     std::vector<int32_t> h_status, h_iters;
     bool host_levels_valid = false;
+    // Levels, status, iterations and steps live in one device block
+    // (d_level_block, level_block_bytes) read back by one copy into pinned
+    // host memory.
+    char* d_level_block = nullptr;
+    size_t level_block_bytes = 0;
+    void* h_pinned = nullptr;
+    size_t h_pinned_bytes = 0;
+    std::vector<double> h_levels;
 
     bool loaded = false, scanned = false, refined = false, normalized = false;
     bool scan_timed = false, refine_timed = false, normalize_timed = false, vjp_timed = false;
@@ -491,6 +499,8 @@ int epseon_ctx_destroy(epseon_ctx* ctx)
             (void)hipStreamSynchronize(st);
     if (ctx->solve_exec != nullptr)
         (void)hipGraphExecDestroy(ctx->solve_exec);
+    if (ctx->h_pinned != nullptr)
+        (void)hipHostFree(ctx->h_pinned);
     ctx->arena.release();
     for (hipEvent_t& ev : ctx->ev)
         if (ev != nullptr)
@@ -900,7 +910,6 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     EPSEON_HIP_TRY(arena_array(a, nw, &d_raw_windows));
     EPSEON_HIP_TRY(arena_array(a, nw, &ctx->d_win_full));
     EPSEON_HIP_TRY(arena_array(a, nw, &ctx->d_win_band));
-    EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_steps));
     EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_brackets));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(count, 1), &ctx->d_rec));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(scan_tasks.size(), 1), &ctx->d_scan_tasks));
@@ -915,9 +924,18 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_nodes));
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_far));
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_resc));
-    EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_levels));
-    EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_status));
-    EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_iters));
+    {
+        // One block: levels (f64), status, iterations, steps (i32), each
+        // 256-byte aligned.
+        const size_t a8 = (nl * sizeof(double) + 255) & ~size_t(255);
+        const size_t a4 = (nl * sizeof(int32_t) + 255) & ~size_t(255);
+        ctx->level_block_bytes = a8 + 3 * a4;
+        EPSEON_HIP_TRY(arena_array(a, ctx->level_block_bytes, &ctx->d_level_block));
+        ctx->d_levels = reinterpret_cast<double*>(ctx->d_level_block);
+        ctx->d_status = reinterpret_cast<int32_t*>(ctx->d_level_block + a8);
+        ctx->d_iters = reinterpret_cast<int32_t*>(ctx->d_level_block + a8 + a4);
+        ctx->d_steps = reinterpret_cast<int32_t*>(ctx->d_level_block + a8 + 2 * a4);
+    }
     EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_norms));
     EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_grad_levels));
     for (int w = 0; w < 4; ++w) {
@@ -1671,20 +1689,41 @@ int epseon_ctx_synchronize(epseon_ctx* ctx)
 
 namespace {
 
-int fetch_level_meta(epseon_ctx* ctx)
+// Read the level block back (one copy into pinned memory, then a stream
+// synchronisation) into h_levels, h_status, h_iters and h_steps.
+int fetch_levels(epseon_ctx* ctx)
 {
     const size_t n = static_cast<size_t>(ctx->total_levels);
+    ctx->h_levels.resize(n);
     ctx->h_status.resize(n);
     ctx->h_iters.resize(n);
     ctx->h_steps.resize(n);
     if (n > 0) {
-        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->h_status.data(), ctx->d_status, n * sizeof(int32_t),
-                                      hipMemcpyDeviceToHost, ctx->stream));
-        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->h_iters.data(), ctx->d_iters, n * sizeof(int32_t),
-                                      hipMemcpyDeviceToHost, ctx->stream));
-        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->h_steps.data(), ctx->d_steps, n * sizeof(int32_t),
-                                      hipMemcpyDeviceToHost, ctx->stream));
+        const size_t bytes = ctx->level_block_bytes;
+        if (ctx->h_pinned_bytes < bytes) {
+            if (ctx->h_pinned != nullptr) {
+                EPSEON_HIP_TRY(hipHostFree(ctx->h_pinned));
+                ctx->h_pinned = nullptr;
+                ctx->h_pinned_bytes = 0;
+            }
+            EPSEON_HIP_TRY(hipHostMalloc(&ctx->h_pinned, bytes, hipHostMallocDefault));
+            ctx->h_pinned_bytes = bytes;
+        }
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->h_pinned, ctx->d_level_block, bytes, hipMemcpyDeviceToHost,
+                                      ctx->stream));
     }
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n > 0) {
+        const char* h = static_cast<const char*>(ctx->h_pinned);
+        std::memcpy(ctx->h_levels.data(), h, n * sizeof(double));
+        std::memcpy(ctx->h_status.data(), h + (reinterpret_cast<char*>(ctx->d_status) - ctx->d_level_block),
+                    n * sizeof(int32_t));
+        std::memcpy(ctx->h_iters.data(), h + (reinterpret_cast<char*>(ctx->d_iters) - ctx->d_level_block),
+                    n * sizeof(int32_t));
+        std::memcpy(ctx->h_steps.data(), h + (reinterpret_cast<char*>(ctx->d_steps) - ctx->d_level_block),
+                    n * sizeof(int32_t));
+    }
+    ctx->host_levels_valid = true;
     return EPSEON_OK;
 }
 
@@ -1859,13 +1898,10 @@ int epseon_get_levels(epseon_ctx* ctx, double* levels, int32_t* status, int32_t*
     if (!ctx->refined)
         return fail(EPSEON_ERR_STATE, "epseon_get_levels before epseon_refine");
     const size_t n = static_cast<size_t>(ctx->total_levels);
-    if (n > 0 && levels)
-        EPSEON_HIP_TRY(hipMemcpyAsync(levels, ctx->d_levels, n * sizeof(double), hipMemcpyDeviceToHost,
-                                      ctx->stream));
-    if (const int rc = fetch_level_meta(ctx); rc != EPSEON_OK)
+    if (const int rc = fetch_levels(ctx); rc != EPSEON_OK)
         return rc;
-    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->host_levels_valid = true;
+    if (n > 0 && levels)
+        std::memcpy(levels, ctx->h_levels.data(), n * sizeof(double));
     if (n > 0 && status)
         std::memcpy(status, ctx->h_status.data(), n * sizeof(int32_t));
     if (n > 0 && iterations)
@@ -1985,12 +2021,9 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
             EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
             stats->refine_ms = ms;
         }
-        if (!ctx->host_levels_valid) {
-            if (const int rc = fetch_level_meta(ctx); rc != EPSEON_OK)
+        if (!ctx->host_levels_valid)
+            if (const int rc = fetch_levels(ctx); rc != EPSEON_OK)
                 return rc;
-            EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
-            ctx->host_levels_valid = true;
-        }
         const std::vector<int32_t>& it = ctx->h_iters;
         const std::vector<int32_t>& st = ctx->h_status;
         double pu = 0.0;

@@ -26,6 +26,15 @@
 // the overflow test every fourth super-block.  Compiled with
 // -ffp-contract=off; every fused operation is an explicit fma, so results are
 // bit-identical to the C oracle.
+//
+// Kernels in this file, in launch order: potential statistics and table
+// shifts (load), band windows (each shot stops 20 guaranteed e-folds past its
+// band top's outermost turning points), segment tables (load), the scan, the
+// bracket pass (one thread per level), and the refinement: lane mode (one
+// level per lane over the whole grid) or segmented mode (one block per level,
+// each lane a segment's transfer matrix, tree product in LDS, 64-lane blocks
+// reading the per-window segment table).  Scan and segmented tasks are binned
+// per XCD so that a potential's wavefronts share one L2.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>

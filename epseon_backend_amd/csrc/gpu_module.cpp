@@ -299,6 +299,7 @@ struct TaskResults {
     std::vector<std::vector<int32_t>> rescales;
     std::vector<int32_t> strides;
     std::vector<int32_t> potential_flags;
+    std::vector<epseon_bracket> brackets;  // per level, potential-major
     double scan_ms = 0, refine_ms = 0, total_ms = 0;
     double scan_point_updates = 0, refine_point_updates = 0;
     bool has_eigenfunctions = false;
@@ -445,6 +446,7 @@ class Task {
         r.levels.assign(np * level_count, std::nan(""));
         r.level_status.assign(np * level_count, EPSEON_LEVEL_UNBOUND);
         r.iterations.assign(np * level_count, 0);
+        r.brackets.assign(np * level_count, epseon_bracket{});
         r.nodes.resize(np);
         r.far_end.resize(np);
         r.rescales.resize(np);
@@ -508,6 +510,8 @@ class Task {
                                          r.level_status.data() + g0 * level_count,
                                          r.iterations.data() + g0 * level_count),
                        "epseon_get_levels");
+            check_capi(epseon_get_brackets(ctx.get(), r.brackets.data() + g0 * level_count),
+                       "epseon_get_brackets");
             (void)nl;
             int64_t ne = 0;
             for (size_t p = g0; p < g1; ++p)
@@ -645,6 +649,30 @@ class TaskHandle {
         return ragged(task_->results().rescales);
     }
     std::vector<int32_t> get_node_strides() const { return task_->results().strides; }
+
+    py::dict get_brackets() const
+    {
+        const TaskResults& r = task_->results();
+        const std::vector<py::ssize_t> shape{static_cast<py::ssize_t>(r.n_potentials),
+                                             static_cast<py::ssize_t>(r.level_count)};
+        py::array_t<double> e_lo(shape), e_hi(shape);
+        py::array_t<int32_t> n_lo(shape), n_hi(shape), found(shape);
+        for (size_t i = 0; i < r.brackets.size(); ++i) {
+            const epseon_bracket& b = r.brackets[i];
+            e_lo.mutable_data()[i] = b.e_lo;
+            e_hi.mutable_data()[i] = b.e_hi;
+            n_lo.mutable_data()[i] = b.nodes_lo;
+            n_hi.mutable_data()[i] = b.nodes_hi;
+            found.mutable_data()[i] = b.found;
+        }
+        py::dict d;
+        d["e_lo"] = e_lo;
+        d["e_hi"] = e_hi;
+        d["nodes_lo"] = n_lo;
+        d["nodes_hi"] = n_hi;
+        d["found"] = found;
+        return d;
+    }
     std::vector<int32_t> get_potential_flags() const { return task_->results().potential_flags; }
 
     std::vector<py::array_t<FP>> get_wavefunctions() const
@@ -1015,6 +1043,9 @@ void bind_handle(py::module_& m, const char* name, const char* doc)
         .def("get_far_end_values", &H::get_far_end_values, "Scan far-end values w_{N-1}.")
         .def("get_rescale_counts", &H::get_rescale_counts, "Scan rescale counts.")
         .def("get_node_strides", &H::get_node_strides, "Node-sampling stride per potential.")
+        .def("get_brackets", &H::get_brackets,
+             "Each level's bracket from the scan: dict of (potentials, levels) arrays e_lo, e_hi, "
+             "nodes_lo, nodes_hi, found.")
         .def("get_potential_flags", &H::get_potential_flags, "Per-potential flags (1 = unresolved).")
         .def("get_timings", &H::get_timings, "Device times and point-update counts.")
         .def("get_wavefunctions", &H::get_wavefunctions,

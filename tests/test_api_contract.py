@@ -386,6 +386,15 @@ def test_drop_in_path_matches_oracle() -> None:
     np.testing.assert_array_equal(np.isnan(levels), ~ok)
     np.testing.assert_array_equal(levels[ok].view(np.int64), olv[ok].view(np.int64))
     np.testing.assert_array_equal(its[ok], oit[ok])
+    # The brackets the refinement started from.
+    br = handle.get_brackets()
+    win = prob.windows(False)
+    for v in range(17):
+        ref = oracle.bracket_level(on, of, orr, prob.e_lo, prob.e_top, v, oracle.band_far_exps(win), win)
+        assert int(br["found"][0, v]) == ref.found
+        if ref.found:
+            assert (br["e_lo"][0, v], br["e_hi"][0, v]) == (ref.e_lo, ref.e_hi)
+            assert (br["nodes_lo"][0, v], br["nodes_hi"][0, v]) == (ref.c_lo, ref.c_hi)
 
 
 @pytest.mark.gpu

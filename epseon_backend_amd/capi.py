@@ -119,6 +119,8 @@ class Stats(ctypes.Structure):
         ("solve_split", ctypes.c_double),
         ("scan_windows", ctypes.c_int32),
         ("solve_graph", ctypes.c_int32),
+        ("refine_levels_per_block", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -149,6 +151,7 @@ class Tuning(ctypes.Structure):
         ("xcd_scan", ctypes.c_int32),
         ("windows", ctypes.c_int32),
         ("graph", ctypes.c_int32),
+        ("segmented_levels", ctypes.c_int32),
         ("solve_split", ctypes.c_double),
     ]
 

@@ -37,7 +37,7 @@ hipError_t launch_brackets(hipStream_t, const PotentialRecord*, const BandWindow
 hipError_t launch_refine(hipStream_t, int32_t, const double*, const PotentialRecord*, const BandWindow*,
                          const WaveTask*, int32_t, int32_t, const Bracket*, double, int32_t, double*,
                          int32_t*, int32_t*, int32_t*);
-hipError_t launch_refine_segmented(hipStream_t, int32_t, const double*, const double*,
+hipError_t launch_refine_segmented(hipStream_t, int32_t, int32_t, const double*, const double*,
                                    const PotentialRecord*, const BandWindow*, const WaveTask*, int32_t,
                                    const int32_t*, int32_t, const Bracket*, double, int32_t, double*,
                                    int32_t*, int32_t*, int32_t*);
@@ -160,6 +160,10 @@ hipError_t arena_array(Arena& a, size_t count, T** out)
 // per level.
 constexpr int64_t kLanePerLevelMinLevels = int64_t(64) * 2048;
 
+// Automatic level pairs in the segmented refinement need at least this many
+// levels per compute unit (three paired blocks per SIMD).
+constexpr int64_t kPairMinLevelsPerCu = 24;
+
 // Automatic windows need at least this many scan wavefronts per SIMD.
 constexpr int64_t kWindowsMinScanWavesPerSimd = 8;
 
@@ -183,6 +187,7 @@ int32_t node_stride(double beta, double e_hi, double v_min)
 }
 
 }  // namespace
+
 }  // namespace epseon::hip
 
 using namespace epseon::hip;
@@ -195,11 +200,13 @@ namespace {
 struct SolveKey {
     uint64_t load = 0;
     int32_t refine_mode = 0, lanes = 0, waves_per_block = 0, xcd_scan = 0, scan_windows = 0, max_iter = 0;
+    int32_t levels_per_block = 1;
     double rel_tol = 0.0;
 
     bool operator==(const SolveKey& o) const
     {
         return load == o.load && refine_mode == o.refine_mode && lanes == o.lanes &&
+               levels_per_block == o.levels_per_block &&
                waves_per_block == o.waves_per_block && xcd_scan == o.xcd_scan && scan_windows == o.scan_windows &&
                max_iter == o.max_iter && rel_tol == o.rel_tol;
     }
@@ -269,6 +276,16 @@ struct epseon_ctx {
     int32_t xcd_max_bin_band = 0;
     int32_t* d_xcd_level_bins = nullptr;
     int32_t xcd_max_level_bin = 0;
+    // Segmented refinement with two levels per block: tasks (potential,
+    // level l) for l = 0, 2, 4, ... of each potential, in task order and
+    // binned per XCD like the level tasks.
+    WaveTask* d_pair_tasks = nullptr;
+    WaveTask* d_xcd_pair_tasks = nullptr;
+    int32_t* d_xcd_pair_bins = nullptr;
+    int32_t xcd_max_pair_bin = 0;
+    int32_t n_pair_tasks = 0;
+    int32_t segmented_levels = 0;  // tuning: levels per segmented block (0: automatic)
+    int32_t last_refine_levels = 1;
     bool xcd_scan = true;
     WaveTask* d_refine_tasks = nullptr;
     WaveTask* d_level_tasks = nullptr;
@@ -280,7 +297,7 @@ struct epseon_ctx {
     // Cumulative task counts and scan work over the potentials in task order
     // (longest grids first): entry k covers the first k potentials.
     std::vector<int32_t> cum_scan_tasks, cum_refine_tasks, cum_level_tasks, cum_lane1_tasks,
-        cum_lane2_tasks;
+        cum_lane2_tasks, cum_pair_tasks;
     std::vector<double> cum_scan_work;
     double solve_split = 0.0;
     // The single-stream solve's launches recorded as a HIP graph (tuning
@@ -824,7 +841,7 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     for (int32_t k = 0; k < count; ++k)
         ctx->order_pos[static_cast<size_t>(order[static_cast<size_t>(k)])] = k;
     // Segmented refinement uses one 64-lane block per level.
-    std::vector<WaveTask> scan_tasks, refine_tasks, level_tasks, lane1_tasks, lane2_tasks;
+    std::vector<WaveTask> scan_tasks, refine_tasks, level_tasks, lane1_tasks, lane2_tasks, pair_tasks;
     std::vector<PsiTask> vjp_lists[4], psi_tasks;
     ctx->cum_psi_tasks.assign(1, 0);
     ctx->cum_lane1_tasks.assign(1, 0);
@@ -832,6 +849,7 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     ctx->cum_scan_tasks.assign(1, 0);
     ctx->cum_refine_tasks.assign(1, 0);
     ctx->cum_level_tasks.assign(1, 0);
+    ctx->cum_pair_tasks.assign(1, 0);
     ctx->cum_scan_work.assign(1, 0.0);
     for (int32_t p : order) {
         const int32_t groups = (ctx->rec[p].level_count + 63) / 64;
@@ -858,6 +876,8 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
             refine_tasks.push_back({p, l});
         for (int32_t l = 0; l < r.level_count; ++l)
             level_tasks.push_back({p, l});
+        for (int32_t l = 0; l < r.level_count; l += 2)
+            pair_tasks.push_back({p, l});
         if (r.level_count > 64)
             for (int32_t l = 0; l < r.level_count; l += 128)
                 lane2_tasks.push_back({p, l});
@@ -868,6 +888,7 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
         ctx->cum_scan_tasks.push_back(static_cast<int32_t>(scan_tasks.size()));
         ctx->cum_refine_tasks.push_back(static_cast<int32_t>(refine_tasks.size()));
         ctx->cum_level_tasks.push_back(static_cast<int32_t>(level_tasks.size()));
+        ctx->cum_pair_tasks.push_back(static_cast<int32_t>(pair_tasks.size()));
         ctx->cum_scan_work.push_back(
             ctx->cum_scan_work.back() +
             (r.flags == EPSEON_POT_OK ? static_cast<double>(r.n_points) * r.n_energies : 0.0));
@@ -883,13 +904,21 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     }
     for (size_t t = 0; t < level_tasks.size(); ++t)
         level_task_work[t] = static_cast<double>(ctx->rec[static_cast<size_t>(level_tasks[t].potential)].n_points);
-    std::vector<int32_t> xcd_bins, xcd_level_bins;
+    std::vector<double> pair_task_work(pair_tasks.size());
+    for (size_t t = 0; t < pair_tasks.size(); ++t) {
+        const PotentialRecord& r = ctx->rec[static_cast<size_t>(pair_tasks[t].potential)];
+        pair_task_work[t] = static_cast<double>(r.n_points) * std::min(2, r.level_count - pair_tasks[t].first);
+    }
+    std::vector<int32_t> xcd_bins, xcd_level_bins, xcd_pair_bins;
     const std::vector<WaveTask> xcd_tasks =
         xcd_binned(scan_tasks, ctx->cum_scan_tasks, scan_task_work, false, xcd_bins, ctx->xcd_max_bin);
     const std::vector<WaveTask> xcd_level_tasks = xcd_binned(level_tasks, ctx->cum_level_tasks, level_task_work,
                                                              false, xcd_level_bins, ctx->xcd_max_level_bin);
+    const std::vector<WaveTask> xcd_pair_tasks = xcd_binned(pair_tasks, ctx->cum_pair_tasks, pair_task_work, false,
+                                                            xcd_pair_bins, ctx->xcd_max_pair_bin);
     ctx->n_refine_tasks = static_cast<int32_t>(refine_tasks.size());
     ctx->n_level_tasks = static_cast<int32_t>(level_tasks.size());
+    ctx->n_pair_tasks = static_cast<int32_t>(pair_tasks.size());
 
     const size_t np = static_cast<size_t>(std::max<int64_t>(pts, 1));
     const size_t ne = static_cast<size_t>(std::max<int64_t>(en, 1));
@@ -917,6 +946,9 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     EPSEON_HIP_TRY(arena_array(a, size_t(9), &ctx->d_xcd_bins));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(xcd_level_tasks.size(), 1), &ctx->d_xcd_level_tasks));
     EPSEON_HIP_TRY(arena_array(a, size_t(9), &ctx->d_xcd_level_bins));
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(pair_tasks.size(), 1), &ctx->d_pair_tasks));
+    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(xcd_pair_tasks.size(), 1), &ctx->d_xcd_pair_tasks));
+    EPSEON_HIP_TRY(arena_array(a, size_t(9), &ctx->d_xcd_pair_bins));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(refine_tasks.size(), 1), &ctx->d_refine_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(level_tasks.size(), 1), &ctx->d_level_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(lane1_tasks.size(), 1), &ctx->d_lane1_tasks));
@@ -971,6 +1003,15 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
                                       xcd_level_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice,
                                       ctx->stream));
     EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_level_bins, xcd_level_bins.data(), 9 * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, ctx->stream));
+    if (!pair_tasks.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_pair_tasks, pair_tasks.data(), pair_tasks.size() * sizeof(WaveTask),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    if (!xcd_pair_tasks.empty())
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_pair_tasks, xcd_pair_tasks.data(),
+                                      xcd_pair_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice,
+                                      ctx->stream));
+    EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_pair_bins, xcd_pair_bins.data(), 9 * sizeof(int32_t),
                                   hipMemcpyHostToDevice, ctx->stream));
     EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_bins, xcd_bins.data(), 9 * sizeof(int32_t),
                                   hipMemcpyHostToDevice, ctx->stream));
@@ -1166,6 +1207,19 @@ int32_t segmented_lanes(const epseon_ctx* ctx)
     return lanes;
 }
 
+// Levels per segmented block: two in 64-lane blocks (each table load then
+// feeds both levels' shots) once the batch has at least kPairMinLevelsPerCu
+// levels per compute unit, so that the pairs still give every SIMD several
+// blocks; else one (unless fixed by the tuning).
+int32_t segmented_levels(const epseon_ctx* ctx, int32_t lanes)
+{
+    if (lanes != 64)
+        return 1;
+    if (ctx->segmented_levels != 0)
+        return ctx->segmented_levels;
+    return ctx->total_levels >= kPairMinLevelsPerCu * ctx->compute_units ? 2 : 1;
+}
+
 // Window set of a scan followed by a refinement in `mode`: the band windows
 // when enabled, except for lane mode (whose wavefronts walk one table for
 // all their levels), which shoots over the whole grid.
@@ -1257,12 +1311,27 @@ hipError_t run_refine(epseon_ctx* ctx, hipStream_t stream, hipStream_t stream_b,
     }
     const double* tab = lanes == 64 && ctx->segtab_ok ? (ctx->scan_windows ? ctx->d_segtab_band : ctx->d_segtab_full)
                                                      : nullptr;
-    if (ctx->xcd_scan && k0 == 0 && k1 == ctx->cum_level_tasks.size() - 1)
-        return launch_refine_segmented(stream, lanes, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_xcd_level_tasks,
+    const bool whole = ctx->xcd_scan && k0 == 0 && k1 == ctx->cum_level_tasks.size() - 1;
+    const int32_t lpb = segmented_levels(ctx, lanes);
+    ctx->last_refine_levels = lpb;
+    if (lpb == 2) {
+        const int32_t p0 = ctx->cum_pair_tasks[k0];
+        const int32_t p1 = ctx->cum_pair_tasks[k1];
+        if (whole)
+            return launch_refine_segmented(stream, lanes, 2, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_xcd_pair_tasks,
+                                           p1 - p0, ctx->d_xcd_pair_bins, ctx->xcd_max_pair_bin, ctx->d_brackets,
+                                           rel_tol, max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters,
+                                           ctx->d_steps);
+        return launch_refine_segmented(stream, lanes, 2, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_pair_tasks + p0,
+                                       p1 - p0, nullptr, 0, ctx->d_brackets, rel_tol, max_iter, ctx->d_levels,
+                                       ctx->d_status, ctx->d_iters, ctx->d_steps);
+    }
+    if (whole)
+        return launch_refine_segmented(stream, lanes, 1, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_xcd_level_tasks,
                                        t1 - t0, ctx->d_xcd_level_bins, ctx->xcd_max_level_bin, ctx->d_brackets,
                                        rel_tol, max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters,
                                        ctx->d_steps);
-    return launch_refine_segmented(stream, lanes, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_level_tasks + t0,
+    return launch_refine_segmented(stream, lanes, 1, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_level_tasks + t0,
                                    t1 - t0, nullptr, 0, ctx->d_brackets, rel_tol, max_iter, ctx->d_levels,
                                    ctx->d_status, ctx->d_iters, ctx->d_steps);
 }
@@ -1426,8 +1495,15 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
         // A graph is recorded when a solve repeats the previous one's key (so
         // a load-then-solve loop, where every solve follows a new load,
         // launches directly) and replayed while the key holds.
-        const SolveKey key{ctx->loads, mode, lanes, ctx->waves_per_block, ctx->xcd_scan ? 1 : 0,
-                           ctx->scan_windows ? 1 : 0, max_iter, rel_tol};
+        const SolveKey key{ctx->loads,
+                           mode,
+                           lanes,
+                           ctx->waves_per_block,
+                           ctx->xcd_scan ? 1 : 0,
+                           ctx->scan_windows ? 1 : 0,
+                           max_iter,
+                           mode == EPSEON_REFINE_SEGMENTED ? segmented_levels(ctx, lanes) : 1,
+                           rel_tol};
         const bool have = ctx->solve_exec != nullptr && ctx->graph_key == key;
         const bool repeat = ctx->has_last_key && ctx->last_key == key;
         ctx->last_key = key;
@@ -1530,6 +1606,7 @@ int epseon_get_tuning(epseon_ctx* ctx, epseon_tuning* t)
     t->xcd_scan = ctx->xcd_scan ? 1 : 0;
     t->windows = ctx->windows;
     t->graph = ctx->graph;
+    t->segmented_levels = ctx->segmented_levels;
     t->solve_split = ctx->solve_split;
     return EPSEON_OK;
 }
@@ -1556,6 +1633,8 @@ int epseon_set_tuning(epseon_ctx* ctx, const epseon_tuning* t)
         return fail(EPSEON_ERR_INVALID, "windows must be 0, 1 or 2");
     if (t->graph != 0 && t->graph != 1)
         return fail(EPSEON_ERR_INVALID, "graph must be 0 or 1");
+    if (t->segmented_levels < 0 || t->segmented_levels > 2)
+        return fail(EPSEON_ERR_INVALID, "segmented_levels must be 0, 1 or 2");
     ctx->refine_mode = t->refine_mode;
     ctx->levels_per_lane = t->levels_per_lane;
     ctx->waves_per_block = t->scan_waves_per_block;
@@ -1563,6 +1642,7 @@ int epseon_set_tuning(epseon_ctx* ctx, const epseon_tuning* t)
     ctx->segmented_lanes = t->segmented_lanes;
     ctx->xcd_scan = t->xcd_scan != 0;
     ctx->graph = t->graph;
+    ctx->segmented_levels = t->segmented_levels;
     ctx->solve_split = t->solve_split;
     return EPSEON_OK;
 }
@@ -1604,6 +1684,8 @@ int epseon_autotune(epseon_ctx* ctx, double rel_tol, int32_t max_iter, int32_t f
         best.refine_mode = choose_refine_mode(ctx);
     if (best.windows == 2)
         best.windows = windows_enabled(ctx) ? 1 : 0;
+    if (best.segmented_levels == 0)
+        best.segmented_levels = segmented_levels(ctx, segmented_lanes(ctx));
     double best_ms = 0.0;
     if (const int rc = epseon_set_tuning(ctx, &best); rc != EPSEON_OK)
         return rc;
@@ -1652,10 +1734,18 @@ int epseon_autotune(epseon_ctx* ctx, double rel_tol, int32_t max_iter, int32_t f
         c.levels_per_lane = 3 - best.levels_per_lane;
         if (const int rc = trial(c); rc != EPSEON_OK)
             return rc;
-    } else if (!(flags & EPSEON_TUNE_KEEP_RESULTS)) {
-        for (int32_t lanes : {64, 128, 256}) {
+    } else {
+        if (!(flags & EPSEON_TUNE_KEEP_RESULTS))
+            for (int32_t lanes : {64, 128, 256}) {
+                epseon_tuning c = best;
+                c.segmented_lanes = lanes;
+                if (const int rc = trial(c); rc != EPSEON_OK)
+                    return rc;
+            }
+        // Level pairs leave every level's arithmetic unchanged.
+        if ((best.segmented_lanes == 0 ? segmented_lanes(ctx) : best.segmented_lanes) == 64) {
             epseon_tuning c = best;
-            c.segmented_lanes = lanes;
+            c.segmented_levels = 3 - best.segmented_levels;
             if (const int rc = trial(c); rc != EPSEON_OK)
                 return rc;
         }
@@ -2041,6 +2131,7 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
         stats->levels_converged = ok;
         stats->refine_mode = ctx->last_refine_mode;
         stats->refine_lanes = ctx->last_refine_lanes;
+        stats->refine_levels_per_block = ctx->last_refine_mode == EPSEON_REFINE_SEGMENTED ? ctx->last_refine_levels : 1;
         // Eigenfunction passes shoot every converged level once over its
         // trimmed grid (n - 2 steps split between the two inward shots).
         double shot_pu = 0.0;

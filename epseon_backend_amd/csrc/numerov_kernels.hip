@@ -1198,11 +1198,26 @@ struct MatrixBody {
     }
 };
 
+// Two levels' transfer matrices through one segment of the same window: the
+// two bodies share every grid value, so each table load feeds both (the walk
+// of one level is bound by those loads, not by its FP64 work).  Each member's
+// arithmetic is that of a lone MatrixBody.
+struct MatrixPair {
+    MatrixBody x, y;
+    __device__ __forceinline__ void operator()(const double u, const int32_t idx)
+    {
+        x(u, idx);
+        y(u, idx);
+    }
+};
+
 // One difference-form solution through a segment, sampling signs at the
-// global sample positions (q + 1) % stride == 0 and the last point.
+// global sample positions (q + 1) % stride == 0 and the last point (the
+// stride is a power of two: mask = stride - 1; last = qn - 1 when qn is not
+// a multiple of the stride, else -1).
 struct CountBody {
     double e, em, w, d, wm;
-    int32_t q0, qn, stride;
+    int32_t q0, mask, last;
     SegCount sc;
     double c10;
     __device__ __forceinline__ void operator()(const double u, const int32_t idx)
@@ -1215,7 +1230,7 @@ struct CountBody {
         w = w + d;
         em = ep;
         const int32_t q = q0 + idx;
-        if ((q + 1) % stride == 0 || (q == qn - 1 && qn % stride != 0)) {
+        if (((q + 1) & mask) == 0 || q == last) {
             const int32_t sb = static_cast<int32_t>(sign_bit(w));
             if (sc.has)
                 sc.internal += sb ^ sc.last;
@@ -1230,6 +1245,15 @@ struct CountBody {
             d = __builtin_ldexp(d, -sh);
             wm = __builtin_ldexp(wm, -sh);
         }
+    }
+};
+
+struct CountPair {
+    CountBody x, y;
+    __device__ __forceinline__ void operator()(const double u, const int32_t idx)
+    {
+        x(u, idx);
+        y(u, idx);
     }
 };
 
@@ -1265,181 +1289,360 @@ __global__ __launch_bounds__(256) void segment_table_kernel(const double* __rest
     }
 }
 
-// Segmented refinement: one T-thread block per level.  Lane s owns steps
-// [q_s, q_{s+1}) of the trimmed grid (S = clamp(Q/32, 1, T) segments) and
-// builds that segment's 2x2 difference-form transfer matrix (columns (w, d) =
-// (1, 0) and (0, 1), renormalised every 32 steps and at the end).  The
-// far-end value is the binary-tree product of the matrices (M_i <- M_{i+d} M_i
-// at d = 1, 2, 4, ...) applied to (w_1, d_1) = (1, 1).  Counted shots chain
-// the matrices sequentially (lane 0) for the segment start vectors and re-run
-// every segment.
-template <int T, bool TABLE>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void numerov_refine_segmented_kernel(
+namespace {
+
+// Shared memory of the segmented refinement, one slot per level of a block:
+// the segment matrices (entries and binary exponents), the chained start
+// vectors and segment node counts of counted shots, and the joined counts.
+template <int T, int K>
+struct SegShared {
+    double m[K][4][T];
+    int32_t e[K][T];
+    double start[K][2][T];
+    SegCount cnt[K][T];
+    int32_t c[K];
+};
+
+// One level's shot geometry in a T-thread block: its band window (Dirichlet
+// points lo, hi on the trimmed grid), the q = n - 2 steps split into `segs`
+// segments, lane s walking [q0, q1) from the table value u_start.
+template <bool TABLE>
+struct SegGeom {
+    const double* u;
+    int32_t lo, hi, seg_row, n, qn, segs, q0, q1;
+    bool active;
+    double u_start;
+    ColumnSource cs;
+
+    __device__ __forceinline__ void init(const double* uv_pot, const double* segtab, const BandWindow& B,
+                                         const int32_t lane, const int32_t T)
+    {
+        lo = B.lo;
+        hi = B.hi;
+        seg_row = B.seg_row;
+        u = uv_pot + B.lo;
+        n = B.hi - B.lo + 1;
+        qn = n - 2;
+        segs = min(T, max(1, qn / 32));
+        active = lane < segs;
+        q0 = active ? static_cast<int32_t>(segment_start(qn, segs, lane)) : 0;
+        q1 = active ? static_cast<int32_t>(segment_start(qn, segs, lane + 1)) : 0;
+        cs = ColumnSource{};
+        if constexpr (TABLE) {
+            // With the segment table, lane s reads column s (row r = point
+            // q0 + r): consecutive lanes load consecutive addresses.  The
+            // resource must be wave-uniform (scalar registers); the
+            // readfirstlane calls state what the compiler cannot prove.
+            const int32_t rows = seg_rows(n, 0, 1);
+            const uint64_t base = reinterpret_cast<uint64_t>(segtab + static_cast<int64_t>(B.seg_row) * 64);
+            const uint32_t blo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(base));
+            const uint32_t bhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(base >> 32));
+            const int32_t bytes = __builtin_amdgcn_readfirstlane(rows * 512);
+            cs.rs = __builtin_amdgcn_make_buffer_rsrc(
+                reinterpret_cast<double*>((static_cast<uint64_t>(bhi) << 32) | blo), static_cast<int16_t>(0),
+                bytes, kBufferResourceWord3);
+            cs.off = lane * 8 + 512;
+            u_start = cs.at(-1);
+        } else {
+            u_start = u[q0];
+        }
+    }
+
+    __device__ __forceinline__ bool same_window(const SegGeom& o) const
+    {
+        return lo == o.lo && hi == o.hi && seg_row == o.seg_row;
+    }
+
+    template <class Body>
+    __device__ __forceinline__ void walk(Body& body) const
+    {
+        if constexpr (TABLE)
+            walk_segment(cs, q1 - q0, body);
+        else
+            walk_segment(RowSource{u + q0 + 1}, q1 - q0, body);
+    }
+};
+
+template <int T, int K>
+__device__ __forceinline__ void store_matrix(SegShared<T, K>& sh, const int32_t slot, const int32_t lane,
+                                             const MatrixBody& mb)
+{
+    const double mx = fmax(fmax(fabs(mb.wa), fabs(mb.da)), fmax(fabs(mb.wb), fabs(mb.db)));
+    const int32_t s = frexp_exponent(mx);
+    sh.m[slot][0][lane] = __builtin_ldexp(mb.wa, -s);
+    sh.m[slot][1][lane] = __builtin_ldexp(mb.wb, -s);
+    sh.m[slot][2][lane] = __builtin_ldexp(mb.da, -s);
+    sh.m[slot][3][lane] = __builtin_ldexp(mb.db, -s);
+    sh.e[slot][lane] = mb.ex + s;
+}
+
+// Start vector of every segment of a counted shot: the matrices chained from
+// (w_1, d_1) = (1, 1), renormalised after each segment (one lane).
+template <int T, int K>
+__device__ __forceinline__ void chain_starts(SegShared<T, K>& sh, const int32_t slot, const int32_t segs)
+{
+    double v0 = 1.0, v1 = 1.0;
+    for (int32_t s = 0; s < segs; ++s) {
+        sh.start[slot][0][s] = v0;
+        sh.start[slot][1][s] = v1;
+        const double a = __builtin_fma(sh.m[slot][0][s], v0, sh.m[slot][1][s] * v1);
+        const double b = __builtin_fma(sh.m[slot][2][s], v0, sh.m[slot][3][s] * v1);
+        const int32_t x = frexp_exponent(fmax(fabs(a), fabs(b)));
+        v0 = __builtin_ldexp(a, -x);
+        v1 = __builtin_ldexp(b, -x);
+    }
+}
+
+// One level of the binary-tree product: M_lane <- M_{lane+d} M_lane.
+template <int T, int K>
+__device__ __forceinline__ void tree_step(SegShared<T, K>& sh, const int32_t slot, const int32_t lane,
+                                          const int32_t d)
+{
+    const double a00 = sh.m[slot][0][lane + d], a01 = sh.m[slot][1][lane + d];
+    const double a10 = sh.m[slot][2][lane + d], a11 = sh.m[slot][3][lane + d];
+    const double b00 = sh.m[slot][0][lane], b01 = sh.m[slot][1][lane];
+    const double b10 = sh.m[slot][2][lane], b11 = sh.m[slot][3][lane];
+    const double c00 = __builtin_fma(a00, b00, a01 * b10);
+    const double c01 = __builtin_fma(a00, b01, a01 * b11);
+    const double c10 = __builtin_fma(a10, b00, a11 * b10);
+    const double c11 = __builtin_fma(a10, b01, a11 * b11);
+    const int32_t s = frexp_exponent(fmax(fmax(fabs(c00), fabs(c01)), fmax(fabs(c10), fabs(c11))));
+    sh.m[slot][0][lane] = __builtin_ldexp(c00, -s);
+    sh.m[slot][1][lane] = __builtin_ldexp(c01, -s);
+    sh.m[slot][2][lane] = __builtin_ldexp(c10, -s);
+    sh.m[slot][3][lane] = __builtin_ldexp(c11, -s);
+    sh.e[slot][lane] = sh.e[slot][lane + d] + sh.e[slot][lane] + s;
+}
+
+// Node count of a counted shot from its segments' counts (one lane).
+template <int T, int K>
+__device__ __forceinline__ int32_t join_counts(const SegShared<T, K>& sh, const int32_t slot, const int32_t segs)
+{
+    int32_t total = 0, prev = 0;
+    for (int32_t s = 0; s < segs; ++s) {
+        const SegCount sc = sh.cnt[slot][s];
+        if (sc.has) {
+            total += sc.internal + (sc.first ^ prev);
+            prev = sc.last;
+        }
+    }
+    return total;
+}
+
+// One segmented shot of KK levels (1, or 2 sharing the window of g): every
+// lane builds its segment's matrix for each energy e[k] (slot k of the
+// shared memory), then the tree product gives the far end F = fw 2^fk of
+// each; for a counted slot the segments are walked again from their chained
+// start vectors and the node count is joined across segment borders.
+template <int T, int K, int KK, bool TABLE>
+__device__ __forceinline__ void segmented_shot(SegShared<T, K>& sh, const SegGeom<TABLE>& g, const int32_t lane,
+                                               const int32_t slot0, const double (&e)[KK],
+                                               const bool (&counted)[KK], const int32_t stride,
+                                               const StepConsts& k, double (&fw)[KK], int32_t (&fk)[KK],
+                                               int32_t (&c)[KK])
+{
+    const int32_t segs = g.segs;
+    if (g.active) {
+        if constexpr (KK == 1) {
+            MatrixBody mb{e[0], e[0] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10};
+            g.walk(mb);
+            store_matrix(sh, slot0, lane, mb);
+        } else {
+            MatrixPair mp{{e[0], e[0] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10},
+                          {e[1], e[1] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10}};
+            g.walk(mp);
+            store_matrix(sh, slot0, lane, mp.x);
+            store_matrix(sh, slot0 + 1, lane, mp.y);
+        }
+    }
+    __syncthreads();
+    bool any_counted = false;
+#pragma unroll
+    for (int q = 0; q < KK; ++q)
+        any_counted = any_counted || counted[q];
+    if (any_counted) {
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < KK; ++q)
+                if (counted[q])
+                    chain_starts(sh, slot0 + q, segs);
+        }
+        __syncthreads();
+    }
+    for (int32_t d = 1; d < segs; d *= 2) {
+        if ((lane & (2 * d - 1)) == 0 && lane + d < segs) {
+#pragma unroll
+            for (int q = 0; q < KK; ++q)
+                tree_step(sh, slot0 + q, lane, d);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < KK; ++q) {
+        fw[q] = __builtin_fma(sh.m[slot0 + q][0][0], 1.0, sh.m[slot0 + q][1][0] * 1.0);
+        fk[q] = sh.e[slot0 + q][0];
+        c[q] = 0;
+    }
+    if (!any_counted)
+        return;
+    if (g.active) {
+        const auto count_body = [&](const int q) {
+            const double w0 = sh.start[slot0 + q][0][lane], d0 = sh.start[slot0 + q][1][lane];
+            return CountBody{e[q], e[q] - g.u_start, w0, d0, w0 - d0, g.q0, stride - 1,
+                             (g.qn & (stride - 1)) != 0 ? g.qn - 1 : -1, {0, 0, 0, 0}, k.c10};
+        };
+        if constexpr (KK == 2) {
+            if (counted[0] && counted[1]) {
+                CountPair cp{count_body(0), count_body(1)};
+                g.walk(cp);
+                sh.cnt[slot0][lane] = cp.x.sc;
+                sh.cnt[slot0 + 1][lane] = cp.y.sc;
+            } else {
+                const int q = counted[0] ? 0 : 1;
+                CountBody cb = count_body(q);
+                g.walk(cb);
+                sh.cnt[slot0 + q][lane] = cb.sc;
+            }
+        } else {
+            CountBody cb = count_body(0);
+            g.walk(cb);
+            sh.cnt[slot0][lane] = cb.sc;
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < KK; ++q)
+            if (counted[q])
+                sh.c[slot0 + q] = join_counts(sh, slot0 + q, segs);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < KK; ++q)
+        if (counted[q])
+            c[q] = sh.c[slot0 + q];
+}
+
+}  // namespace
+
+// Segmented refinement: one T-thread block per level (K = 1) or per pair of
+// levels l, l + 1 (K = 2).  Lane s owns steps [q_s, q_{s+1}) of a level's
+// window (S = clamp(Q/32, 1, T) segments) and builds that segment's 2x2
+// difference-form transfer matrix (columns (w, d) = (1, 0) and (0, 1),
+// renormalised every 32 steps and at the end).  The far-end value is the
+// binary-tree product of the matrices (M_i <- M_{i+d} M_i at d = 1, 2, 4, ...)
+// applied to (w_1, d_1) = (1, 1).  Counted shots chain the matrices
+// sequentially (lane 0) for the segment start vectors and re-run every
+// segment.  The two levels of a pair whose shots use the same window walk it
+// together, each grid value loaded once for both; otherwise (and once one
+// of them has converged) each shoots alone.  Every level's arithmetic is
+// that of a block of its own.
+// Paired blocks carry two levels' refinement state, so they are compiled for
+// two wavefronts per SIMD (256 VGPRs); their walks have twice the
+// independent FP64 chains of a single level's.
+template <int T, bool TABLE, int K>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(K == 2 ? 2 : 3))) void numerov_refine_segmented_kernel(
     const double* __restrict__ uv, const double* __restrict__ segtab, const PotentialRecord* __restrict__ pots,
     const BandWindow* __restrict__ windows, const WaveTask* __restrict__ tasks, const int32_t n_tasks,
     const int32_t* __restrict__ bins, const Bracket* __restrict__ brackets, const double rel_tol,
     const int32_t max_iter, double* __restrict__ levels, int32_t* __restrict__ status,
     int32_t* __restrict__ iterations, int32_t* __restrict__ steps)
 {
-    __shared__ double s_m[4][T];
-    __shared__ int32_t s_e[T];
-    __shared__ double s_start[2][T];
-    __shared__ SegCount s_cnt[T];
-    __shared__ int32_t s_c;
+    static_assert(K == 1 || K == 2, "one or two levels per block");
+    __shared__ SegShared<T, K> sh;
 
     WaveTask task;
     if (!fetch_block_task(tasks, n_tasks, bins, task))
         return;
     const PotentialRecord& P = pots[task.potential];
     const int32_t lane = static_cast<int32_t>(threadIdx.x);
-    const int64_t out = P.level_offset + task.first;
-    if (P.flags != 0) {
-        if (lane == 0)
-            write_level(levels, status, iterations, out, __builtin_nan(""), 3, 0);
-        return;
-    }
     const BandWindow* W = windows + task.potential * kBands;
-    Refiner r{};
-    r.v = P.level_min + task.first;
-    r.b = brackets[out];
-    r.start_model();
-    if (!r.b.found) {
+    const double* uv_pot = uv + P.ut_offset;
+    Refiner r[K];
+    SegGeom<TABLE> g[K];
+    bool live[K];
+    int32_t st[K];
+    int64_t out[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const int32_t l = task.first + q;
+        out[q] = P.level_offset + l;
+        live[q] = false;
+        st[q] = 0;
+        r[q] = Refiner{};
+        if (l >= P.level_count)
+            continue;
+        if (P.flags != 0) {
+            if (lane == 0)
+                write_level(levels, status, iterations, out[q], __builtin_nan(""), 3, 0);
+            continue;
+        }
+        r[q].v = P.level_min + l;
+        r[q].b = brackets[out[q]];
+        r[q].start_model();
+        if (!r[q].b.found) {
+            if (lane == 0)
+                write_level(levels, status, iterations, out[q], __builtin_nan(""), 1, 0);
+            continue;
+        }
+        live[q] = true;
+        g[q].init(uv_pot, segtab, W[r[q].b.band], lane, T);
         if (lane == 0)
-            write_level(levels, status, iterations, out, __builtin_nan(""), 1, 0);
-        return;
+            steps[out[q]] = g[q].n - 2;
     }
-    // The level's band window: Dirichlet points lo and hi of the trimmed grid.
-    const BandWindow& B = W[r.b.band];
-    const double* u = uv + P.ut_offset + B.lo;
-    const int32_t n = B.hi - B.lo + 1;
-    if (lane == 0)
-        steps[out] = n - 2;
-    const int32_t qn = n - 2;
-    const int32_t segs = min(T, max(1, qn / 32));
-    const bool active = lane < segs;
-    const int32_t q0 = active ? static_cast<int32_t>(segment_start(qn, segs, lane)) : 0;
-    const int32_t q1 = active ? static_cast<int32_t>(segment_start(qn, segs, lane + 1)) : 0;
-    // With the segment table, lane s reads column s (row r = point q0 + r):
-    // consecutive lanes load consecutive addresses.
-    ColumnSource cs{};
-    double u_start;
-    if constexpr (TABLE) {
-        // The resource must be wave-uniform (scalar registers); the
-        // readfirstlane calls state what the compiler cannot prove.
-        const int32_t rows = seg_rows(n, 0, 1);
-        const uint64_t base = reinterpret_cast<uint64_t>(segtab + static_cast<int64_t>(B.seg_row) * 64);
-        const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(base));
-        const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(base >> 32));
-        const int32_t bytes = __builtin_amdgcn_readfirstlane(rows * 512);
-        cs.rs = __builtin_amdgcn_make_buffer_rsrc(
-            reinterpret_cast<double*>((static_cast<uint64_t>(hi) << 32) | lo), static_cast<int16_t>(0),
-            bytes, kBufferResourceWord3);
-        cs.off = lane * 8 + 512;
-        u_start = cs.at(-1);
-    } else {
-        u_start = u[q0];
-    }
-    const RowSource rsrc{u + q0 + 1};
+    bool together = false;
+    if constexpr (K == 2)
+        together = live[0] && live[1] && g[0].same_window(g[1]);
     const StepConsts k = step_consts();
 
-    int32_t st = 0;
     for (;;) {
-        if (r.converged(rel_tol, P.abs_tol))
+        double em[K];
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            em[q] = 0.0;
+            if (!live[q])
+                continue;
+            if (r[q].converged(rel_tol, P.abs_tol)) {
+                live[q] = false;
+            } else if (r[q].it >= max_iter) {
+                st[q] = 2;
+                live[q] = false;
+            } else {
+                em[q] = r[q].probe(rel_tol, P.abs_tol);
+                any = true;
+            }
+            if (!live[q] && lane == 0)
+                write_level(levels, status, iterations, out[q], r[q].level(), st[q], r[q].it);
+        }
+        if (!any)
             break;
-        if (r.it >= max_iter) {
-            st = 2;
-            break;
-        }
-        const double em = r.probe(rel_tol, P.abs_tol);
-        const double e = P.beta * em;
-
-        if (active) {
-            MatrixBody mb{e, e - u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10};
-            if constexpr (TABLE)
-                walk_segment(cs, q1 - q0, mb);
-            else
-                walk_segment(rsrc, q1 - q0, mb);
-            const double mx = fmax(fmax(fabs(mb.wa), fabs(mb.da)), fmax(fabs(mb.wb), fabs(mb.db)));
-            const int32_t sh = frexp_exponent(mx);
-            s_m[0][lane] = __builtin_ldexp(mb.wa, -sh);
-            s_m[1][lane] = __builtin_ldexp(mb.wb, -sh);
-            s_m[2][lane] = __builtin_ldexp(mb.da, -sh);
-            s_m[3][lane] = __builtin_ldexp(mb.db, -sh);
-            s_e[lane] = mb.ex + sh;
-        }
-        __syncthreads();
-
-        const bool counted = !r.isolated;
-        if (counted) {
-            if (lane == 0) {
-                double v0 = 1.0, v1 = 1.0;
-                for (int32_t s = 0; s < segs; ++s) {
-                    s_start[0][s] = v0;
-                    s_start[1][s] = v1;
-                    const double a = __builtin_fma(s_m[0][s], v0, s_m[1][s] * v1);
-                    const double b = __builtin_fma(s_m[2][s], v0, s_m[3][s] * v1);
-                    const int32_t sh = frexp_exponent(fmax(fabs(a), fabs(b)));
-                    v0 = __builtin_ldexp(a, -sh);
-                    v1 = __builtin_ldexp(b, -sh);
-                }
+        if constexpr (K == 2) {
+            if (together && live[0] && live[1]) {
+                const double e[2] = {P.beta * em[0], P.beta * em[1]};
+                const bool counted[2] = {!r[0].isolated, !r[1].isolated};
+                double fw[2];
+                int32_t fk[2], c[2];
+                segmented_shot<T, K, 2, TABLE>(sh, g[0], lane, 0, e, counted, P.stride, k, fw, fk, c);
+                r[0].update(em[0], fw[0], fk[0], c[0]);
+                r[1].update(em[1], fw[1], fk[1], c[1]);
+                __syncthreads();
+                continue;
             }
+        }
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            if (!live[q])
+                continue;
+            const double e[1] = {P.beta * em[q]};
+            const bool counted[1] = {!r[q].isolated};
+            double fw[1];
+            int32_t fk[1], c[1];
+            segmented_shot<T, K, 1, TABLE>(sh, g[q], lane, q, e, counted, P.stride, k, fw, fk, c);
+            r[q].update(em[q], fw[0], fk[0], c[0]);
             __syncthreads();
         }
-
-        for (int32_t d = 1; d < segs; d *= 2) {
-            if ((lane & (2 * d - 1)) == 0 && lane + d < segs) {
-                const double a00 = s_m[0][lane + d], a01 = s_m[1][lane + d];
-                const double a10 = s_m[2][lane + d], a11 = s_m[3][lane + d];
-                const double b00 = s_m[0][lane], b01 = s_m[1][lane];
-                const double b10 = s_m[2][lane], b11 = s_m[3][lane];
-                const double c00 = __builtin_fma(a00, b00, a01 * b10);
-                const double c01 = __builtin_fma(a00, b01, a01 * b11);
-                const double c10 = __builtin_fma(a10, b00, a11 * b10);
-                const double c11 = __builtin_fma(a10, b01, a11 * b11);
-                const int32_t sh = frexp_exponent(
-                    fmax(fmax(fabs(c00), fabs(c01)), fmax(fabs(c10), fabs(c11))));
-                s_m[0][lane] = __builtin_ldexp(c00, -sh);
-                s_m[1][lane] = __builtin_ldexp(c01, -sh);
-                s_m[2][lane] = __builtin_ldexp(c10, -sh);
-                s_m[3][lane] = __builtin_ldexp(c11, -sh);
-                s_e[lane] = s_e[lane + d] + s_e[lane] + sh;
-            }
-            __syncthreads();
-        }
-        const double fw = __builtin_fma(s_m[0][0], 1.0, s_m[1][0] * 1.0);
-        const int32_t fk = s_e[0];
-
-        int32_t c = 0;
-        if (counted) {
-            if (active) {
-                const double w0 = s_start[0][lane], d0 = s_start[1][lane];
-                CountBody cb{e, e - u_start, w0, d0, w0 - d0, q0, qn, P.stride, {0, 0, 0, 0}, k.c10};
-                if constexpr (TABLE)
-                    walk_segment(cs, q1 - q0, cb);
-                else
-                    walk_segment(rsrc, q1 - q0, cb);
-                s_cnt[lane] = cb.sc;
-            }
-            __syncthreads();
-            if (lane == 0) {
-                int32_t total = 0, prev = 0;
-                for (int32_t s = 0; s < segs; ++s) {
-                    const SegCount sc = s_cnt[s];
-                    if (sc.has) {
-                        total += sc.internal + (sc.first ^ prev);
-                        prev = sc.last;
-                    }
-                }
-                s_c = total;
-            }
-            __syncthreads();
-            c = s_c;
-        }
-        r.update(em, fw, fk, c);
-        __syncthreads();
     }
-    if (lane == 0)
-        write_level(levels, status, iterations, out, r.level(), st, r.it);
 }
 
 // Bracket pass: one thread per level task (potential, level index) finds
@@ -1540,9 +1743,10 @@ hipError_t launch_refine(hipStream_t stream, int32_t levels_per_lane, const doub
 }
 
 // segtab != nullptr selects the segment tables for 64-lane blocks.
+// levels_per_block 2 (64-lane blocks only) takes tasks of two levels each.
 // bins == nullptr: tasks[0, n_tasks) in order; else `bins` (device, 9
 // offsets) splits the list per XCD and `max_bin` is the largest bin's size.
-hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, const double* uv,
+hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, int32_t levels_per_block, const double* uv,
                                    const double* segtab, const PotentialRecord* pots,
                                    const BandWindow* windows, const WaveTask* tasks, int32_t n_tasks,
                                    const int32_t* bins, int32_t max_bin, const Bracket* brackets,
@@ -1551,23 +1755,31 @@ hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, const doub
 {
     if (n_tasks == 0)
         return hipSuccess;
+    if (levels_per_block != 1 && !(levels_per_block == 2 && lanes == 64))
+        return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(bins == nullptr ? n_tasks : 8 * max_bin));
-#define EPSEON_SEGMENTED(T, TAB)                                                                   \
-    hipLaunchKernelGGL((numerov_refine_segmented_kernel<T, TAB>), grid, dim3(T), 0, stream, uv,   \
-                       segtab, pots, windows, tasks, n_tasks, bins, brackets, rel_tol, max_iter,   \
+#define EPSEON_SEGMENTED(T, TAB, K)                                                                   \
+    hipLaunchKernelGGL((numerov_refine_segmented_kernel<T, TAB, K>), grid, dim3(T), 0, stream, uv,   \
+                       segtab, pots, windows, tasks, n_tasks, bins, brackets, rel_tol, max_iter,      \
                        levels, status, iterations, steps)
     switch (lanes) {
     case 256:
-        EPSEON_SEGMENTED(256, false);
+        EPSEON_SEGMENTED(256, false, 1);
         break;
     case 128:
-        EPSEON_SEGMENTED(128, false);
+        EPSEON_SEGMENTED(128, false, 1);
         break;
     case 64:
-        if (segtab != nullptr)
-            EPSEON_SEGMENTED(64, true);
-        else
-            EPSEON_SEGMENTED(64, false);
+        if (levels_per_block == 2) {
+            if (segtab != nullptr)
+                EPSEON_SEGMENTED(64, true, 2);
+            else
+                EPSEON_SEGMENTED(64, false, 2);
+        } else if (segtab != nullptr) {
+            EPSEON_SEGMENTED(64, true, 1);
+        } else {
+            EPSEON_SEGMENTED(64, false, 1);
+        }
         break;
     default:
         return hipErrorInvalidValue;

@@ -127,6 +127,8 @@ typedef struct epseon_stats {
     double solve_split;           /* scan-work fraction of the first pipeline group (0: sequential) */
     int32_t scan_windows;         /* 1 when the last scan (and its refinement) used band windows */
     int32_t solve_graph;          /* 1 when the last epseon_solve replayed a captured HIP graph */
+    int32_t refine_levels_per_block; /* levels sharing a block in the last segmented refine (1 or 2) */
+    int32_t reserved;
 } epseon_stats;
 
 typedef struct epseon_ctx epseon_ctx;
@@ -270,6 +272,12 @@ typedef struct epseon_tuning {
                                      one.  Pipelined (solve_split > 0) and paired-lane solves
                                      always launch directly.  Results are bit-identical either
                                      way. */
+    int32_t segmented_levels;     /* segmented mode with 64-lane blocks: levels per block.  2: the
+                                     levels l, l + 1 of a potential share a block, and while both
+                                     shoot in the same window every grid value is loaded once for
+                                     both; 1: one level per block; 0 (default): 2 when the batch has
+                                     at least 24 levels per compute unit, else 1.  Results are
+                                     bit-identical either way. */
     double solve_split;           /* see epseon_set_solve_split */
 } epseon_tuning;
 

@@ -55,11 +55,11 @@ def _compare(prob, nodes, far, resc, lv, it, levels, seg, lanes, windows=None):
     np.testing.assert_array_equal(it[ok], oit[ok])
 
 
-@pytest.mark.parametrize("mode", ["lane", "segmented"])
+@pytest.mark.parametrize("mode", ["lane", "segmented", "segmented-pairs"])
 def test_batched_subset_parity(capi, mode):
     specs = batched(4096)[:40]
     _log(f"case: first 40 potentials of the batched workload at full size, {mode} refine")
-    seg = mode == "segmented"
+    seg = mode != "lane"
     tables = [s.table() for s in specs]
     counts = [s.bound_levels() + 2 for s in specs]
     with capi.Context(0) as ctx:
@@ -68,6 +68,9 @@ def test_batched_subset_parity(capi, mode):
         t = ctx.tuning()
         t.refine_mode = capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE
         t.windows = int(seg)
+        # Two levels per 64-lane block (each table load feeding both shots);
+        # every level's arithmetic is that of a block of its own.
+        t.segmented_levels = 2 if mode == "segmented-pairs" else 1
         ctx.set_tuning(t)
         ctx.load([_pot(capi, s, t, c) for s, t, c in zip(specs, tables, counts)])
         ctx.solve(1e-10, 200)
@@ -75,6 +78,8 @@ def test_batched_subset_parity(capi, mode):
         lv, st, it = ctx.levels()
         lanes = ctx.stats().refine_lanes
         assert ctx.stats().scan_windows == int(seg)
+        if seg:
+            assert ctx.stats().refine_levels_per_block == (2 if mode == "segmented-pairs" else 1)
         flags = [ctx.potential_info(i).flags for i in range(len(specs))]
     eo = lo = 0
     for i, s in enumerate(specs):
@@ -106,6 +111,7 @@ def test_batched_full_scale(capi):
          f"refine {stats.refine_ms:.2f} ms, levels converged {stats.levels_converged}, "
          f"unresolved potentials {int(flags.sum())}")
     assert stats.refine_mode == capi.REFINE_SEGMENTED and stats.refine_lanes == 64 and stats.scan_windows == 1
+    assert stats.refine_levels_per_block == 2
     assert np.all(st != 2), "a level hit the iteration cap"
     eo = lo = 0
     resolved_checked = 0
@@ -183,8 +189,9 @@ def test_result_preserving_tunings_are_bit_identical(capi):
 
 
 def test_result_preserving_tunings_windowed(capi):
-    """The same schedule knobs with windowed shots and segmented refinement:
-    bit-identical outputs, equal to the oracle's windowed solve."""
+    """The same schedule knobs with windowed shots and segmented refinement,
+    one or two levels per block: bit-identical outputs, equal to the oracle's
+    windowed solve."""
     specs = batched(4096)[:300]
     _log("case: first 300 batched potentials, segmented refine with band windows, tunings varied")
     counts = [s.bound_levels() + 2 for s in specs]
@@ -192,15 +199,18 @@ def test_result_preserving_tunings_windowed(capi):
     outs = []
     with capi.Context(0) as ctx:
         ctx.load([_pot(capi, s, t, c) for s, t, c in zip(specs, tables, counts)])
-        for wpb, split, xcd in ((1, 0.0, 1), (2, 0.3, 0), (4, 0.0, 0)):
+        for wpb, split, xcd, lpb in ((1, 0.0, 1, 1), (2, 0.3, 0, 2), (4, 0.0, 0, 1), (1, 0.0, 1, 2),
+                                     (2, 0.0, 0, 2)):
             t = capi.Tuning(refine_mode=capi.REFINE_SEGMENTED, levels_per_lane=1, scan_waves_per_block=wpb,
-                            segmented_lanes=64, xcd_scan=xcd, windows=1, solve_split=split)
+                            segmented_lanes=64, xcd_scan=xcd, windows=1, segmented_levels=lpb,
+                            solve_split=split)
             ctx.set_tuning(t)
             ctx.solve(1e-10, 200)
             nodes, far, resc = ctx.scan_results()
             lv, st, it = ctx.levels()
             stats = ctx.stats()
-            _log(f"  wpb {wpb} split {split} xcd {xcd}: solve {stats.solve_ms:.2f} ms")
+            assert stats.refine_levels_per_block == lpb
+            _log(f"  wpb {wpb} split {split} xcd {xcd} levels/block {lpb}: solve {stats.solve_ms:.2f} ms")
             outs.append((nodes, far.view(np.int64), resc, lv.view(np.int64), st, it))
     for other in outs[1:]:
         for a, b in zip(outs[0], other):

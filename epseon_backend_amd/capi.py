@@ -204,6 +204,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "epseon_normalize": (ctypes.c_int, [ctypes.c_void_p]),
     "epseon_get_wavefunctions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]),
     "epseon_level_vjp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
+    "epseon_level_vjp_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "epseon_get_all_wavefunctions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
     "epseon_get_eigenfunction_size": (ctypes.c_int, [ctypes.c_void_p, _c_int64_p]),
 }
@@ -540,3 +541,10 @@ class Context:
         buffers of this context's device."""
         _check(self._lib.epseon_level_vjp(self._h, ctypes.c_void_p(grad_levels_ptr),
                                           ctypes.c_void_p(grad_v_ptr), MEM_DEVICE))
+
+    def level_vjp_noalloc(self, grad_levels_ptr: int, grad_v_ptr: int) -> None:
+        """level_vjp_device without any allocation: the normalisation's
+        checkpoints live in the library's per-device scratch array
+        (epseon_level_vjp_noalloc; calls overlapping on one device race)."""
+        _check(self._lib.epseon_level_vjp_noalloc(self._h, ctypes.c_void_p(grad_levels_ptr),
+                                                  ctypes.c_void_p(grad_v_ptr)))

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <string>
@@ -187,6 +188,43 @@ int32_t node_stride(double beta, double e_hi, double v_min)
 }
 
 }  // namespace
+
+// Scratch of epseon_level_vjp_noalloc: the normalisation pass's segment maps
+// and shot checkpoints, in a file-scope device array instead of the arena.
+// Every device the library runs on holds its own copy until the process
+// ends.  Host code never uses the array's name (a host-side stand-in): each
+// device's address is taken once with hipGetSymbolAddress, with that device
+// current, when the first context opens on it, and kept in scratch_ptr.
+constexpr size_t kNoAllocScratchBytes = EPSEON_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Address of `scratch` on the current device `device` (once per device).
+hipError_t scratch_setup(int32_t device)
+{
+    if (device < 0 || device >= kMaxScratchDevices)
+        return hipSuccess;  // no scratch: epseon_level_vjp_noalloc reports it
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (g_scratch_ptr[device] != nullptr)
+        return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+        return e;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+void* scratch_of(int32_t device)
+{
+    if (device < 0 || device >= kMaxScratchDevices)
+        return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    return g_scratch_ptr[device];
+}
 
 }  // namespace epseon::hip
 
@@ -495,6 +533,8 @@ int epseon_ctx_create(int32_t device, uint64_t block_bytes, epseon_ctx** out)
         e = hipEventCreateWithFlags(&ctx->ev_tables, hipEventDisableTiming);
     if (e == hipSuccess)
         e = hipDeviceGetAttribute(&ctx->compute_units, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess)
+        e = scratch_setup(device);
     if (e != hipSuccess) {
         const int rc = hip_fail(e, "context creation");
         epseon_ctx_destroy(ctx);
@@ -1978,6 +2018,57 @@ int epseon_level_vjp(epseon_ctx* ctx, const double* grad_levels, double* grad_v,
     } else {
         EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
+    return EPSEON_OK;
+}
+
+int epseon_level_vjp_noalloc(epseon_ctx* ctx, const double* grad_levels_device, double* grad_v_device)
+{
+    if (const int rc = check_ctx(ctx); rc != EPSEON_OK)
+        return rc;
+    if (!ctx->refined)
+        return fail(EPSEON_ERR_STATE, "epseon_level_vjp_noalloc before epseon_refine");
+    if (grad_v_device == nullptr || (grad_levels_device == nullptr && ctx->total_levels > 0))
+        return fail(EPSEON_ERR_INVALID, "null gradient pointer");
+    // Segment maps and checkpoints of the normalisation pass, both carved
+    // from the device's scratch array.
+    const size_t slots = static_cast<size_t>(std::max<int64_t>(ctx->ckpt_total, 1));
+    const size_t ckpt_bytes = (slots * sizeof(PsiCheckpoint) + 255) & ~size_t(255);
+    const size_t need = ckpt_bytes + slots * sizeof(PsiSegMatrix);
+    if (need > kNoAllocScratchBytes) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "epseon_level_vjp_noalloc: the batch needs %zu bytes of scratch (%zu eigenfunction-shot "
+                      "segments), above the %zu-byte maximum",
+                      need, slots, kNoAllocScratchBytes);
+        return fail(EPSEON_ERR_INVALID, msg);
+    }
+    void* base = scratch_of(ctx->device);
+    if (base == nullptr)
+        return fail(EPSEON_ERR_DEVICE, "epseon_level_vjp_noalloc: no scratch on this device");
+    PsiCheckpoint* ckpt = static_cast<PsiCheckpoint*>(base);
+    PsiSegMatrix* mats = reinterpret_cast<PsiSegMatrix*>(static_cast<char*>(base) + ckpt_bytes);
+    // The scratch holds whatever the last call on this device left, so the
+    // normalisation always runs; every slot a kernel reads is written first.
+    // The context's own checkpoints (epseon_normalize) are left as they are.
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
+    EPSEON_HIP_TRY(launch_normalize(ctx->stream, ctx->d_uv, ctx->d_uvr, ctx->d_rec, ctx->d_psi_tasks,
+                                    ctx->cum_psi_tasks.back(), ctx->d_refine_tasks, ctx->n_refine_tasks,
+                                    ctx->d_levels, ctx->d_status, mats, ckpt, ctx->d_norms));
+    const hipStream_t streams[4] = {ctx->stream, ctx->stream2, ctx->stream3, ctx->stream4};
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+    for (int w = 1; w < 4; ++w)
+        EPSEON_HIP_TRY(hipStreamWaitEvent(streams[w], ctx->ev_fork, 0));
+    for (int w = 0; w < 4; ++w)
+        EPSEON_HIP_TRY(launch_level_vjp(streams[w], w + 1, ctx->d_uv, ctx->d_uvr, ctx->d_rec,
+                                        ctx->d_vjp_lists[w], ctx->n_vjp[w], ctx->d_levels, ctx->d_status,
+                                        ctx->d_norms, ckpt, grad_levels_device, grad_v_device));
+    for (int w = 1; w < 4; ++w) {
+        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_join_aux[w - 1], streams[w]));
+        EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join_aux[w - 1], 0));
+    }
+    EPSEON_HIP_TRY(hipEventRecord(ctx->ev[7], ctx->stream));
+    ctx->vjp_timed = true;
+    EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return EPSEON_OK;
 }
 

@@ -345,6 +345,26 @@ int epseon_get_eigenfunction_size(epseon_ctx* ctx, int64_t* elements);
  * No counterpart in the reference (it has no gradient path). */
 int epseon_level_vjp(epseon_ctx* ctx, const double* grad_levels, double* grad_v, int32_t mem);
 
+/* Maximum scratch of epseon_level_vjp_noalloc, bytes per device. */
+#define EPSEON_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* epseon_level_vjp for device pointers that allocates nothing: no hipMalloc,
+ * hipFree or other allocating call, on the device or the host.  The
+ * normalisation pass's segment maps and shot checkpoints (80 + 32 bytes per
+ * eigenfunction-shot segment of every level, segments from
+ * epseon_potential_info point counts) live in a library-owned device array of
+ * EPSEON_NOALLOC_SCRATCH_BYTES per device, held until the process ends;
+ * batches that need more are rejected with EPSEON_ERR_INVALID.  The
+ * normalisation always runs (the array keeps the last call's contents), the
+ * context's own eigenfunction state is left as it was, and the result equals
+ * epseon_level_vjp(ctx, grad_levels_device, grad_v_device, EPSEON_MEM_DEVICE)
+ * bit for bit.  Every call on a device shares that device's array: calls
+ * that overlap on one device (from several threads or contexts) race.
+ * Synchronises the stream.  No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (serving loops, code that must
+ * not allocate between set-up and teardown). */
+int epseon_level_vjp_noalloc(epseon_ctx* ctx, const double* grad_levels_device, double* grad_v_device);
+
 /* Copy the refined levels out, in potential order, level_count per potential
  * (the reference's per-slot output buffer, algorithm_config.hpp:189).  Any of
  * the pointers may be NULL.  Synchronises the stream. */

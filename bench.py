@@ -1,18 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark of the Numerov-shooting hot path on one MI355X.
 
-Workload (default ``single``): one heavy Morse potential (mu = 63.452 amu,
-D_e = 12440 cm^-1, a = 1.8676 1/A, r_e = 2.666 A), an FP64 grid of 200,000
-points on r in [2, 8] A, 65,536 trial energies uniform in (min V, D_e), every
-bound level refined to 1e-10 relative.  A step is one pass of the hot path over
-that input: Numerov scan of all trial energies, node-count bracketing and
-refinement of every level, levels copied back.  The potential table is
-uploaded once before timing (inputs resident in HBM).
-
-``--workload batched`` runs the batched case instead: 4,096 seeded Morse
-potentials with ragged grids (2k-64k points) and trial counts (256-8192); it
+Workload (default ``batched``): 4,096 seeded Morse potentials (mu log-U[0.5,
+100] amu, D_e U[5e3, 5e4] cm^-1, a U[1, 2.5] 1/A, r_e U[0.7, 3] A) with
+ragged FP64 grids (log-U[2,000, 64,000] points on [r_e - 0.6, r_e + 6/a] A)
+and trial counts (log-U[256, 8,192]), every bound level refined to 1e-10
+relative, launch configuration from the autotuner.  A step is one pass of the
+hot path over that input: Numerov scan of all trial energies, node-count
+bracketing and refinement of every level, levels copied back.  The potential
+tables are uploaded once before timing (inputs resident in HBM).  The run
 also times the same solve done potential by potential (one context and one
-launch sequence per potential) on a seeded sample of 64 potentials.
+launch sequence per potential) on a seeded sample of 64 potentials, and the
+single heavy Morse potential below for a few steps.
+
+``--workload single`` runs one heavy Morse potential (mu = 63.452 amu,
+D_e = 12440 cm^-1, a = 1.8676 1/A, r_e = 2.666 A) on an FP64 grid of 200,000
+points on r in [2, 8] A with 65,536 trial energies uniform in (min V, D_e).
 
 ``--workload stress`` runs 64 double-well potentials (Morse plus a Gaussian
 barrier) on 1,000,000-point grids with 262,144 trial energies: a step is the
@@ -28,9 +31,9 @@ carries the grad/fwd wall-time ratio.
 
 Prints one JSON line: metric/value (whole-job Numerov point-updates per
 second over scan + refinement shots), eigenlevels/s, the roofline of the
-dominant kernel (the scan, timed with HIP events on its own stream), and a
-CPU baseline (the C oracle on the host cores, on a bounded sample of the same
-workload).
+dominant kernel (the scan, timed with HIP events on the stream it runs on),
+the whole solve's FP64 fraction, and a CPU baseline (the C oracle on the host
+cores, on a stated subset of the same workload, all cores and one core).
 """
 from __future__ import annotations
 
@@ -99,32 +102,78 @@ def batched_workload(capi, count: int = 4096):
     return pots, config
 
 
-def cpu_baseline(target_seconds: float = 12.0) -> dict:
-    """The C oracle (OpenMP over trial energies) on a bounded sample of the
-    single-potential workload: the same grid with fewer trial energies."""
-    import oracle
-    from epseon_backend_amd.workloads import I2
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    c = {"n": I2.n, "m": I2.m}
-    prob = oracle.Problem(v=I2.table(), r_min=I2.r_min, r_max=I2.r_max, mu=I2.mu, e_top=I2.de, m=64)
-    ut, b, stride = prob.pt, prob.beta, prob.stride
+
+def _oracle_solve_work(oracle, prob, n_levels: int):
+    """The oracle's scan + refinement of one potential as the GPU runs it on
+    the batch (band windows, segmented shots of 64 segments): seconds,
+    performed point-updates (counted as the library counts them) and
+    converged levels."""
+    t0 = time.perf_counter()
+    nodes, far, resc, lv, it = oracle.solve(prob, 0, n_levels, 1e-10, 200, segmented=True, lanes=64, windows=True)
+    dt = time.perf_counter() - t0
+    win = prob.windows(True)
+    steps = win[:, 1] - win[:, 0] - 1
+    m = prob.m
+    pu = 0.0
+    for j0 in range(0, m, 64):
+        pu += min(64, m - j0) * float(steps[oracle.group_band(j0, m, prob.e_lo, prob.e_top)])
+    fe = oracle.band_far_exps(win)
+    for v in range(n_levels):
+        if it[v] > 0:
+            br = oracle.bracket_level(nodes, far, resc, prob.e_lo, prob.e_top, v, fe, win)
+            pu += float(it[v]) * float(steps[br.band])
+    return dt, pu, int(np.sum(np.isfinite(lv)))
+
+
+def cpu_baseline(all_cores_count: int = 256, one_core_count: int = 32) -> dict:
+    """The C oracle (FP64, OpenMP over trial energies and levels) on subsets of
+    the batched workload: its first ``all_cores_count`` potentials on every
+    host core, its first ``one_core_count`` on one core."""
+    import oracle
+    from epseon_backend_amd.workloads import batched
+
+    specs = batched(4096)
     threads = oracle.num_threads()
-    m = 64 * threads
-    t0 = time.perf_counter()
-    oracle.scan(ut, b, prob.e_lo, prob.e_top, m, stride)
-    dt = time.perf_counter() - t0
-    m = int(min(c["m"], max(m, m * target_seconds / max(dt, 1e-3))))
-    t0 = time.perf_counter()
-    oracle.scan(ut, b, prob.e_lo, prob.e_top, m, stride)
-    dt = time.perf_counter() - t0
-    pu = (c["n"] - 2) * m
+
+    def run(count: int, nthreads: int) -> tuple[float, float, int, int]:
+        oracle.set_num_threads(nthreads)
+        secs = pu = 0.0
+        levels = 0
+        for s in specs[:count]:
+            prob = oracle.Problem(v=s.table(), r_min=s.r_min, r_max=s.r_max, mu=s.mu, e_top=s.de, m=s.m)
+            if not prob.usable:
+                continue
+            dt, w, nl = _oracle_solve_work(oracle, prob, s.bound_levels() + 2)
+            secs += dt
+            pu += w
+            levels += nl
+        return secs, pu, levels, count
+
+    s_all, pu_all, lv_all, n_all = run(all_cores_count, threads)
+    s_one, pu_one, lv_one, n_one = run(one_core_count, 1)
+    oracle.set_num_threads(threads)
     return {
-        "value": pu / dt,
+        "value": pu_all / s_all,
         "unit": "point-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"Numerov scan of {m} of the 65536 trial energies over the full 200000-point grid "
-        f"({pu:.3e} point-updates in {dt:.2f} s)",
+        "sample": f"oracle scan + bracket + refinement (band windows, segmented shots) of the first {n_all} "
+        f"of the 4096 batched potentials on {threads} threads ({pu_all:.3e} point-updates, {lv_all} levels "
+        f"in {s_all:.2f} s), and of the first {n_one} on 1 thread ({pu_one:.3e} point-updates, {lv_one} "
+        f"levels in {s_one:.2f} s)",
+        "eigenlevels_per_s": lv_all / s_all,
+        "single_core_value": pu_one / s_one,
+        "single_core_eigenlevels_per_s": lv_one / s_one,
+        "cpu_model": _cpu_model(),
     }
 
 
@@ -330,16 +379,65 @@ def run_stress(capi, args) -> int:
     return 0
 
 
-def load_traffic(workload: str) -> float | None:
+def load_traffic(workload: str) -> tuple[float | None, str | None]:
     """HBM bytes per scan launch measured by rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel calibration), if committed."""
+    WRITE_SIZE, scaled by the copy-kernel calibration) in a separate profiled
+    run, read from the committed summary: (bytes, file) or (None, None)."""
     path = REPO / "profiles" / f"scan_{workload}_traffic.json"
     if not path.exists():
-        return None
+        return None, None
     try:
-        return float(json.loads(path.read_text())["bytes_per_launch"])
+        return float(json.loads(path.read_text())["bytes_per_launch"]), str(path.relative_to(REPO))
     except (KeyError, ValueError, OSError):
-        return None
+        return None, None
+
+
+def single_summary(capi, rel_tol: float, warmup: int = 20, steps: int = 10) -> dict:
+    """A few steps of the single heavy Morse workload (200,000 points, 65,536
+    trial energies): its own context, timed like the main workload."""
+    pots, config = single_workload(capi)
+    ctx = capi.Context(0)
+    ctx.load(pots)
+    for _ in range(warmup):
+        ctx.solve(rel_tol, 200)
+        ctx.levels()
+    pu = 0.0
+    levels = 0
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.solve(rel_tol, 200)
+        lv, _, _ = ctx.levels()
+        st = ctx.stats()
+        pu += st.scan_point_updates + st.refine_point_updates
+        levels += int(st.levels_converged)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    from epseon_backend_amd.workloads import I2
+
+    # Levels well inside the box against the analytic Morse levels (the
+    # discretisation error at this step is about 1e-11 relative).
+    ana = I2.analytic(np.arange(100))
+    err = float(np.max(np.abs(lv[:100] - ana) / ana))
+    ctx.scan()
+    ctx.refine(rel_tol, 200)
+    st = ctx.stats()
+    ctx.close()
+    scan_tflops = FLOP_PER_POINT_UPDATE * st.scan_point_updates / (st.scan_ms * 1e-3) / 1e12
+    return {
+        "workload": config["workload"],
+        "ms_per_step": wall / steps * 1e3,
+        "point_updates_per_s": pu / wall,
+        "eigenlevels_per_s": levels / wall,
+        "levels_per_step": levels // steps,
+        "scan_ms": st.scan_ms,
+        "refine_ms": st.refine_ms,
+        "scan_frac": scan_tflops / FP64_VECTOR_PEAK_TFLOPS,
+        "analytic_max_rel_err_v_lt_100": err,
+        "check_ok": bool(err < 1e-9),
+        "steps": steps,
+        "warmup": warmup,
+    }
 
 
 def main() -> int:
@@ -349,10 +447,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=None,
                     help="untimed steps before timing (default: 20 single, 10 batched and fit, 0 stress; "
                          "the clock settles over the first ~40 ms of back-to-back solves)")
-    ap.add_argument("--workload", choices=["single", "batched", "fit", "stress"], default="single")
+    ap.add_argument("--workload", choices=["single", "batched", "fit", "stress"], default="batched")
     ap.add_argument("--rel-tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="batched: keep the default launch configuration")
+    ap.add_argument("--no-single", action="store_true", help="batched: skip the single heavy potential's steps")
     ap.add_argument(
         "--copy-calibration",
         action="store_true",
@@ -423,6 +522,15 @@ def main() -> int:
         levels_ok += int(s.levels_converged)
     ctx.synchronize()
     wall = time.perf_counter() - t0
+    # Sanity of the last step's levels: every level converged within the
+    # iteration cap, and each potential's levels strictly increasing.
+    check_ok = bool(np.all(st != capi.LEVEL_MAXITER))
+    o = 0
+    for p in pots:
+        x = lv[o : o + p.level_count]
+        x = x[np.isfinite(x)]
+        check_ok = check_ok and bool(np.all(np.diff(x) > 0))
+        o += p.level_count
     # Roofline of the scan kernel: isolated scans (one launch each), timed
     # with HIP events on the context's stream; the solve above may overlap
     # scan and refinement on two streams.
@@ -441,6 +549,7 @@ def main() -> int:
     refine_ms = iso_refine_ms
     avg_scan_s = float(np.mean(scan_ms)) * 1e-3
     achieved = FLOP_PER_POINT_UPDATE * (scan_pu / args.steps) / avg_scan_s / 1e12
+    traffic, traffic_source = load_traffic(args.workload)
     out = {
         "metric": METRIC,
         "value": value,
@@ -470,22 +579,40 @@ def main() -> int:
             "peak": FP64_VECTOR_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
-            "traffic": load_traffic(args.workload),
+            "traffic": traffic,
+            "kernel": "numerov_scan_kernel",
+            "timing": "mean of 3 isolated scan launches, HIP events on the launch stream",
+            "traffic_source": (f"{traffic_source}: rocprofv3 FETCH_SIZE + WRITE_SIZE of a separate profiled run "
+                               "(committed constant, not measured in this run)") if traffic_source else None,
         },
     }
     mode = "segmented" if s.refine_mode == capi.REFINE_SEGMENTED else "lane"
     refine_achieved = (REFINE_FLOP_PER_POINT_UPDATE[mode] * (refine_pu / args.steps)
                        / (float(np.mean(refine_ms)) * 1e-3) / 1e12)
     out["refine_roofline"] = {
-        "kernel": f"numerov_refine ({mode})",
+        "kernel": f"numerov_refine ({mode}, {int(s.refine_levels_per_block)} level(s) per block)",
         "bound": "valu",
         "achieved": refine_achieved,
         "peak": FP64_VECTOR_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": refine_achieved / FP64_VECTOR_PEAK_TFLOPS,
     }
+    # The whole solve (scan, brackets, refinement; device time of the timed
+    # solves): 7 FLOP per scan point-update, the refinement's per its mode.
+    solve_flop = (FLOP_PER_POINT_UPDATE * scan_pu + REFINE_FLOP_PER_POINT_UPDATE[mode] * refine_pu) / args.steps
+    solve_achieved = solve_flop / (float(np.mean(solve_ms)) * 1e-3) / 1e12
+    out["solve_roofline"] = {
+        "bound": "valu",
+        "achieved": solve_achieved,
+        "peak": FP64_VECTOR_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": solve_achieved / FP64_VECTOR_PEAK_TFLOPS,
+    }
     out["scan_windows"] = int(s.scan_windows)
-    if not args.no_cpu_baseline and args.workload == "single":
+    out["levels_check_ok"] = check_ok
+    if args.workload == "batched" and not args.no_single:
+        out["single_heavy_morse"] = single_summary(capi, args.rel_tol)
+    if not args.no_cpu_baseline and args.workload == "batched":
         out["cpu_baseline"] = cpu_baseline()
     if tuning is not None:
         out["tuning"] = tuning
@@ -495,7 +622,10 @@ def main() -> int:
         out["batched_vs_per_potential"] = out["eigenlevels_per_s"] / pp["eigenlevels_per_s"]
     sys.stdout.write(json.dumps(out) + "\n")
     sys.stdout.flush()
-    return 0
+    ok = check_ok and out.get("single_heavy_morse", {}).get("check_ok", True)
+    if not ok:
+        sys.stderr.write("bench.py: level checks failed (see levels_check_ok / single_heavy_morse.check_ok)\n")
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":

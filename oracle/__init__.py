@@ -68,6 +68,7 @@ def _lib() -> ctypes.CDLL:
     d, i32, i64 = ctypes.c_double, ctypes.c_int32, ctypes.c_int64
     sig = {
         "eo_num_threads": (i32, []),
+        "eo_set_num_threads": (None, [i32]),
         "eo_grid_step": (d, [d, d, i64]),
         "eo_beta": (d, [d, d]),
         "eo_shifted_potential": (None, [_dp, i64, d, _dp]),
@@ -125,6 +126,11 @@ def _f64(a: np.ndarray) -> np.ndarray:
 def num_threads() -> int:
     """OpenMP threads the oracle's parallel loops use."""
     return int(_lib().eo_num_threads())
+
+
+def set_num_threads(n: int) -> None:
+    """Threads of the oracle's parallel loops from now on."""
+    _lib().eo_set_num_threads(int(n))
 
 
 def grid_step(r_min: float, r_max: float, n: int) -> float:

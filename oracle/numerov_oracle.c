@@ -83,6 +83,16 @@ int32_t eo_num_threads(void)
 #endif
 }
 
+/* Set the threads of the OpenMP loops below (timing a single-core run). */
+void eo_set_num_threads(int32_t n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n < 1 ? 1 : n);
+#else
+    (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------------ grid */
 
 double eo_grid_step(double r_min, double r_max, int64_t n)

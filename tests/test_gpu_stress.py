@@ -1,14 +1,17 @@
-"""Stress workload on the GPU at full size: double-well potentials (Morse plus
-a Gaussian barrier) on 1,000,000-point grids with 262,144 trial energies.
+"""Stress workload on the GPU at full size: all 64 double-well potentials
+(Morse plus a Gaussian barrier) on 1,000,000-point grids with 262,144 trial
+energies, every bound level, every eigenfunction written to device memory
+and the adjoint of the whole batch.
 
 The oracle's full scan at this size takes minutes on the CPU, so parity is
-checked where it is exact and cheap: sampled trial energies (node count,
-far-end value and rescale count bit-exact from single oracle shots), sampled
-levels (the oracle refines from the GPU's own scan arrays and must land on the
-same bits in the same number of iterations), eigenfunctions of sampled
-levels, and the adjoint of one potential.  Properties of the whole output
-(monotone counts, one level per node count below the window top, increasing
-levels) cover the rest.
+checked where it is exact and cheap: sampled trial energies of every
+potential (node count, far-end value and rescale count bit-exact from single
+oracle shots), sampled levels of every potential (the oracle refines from the
+GPU's own scan arrays and must land on the same bits in the same number of
+iterations), eigenfunctions of sampled levels and the adjoint of sampled
+potentials.  Properties of the whole output (monotone counts, one level per
+node count below the window top, increasing levels, unit-norm eigenfunctions)
+cover the rest.
 """
 from __future__ import annotations
 
@@ -39,8 +42,10 @@ def capi():
 
 
 def test_double_wells_full_size(capi):
-    specs = double_wells(64)[:2]
-    _log("case: 2 double-well potentials, 1,000,000 points, 262,144 trial energies, all levels")
+    import torch
+
+    specs = double_wells(64)
+    _log("case: all 64 double-well potentials, 1,000,000 points, 262,144 trial energies, all levels")
     tables = [s.table() for s in specs]
     counts = [s.bound_levels() + 2 for s in specs]
     rng = np.random.default_rng(17)
@@ -52,45 +57,74 @@ def test_double_wells_full_size(capi):
         nodes, far, resc = ctx.scan_results()
         lv, st, it = ctx.levels()
         stats = ctx.stats()
-        _log(f"  solve {stats.solve_ms:.1f} ms, refine mode {stats.refine_mode} lanes {stats.refine_lanes}")
-        psi0 = ctx.wavefunctions(0)
+        _log(f"  solve {stats.solve_ms:.1f} ms, refine mode {stats.refine_mode} lanes {stats.refine_lanes}, "
+             f"windows {stats.scan_windows}, converged {stats.levels_converged}")
+        # Every eigenfunction of the batch in device memory (one launch), and
+        # the adjoint of the whole batch with eigenfunctions recomputed.
+        psi = torch.empty(ctx.eigenfunction_size(), dtype=torch.float64, device="cuda")
+        ctx.all_wavefunctions_device(psi.data_ptr())
         g = rng.normal(size=lv.shape)
-        gv = ctx.level_vjp(g)
+        gd = torch.from_numpy(g).cuda()
+        gv_dev = torch.empty(ctx.v_span, dtype=torch.float64, device="cuda")
+        ctx.level_vjp_device(gd.data_ptr(), gv_dev.data_ptr())
         stats = ctx.stats()
         _log(f"  normalize {stats.normalize_ms:.1f} ms, vjp {stats.vjp_ms:.1f} ms")
+    gv = gv_dev.cpu().numpy()
     seg = stats.refine_mode == capi.REFINE_SEGMENTED
-    eo = lo = 0
+    windows = bool(stats.scan_windows)
+    eo = lo = po = 0
+    psi_base = []
     for p, s in enumerate(specs):
         prob = problem_from_spec(s)
+        prob.v = tables[p]
         assert prob.usable
         nd = nodes[eo : eo + s.m]
         fr = far[eo : eo + s.m]
         rs = resc[eo : eo + s.m]
         lev = lv[lo : lo + counts[p]]
-        found = lev[st[lo : lo + counts[p]] == 0]
-        assert np.all(np.diff(nd) >= 0)
-        assert np.all(np.diff(found) > 0)
-        assert len(found) == min(counts[p], int(nd[-1]))
+        ok = st[lo : lo + counts[p]] == 0
+        found = lev[ok]
+        assert np.all(np.diff(nd) >= 0), p
+        assert np.all(np.diff(found) > 0), p
+        assert len(found) == min(counts[p], int(nd[-1])), p
+        assert np.all(ok[: len(found)]), p
         # Sampled trial energies: single oracle shots in the scan group's
         # band window, bit-exact.
-        windows = bool(stats.scan_windows)
-        for j in rng.choice(s.m, 24, replace=False):
+        for j in rng.choice(s.m, 6, replace=False):
             c, w, k = prob.scan_shot(int(j), windows)
             assert (c, k) == (int(nd[j]), int(rs[j])), (p, j)
             assert np.float64(w).view(np.int64) == fr[j].view(np.int64), (p, j)
         # Sampled levels: the oracle refines from the GPU's scan arrays.
-        for v in sorted(rng.choice(len(found), 4, replace=False)):
+        for v in sorted(rng.choice(len(found), 2, replace=False)):
             ref, its = prob.refine(nd, fr, rs, int(v), 1e-10, 200, segmented=seg, lanes=stats.refine_lanes,
                                    on=windows)
             assert np.float64(ref).view(np.int64) == lev[v].view(np.int64), (p, v)
             assert its == int(it[lo + v]), (p, v)
+        psi_base.append(po)
         eo += s.m
         lo += counts[p]
-    # Eigenfunctions of sampled levels of potential 0 and its adjoint.
-    prob = problem_from_spec(specs[0])
-    for v in (0, counts[0] // 3, len(lv[: counts[0]][st[: counts[0]] == 0]) - 1):
-        ref = oracle.eigenfunction(prob, lv[v])
-        np.testing.assert_allclose(psi0[v], ref, rtol=0, atol=1e-12 * np.abs(ref).max(), err_msg=f"level {v}")
-    lev0 = np.where(st[: counts[0]] == 0, lv[: counts[0]], np.nan)
-    ref = oracle.level_vjp(prob, lev0, g[: counts[0]])
-    np.testing.assert_allclose(gv[: specs[0].n], ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+        po += counts[p] * s.n
+    assert po == psi.numel()
+    # Eigenfunctions of sampled levels of sampled potentials against the
+    # oracle at the GPU's levels; unit norm on the grid.
+    lo_of = np.concatenate([[0], np.cumsum(counts)])
+    for p in sorted(rng.choice(len(specs), 4, replace=False)):
+        s = specs[p]
+        prob = problem_from_spec(s)
+        prob.v = tables[p]
+        nf = int(np.sum(st[lo_of[p] : lo_of[p + 1]] == 0))
+        for v in (0, nf // 2, nf - 1):
+            row = psi[psi_base[p] + v * s.n : psi_base[p] + (v + 1) * s.n].cpu().numpy()
+            ref = oracle.eigenfunction(prob, lv[lo_of[p] + v])
+            np.testing.assert_allclose(row, ref, rtol=0, atol=1e-12 * np.abs(ref).max(), err_msg=f"{p}/{v}")
+            assert abs(float(np.sum(row * row) * prob.h) - 1.0) < 1e-12
+    # The adjoint of sampled potentials against the oracle's.
+    off = np.concatenate([[0], np.cumsum([s.n for s in specs])])
+    for p in sorted(rng.choice(len(specs), 2, replace=False)):
+        s = specs[p]
+        prob = problem_from_spec(s)
+        prob.v = tables[p]
+        levp = np.where(st[lo_of[p] : lo_of[p + 1]] == 0, lv[lo_of[p] : lo_of[p + 1]], np.nan)
+        ref = oracle.level_vjp(prob, levp, g[lo_of[p] : lo_of[p + 1]])
+        np.testing.assert_allclose(gv[off[p] : off[p + 1]], ref, rtol=0, atol=1e-11 * np.abs(ref).max(),
+                                   err_msg=f"potential {p}")

@@ -29,6 +29,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <exception>
 #include <memory>
 #include <mutex>
@@ -38,6 +40,7 @@
 #include <string>
 #include <thread>
 #include <tuple>
+#include <type_traits>
 #include <utility>
 #include <variant>
 #include <vector>
@@ -212,6 +215,211 @@ struct TabulatedPotentialConfig {
     double asymptote = 0;
 };
 
+// Reference: PotentialFileLoader (potential_source.hpp:43-100), whose data
+// getter returns {} (:89-91) and whose file format the reference never
+// defines.  Format read here, one potential per file:
+//   * text: one grid point per line, "r V" (r in Angstrom, V in cm^-1,
+//     separated by spaces, tabs or one comma); blank lines and lines starting
+//     with '#' are skipped, except "# asymptote <value>", which sets the
+//     energy the levels are searched below (default: V at the last point);
+//   * .npy (by file name): a little-endian float64 array of shape (N, 2),
+//     C order, rows (r, V); asymptote: V at the last point.
+// The r column must be strictly increasing and uniform: every r_i within
+// 1e-4 h of r_0 + i h, h = (r_{N-1} - r_0)/(N - 1).  At least 4 points, all
+// values finite.
+struct LoadedPotential {
+    std::vector<double> r, v;
+    double asymptote = 0;
+};
+
+[[noreturn]] void file_error(const std::string& file, const std::string& what)
+{
+    throw std::runtime_error("Potential file " + file + ": " + what);
+}
+
+LoadedPotential parse_potential_text(const std::string& file)
+{
+    FILE* f = std::fopen(file.c_str(), "rb");
+    if (f == nullptr)
+        file_error(file, "cannot be opened");
+    std::string text;
+    char buf[1 << 16];
+    size_t got = 0;
+    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0)
+        text.append(buf, got);
+    std::fclose(f);
+    LoadedPotential out;
+    std::optional<double> asymptote;
+    size_t line_no = 0, pos = 0;
+    while (pos <= text.size()) {
+        size_t end = text.find('\n', pos);
+        if (end == std::string::npos)
+            end = text.size();
+        std::string line = text.substr(pos, end - pos);
+        pos = end + 1;
+        ++line_no;
+        if (!line.empty() && line.back() == '\r')
+            line.pop_back();
+        size_t b = line.find_first_not_of(" \t");
+        if (b == std::string::npos)
+            continue;
+        if (line[b] == '#') {
+            std::string rest = line.substr(b + 1);
+            size_t k = rest.find_first_not_of(" \t");
+            if (k != std::string::npos && rest.compare(k, 9, "asymptote") == 0) {
+                const char* c = rest.c_str() + k + 9;
+                char* stop = nullptr;
+                const double x = std::strtod(c, &stop);
+                if (stop == c || !std::isfinite(x))
+                    file_error(file, "line " + std::to_string(line_no) + ": bad asymptote value");
+                asymptote = x;
+            }
+            continue;
+        }
+        const char* c = line.c_str() + b;
+        char* stop = nullptr;
+        const double r = std::strtod(c, &stop);
+        if (stop == c)
+            file_error(file, "line " + std::to_string(line_no) + ": expected two numbers \"r V\"");
+        c = stop;
+        while (*c == ' ' || *c == '\t')
+            ++c;
+        if (*c == ',')
+            ++c;
+        const double v = std::strtod(c, &stop);
+        if (stop == c)
+            file_error(file, "line " + std::to_string(line_no) + ": expected two numbers \"r V\"");
+        c = stop;
+        while (*c == ' ' || *c == '\t')
+            ++c;
+        if (*c != '\0')
+            file_error(file, "line " + std::to_string(line_no) + ": unexpected text after \"r V\"");
+        out.r.push_back(r);
+        out.v.push_back(v);
+    }
+    if (out.v.empty())
+        file_error(file, "no data lines");
+    out.asymptote = asymptote.value_or(out.v.back());
+    return out;
+}
+
+LoadedPotential parse_potential_npy(const std::string& file)
+{
+    FILE* f = std::fopen(file.c_str(), "rb");
+    if (f == nullptr)
+        file_error(file, "cannot be opened");
+    std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
+    unsigned char magic[8];
+    if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "\x93NUMPY", 6) != 0)
+        file_error(file, "not a .npy file");
+    size_t header_len = 0;
+    if (magic[6] == 1) {
+        unsigned char l[2];
+        if (std::fread(l, 1, 2, f) != 2)
+            file_error(file, "truncated header");
+        header_len = l[0] | (size_t(l[1]) << 8);
+    } else if (magic[6] == 2 || magic[6] == 3) {
+        unsigned char l[4];
+        if (std::fread(l, 1, 4, f) != 4)
+            file_error(file, "truncated header");
+        header_len = l[0] | (size_t(l[1]) << 8) | (size_t(l[2]) << 16) | (size_t(l[3]) << 24);
+    } else {
+        file_error(file, "unsupported .npy version");
+    }
+    std::string header(header_len, '\0');
+    if (std::fread(header.data(), 1, header_len, f) != header_len)
+        file_error(file, "truncated header");
+    auto field = [&](const char* key) -> std::string {
+        const size_t k = header.find(std::string("'") + key + "'");
+        if (k == std::string::npos)
+            file_error(file, std::string("header lacks '") + key + "'");
+        size_t c = header.find(':', k);
+        return c == std::string::npos ? std::string() : header.substr(c + 1);
+    };
+    const std::string descr = field("descr");
+    if (descr.find("'<f8'") == std::string::npos)
+        file_error(file, "the array must be little-endian float64 ('<f8')");
+    if (field("fortran_order").find("False") == std::string::npos)
+        file_error(file, "the array must be in C order");
+    const std::string shape = field("shape");
+    const size_t open = shape.find('('), close = shape.find(')');
+    if (open == std::string::npos || close == std::string::npos)
+        file_error(file, "bad shape");
+    long long rows = -1, cols = -1;
+    if (std::sscanf(shape.substr(open + 1, close - open - 1).c_str(), "%lld , %lld", &rows, &cols) != 2 ||
+        cols != 2 || rows < 1)
+        file_error(file, "the array must have shape (N, 2)");
+    std::vector<double> data(static_cast<size_t>(rows) * 2);
+    if (std::fread(data.data(), sizeof(double), data.size(), f) != data.size())
+        file_error(file, "truncated data");
+    LoadedPotential out;
+    out.r.resize(static_cast<size_t>(rows));
+    out.v.resize(static_cast<size_t>(rows));
+    for (size_t i = 0; i < static_cast<size_t>(rows); ++i) {
+        out.r[i] = data[2 * i];
+        out.v[i] = data[2 * i + 1];
+    }
+    out.asymptote = out.v.back();
+    return out;
+}
+
+bool ends_with(const std::string& s, const std::string& suffix)
+{
+    return s.size() >= suffix.size() && s.compare(s.size() - suffix.size(), suffix.size(), suffix) == 0;
+}
+
+LoadedPotential read_potential_file(const std::string& file)
+{
+    LoadedPotential p = ends_with(file, ".npy") ? parse_potential_npy(file) : parse_potential_text(file);
+    const size_t n = p.v.size();
+    if (n < 4)
+        file_error(file, "needs at least 4 grid points, has " + std::to_string(n));
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p.r[i]) || !std::isfinite(p.v[i]))
+            file_error(file, "point " + std::to_string(i) + " is not finite");
+    const double h = (p.r[n - 1] - p.r[0]) / static_cast<double>(n - 1);
+    if (!(h > 0.0))
+        file_error(file, "r must increase");
+    for (size_t i = 0; i < n; ++i)
+        if (std::fabs(p.r[i] - (p.r[0] + static_cast<double>(i) * h)) > 1e-4 * h)
+            file_error(file, "r is not a uniform grid (point " + std::to_string(i) + ")");
+    return p;
+}
+
+class PotentialFileLoader {
+  public:
+    PotentialFileLoader() = default;
+    explicit PotentialFileLoader(std::vector<std::string> file_names) : file_names_(std::move(file_names)) {}
+
+    const std::vector<std::string>& file_names() const { return file_names_; }
+
+    // Reference: equals (potential_source.hpp:80-87): same file names.
+    bool equals(const PotentialFileLoader& other) const { return file_names_ == other.file_names_; }
+
+    // Reference: get_potential_data (potential_source.hpp:89-91): the V
+    // values of every file, in file order.
+    std::vector<std::vector<double>> get_potential_data() const
+    {
+        std::vector<std::vector<double>> out;
+        out.reserve(file_names_.size());
+        for (const std::string& f : file_names_)
+            out.push_back(read_potential_file(f).v);
+        return out;
+    }
+
+    std::vector<LoadedPotential> read() const
+    {
+        std::vector<LoadedPotential> out;
+        out.reserve(file_names_.size());
+        for (const std::string& f : file_names_)
+            out.push_back(read_potential_file(f));
+        return out;
+    }
+
+  private:
+    std::vector<std::string> file_names_;
+};
+
 // One potential after the configurator's precision has been applied.
 struct PotentialSpec {
     enum class Kind { Morse, DoubleWell, Tabulated } kind = Kind::Morse;
@@ -288,29 +496,41 @@ uint32_t point_count(const PotentialSpec& s)
                                                     : s.point_count;
 }
 
+// Scan results of every trial energy of a task, potential-major.
+struct ScanArrays {
+    std::vector<int32_t> nodes, rescales;
+    std::vector<double> far_end;
+};
+
+using EigenRows = std::shared_ptr<const std::vector<double>>;
+
 struct TaskResults {
     int32_t n_potentials = 0;
     int32_t level_count = 0;
     std::vector<double> levels;
     std::vector<int32_t> level_status;
     std::vector<int32_t> iterations;
-    std::vector<std::vector<int32_t>> nodes;
-    std::vector<std::vector<double>> far_end;
-    std::vector<std::vector<int32_t>> rescales;
+    std::vector<int64_t> energy_offsets;  // first trial energy of each potential, plus the total
+    std::vector<int64_t> point_counts;    // grid points of each potential
     std::vector<int32_t> strides;
     std::vector<int32_t> potential_flags;
     std::vector<epseon_bracket> brackets;  // per level, potential-major
-    double scan_ms = 0, refine_ms = 0, total_ms = 0;
+    double solve_ms = 0, total_ms = 0;
     double scan_point_updates = 0, refine_point_updates = 0;
     bool has_eigenfunctions = false;
-    std::vector<std::vector<double>> eigenfunctions;  // per potential, level_count x n_points
-    std::vector<int64_t> eigenfunction_points;
+    // Filled group by group when the task ran in several groups (the device
+    // context is reloaded per group); for a one-group task read from the
+    // context the task keeps, on first request.
+    std::shared_ptr<const ScanArrays> scan;
+    std::vector<EigenRows> eigenfunctions;  // per potential, level_count x n_points
 };
 
 class TaskCancelled : public std::exception {
   public:
     const char* what() const noexcept override { return "task cancelled"; }
 };
+
+using CtxPtr = std::unique_ptr<epseon_ctx, int (*)(epseon_ctx*)>;
 
 class Task {
   public:
@@ -339,6 +559,8 @@ class Task {
     bool is_started() const { return started_.load(std::memory_order_acquire); }
     bool is_running() const { return is_started() && !is_done(); }
 
+    // Reference: TaskHandle::cancel (task_handle.hpp:136-144): a stop request
+    // the worker honours at its next phase boundary.
     bool cancel()
     {
         if (is_running())
@@ -373,6 +595,48 @@ class Task {
         return results_;
     }
 
+    // Scan arrays of the whole task (read from the device on first request
+    // when the task ran as one group).
+    std::shared_ptr<const ScanArrays> scan_arrays()
+    {
+        const TaskResults& r = results();
+        std::lock_guard<std::mutex> lk(fetch_mutex_);
+        if (!results_.scan) {
+            auto sa = std::make_shared<ScanArrays>();
+            const size_t ne = static_cast<size_t>(r.energy_offsets.back());
+            sa->nodes.resize(ne);
+            sa->far_end.resize(ne);
+            sa->rescales.resize(ne);
+            check_capi(epseon_get_scan(ctx_.get(), sa->nodes.data(), sa->far_end.data(), sa->rescales.data()),
+                       "epseon_get_scan");
+            results_.scan = std::move(sa);
+        }
+        return results_.scan;
+    }
+
+    // Eigenfunctions of potential p (read from the device on first request
+    // when the task ran as one group).
+    EigenRows eigenfunctions(size_t p)
+    {
+        const TaskResults& r = results();
+        if (!r.has_eigenfunctions)
+            throw std::runtime_error(
+                "Eigenfunctions were not requested: call set_eigenfunction_output(True) before submit_task.");
+        if (p >= r.eigenfunctions.size())
+            throw py::index_error("potential index out of range");
+        std::lock_guard<std::mutex> lk(fetch_mutex_);
+        if (!results_.eigenfunctions[p]) {
+            auto rows = std::make_shared<std::vector<double>>(static_cast<size_t>(r.level_count) *
+                                                              static_cast<size_t>(r.point_counts[p]));
+            if (!rows->empty())
+                check_capi(epseon_get_wavefunctions(ctx_.get(), static_cast<int32_t>(p), rows->data(),
+                                                    EPSEON_MEM_HOST),
+                           "epseon_get_wavefunctions");
+            results_.eigenfunctions[p] = std::move(rows);
+        }
+        return results_.eigenfunctions[p];
+    }
+
     Precision precision() const { return spec_.precision; }
 
   private:
@@ -395,12 +659,15 @@ class Task {
             set_status("done");
         } catch (const TaskCancelled&) {
             cancelled_ = true;
+            ctx_.reset();
             set_status("cancelled");
         } catch (const std::exception& e) {
             error_ = std::make_exception_ptr(std::runtime_error(e.what()));
+            ctx_.reset();
             set_status(std::string("failed: ") + e.what());
         } catch (...) {
             error_ = std::make_exception_ptr(std::runtime_error("unknown error in task worker"));
+            ctx_.reset();
             set_status("failed: unknown error");
         }
         done_.store(true, std::memory_order_release);
@@ -438,7 +705,8 @@ class Task {
                     std::to_string(s.hardware.potential_buffer_size) + ".");
         const int64_t default_m =
             s.options.trial_energies.value_or(std::max<int64_t>(256, ((16 * level_count + 63) / 64) * 64));
-        const size_t group = s.hardware.group_size ? s.hardware.group_size : np;
+        const size_t group = s.hardware.group_size ? s.hardware.group_size : std::max<size_t>(np, 1);
+        const bool one_group = np <= group;
 
         TaskResults r;
         r.n_potentials = static_cast<int32_t>(np);
@@ -447,22 +715,33 @@ class Task {
         r.level_status.assign(np * level_count, EPSEON_LEVEL_UNBOUND);
         r.iterations.assign(np * level_count, 0);
         r.brackets.assign(np * level_count, epseon_bracket{});
-        r.nodes.resize(np);
-        r.far_end.resize(np);
-        r.rescales.resize(np);
         r.strides.assign(np, 0);
         r.potential_flags.assign(np, 0);
+        r.point_counts.assign(np, 0);
+        r.energy_offsets.assign(np + 1, 0);
+        for (size_t p = 0; p < np; ++p) {
+            r.point_counts[p] = point_count(s.potentials[p]);
+            r.energy_offsets[p + 1] =
+                r.energy_offsets[p] + (s.options.trial_energy_counts.empty() ? default_m
+                                                                             : s.options.trial_energy_counts[p]);
+        }
         r.has_eigenfunctions = s.options.eigenfunctions;
-        if (r.has_eigenfunctions) {
-            r.eigenfunctions.resize(np);
-            r.eigenfunction_points.assign(np, 0);
+        if (r.has_eigenfunctions)
+            r.eigenfunctions.assign(np, nullptr);
+        std::shared_ptr<ScanArrays> scan;
+        if (!one_group) {
+            scan = std::make_shared<ScanArrays>();
+            const size_t ne = static_cast<size_t>(r.energy_offsets.back());
+            scan->nodes.resize(ne);
+            scan->far_end.resize(ne);
+            scan->rescales.resize(ne);
         }
 
         set_status("opening device");
         epseon_ctx* raw = nullptr;
         check_capi(epseon_ctx_create(device_, s.hardware.allocation_block_size, &raw),
                    "epseon_ctx_create");
-        std::unique_ptr<epseon_ctx, int (*)(epseon_ctx*)> ctx(raw, epseon_ctx_destroy);
+        CtxPtr ctx(raw, epseon_ctx_destroy);
 
         const auto t_start = std::chrono::steady_clock::now();
         for (size_t g0 = 0; g0 < np; g0 += group) {
@@ -477,13 +756,13 @@ class Task {
                 total += point_count(s.potentials[p]);
             }
             std::vector<double> v(static_cast<size_t>(std::max<int64_t>(total, 1)));
-            tabulate(g0, g1, offsets, v);
+            tabulate(g0, g1, offsets, v, st);
+            poll(st);
             for (size_t p = g0; p < g1; ++p) {
                 const PotentialSpec& ps = s.potentials[p];
                 epseon_potential_desc& d = desc[p - g0];
                 d.n_points = point_count(ps);
-                d.n_energies = s.options.trial_energy_counts.empty() ? default_m
-                                                                     : s.options.trial_energy_counts[p];
+                d.n_energies = r.energy_offsets[p + 1] - r.energy_offsets[p];
                 d.r_min = ps.min_r;
                 d.r_max = ps.max_r;
                 d.reduced_mass = s.options.reduced_masses.empty() ? mu : s.options.reduced_masses[p];
@@ -491,86 +770,81 @@ class Task {
                 d.level_min = static_cast<int32_t>(s.vibwa.min_level);
                 d.level_count = level_count;
             }
-            poll(st);
             set_status("uploading potentials");
             check_capi(epseon_load_potentials(ctx.get(), static_cast<int32_t>(g1 - g0), desc.data(),
                                               v.data(), offsets.data()),
                        "epseon_load_potentials");
             poll(st);
-            set_status("scanning trial energies");
-            check_capi(epseon_scan(ctx.get()), "epseon_scan");
-            check_capi(epseon_ctx_synchronize(ctx.get()), "epseon_ctx_synchronize");
-            poll(st);
-            set_status("refining levels");
-            check_capi(epseon_refine(ctx.get(), s.options.rel_tol, s.options.max_iterations),
-                       "epseon_refine");
+            set_status("solving: scan, brackets, refinement");
+            check_capi(epseon_solve(ctx.get(), s.options.rel_tol, s.options.max_iterations), "epseon_solve");
             set_status("reading results");
-            const size_t nl = (g1 - g0) * static_cast<size_t>(level_count);
-            check_capi(epseon_get_levels(ctx.get(), r.levels.data() + g0 * level_count,
-                                         r.level_status.data() + g0 * level_count,
-                                         r.iterations.data() + g0 * level_count),
+            const size_t l0 = g0 * static_cast<size_t>(level_count);
+            check_capi(epseon_get_levels(ctx.get(), r.levels.data() + l0, r.level_status.data() + l0,
+                                         r.iterations.data() + l0),
                        "epseon_get_levels");
-            check_capi(epseon_get_brackets(ctx.get(), r.brackets.data() + g0 * level_count),
-                       "epseon_get_brackets");
-            (void)nl;
-            int64_t ne = 0;
-            for (size_t p = g0; p < g1; ++p)
-                ne += desc[p - g0].n_energies;
-            std::vector<int32_t> nodes(static_cast<size_t>(ne)), resc(static_cast<size_t>(ne));
-            std::vector<double> far(static_cast<size_t>(ne));
-            check_capi(epseon_get_scan(ctx.get(), nodes.data(), far.data(), resc.data()),
-                       "epseon_get_scan");
-            int64_t o = 0;
+            check_capi(epseon_get_brackets(ctx.get(), r.brackets.data() + l0), "epseon_get_brackets");
             for (size_t p = g0; p < g1; ++p) {
-                const int64_t m = desc[p - g0].n_energies;
-                r.nodes[p].assign(nodes.begin() + o, nodes.begin() + o + m);
-                r.far_end[p].assign(far.begin() + o, far.begin() + o + m);
-                r.rescales[p].assign(resc.begin() + o, resc.begin() + o + m);
-                o += m;
                 epseon_potential_info info{};
                 check_capi(epseon_potential_info_get(ctx.get(), static_cast<int32_t>(p - g0), &info),
                            "epseon_potential_info_get");
                 r.strides[p] = info.stride;
                 r.potential_flags[p] = info.flags;
             }
+            epseon_stats stats{};
+            check_capi(epseon_get_stats(ctx.get(), &stats), "epseon_get_stats");
+            r.solve_ms += stats.solve_ms;
+            r.scan_point_updates += stats.scan_point_updates;
+            r.refine_point_updates += stats.refine_point_updates;
+            if (one_group)
+                break;
+            // Several groups: this group's arrays are read before the next load.
+            const size_t e0 = static_cast<size_t>(r.energy_offsets[g0]);
+            check_capi(epseon_get_scan(ctx.get(), scan->nodes.data() + e0, scan->far_end.data() + e0,
+                                       scan->rescales.data() + e0),
+                       "epseon_get_scan");
             if (s.options.eigenfunctions) {
                 poll(st);
                 set_status("normalising eigenfunctions");
-                int64_t total_psi = 0;
-                check_capi(epseon_get_eigenfunction_size(ctx.get(), &total_psi),
-                           "epseon_get_eigenfunction_size");
-                std::vector<double> psi(static_cast<size_t>(std::max<int64_t>(total_psi, 1)));
-                check_capi(epseon_get_all_wavefunctions(ctx.get(), psi.data(), EPSEON_MEM_HOST),
-                           "epseon_get_all_wavefunctions");
-                int64_t q = 0;
                 for (size_t p = g0; p < g1; ++p) {
-                    const int64_t cnt = static_cast<int64_t>(level_count) * desc[p - g0].n_points;
-                    r.eigenfunctions[p].assign(psi.begin() + q, psi.begin() + q + cnt);
-                    r.eigenfunction_points[p] = desc[p - g0].n_points;
-                    q += cnt;
+                    auto rows = std::make_shared<std::vector<double>>(static_cast<size_t>(level_count) *
+                                                                      static_cast<size_t>(r.point_counts[p]));
+                    if (!rows->empty())
+                        check_capi(epseon_get_wavefunctions(ctx.get(), static_cast<int32_t>(p - g0), rows->data(),
+                                                            EPSEON_MEM_HOST),
+                                   "epseon_get_wavefunctions");
+                    r.eigenfunctions[p] = std::move(rows);
                 }
             }
-            epseon_stats stats{};
-            check_capi(epseon_get_stats(ctx.get(), &stats), "epseon_get_stats");
-            r.scan_ms += stats.scan_ms;
-            r.refine_ms += stats.refine_ms;
-            r.scan_point_updates += stats.scan_point_updates;
-            r.refine_point_updates += stats.refine_point_updates;
         }
+        if (one_group && s.options.eigenfunctions && np > 0) {
+            // Normalise now, so that reading the eigenfunctions later is a copy.
+            poll(st);
+            set_status("normalising eigenfunctions");
+            check_capi(epseon_normalize(ctx.get()), "epseon_normalize");
+            check_capi(epseon_ctx_synchronize(ctx.get()), "epseon_ctx_synchronize");
+        }
+        r.scan = scan;
         r.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
                          .count();
         results_ = std::move(r);
+        // A one-group task keeps its context: scan arrays and eigenfunctions
+        // are read from the device when first requested.
+        if (one_group)
+            ctx_ = std::move(ctx);
     }
 
     // V_i on each grid, host FP64 (std::exp), split over host threads for
-    // large batches.
-    void tabulate(size_t g0, size_t g1, const std::vector<int64_t>& offsets, std::vector<double>& v)
+    // large batches; the workers stop early when a stop is requested.
+    void tabulate(size_t g0, size_t g1, const std::vector<int64_t>& offsets, std::vector<double>& v,
+                  const std::stop_token& st)
     {
         const size_t n = g1 - g0;
         const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         const unsigned workers = static_cast<unsigned>(std::min<size_t>(hw, n));
         auto body = [&](unsigned w) {
             for (size_t q = w; q < n; q += workers) {
+                if (st.stop_requested())
+                    return;
                 const PotentialSpec& ps = spec_.potentials[g0 + q];
                 const uint32_t npts = point_count(ps);
                 const double h = (ps.max_r - ps.min_r) / static_cast<double>(npts - 1);
@@ -597,6 +871,8 @@ class Task {
     bool cancelled_ = false;
     std::exception_ptr error_;
     TaskResults results_;
+    CtxPtr ctx_{nullptr, epseon_ctx_destroy};
+    std::mutex fetch_mutex_;
     mutable std::mutex status_mutex_;
     std::string status_ = "pending";
     std::mutex join_mutex_;
@@ -604,6 +880,19 @@ class Task {
 };
 
 // ------------------------------------------------------ python wrappers
+
+// Read-only numpy view of `count` elements at `offset` of a shared buffer,
+// shaped `shape`; the array keeps the buffer alive.
+template <typename T>
+py::array_t<T> view_of(const std::shared_ptr<const std::vector<T>>& owner, size_t offset,
+                       const std::vector<py::ssize_t>& shape)
+{
+    auto* keep = new std::shared_ptr<const std::vector<T>>(owner);
+    py::capsule base(keep, [](void* q) { delete static_cast<std::shared_ptr<const std::vector<T>>*>(q); });
+    py::array_t<T> a(shape, owner->data() + offset, base);
+    a.attr("setflags")(py::arg("write") = false);
+    return a;
+}
 
 template <typename FP>
 class TaskHandle {
@@ -638,15 +927,18 @@ class TaskHandle {
 
     std::vector<py::array_t<int32_t>> get_node_counts() const
     {
-        return ragged(task_->results().nodes);
+        const auto sa = task_->scan_arrays();
+        return ragged(std::shared_ptr<const std::vector<int32_t>>(sa, &sa->nodes));
     }
     std::vector<py::array_t<double>> get_far_end_values() const
     {
-        return ragged(task_->results().far_end);
+        const auto sa = task_->scan_arrays();
+        return ragged(std::shared_ptr<const std::vector<double>>(sa, &sa->far_end));
     }
     std::vector<py::array_t<int32_t>> get_rescale_counts() const
     {
-        return ragged(task_->results().rescales);
+        const auto sa = task_->scan_arrays();
+        return ragged(std::shared_ptr<const std::vector<int32_t>>(sa, &sa->rescales));
     }
     std::vector<int32_t> get_node_strides() const { return task_->results().strides; }
 
@@ -675,22 +967,39 @@ class TaskHandle {
     }
     std::vector<int32_t> get_potential_flags() const { return task_->results().potential_flags; }
 
-    std::vector<py::array_t<FP>> get_wavefunctions() const
+    // One potential's eigenfunctions, (levels, points).  Float64 handles get
+    // a read-only view of the task's copy; float32 handles a rounded copy.
+    py::array_t<FP> wavefunctions_of(size_t p) const
+    {
+        const TaskResults& r = task_->results();
+        const EigenRows rows = task_->eigenfunctions(p);
+        const std::vector<py::ssize_t> shape{static_cast<py::ssize_t>(r.level_count),
+                                             static_cast<py::ssize_t>(r.point_counts[p])};
+        if constexpr (std::is_same_v<FP, double>) {
+            return view_of<double>(rows, 0, shape);
+        } else {
+            py::array_t<FP> a(shape);
+            FP* d = a.mutable_data();
+            for (size_t i = 0; i < rows->size(); ++i)
+                d[i] = static_cast<FP>((*rows)[i]);
+            return a;
+        }
+    }
+
+    py::object get_wavefunctions(std::optional<int64_t> potential) const
     {
         const TaskResults& r = task_->results();
         if (!r.has_eigenfunctions)
             throw std::runtime_error(
                 "Eigenfunctions were not requested: call set_eigenfunction_output(True) before submit_task.");
-        std::vector<py::array_t<FP>> out;
-        out.reserve(r.eigenfunctions.size());
-        for (size_t p = 0; p < r.eigenfunctions.size(); ++p) {
-            py::array_t<FP> a({static_cast<py::ssize_t>(r.level_count),
-                               static_cast<py::ssize_t>(r.eigenfunction_points[p])});
-            FP* d = a.mutable_data();
-            for (size_t i = 0; i < r.eigenfunctions[p].size(); ++i)
-                d[i] = static_cast<FP>(r.eigenfunctions[p][i]);
-            out.push_back(std::move(a));
+        if (potential.has_value()) {
+            if (*potential < 0 || *potential >= r.n_potentials)
+                throw py::index_error("potential index out of range");
+            return wavefunctions_of(static_cast<size_t>(*potential));
         }
+        py::list out;
+        for (size_t p = 0; p < static_cast<size_t>(r.n_potentials); ++p)
+            out.append(wavefunctions_of(p));
         return out;
     }
 
@@ -698,8 +1007,7 @@ class TaskHandle {
     {
         const TaskResults& r = task_->results();
         py::dict d;
-        d["scan_ms"] = r.scan_ms;
-        d["refine_ms"] = r.refine_ms;
+        d["solve_ms"] = r.solve_ms;
         d["total_ms"] = r.total_ms;
         d["scan_point_updates"] = r.scan_point_updates;
         d["refine_point_updates"] = r.refine_point_updates;
@@ -716,15 +1024,16 @@ class TaskHandle {
         return out;
     }
 
+    // One read-only view per potential into the task's scan array.
     template <typename T>
-    static std::vector<py::array_t<T>> ragged(const std::vector<std::vector<T>>& rows)
+    std::vector<py::array_t<T>> ragged(const std::shared_ptr<const std::vector<T>>& all) const
     {
+        const TaskResults& r = task_->results();
         std::vector<py::array_t<T>> out;
-        out.reserve(rows.size());
-        for (const auto& row : rows) {
-            py::array_t<T> a(static_cast<py::ssize_t>(row.size()));
-            std::copy(row.begin(), row.end(), a.mutable_data());
-            out.push_back(std::move(a));
+        out.reserve(static_cast<size_t>(r.n_potentials));
+        for (size_t p = 0; p < static_cast<size_t>(r.n_potentials); ++p) {
+            const int64_t e0 = r.energy_offsets[p];
+            out.push_back(view_of<T>(all, static_cast<size_t>(e0), {static_cast<py::ssize_t>(r.energy_offsets[p + 1] - e0)}));
         }
         return out;
     }
@@ -818,6 +1127,17 @@ class TaskConfigurator {
         }
         state_->potentials = std::move(v);
         return *this;
+    }
+
+    // Additive: potentials read from files (reference: the configurator's
+    // setPotentialSource with a PotentialFileLoader, task_configurator.hpp:
+    // 98-101).  The files are read now, so a bad file raises here.
+    TaskConfigurator& set_potential_file_loader(const PotentialFileLoader& loader)
+    {
+        std::vector<TabulatedPotentialConfig> v;
+        for (LoadedPotential& p : loader.read())
+            v.push_back(TabulatedPotentialConfig{std::move(p.v), p.r.front(), p.r.back(), p.asymptote});
+        return set_tabulated_potential(v);
     }
 
     // Reference: set_vibwa_algorithm (api.hpp:222-241).
@@ -1010,6 +1330,9 @@ void bind_configurator(py::module_& m, const char* name)
              "Set Morse potentials with a Gaussian barrier.", py::return_value_policy::reference)
         .def("set_tabulated_potential", &C::set_tabulated_potential, py::arg("configurations"),
              "Set tabulated potentials on uniform grids.", py::return_value_policy::reference)
+        .def("set_potential_file_loader", &C::set_potential_file_loader, py::arg("loader"),
+             "Set potentials read from the loader's files (text \"r V\" columns or (N, 2) float64 .npy).",
+             py::return_value_policy::reference)
         .def("set_trial_energies", &C::set_trial_energies, py::arg("count"),
              "Number of trial energies scanned per potential.", py::return_value_policy::reference)
         .def("set_trial_energy_counts", &C::set_trial_energy_counts, py::arg("counts"),
@@ -1048,9 +1371,9 @@ void bind_handle(py::module_& m, const char* name, const char* doc)
              "nodes_lo, nodes_hi, found.")
         .def("get_potential_flags", &H::get_potential_flags, "Per-potential flags (1 = unresolved).")
         .def("get_timings", &H::get_timings, "Device times and point-update counts.")
-        .def("get_wavefunctions", &H::get_wavefunctions,
-             "Normalised eigenfunctions, one (levels, points) array per potential; zero rows for absent "
-             "levels.");
+        .def("get_wavefunctions", &H::get_wavefunctions, py::arg("potential") = py::none(),
+             "Normalised eigenfunctions: a (levels, points) array for `potential`, or one per potential "
+             "when it is None; zero rows for absent levels.  Read from the device on first request.");
 }
 
 }  // namespace epseon::gpu
@@ -1140,6 +1463,26 @@ PYBIND11_MODULE(_libepseon_gpu, m)
                  return TabulatedPotentialConfig{std::move(values), min_r, max_r, asymptote};
              }),
              py::arg("values"), py::arg("min_r"), py::arg("max_r"), py::arg("asymptote"));
+
+    py::class_<PotentialFileLoader>(m, "PotentialFileLoader",
+                                    "Potential source reading one tabulated V(r) per file.")
+        .def(py::init<>())
+        .def(py::init<std::vector<std::string>>(), py::arg("file_names"))
+        .def_property_readonly("file_names", &PotentialFileLoader::file_names)
+        .def("equals", &PotentialFileLoader::equals, py::arg("other"))
+        .def("__eq__", &PotentialFileLoader::equals)
+        .def(
+            "get_potential_data",
+            [](const PotentialFileLoader& l) {
+                std::vector<py::array_t<double>> out;
+                for (const std::vector<double>& v : l.get_potential_data()) {
+                    py::array_t<double> a(static_cast<py::ssize_t>(v.size()));
+                    std::copy(v.begin(), v.end(), a.mutable_data());
+                    out.push_back(std::move(a));
+                }
+                return out;
+            },
+            "V values of every file, one array per file.");
 
     bind_configurator<float>(m, "TaskConfiguratorFloat32");
     bind_configurator<double>(m, "TaskConfiguratorFloat64");

@@ -35,11 +35,11 @@ class PhysicalDeviceProperties:
 
 class MemoryHeap:
     size: int
-    flags: int
+    flags: list[str]
 
 class MemoryType:
     heap_index: int
-    flags: int
+    flags: list[str]
 
 class PhysicalDeviceMemoryProperties:
     memory_heaps: list[MemoryHeap]
@@ -83,6 +83,17 @@ class DoubleWellPotentialConfig:  # additive
 class TabulatedPotentialConfig:  # additive
     def __init__(self, values: Sequence[float], min_r: float, max_r: float, asymptote: float) -> None: ...
 
+class PotentialFileLoader:
+    """One tabulated V(r) per file: text "r V" lines (optional "# asymptote
+    <x>") or a (N, 2) float64 .npy; uniform r grid, at least 4 points."""
+    file_names: list[str]
+    @overload
+    def __init__(self) -> None: ...
+    @overload
+    def __init__(self, file_names: Sequence[str]) -> None: ...
+    def equals(self, other: PotentialFileLoader) -> bool: ...
+    def get_potential_data(self) -> list[npt.NDArray[np.float64]]: ...
+
 class _TaskConfigurator:
     def __init__(self) -> None: ...
     def set_hardware_config(
@@ -103,6 +114,7 @@ class _TaskConfigurator:
     def set_morse_potential_batch(self, configurations: Sequence[MorsePotentialConfig]) -> _TaskConfigurator: ...
     def set_double_well_potential(self, configurations: Sequence[DoubleWellPotentialConfig]) -> _TaskConfigurator: ...
     def set_tabulated_potential(self, configurations: Sequence[TabulatedPotentialConfig]) -> _TaskConfigurator: ...
+    def set_potential_file_loader(self, loader: PotentialFileLoader) -> _TaskConfigurator: ...
     def set_trial_energies(self, count: int) -> _TaskConfigurator: ...
     def set_trial_energy_counts(self, counts: Sequence[int]) -> _TaskConfigurator: ...
     def set_reduced_masses(self, masses: Sequence[float]) -> _TaskConfigurator: ...
@@ -125,16 +137,23 @@ class _TaskHandle:
     def get_far_end_values(self) -> list[npt.NDArray[np.float64]]: ...
     def get_rescale_counts(self) -> list[npt.NDArray[np.int32]]: ...
     def get_node_strides(self) -> list[int]: ...
+    def get_brackets(self) -> dict[str, npt.NDArray]: ...
     def get_potential_flags(self) -> list[int]: ...
     def get_timings(self) -> dict[str, float]: ...
 
 class TaskHandleFloat32(_TaskHandle):
     def get_levels(self) -> npt.NDArray[np.float32]: ...  # additive
-    def get_wavefunctions(self) -> list[npt.NDArray[np.float32]]: ...  # additive
+    @overload
+    def get_wavefunctions(self, potential: None = None) -> list[npt.NDArray[np.float32]]: ...  # additive
+    @overload
+    def get_wavefunctions(self, potential: int) -> npt.NDArray[np.float32]: ...
 
 class TaskHandleFloat64(_TaskHandle):
     def get_levels(self) -> npt.NDArray[np.float64]: ...  # additive
-    def get_wavefunctions(self) -> list[npt.NDArray[np.float64]]: ...  # additive
+    @overload
+    def get_wavefunctions(self, potential: None = None) -> list[npt.NDArray[np.float64]]: ...  # additive
+    @overload
+    def get_wavefunctions(self, potential: int) -> npt.NDArray[np.float64]: ...
 
 class ComputeDeviceInterface:
     device_id: int

@@ -451,3 +451,38 @@ def test_drop_in_tabulated_double_wells_match_oracle() -> None:
         assert ok.sum() >= 10
         np.testing.assert_array_equal(status[p][ok], 0)
         np.testing.assert_array_equal(got_levels[p][ok].view(np.int64), olv[ok].view(np.int64))
+
+
+def test_stub_declares_exactly_the_bound_members() -> None:
+    """The .pyi stub names every public class and method the module binds,
+    and nothing it does not bind."""
+    import ast
+
+    m = _gpu()
+    stub = Path(m.__file__).with_name("_libepseon_gpu.pyi")
+    tree = ast.parse(stub.read_text())
+    declared: dict[str, set[str]] = {}
+    bases: dict[str, list[str]] = {}
+    for node in tree.body:
+        if isinstance(node, ast.ClassDef):
+            names = set()
+            for item in node.body:
+                if isinstance(item, ast.FunctionDef) and not item.name.startswith("__"):
+                    names.add(item.name)
+                elif isinstance(item, ast.AnnAssign) and isinstance(item.target, ast.Name):
+                    names.add(item.target.id)
+            declared[node.name] = names
+            bases[node.name] = [b.id for b in node.bases if isinstance(b, ast.Name)]
+
+    def all_names(cls: str) -> set[str]:
+        out = set(declared.get(cls, set()))
+        for b in bases.get(cls, []):
+            out |= all_names(b)
+        return out
+
+    public = {k for k in declared if not k.startswith("_")}
+    bound = {k for k in dir(m) if not k.startswith("_") and isinstance(getattr(m, k), type)}
+    assert public == bound, (public ^ bound)
+    for cls in public:
+        have = {k for k in dir(getattr(m, cls)) if not k.startswith("_")}
+        assert all_names(cls) == have, (cls, all_names(cls) ^ have)

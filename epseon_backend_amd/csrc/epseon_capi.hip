@@ -294,6 +294,11 @@ struct epseon_ctx {
     double* d_segtab_full = nullptr;
     double* d_segtab_band = nullptr;
     bool segtab_ok = false;  // every table fits one 32-bit-addressed buffer
+    // Jobs of both table sets (whole-grid first, then the band windows'); a
+    // set is built when a refinement first needs it.
+    TableJob* d_table_jobs = nullptr;
+    int32_t n_jobs_full = 0, n_jobs_band = 0, table_max_rows = 0;
+    bool tables_full = false, tables_band = false;
     int32_t* d_steps = nullptr;  // recurrence steps of each level's shots (its window's n - 2)
     Bracket* d_brackets = nullptr;  // each level's bracket from the last scan
     PotentialRecord* d_rec = nullptr;
@@ -576,6 +581,25 @@ int epseon_ctx_destroy(epseon_ctx* ctx)
 }
 
 namespace {
+
+int32_t choose_refine_mode(const epseon_ctx* ctx);
+int32_t segmented_lanes(const epseon_ctx* ctx);
+bool windows_for(const epseon_ctx* ctx, int32_t mode);
+
+// Build the segment tables of the band windows (band) or of the whole
+// grids on `stream`, once per load.
+hipError_t build_tables(epseon_ctx* ctx, hipStream_t stream, bool band)
+{
+    bool& built = band ? ctx->tables_band : ctx->tables_full;
+    if (built || !ctx->segtab_ok)
+        return hipSuccess;
+    built = true;
+    if (band)
+        return launch_segment_tables(stream, ctx->d_uv, ctx->d_rec, ctx->d_table_jobs + ctx->n_jobs_full,
+                                     ctx->n_jobs_band, ctx->table_max_rows, ctx->d_segtab_band);
+    return launch_segment_tables(stream, ctx->d_uv, ctx->d_rec, ctx->d_table_jobs, ctx->n_jobs_full,
+                                 ctx->table_max_rows, ctx->d_segtab_full);
+}
 
 // Shared body of epseon_load_potentials (v_host) and
 // epseon_load_potentials_device (v_dev).
@@ -1145,6 +1169,11 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     EPSEON_HIP_TRY(arena_array(a, static_cast<size_t>(std::max<int64_t>(band_total, 1)) * 64, &ctx->d_segtab_band));
     TableJob* d_jobs = nullptr;
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(jobs_full.size() + jobs_band.size(), 1), &d_jobs));
+    ctx->d_table_jobs = d_jobs;
+    ctx->n_jobs_full = static_cast<int32_t>(jobs_full.size());
+    ctx->n_jobs_band = static_cast<int32_t>(jobs_band.size());
+    ctx->table_max_rows = max_rows;
+    ctx->tables_full = ctx->tables_band = false;
     if (count > 0) {
         EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_win_full, win_full.data(), win_full.size() * sizeof(BandWindow),
                                       hipMemcpyHostToDevice, ctx->stream));
@@ -1159,17 +1188,16 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
                                       jobs_band.size() * sizeof(TableJob), hipMemcpyHostToDevice,
                                       ctx->stream));
     if (ctx->segtab_ok && jobs_full.size() + jobs_band.size() > 0) {
-        // Built on stream2, overlapping the next scan; the refinement waits.
-        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
-        EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
-        EPSEON_HIP_TRY(launch_segment_tables(ctx->stream2, ctx->d_uv, ctx->d_rec, d_jobs,
-                                             static_cast<int32_t>(jobs_full.size()), max_rows,
-                                             ctx->d_segtab_full));
-        EPSEON_HIP_TRY(launch_segment_tables(ctx->stream2, ctx->d_uv, ctx->d_rec, d_jobs + jobs_full.size(),
-                                             static_cast<int32_t>(jobs_band.size()), max_rows,
-                                             ctx->d_segtab_band));
-        EPSEON_HIP_TRY(hipEventRecord(ctx->ev_tables, ctx->stream2));
-        ctx->tables_pending = true;
+        // The set the current tuning's solve will use, built on stream2 while
+        // the next scan runs; the refinement waits for it (tables_pending).
+        const int32_t mode = choose_refine_mode(ctx);
+        if (mode == EPSEON_REFINE_SEGMENTED && segmented_lanes(ctx) == 64) {
+            EPSEON_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+            EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+            EPSEON_HIP_TRY(build_tables(ctx, ctx->stream2, windows_for(ctx, mode)));
+            EPSEON_HIP_TRY(hipEventRecord(ctx->ev_tables, ctx->stream2));
+            ctx->tables_pending = true;
+        }
     }
     // The host buffers are pageable and owned by the caller: finish the
     // copies before returning.
@@ -1297,13 +1325,19 @@ hipError_t run_scan(epseon_ctx* ctx, hipStream_t stream, size_t k0, size_t k1)
                        ctx->waves_per_block, ctx->d_nodes, ctx->d_far, ctx->d_resc);
 }
 
-// Make `stream` wait for the segment tables of the last load (once).
-hipError_t wait_tables(epseon_ctx* ctx, hipStream_t stream)
+// Make `stream` wait for the segment tables of the last load (once), and,
+// for a refinement in 64-lane segmented blocks (`mode`, `lanes`), build the
+// set the last scan's windows need if the load did not.
+hipError_t wait_tables(epseon_ctx* ctx, hipStream_t stream, int32_t mode, int32_t lanes)
 {
-    if (!ctx->tables_pending)
+    if (ctx->tables_pending) {
+        ctx->tables_pending = false;
+        if (const hipError_t e = hipStreamWaitEvent(stream, ctx->ev_tables, 0); e != hipSuccess)
+            return e;
+    }
+    if (mode != EPSEON_REFINE_SEGMENTED || lanes != 64)
         return hipSuccess;
-    ctx->tables_pending = false;
-    return hipStreamWaitEvent(stream, ctx->ev_tables, 0);
+    return build_tables(ctx, stream, ctx->scan_windows);
 }
 
 // Refinement of the levels of potentials [k0, k1) in task order, in the
@@ -1414,7 +1448,7 @@ int capture_solve(epseon_ctx* ctx, int32_t mode, int32_t lanes, double rel_tol, 
     }
     const size_t np = n_potentials(ctx);
     // No wait on an event recorded outside the capture inside it.
-    EPSEON_HIP_TRY(wait_tables(ctx, ctx->stream));
+    EPSEON_HIP_TRY(wait_tables(ctx, ctx->stream, mode, lanes));
     EPSEON_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
     hipError_t e = run_scan(ctx, ctx->stream, 0, np);
     if (e == hipSuccess)
@@ -1468,7 +1502,7 @@ int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     const int32_t lanes = segmented_lanes(ctx);
     ctx->last_refine_mode = mode;
     ctx->last_refine_lanes = mode == EPSEON_REFINE_LANE ? 1 : lanes;
-    EPSEON_HIP_TRY(wait_tables(ctx, ctx->stream));
+    EPSEON_HIP_TRY(wait_tables(ctx, ctx->stream, mode, lanes));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
     if (mode == EPSEON_REFINE_LANE && ctx->levels_per_lane == 2) {
         // Paired and single lane tasks run concurrently on the two streams.
@@ -1561,7 +1595,7 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
         } else {
             EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve0, ctx->stream));
             EPSEON_HIP_TRY(run_scan(ctx, ctx->stream, 0, np));
-            EPSEON_HIP_TRY(wait_tables(ctx, ctx->stream));
+            EPSEON_HIP_TRY(wait_tables(ctx, ctx->stream, mode, lanes));
             EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream, mode, lanes, rel_tol, max_iter, 0, np));
             EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve1, ctx->stream));
         }
@@ -1600,7 +1634,7 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     EPSEON_HIP_TRY(run_scan(ctx, ctx->stream, 0, k));
     EPSEON_HIP_TRY(run_scan(ctx, ctx->stream2, k, np));
     // stream2 built the tables before its scan; the first stream waits.
-    EPSEON_HIP_TRY(wait_tables(ctx, ctx->stream));
+    EPSEON_HIP_TRY(wait_tables(ctx, ctx->stream, mode, lanes));
     EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream, mode, lanes, rel_tol, max_iter, 0, k));
     EPSEON_HIP_TRY(run_refine(ctx, ctx->stream2, ctx->stream2, mode, lanes, rel_tol, max_iter, k, np));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev_join, ctx->stream2));

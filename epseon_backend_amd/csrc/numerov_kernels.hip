@@ -1165,7 +1165,10 @@ __device__ __forceinline__ void walk_segment(const Src& src, const int32_t len, 
 
 // Two difference-form columns through one segment: the segment's transfer
 // matrix on (w, d).  All six carried values are renormalised after every
-// 32nd step.
+// 128th step: a step grows them by less than 2^2.87 and shrinks the largest
+// by less than 2^4.5 (its matrix has determinant f_{i-1}/f_{i+1} > 1/3 and
+// norm below 7.4), so 128 steps stay inside the normal range, and scaling by
+// a power of two is exact.
 struct MatrixBody {
     double e, em;
     double wa, da, pa, wb, db, pb;
@@ -1183,7 +1186,7 @@ struct MatrixBody {
         wa = wa + da;
         wb = wb + db;
         em = ep;
-        if ((idx & 31) == 31) {
+        if ((idx & 127) == 127) {
             const double mx = fmax(fmax(fmax(fabs(wa), fabs(da)), fmax(fabs(pa), fabs(wb))),
                                    fmax(fabs(db), fabs(pb)));
             const int32_t sh = frexp_exponent(mx);
@@ -1239,7 +1242,7 @@ struct CountBody {
             sc.last = sb;
             sc.has = 1;
         }
-        if ((idx & 31) == 31) {
+        if ((idx & 127) == 127) {
             const int32_t sh = frexp_exponent(fmax(fmax(fabs(w), fabs(d)), fabs(wm)));
             w = __builtin_ldexp(w, -sh);
             d = __builtin_ldexp(d, -sh);
@@ -1262,12 +1265,19 @@ struct CountPair {
 // Segment tables of the 64-lane segmented refinement: for job J, rows
 // r < seg_rows(J.n) of 64 columns from row J.seg_row, column s holding the
 // window's uv at point min(q0_s + r, n - 1) for the segments s < seg_count(n)
-// (zero in the unused columns).
+// (zero in the unused columns).  One workgroup per 64-row tile of a job: each
+// wavefront reads 64 consecutive points of one segment per load (512
+// contiguous bytes), the tile is transposed through LDS, and each wavefront
+// then stores 64 consecutive values of one row.
+constexpr int32_t kTableTile = 64;
 __global__ __launch_bounds__(256) void segment_table_kernel(const double* __restrict__ uv,
                                                             const PotentialRecord* __restrict__ pots,
                                                             const TableJob* __restrict__ jobs,
                                                             const int32_t n_jobs, double* __restrict__ segtab)
 {
+    __shared__ double tile[64][kTableTile + 1];
+    const int32_t lane = static_cast<int32_t>(threadIdx.x & 63);
+    const int32_t wave = static_cast<int32_t>(threadIdx.x >> 6);
     for (int32_t jb = static_cast<int32_t>(blockIdx.y); jb < n_jobs; jb += static_cast<int32_t>(gridDim.y)) {
         const TableJob J = jobs[jb];
         const int32_t n = J.n;
@@ -1276,15 +1286,22 @@ __global__ __launch_bounds__(256) void segment_table_kernel(const double* __rest
         const int32_t segs = seg_count(n);
         const double* u = uv + pots[J.potential].ut_offset + J.lo;
         double* out = segtab + static_cast<int64_t>(J.seg_row) * 64;
-        for (int32_t i = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x); i < rows * 64;
-             i += static_cast<int32_t>(gridDim.x * blockDim.x)) {
-            const int32_t r = i >> 6, sg = i & 63;
-            double x = 0.0;
-            if (sg < segs) {
-                const int64_t q = segment_start(qn, segs, sg) + r;
-                x = u[q < n - 1 ? q : n - 1];
+        for (int32_t r0 = static_cast<int32_t>(blockIdx.x) * kTableTile; r0 < rows;
+             r0 += static_cast<int32_t>(gridDim.x) * kTableTile) {
+            // Columns: wavefront w reads segments w, w + 4, ...; lane = row.
+            for (int32_t sg = wave; sg < 64; sg += 4) {
+                double x = 0.0;
+                if (sg < segs) {
+                    const int64_t q = segment_start(qn, segs, sg) + r0 + lane;
+                    x = u[q < n - 1 ? q : n - 1];
+                }
+                tile[sg][lane] = x;
             }
-            out[i] = x;
+            __syncthreads();
+            // Rows: wavefront w stores rows r0 + w, r0 + w + 4, ...; lane = column.
+            for (int32_t r = wave; r < kTableTile && r0 + r < rows; r += 4)
+                out[static_cast<int64_t>(r0 + r) * 64 + lane] = tile[lane][r];
+            __syncthreads();
         }
     }
 }
@@ -1346,11 +1363,6 @@ struct SegGeom {
         } else {
             u_start = u[q0];
         }
-    }
-
-    __device__ __forceinline__ bool same_window(const SegGeom& o) const
-    {
-        return lo == o.lo && hi == o.hi && seg_row == o.seg_row;
     }
 
     template <class Body>
@@ -1525,11 +1537,24 @@ __device__ __forceinline__ void segmented_shot(SegShared<T, K>& sh, const SegGeo
 
 }  // namespace
 
+// Refinement state of a segmented block, in LDS: thread 0 runs the state
+// machine (probe and update) between shots and publishes the probes, so the
+// walks carry no refinement state in registers.
+template <int K>
+struct SegControl {
+    Refiner r[K];
+    double em[K];
+    int32_t st[K];
+    int32_t live[K];
+    int32_t counted[K];
+    int32_t action;  // 0: all done; 1: the pair shoots together; 2: live levels shoot alone
+};
+
 // Segmented refinement: one T-thread block per level (K = 1) or per pair of
 // levels l, l + 1 (K = 2).  Lane s owns steps [q_s, q_{s+1}) of a level's
 // window (S = clamp(Q/32, 1, T) segments) and builds that segment's 2x2
 // difference-form transfer matrix (columns (w, d) = (1, 0) and (0, 1),
-// renormalised every 32 steps and at the end).  The far-end value is the
+// renormalised every 128 steps and at the end).  The far-end value is the
 // binary-tree product of the matrices (M_i <- M_{i+d} M_i at d = 1, 2, 4, ...)
 // applied to (w_1, d_1) = (1, 1).  Counted shots chain the matrices
 // sequentially (lane 0) for the segment start vectors and re-run every
@@ -1537,11 +1562,8 @@ __device__ __forceinline__ void segmented_shot(SegShared<T, K>& sh, const SegGeo
 // together, each grid value loaded once for both; otherwise (and once one
 // of them has converged) each shoots alone.  Every level's arithmetic is
 // that of a block of its own.
-// Paired blocks carry two levels' refinement state, so they are compiled for
-// two wavefronts per SIMD (256 VGPRs); their walks have twice the
-// independent FP64 chains of a single level's.
 template <int T, bool TABLE, int K>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(K == 2 ? 2 : 3))) void numerov_refine_segmented_kernel(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void numerov_refine_segmented_kernel(
     const double* __restrict__ uv, const double* __restrict__ segtab, const PotentialRecord* __restrict__ pots,
     const BandWindow* __restrict__ windows, const WaveTask* __restrict__ tasks, const int32_t n_tasks,
     const int32_t* __restrict__ bins, const Bracket* __restrict__ brackets, const double rel_tol,
@@ -1550,6 +1572,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(K == 2 ? 2 : 
 {
     static_assert(K == 1 || K == 2, "one or two levels per block");
     __shared__ SegShared<T, K> sh;
+    __shared__ SegControl<K> ctl;
 
     WaveTask task;
     if (!fetch_block_task(tasks, n_tasks, bins, task))
@@ -1557,89 +1580,109 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(K == 2 ? 2 : 
     const PotentialRecord& P = pots[task.potential];
     const int32_t lane = static_cast<int32_t>(threadIdx.x);
     const BandWindow* W = windows + task.potential * kBands;
-    const double* uv_pot = uv + P.ut_offset;
-    Refiner r[K];
-    SegGeom<TABLE> g[K];
-    bool live[K];
-    int32_t st[K];
-    int64_t out[K];
+    if (lane == 0) {
 #pragma unroll
-    for (int q = 0; q < K; ++q) {
-        const int32_t l = task.first + q;
-        out[q] = P.level_offset + l;
-        live[q] = false;
-        st[q] = 0;
-        r[q] = Refiner{};
-        if (l >= P.level_count)
-            continue;
-        if (P.flags != 0) {
-            if (lane == 0)
-                write_level(levels, status, iterations, out[q], __builtin_nan(""), 3, 0);
-            continue;
+        for (int q = 0; q < K; ++q) {
+            const int32_t l = task.first + q;
+            const int64_t out = P.level_offset + l;
+            Refiner& r = ctl.r[q];
+            r = Refiner{};
+            ctl.live[q] = 0;
+            ctl.st[q] = 0;
+            if (l >= P.level_count)
+                continue;
+            if (P.flags != 0) {
+                write_level(levels, status, iterations, out, __builtin_nan(""), 3, 0);
+                continue;
+            }
+            r.v = P.level_min + l;
+            r.b = brackets[out];
+            r.start_model();
+            if (!r.b.found) {
+                write_level(levels, status, iterations, out, __builtin_nan(""), 1, 0);
+                continue;
+            }
+            ctl.live[q] = 1;
+            steps[out] = W[r.b.band].hi - W[r.b.band].lo - 1;
         }
-        r[q].v = P.level_min + l;
-        r[q].b = brackets[out[q]];
-        r[q].start_model();
-        if (!r[q].b.found) {
-            if (lane == 0)
-                write_level(levels, status, iterations, out[q], __builtin_nan(""), 1, 0);
-            continue;
-        }
-        live[q] = true;
-        g[q].init(uv_pot, segtab, W[r[q].b.band], lane, T);
-        if (lane == 0)
-            steps[out[q]] = g[q].n - 2;
     }
+    __syncthreads();
+    const double* uv_pot = uv + P.ut_offset;
+    // A shot's geometry is rebuilt from its band window before each shot
+    // (nothing of it stays in registers between shots).
     bool together = false;
-    if constexpr (K == 2)
-        together = live[0] && live[1] && g[0].same_window(g[1]);
+    if constexpr (K == 2) {
+        if (ctl.live[0] && ctl.live[1]) {
+            const BandWindow& a = W[ctl.r[0].b.band];
+            const BandWindow& b = W[ctl.r[1].b.band];
+            together = a.lo == b.lo && a.hi == b.hi && a.seg_row == b.seg_row;
+        }
+    }
     const StepConsts k = step_consts();
 
     for (;;) {
-        double em[K];
-        bool any = false;
+        if (lane == 0) {
+            bool any = false;
 #pragma unroll
-        for (int q = 0; q < K; ++q) {
-            em[q] = 0.0;
-            if (!live[q])
-                continue;
-            if (r[q].converged(rel_tol, P.abs_tol)) {
-                live[q] = false;
-            } else if (r[q].it >= max_iter) {
-                st[q] = 2;
-                live[q] = false;
-            } else {
-                em[q] = r[q].probe(rel_tol, P.abs_tol);
-                any = true;
+            for (int q = 0; q < K; ++q) {
+                if (!ctl.live[q])
+                    continue;
+                Refiner& r = ctl.r[q];
+                bool done = false;
+                if (r.converged(rel_tol, P.abs_tol)) {
+                    done = true;
+                } else if (r.it >= max_iter) {
+                    ctl.st[q] = 2;
+                    done = true;
+                } else {
+                    ctl.em[q] = r.probe(rel_tol, P.abs_tol);
+                    ctl.counted[q] = r.isolated ? 0 : 1;
+                    any = true;
+                }
+                if (done) {
+                    ctl.live[q] = 0;
+                    write_level(levels, status, iterations, P.level_offset + task.first + q, r.level(), ctl.st[q],
+                                r.it);
+                }
             }
-            if (!live[q] && lane == 0)
-                write_level(levels, status, iterations, out[q], r[q].level(), st[q], r[q].it);
+            ctl.action = !any ? 0 : ((K == 2 && together && ctl.live[0] && ctl.live[K - 1]) ? 1 : 2);
         }
-        if (!any)
+        __syncthreads();
+        const int32_t action = ctl.action;
+        if (action == 0)
             break;
         if constexpr (K == 2) {
-            if (together && live[0] && live[1]) {
-                const double e[2] = {P.beta * em[0], P.beta * em[1]};
-                const bool counted[2] = {!r[0].isolated, !r[1].isolated};
+            if (action == 1) {
+                const double em0 = ctl.em[0], em1 = ctl.em[1];
+                const double e[2] = {P.beta * em0, P.beta * em1};
+                const bool counted[2] = {ctl.counted[0] != 0, ctl.counted[1] != 0};
                 double fw[2];
                 int32_t fk[2], c[2];
-                segmented_shot<T, K, 2, TABLE>(sh, g[0], lane, 0, e, counted, P.stride, k, fw, fk, c);
-                r[0].update(em[0], fw[0], fk[0], c[0]);
-                r[1].update(em[1], fw[1], fk[1], c[1]);
+                SegGeom<TABLE> g;
+                g.init(uv_pot, segtab, W[ctl.r[0].b.band], lane, T);
+                segmented_shot<T, K, 2, TABLE>(sh, g, lane, 0, e, counted, P.stride, k, fw, fk, c);
+                if (lane == 0) {
+                    ctl.r[0].update(em0, fw[0], fk[0], c[0]);
+                    ctl.r[1].update(em1, fw[1], fk[1], c[1]);
+                }
                 __syncthreads();
                 continue;
             }
         }
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            if (!live[q])
+            if (!ctl.live[q])
                 continue;
-            const double e[1] = {P.beta * em[q]};
-            const bool counted[1] = {!r[q].isolated};
+            const double em = ctl.em[q];
+            const double e[1] = {P.beta * em};
+            const bool counted[1] = {ctl.counted[q] != 0};
             double fw[1];
             int32_t fk[1], c[1];
-            segmented_shot<T, K, 1, TABLE>(sh, g[q], lane, q, e, counted, P.stride, k, fw, fk, c);
-            r[q].update(em[q], fw[0], fk[0], c[0]);
+            SegGeom<TABLE> g;
+            g.init(uv_pot, segtab, W[ctl.r[q].b.band], lane, T);
+            segmented_shot<T, K, 1, TABLE>(sh, g, lane, q, e, counted, P.stride, k, fw, fk, c);
+            if (lane == 0)
+                ctl.r[q].update(em, fw[0], fk[0], c[0]);
             __syncthreads();
         }
     }
@@ -1793,7 +1836,7 @@ hipError_t launch_segment_tables(hipStream_t stream, const double* uv, const Pot
 {
     if (n_jobs == 0 || max_rows == 0)
         return hipSuccess;
-    const int32_t bx = (max_rows * 64 + 255) / 256;
+    const int32_t bx = (max_rows + kTableTile - 1) / kTableTile;
     dim3 grid(static_cast<unsigned>(bx < 1024 ? bx : 1024), static_cast<unsigned>(n_jobs < 65535 ? n_jobs : 65535));
     hipLaunchKernelGGL(segment_table_kernel, grid, dim3(256), 0, stream, uv, pots, jobs, n_jobs, segtab);
     return hipGetLastError();

@@ -655,9 +655,10 @@ int32_t eo_shoot_diff(const double* uv, int64_t n, double e, int32_t stride, dou
  * segment s covers steps [q_s, q_{s+1}), q_s = (s*Q)/S.  Each segment yields
  * the 2x2 matrix 2^E [[m00 m01][m10 m11]] taking (w, d) at the segment start
  * to (w, d) at its end: the two columns start from (w, d) = (1, 0) and (0, 1)
- * with w_prev = w - d, and after every 32nd step of the segment and at its end
- * all carried values are divided by 2^x, x the frexp exponent of their
- * largest magnitude.  The chain v <- M_s v from v = (w_1, d_1) = (1, 1),
+ * with w_prev = w - d, and after every 128th step of the segment and at its
+ * end all carried values are divided by 2^x, x the frexp exponent of their
+ * largest magnitude (a step grows them by less than 2^2.87 and shrinks the
+ * largest by less than 2^4.5, so 128 steps stay in the normal range).  The chain v <- M_s v from v = (w_1, d_1) = (1, 1),
  * renormalised the same way after each product, ends at F = w_{N-1} = v0 2^E.
  * The far-end value comes from the product of all segment matrices, formed
  * as a fixed binary tree (at distance d = 1, 2, 4, ...: M_i <- M_{i+d} M_i for
@@ -723,7 +724,7 @@ void eo_segment_matrix(const double* uv, int64_t q0, int64_t q1, double e, eo_ma
             x[c] = x[c] + x[c + 1];
         }
         em = ep;
-        if (((q - q0) & 31) == 31) {
+        if (((q - q0) & 127) == 127) {
             const int32_t sh = eo_max_exponent(x, 6);
             eo_scale(x, 6, sh);
             ex += sh;
@@ -790,7 +791,7 @@ void eo_segment_count_nodes(const double* uv, int64_t n, int64_t q0, int64_t q1,
             c.last = (int32_t)sb;
             c.has = 1;
         }
-        if (((q - q0) & 31) == 31)
+        if (((q - q0) & 127) == 127)
             eo_scale(x, 3, eo_max_exponent(x, 3));
     }
     *out = c;

@@ -202,6 +202,37 @@ def per_potential_launches(capi, pots, rel_tol: float, sample: int = 64) -> dict
     return {"eigenlevels_per_s": levels / wall, "potentials": sample, "levels": levels, "wall_s": wall}
 
 
+def run_loads(capi, args) -> int:
+    """Loads of the fitting batch with nothing else in flight (potential
+    statistics, shifted tables, band windows and segment tables, host layout
+    included): the load part of every forward pass of the fit, timed alone."""
+    from epseon_backend_amd.workloads import DIFF_SEED, differentiation
+
+    specs = differentiation(256)
+    pots = [capi.Potential(values=s.table(), r_min=s.r_min, r_max=s.r_max, reduced_mass=s.mu, e_top=s.de,
+                           n_energies=s.m, level_min=0, level_count=41) for s in specs]
+    ctx = capi.Context(0)
+    ms = []
+    for i in range(args.loads_only + 1):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.load(pots)
+        ctx.synchronize()
+        if i > 0:
+            ms.append((time.perf_counter() - t0) * 1e3)
+    ctx.close()
+    out = {
+        "workload": f"loads of the fitting batch: 256 Morse potentials (seed {DIFF_SEED}) x 50000 points, "
+        "host V uploaded each time",
+        "loads": args.loads_only,
+        "load_ms": ms,
+        "mean_load_ms": float(np.mean(ms)),
+    }
+    sys.stdout.write(json.dumps(out) + "\n")
+    sys.stdout.flush()
+    return 0
+
+
 def run_fit(capi, args) -> int:
     """Differentiation workload: forward + backward through the adjoint."""
     import torch
@@ -452,6 +483,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="batched: keep the default launch configuration")
     ap.add_argument("--no-single", action="store_true", help="batched: skip the single heavy potential's steps")
+    ap.add_argument("--loads-only", type=int, default=0, metavar="K",
+                    help="fit: only load the fitting batch K times (after one untimed load) and report the "
+                         "load times, each load's kernels running alone")
     ap.add_argument(
         "--copy-calibration",
         action="store_true",
@@ -489,7 +523,7 @@ def main() -> int:
         return 1
 
     if args.workload == "fit":
-        return run_fit(capi, args)
+        return run_loads(capi, args) if args.loads_only > 0 else run_fit(capi, args)
     if args.workload == "stress":
         return run_stress(capi, args)
     pots, config = single_workload(capi) if args.workload == "single" else batched_workload(capi)

@@ -1331,8 +1331,20 @@ struct SegGeom {
     double u_start;
     ColumnSource cs;
 
+    // Lane `lane`'s steps [q0, q1) of the window (64-bit divisions: done once
+    // per level, not per shot).
+    static __device__ __forceinline__ int2 bounds(const BandWindow& B, const int32_t lane, const int32_t T)
+    {
+        const int32_t q = B.hi - B.lo - 1;
+        const int32_t sg = min(T, max(1, q / 32));
+        if (lane >= sg)
+            return make_int2(0, 0);
+        return make_int2(static_cast<int32_t>(segment_start(q, sg, lane)),
+                         static_cast<int32_t>(segment_start(q, sg, lane + 1)));
+    }
+
     __device__ __forceinline__ void init(const double* uv_pot, const double* segtab, const BandWindow& B,
-                                         const int32_t lane, const int32_t T)
+                                         const int2 qb, const int32_t lane, const int32_t T)
     {
         lo = B.lo;
         hi = B.hi;
@@ -1342,8 +1354,8 @@ struct SegGeom {
         qn = n - 2;
         segs = min(T, max(1, qn / 32));
         active = lane < segs;
-        q0 = active ? static_cast<int32_t>(segment_start(qn, segs, lane)) : 0;
-        q1 = active ? static_cast<int32_t>(segment_start(qn, segs, lane + 1)) : 0;
+        q0 = qb.x;
+        q1 = qb.y;
         cs = ColumnSource{};
         if constexpr (TABLE) {
             // With the segment table, lane s reads column s (row r = point
@@ -1375,17 +1387,56 @@ struct SegGeom {
     }
 };
 
-template <int T, int K>
-__device__ __forceinline__ void store_matrix(SegShared<T, K>& sh, const int32_t slot, const int32_t lane,
-                                             const MatrixBody& mb)
+// A lane's segment matrix in registers (entries m00 = w column of (1, 0),
+// m01 = w column of (0, 1), m10, m11 the d rows), times 2^e.
+struct SegMat {
+    double m00, m01, m10, m11;
+    int32_t e;
+};
+
+// The walk's matrix divided by the frexp exponent of its largest entry.
+__device__ __forceinline__ SegMat normalized(const MatrixBody& mb)
 {
     const double mx = fmax(fmax(fabs(mb.wa), fabs(mb.da)), fmax(fabs(mb.wb), fabs(mb.db)));
     const int32_t s = frexp_exponent(mx);
-    sh.m[slot][0][lane] = __builtin_ldexp(mb.wa, -s);
-    sh.m[slot][1][lane] = __builtin_ldexp(mb.wb, -s);
-    sh.m[slot][2][lane] = __builtin_ldexp(mb.da, -s);
-    sh.m[slot][3][lane] = __builtin_ldexp(mb.db, -s);
-    sh.e[slot][lane] = mb.ex + s;
+    return SegMat{__builtin_ldexp(mb.wa, -s), __builtin_ldexp(mb.wb, -s), __builtin_ldexp(mb.da, -s),
+                  __builtin_ldexp(mb.db, -s), mb.ex + s};
+}
+
+template <int T, int K>
+__device__ __forceinline__ void store_seg(SegShared<T, K>& sh, const int32_t slot, const int32_t lane,
+                                          const SegMat& a)
+{
+    sh.m[slot][0][lane] = a.m00;
+    sh.m[slot][1][lane] = a.m01;
+    sh.m[slot][2][lane] = a.m10;
+    sh.m[slot][3][lane] = a.m11;
+    sh.e[slot][lane] = a.e;
+}
+
+// The binary-tree product of a wavefront's segment matrices in registers:
+// at d = 1, 2, 4, ... lane i (a multiple of 2d) takes M_{i+d} M_i, the
+// partner's matrix arriving by a cross-lane shuffle; the same operations as
+// tree_step.  Lane 0 ends with the product of all segments.
+__device__ __forceinline__ void tree_in_registers(SegMat& a, const int32_t lane, const int32_t segs)
+{
+    for (int32_t d = 1; d < segs; d *= 2) {
+        const double b00 = __shfl_down(a.m00, d, 64), b01 = __shfl_down(a.m01, d, 64);
+        const double b10 = __shfl_down(a.m10, d, 64), b11 = __shfl_down(a.m11, d, 64);
+        const int32_t be = __shfl_down(a.e, d, 64);
+        if ((lane & (2 * d - 1)) == 0 && lane + d < segs) {
+            const double c00 = __builtin_fma(b00, a.m00, b01 * a.m10);
+            const double c01 = __builtin_fma(b00, a.m01, b01 * a.m11);
+            const double c10 = __builtin_fma(b10, a.m00, b11 * a.m10);
+            const double c11 = __builtin_fma(b10, a.m01, b11 * a.m11);
+            const int32_t s = frexp_exponent(fmax(fmax(fabs(c00), fabs(c01)), fmax(fabs(c10), fabs(c11))));
+            a.m00 = __builtin_ldexp(c00, -s);
+            a.m01 = __builtin_ldexp(c01, -s);
+            a.m10 = __builtin_ldexp(c10, -s);
+            a.m11 = __builtin_ldexp(c11, -s);
+            a.e = be + a.e + s;
+        }
+    }
 }
 
 // Start vector of every segment of a counted shot: the matrices chained from
@@ -1445,7 +1496,9 @@ __device__ __forceinline__ int32_t join_counts(const SegShared<T, K>& sh, const 
 // lane builds its segment's matrix for each energy e[k] (slot k of the
 // shared memory), then the tree product gives the far end F = fw 2^fk of
 // each; for a counted slot the segments are walked again from their chained
-// start vectors and the node count is joined across segment borders.
+// start vectors and the node count is joined across segment borders.  In
+// 64-lane blocks (one wavefront) the tree runs in registers and the
+// matrices go to LDS only for a counted shot's chain.
 template <int T, int K, int KK, bool TABLE>
 __device__ __forceinline__ void segmented_shot(SegShared<T, K>& sh, const SegGeom<TABLE>& g, const int32_t lane,
                                                const int32_t slot0, const double (&e)[KK],
@@ -1454,24 +1507,37 @@ __device__ __forceinline__ void segmented_shot(SegShared<T, K>& sh, const SegGeo
                                                int32_t (&c)[KK])
 {
     const int32_t segs = g.segs;
-    if (g.active) {
-        if constexpr (KK == 1) {
-            MatrixBody mb{e[0], e[0] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10};
-            g.walk(mb);
-            store_matrix(sh, slot0, lane, mb);
-        } else {
-            MatrixPair mp{{e[0], e[0] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10},
-                          {e[1], e[1] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10}};
-            g.walk(mp);
-            store_matrix(sh, slot0, lane, mp.x);
-            store_matrix(sh, slot0 + 1, lane, mp.y);
-        }
-    }
-    __syncthreads();
     bool any_counted = false;
 #pragma unroll
     for (int q = 0; q < KK; ++q)
         any_counted = any_counted || counted[q];
+    SegMat a[KK];
+#pragma unroll
+    for (int q = 0; q < KK; ++q)
+        a[q] = SegMat{1.0, 0.0, 0.0, 1.0, 0};
+    if (g.active) {
+        if constexpr (KK == 1) {
+            MatrixBody mb{e[0], e[0] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10};
+            g.walk(mb);
+            a[0] = normalized(mb);
+        } else {
+            MatrixPair mp{{e[0], e[0] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10},
+                          {e[1], e[1] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10}};
+            g.walk(mp);
+            a[0] = normalized(mp.x);
+            a[1] = normalized(mp.y);
+        }
+    }
+    constexpr bool kRegisterTree = T == 64;
+    if (!kRegisterTree || any_counted) {
+        if (g.active) {
+#pragma unroll
+            for (int q = 0; q < KK; ++q)
+                if (!kRegisterTree || counted[q])
+                    store_seg(sh, slot0 + q, lane, a[q]);
+        }
+        __syncthreads();
+    }
     if (any_counted) {
         if (lane == 0) {
 #pragma unroll
@@ -1481,20 +1547,31 @@ __device__ __forceinline__ void segmented_shot(SegShared<T, K>& sh, const SegGeo
         }
         __syncthreads();
     }
-    for (int32_t d = 1; d < segs; d *= 2) {
-        if ((lane & (2 * d - 1)) == 0 && lane + d < segs) {
+    if constexpr (kRegisterTree) {
 #pragma unroll
-            for (int q = 0; q < KK; ++q)
-                tree_step(sh, slot0 + q, lane, d);
+        for (int q = 0; q < KK; ++q) {
+            tree_in_registers(a[q], lane, segs);
+            fw[q] = __shfl(__builtin_fma(a[q].m00, 1.0, a[q].m01 * 1.0), 0, 64);
+            fk[q] = __shfl(a[q].e, 0, 64);
         }
-        __syncthreads();
+    } else {
+        for (int32_t d = 1; d < segs; d *= 2) {
+            if ((lane & (2 * d - 1)) == 0 && lane + d < segs) {
+#pragma unroll
+                for (int q = 0; q < KK; ++q)
+                    tree_step(sh, slot0 + q, lane, d);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < KK; ++q) {
+            fw[q] = __builtin_fma(sh.m[slot0 + q][0][0], 1.0, sh.m[slot0 + q][1][0] * 1.0);
+            fk[q] = sh.e[slot0 + q][0];
+        }
     }
 #pragma unroll
-    for (int q = 0; q < KK; ++q) {
-        fw[q] = __builtin_fma(sh.m[slot0 + q][0][0], 1.0, sh.m[slot0 + q][1][0] * 1.0);
-        fk[q] = sh.e[slot0 + q][0];
+    for (int q = 0; q < KK; ++q)
         c[q] = 0;
-    }
     if (!any_counted)
         return;
     if (g.active) {
@@ -1608,8 +1685,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
     }
     __syncthreads();
     const double* uv_pot = uv + P.ut_offset;
-    // A shot's geometry is rebuilt from its band window before each shot
-    // (nothing of it stays in registers between shots).
+    // A shot's geometry is rebuilt from its band window before each shot;
+    // only each lane's segment bounds stay in registers between shots.
+    int2 qb[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+        qb[q] = ctl.live[q] ? SegGeom<TABLE>::bounds(W[ctl.r[q].b.band], lane, T) : make_int2(0, 0);
     bool together = false;
     if constexpr (K == 2) {
         if (ctl.live[0] && ctl.live[1]) {
@@ -1659,7 +1740,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
                 double fw[2];
                 int32_t fk[2], c[2];
                 SegGeom<TABLE> g;
-                g.init(uv_pot, segtab, W[ctl.r[0].b.band], lane, T);
+                g.init(uv_pot, segtab, W[ctl.r[0].b.band], qb[0], lane, T);
                 segmented_shot<T, K, 2, TABLE>(sh, g, lane, 0, e, counted, P.stride, k, fw, fk, c);
                 if (lane == 0) {
                     ctl.r[0].update(em0, fw[0], fk[0], c[0]);
@@ -1679,7 +1760,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
             double fw[1];
             int32_t fk[1], c[1];
             SegGeom<TABLE> g;
-            g.init(uv_pot, segtab, W[ctl.r[q].b.band], lane, T);
+            g.init(uv_pot, segtab, W[ctl.r[q].b.band], qb[q], lane, T);
             segmented_shot<T, K, 1, TABLE>(sh, g, lane, q, e, counted, P.stride, k, fw, fk, c);
             if (lane == 0)
                 ctl.r[q].update(em, fw[0], fk[0], c[0]);

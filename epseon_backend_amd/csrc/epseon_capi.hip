@@ -34,14 +34,14 @@ hipError_t launch_scan(hipStream_t, const double*, const PotentialRecord*, const
                        const WaveTask*, int32_t, const int32_t*, int32_t, int32_t, int32_t*, double*,
                        int32_t*);
 hipError_t launch_brackets(hipStream_t, const PotentialRecord*, const BandWindow*, const WaveTask*, int32_t,
-                           const int32_t*, const double*, const int32_t*, Bracket*);
+                           const int32_t*, const double*, const int32_t*, Bracket*, RefinePoints*);
 hipError_t launch_refine(hipStream_t, int32_t, const double*, const PotentialRecord*, const BandWindow*,
-                         const WaveTask*, int32_t, int32_t, const Bracket*, double, int32_t, double*,
-                         int32_t*, int32_t*, int32_t*);
+                         const WaveTask*, int32_t, int32_t, const Bracket*, RefinePoints*, double, int32_t,
+                         double*, int32_t*, int32_t*, int32_t*);
 hipError_t launch_refine_segmented(hipStream_t, int32_t, int32_t, const double*, const double*,
                                    const PotentialRecord*, const BandWindow*, const WaveTask*, int32_t,
-                                   const int32_t*, int32_t, const Bracket*, double, int32_t, double*,
-                                   int32_t*, int32_t*, int32_t*);
+                                   const int32_t*, int32_t, const Bracket*, RefinePoints*, double, int32_t,
+                                   double*, int32_t*, int32_t*, int32_t*);
 hipError_t launch_segment_tables(hipStream_t, const double*, const PotentialRecord*, const TableJob*,
                                  int32_t, int32_t, double*);
 hipError_t launch_band_windows(hipStream_t, const double*, const PotentialRecord*, int32_t, int2*);
@@ -301,6 +301,7 @@ struct epseon_ctx {
     bool tables_full = false, tables_band = false;
     int32_t* d_steps = nullptr;  // recurrence steps of each level's shots (its window's n - 2)
     Bracket* d_brackets = nullptr;  // each level's bracket from the last scan
+    RefinePoints* d_points = nullptr;  // each level's interpolation points (bracket pass, refinement)
     PotentialRecord* d_rec = nullptr;
     WaveTask* d_scan_tasks = nullptr;
     // Scan tasks split into one bin per XCD (whole potentials, balanced by
@@ -1004,6 +1005,7 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     EPSEON_HIP_TRY(arena_array(a, nw, &ctx->d_win_full));
     EPSEON_HIP_TRY(arena_array(a, nw, &ctx->d_win_band));
     EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_brackets));
+    EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_points));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(count, 1), &ctx->d_rec));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(scan_tasks.size(), 1), &ctx->d_scan_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(xcd_tasks.size(), 1), &ctx->d_xcd_scan_tasks));
@@ -1352,23 +1354,24 @@ hipError_t run_refine(epseon_ctx* ctx, hipStream_t stream, hipStream_t stream_b,
     const int32_t t0 = ctx->cum_level_tasks[k0];
     const int32_t t1 = ctx->cum_level_tasks[k1];
     if (const hipError_t e = launch_brackets(stream, ctx->d_rec, win, ctx->d_level_tasks + t0, t1 - t0,
-                                             ctx->d_nodes, ctx->d_far, ctx->d_resc, ctx->d_brackets);
+                                             ctx->d_nodes, ctx->d_far, ctx->d_resc, ctx->d_brackets,
+                                             ctx->d_points);
         e != hipSuccess)
         return e;
     if (mode == EPSEON_REFINE_LANE && ctx->levels_per_lane == 1) {
         const int32_t r0 = ctx->cum_refine_tasks[k0];
         const int32_t r1 = ctx->cum_refine_tasks[k1];
         return launch_refine(stream, 1, ctx->d_uv, ctx->d_rec, win, ctx->d_refine_tasks + r0, r1 - r0,
-                             ctx->waves_per_block, ctx->d_brackets, rel_tol, max_iter, ctx->d_levels,
-                             ctx->d_status, ctx->d_iters, ctx->d_steps);
+                             ctx->waves_per_block, ctx->d_brackets, ctx->d_points, rel_tol, max_iter,
+                             ctx->d_levels, ctx->d_status, ctx->d_iters, ctx->d_steps);
     }
     if (mode == EPSEON_REFINE_LANE) {
         const int32_t a0 = ctx->cum_lane2_tasks[k0];
         const int32_t a1 = ctx->cum_lane2_tasks[k1];
         const int32_t b0 = ctx->cum_lane1_tasks[k0];
         const int32_t b1 = ctx->cum_lane1_tasks[k1];
-        hipError_t e = launch_refine(stream, 2, ctx->d_uv, ctx->d_rec, win, ctx->d_lane2_tasks + a0,
-                                     a1 - a0, ctx->waves_per_block, ctx->d_brackets, rel_tol, max_iter,
+        hipError_t e = launch_refine(stream, 2, ctx->d_uv, ctx->d_rec, win, ctx->d_lane2_tasks + a0, a1 - a0,
+                                     ctx->waves_per_block, ctx->d_brackets, ctx->d_points, rel_tol, max_iter,
                                      ctx->d_levels, ctx->d_status, ctx->d_iters, ctx->d_steps);
         if (e != hipSuccess)
             return e;
@@ -1380,8 +1383,8 @@ hipError_t run_refine(epseon_ctx* ctx, hipStream_t stream, hipStream_t stream_b,
                 return e;
         }
         return launch_refine(stream_b, 1, ctx->d_uv, ctx->d_rec, win, ctx->d_lane1_tasks + b0, b1 - b0,
-                             ctx->waves_per_block, ctx->d_brackets, rel_tol, max_iter, ctx->d_levels,
-                             ctx->d_status, ctx->d_iters, ctx->d_steps);
+                             ctx->waves_per_block, ctx->d_brackets, ctx->d_points, rel_tol, max_iter,
+                             ctx->d_levels, ctx->d_status, ctx->d_iters, ctx->d_steps);
     }
     const double* tab = lanes == 64 && ctx->segtab_ok ? (ctx->scan_windows ? ctx->d_segtab_band : ctx->d_segtab_full)
                                                      : nullptr;
@@ -1394,20 +1397,20 @@ hipError_t run_refine(epseon_ctx* ctx, hipStream_t stream, hipStream_t stream_b,
         if (whole)
             return launch_refine_segmented(stream, lanes, 2, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_xcd_pair_tasks,
                                            p1 - p0, ctx->d_xcd_pair_bins, ctx->xcd_max_pair_bin, ctx->d_brackets,
-                                           rel_tol, max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters,
-                                           ctx->d_steps);
+                                           ctx->d_points, rel_tol, max_iter, ctx->d_levels, ctx->d_status,
+                                           ctx->d_iters, ctx->d_steps);
         return launch_refine_segmented(stream, lanes, 2, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_pair_tasks + p0,
-                                       p1 - p0, nullptr, 0, ctx->d_brackets, rel_tol, max_iter, ctx->d_levels,
-                                       ctx->d_status, ctx->d_iters, ctx->d_steps);
+                                       p1 - p0, nullptr, 0, ctx->d_brackets, ctx->d_points, rel_tol, max_iter,
+                                       ctx->d_levels, ctx->d_status, ctx->d_iters, ctx->d_steps);
     }
     if (whole)
         return launch_refine_segmented(stream, lanes, 1, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_xcd_level_tasks,
                                        t1 - t0, ctx->d_xcd_level_bins, ctx->xcd_max_level_bin, ctx->d_brackets,
-                                       rel_tol, max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters,
+                                       ctx->d_points, rel_tol, max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters,
                                        ctx->d_steps);
     return launch_refine_segmented(stream, lanes, 1, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_level_tasks + t0,
-                                   t1 - t0, nullptr, 0, ctx->d_brackets, rel_tol, max_iter, ctx->d_levels,
-                                   ctx->d_status, ctx->d_iters, ctx->d_steps);
+                                   t1 - t0, nullptr, 0, ctx->d_brackets, ctx->d_points, rel_tol, max_iter,
+                                   ctx->d_levels, ctx->d_status, ctx->d_iters, ctx->d_steps);
 }
 
 int check_refine_args(double rel_tol, int32_t max_iter)

@@ -793,14 +793,19 @@ __device__ __forceinline__ void model_apply(const double b, const double e0, con
 // nodes and the one below it (or the window floor, below which no level
 // exists since every eigenvalue exceeds min V).  Exponents are binary.  When
 // the bracket holds exactly level v and a third scan point of the same count
-// regime exists above or below it, `rate` is the model rate b.
+// regime exists above or below it, `rate` is the model rate b.  `pts` gets
+// the level's interpolation points: scan points j - 4 .. j + 3 (j the upper
+// end) whose group shares the level's window, the lower end only when its
+// value is of that window.
 // W: the potential's kBands band windows (those the scan used).
 __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const BandWindow* W,
                                                 const int32_t v, const int32_t* __restrict__ nodes,
                                                 const double* __restrict__ far_end,
-                                                const int32_t* __restrict__ rescales)
+                                                const int32_t* __restrict__ rescales, RefinePoints& pts)
 {
     Bracket b{};
+    pts.n = 0;
+    pts.ih = 0;
     const int32_t m = P.n_energies;
     const int32_t* nd = nodes + P.energy_offset;
     int32_t lo = 0, hi = m;
@@ -837,6 +842,23 @@ __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const 
         b.lo_ok = 1;
         b.lo_foreign = same_window(W, bl, band) ? 0 : 1;
     }
+    int32_t np = 0;
+    for (int32_t q = j - 4; q <= j + 3; ++q) {
+        if (q < 0 || q >= m)
+            continue;
+        if (q == j - 1 && !(b.lo_ok && !b.lo_foreign))
+            continue;
+        const int32_t bq = group_band(P, q);
+        if (q != j && q != j - 1 && !same_window(W, bq, band))
+            continue;
+        if (q == j)
+            pts.ih = np;
+        pts.e[np] = __builtin_fma(static_cast<double>(q + 1), P.de, P.e_lo);
+        pts.w[np] = far_end[P.energy_offset + q];
+        pts.k[np] = W[bq].far_exp - rescales[P.energy_offset + q] * kRescaleLog2;
+        ++np;
+    }
+    pts.n = np;
     b.rate = 0.0;
     if (b.lo_ok && !b.lo_foreign && b.cl == v && b.ch == v + 1) {
         const int64_t o = P.energy_offset;
@@ -850,11 +872,107 @@ __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const 
     return b;
 }
 
+// Insert a shot at em (inside the bracket: at index ih in energy order).  A
+// full list first drops the outermost point farther from the bracket centre c
+// (the last one on a tie).
+__device__ __forceinline__ void points_insert(RefinePoints& p, const double c, const double em, const double w,
+                                              const int32_t k)
+{
+    int32_t n = p.n;
+    int32_t ih = p.ih;
+    if (n == kPts) {
+        if (c - p.e[0] > p.e[n - 1] - c) {
+            for (int32_t i = 1; i < n; ++i) {
+                p.e[i - 1] = p.e[i];
+                p.w[i - 1] = p.w[i];
+                p.k[i - 1] = p.k[i];
+            }
+            --ih;
+        }
+        --n;
+    }
+    for (int32_t i = n; i > ih; --i) {
+        p.e[i] = p.e[i - 1];
+        p.w[i] = p.w[i - 1];
+        p.k[i] = p.k[i - 1];
+    }
+    p.e[ih] = em;
+    p.w[ih] = w;
+    p.k[ih] = k;
+    p.n = n + 1;
+    p.ih = ih;
+}
+
+// Root in (0, 1) of the polynomial through the points in x = (E - el)/width,
+// y = w 2^(k - max k) (Newton divided differences), by at most 8 Newton steps
+// from x0, stopping once a step is at most 2^-40.  false when there are fewer
+// than 3 points, a derivative vanishes or a step leaves (0, 1).  The loops
+// run over all kPts slots (predicated on the count) so that the table stays
+// in registers.
+__device__ __forceinline__ bool points_root(const RefinePoints& p, const double el, const double width,
+                                            const double x0, double& root)
+{
+    const int32_t n = p.n;
+    if (n < 3)
+        return false;
+    const double inv = 1.0 / width;
+    int32_t kmax = p.k[0];
+#pragma unroll
+    for (int32_t i = 1; i < kPts; ++i)
+        if (i < n)
+            kmax = p.k[i] > kmax ? p.k[i] : kmax;
+    double x[kPts], c[kPts];
+#pragma unroll
+    for (int32_t i = 0; i < kPts; ++i) {
+        x[i] = 0.0;
+        c[i] = 0.0;
+        if (i < n) {
+            x[i] = (p.e[i] - el) * inv;
+            c[i] = __builtin_ldexp(p.w[i], p.k[i] - kmax);
+        }
+    }
+#pragma unroll
+    for (int32_t j = 1; j < kPts; ++j)
+#pragma unroll
+        for (int32_t i = kPts - 1; i >= j; --i)
+            if (i < n)
+                c[i] = (c[i] - c[i - 1]) / (x[i] - x[i - j]);
+    double top = c[2];
+#pragma unroll
+    for (int32_t i = 3; i < kPts; ++i)
+        top = (i == n - 1) ? c[i] : top;
+    double t = x0;
+    for (int32_t it = 0; it < 8; ++it) {
+        double f = top, d = 0.0;
+#pragma unroll
+        for (int32_t i = kPts - 2; i >= 0; --i) {
+            if (i <= n - 2) {
+                const double u = t - x[i];
+                d = __builtin_fma(d, u, f);
+                f = __builtin_fma(f, u, c[i]);
+            }
+        }
+        if (!(d != 0.0))
+            return false;
+        const double dt = f / d;
+        t = t - dt;
+        if (!(t > 0.0 && t < 1.0))
+            return false;
+        if (fabs(dt) <= 0x1p-40)
+            break;
+    }
+    root = t;
+    return true;
+}
+
 // Refinement state machine shared by both refine kernels.  While the
 // bracket holds more than level v (or its floor has no shot) the probe is the
-// count-interpolated point and needs a counted shot; once it holds exactly level v the probe
-// is the regula-falsi point on F = w 2^k (Anderson-Bjorck scaling), kept at least tol/2
-// inside, and the probe's count follows from the sign of F.  Stops when the
+// count-interpolated point and needs a counted shot; once it holds exactly
+// level v the probe is the root of the polynomial through the level's
+// interpolation points (points_root), started from the regula-falsi point of
+// the ends (Anderson-Bjorck scaling), which is the probe when the root search
+// fails; every probe is kept at least tol/2 inside, and the probe's count
+// follows from the sign of F.  Every shot joins the points.  Stops when the
 // width is <= rel_tol max(|E_lo|, |E_hi|) + abs_tol.
 struct Refiner {
     Bracket b;
@@ -881,7 +999,7 @@ struct Refiner {
         return (b.eh - b.el) <= tol;
     }
 
-    __device__ __forceinline__ double probe(const double rel_tol, const double abs_tol)
+    __device__ __forceinline__ double probe(const double rel_tol, const double abs_tol, const RefinePoints& p)
     {
         const double width = b.eh - b.el;
         const double tol = __builtin_fma(rel_tol, fmax(fabs(b.el), fabs(b.eh)), abs_tol);
@@ -895,7 +1013,11 @@ struct Refiner {
         } else if (isolated) {
             const double r = __builtin_ldexp(b.wh / b.wl, b.kh - b.kl);
             const double s = 1.0 / (1.0 - r);
-            em = __builtin_fma(s, width, b.el);
+            double x;
+            if (points_root(p, b.el, width, fmin(fmax(s, 0.0), 1.0), x))
+                em = __builtin_fma(x, width, b.el);
+            else
+                em = __builtin_fma(s, width, b.el);
             const double d = 0.5 * tol;
             if (!(em >= b.el + d))
                 em = b.el + d;
@@ -914,13 +1036,16 @@ struct Refiner {
     }
 
     // Take the probe's result; `c` is ignored when the bracket is isolated.
-    __device__ __forceinline__ void update(const double em, double w, int32_t k, int32_t c)
+    __device__ __forceinline__ void update(const double em, double w, int32_t k, int32_t c, RefinePoints& p)
     {
         if (isolated)
             c = reshoot ? v : ((static_cast<int32_t>(sign_bit(w)) == (v & 1)) ? v : v + 1);
         if (rate != 0.0)
             model_apply(rate, e0, em, w, k);
         ++it;
+        points_insert(p, b.el + 0.5 * (b.eh - b.el), em, w, k);
+        if (c <= v)
+            ++p.ih;
         // Anderson-Bjorck: when the same end is replaced twice in a row, the
         // kept end's value is scaled by m = 1 - F_new/F_replaced (0.5 when
         // m <= 0 or undefined).
@@ -952,6 +1077,16 @@ struct Refiner {
     __device__ __forceinline__ double level() const { return b.el + 0.5 * (b.eh - b.el); }
 };
 
+// The model transform of interpolation point i (as of the bracket ends).
+__device__ __forceinline__ void point_model(RefinePoints& p, const int32_t i, const double rate, const double e0)
+{
+    double w = p.w[i];
+    int32_t k = p.k[i];
+    model_apply(rate, e0, p.e[i], w, k);
+    p.w[i] = w;
+    p.k[i] = k;
+}
+
 __device__ __forceinline__ void write_level(double* levels, int32_t* status, int32_t* iterations,
                                             const int64_t out, const double lv, const int32_t st,
                                             const int32_t it)
@@ -974,9 +1109,9 @@ template <int K>
 __global__ __launch_bounds__(256) void numerov_refine_kernel(
     const double* __restrict__ uv, const PotentialRecord* __restrict__ pots,
     const BandWindow* __restrict__ windows, const WaveTask* __restrict__ tasks, const int32_t n_tasks,
-    const Bracket* __restrict__ brackets, const double rel_tol, const int32_t max_iter,
-    double* __restrict__ levels, int32_t* __restrict__ status, int32_t* __restrict__ iterations,
-    int32_t* __restrict__ steps)
+    const Bracket* __restrict__ brackets, RefinePoints* __restrict__ points, const double rel_tol,
+    const int32_t max_iter, double* __restrict__ levels, int32_t* __restrict__ status,
+    int32_t* __restrict__ iterations, int32_t* __restrict__ steps)
 {
     static_assert(K == 1 || K == 2, "one or two levels per lane");
     const int32_t tid = wave_task_index();
@@ -1009,6 +1144,11 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
             write_level(levels, status, iterations, out[q], __builtin_nan(""), 1, 0);
             continue;
         }
+        if (r[q].rate != 0.0) {
+            RefinePoints& pq = points[out[q]];
+            for (int32_t i = 0; i < pq.n; ++i)
+                point_model(pq, i, r[q].rate, r[q].e0);
+        }
         live[q] = true;
     }
     const double* u = uv + P.ut_offset + W[0].lo;
@@ -1028,7 +1168,7 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
                 write_level(levels, status, iterations, out[q], r[q].level(), 2, r[q].it);
                 live[q] = false;
             } else {
-                em[q] = r[q].probe(rel_tol, P.abs_tol);
+                em[q] = r[q].probe(rel_tol, P.abs_tol, points[out[q]]);
                 any = true;
             }
         }
@@ -1038,7 +1178,7 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
             double w;
             int32_t k;
             const int32_t c = shoot<DiffForm>(u, n, P.beta * em[0], P.stride, w, k);
-            r[0].update(em[0], w, k * kRescaleLog2, c);
+            r[0].update(em[0], w, k * kRescaleLog2, c, points[out[0]]);
         } else {
             const double ea = live[0] ? em[0] : em[1];
             const double eb = live[1] ? em[1] : em[0];
@@ -1046,9 +1186,9 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
             DiffPair::Counter c;
             shoot<DiffPair>(u, n, DiffPair::Energy{P.beta * ea, P.beta * eb}, P.stride, s, c);
             if (live[0])
-                r[0].update(em[0], s.x.w, s.x.rescales * kRescaleLog2, c.a.nodes);
+                r[0].update(em[0], s.x.w, s.x.rescales * kRescaleLog2, c.a.nodes, points[out[0]]);
             if (live[1])
-                r[1].update(em[1], s.y.w, s.y.rescales * kRescaleLog2, c.b.nodes);
+                r[1].update(em[1], s.y.w, s.y.rescales * kRescaleLog2, c.b.nodes, points[out[1]]);
         }
     }
 }
@@ -1614,17 +1754,17 @@ __device__ __forceinline__ void segmented_shot(SegShared<T, K>& sh, const SegGeo
 
 }  // namespace
 
-// Refinement state of a segmented block, in LDS: thread 0 runs the state
-// machine (probe and update) between shots and publishes the probes, so the
-// walks carry no refinement state in registers.
+// Refinement state of a segmented block, in LDS: thread q < K runs level
+// q's state machine (probe and update) between shots and publishes its probe,
+// so the walks carry no refinement state in registers.
 template <int K>
 struct SegControl {
     Refiner r[K];
+    RefinePoints pts[K];
     double em[K];
     int32_t st[K];
     int32_t live[K];
     int32_t counted[K];
-    int32_t action;  // 0: all done; 1: the pair shoots together; 2: live levels shoot alone
 };
 
 // Segmented refinement: one T-thread block per level (K = 1) or per pair of
@@ -1643,8 +1783,8 @@ template <int T, bool TABLE, int K>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void numerov_refine_segmented_kernel(
     const double* __restrict__ uv, const double* __restrict__ segtab, const PotentialRecord* __restrict__ pots,
     const BandWindow* __restrict__ windows, const WaveTask* __restrict__ tasks, const int32_t n_tasks,
-    const int32_t* __restrict__ bins, const Bracket* __restrict__ brackets, const double rel_tol,
-    const int32_t max_iter, double* __restrict__ levels, int32_t* __restrict__ status,
+    const int32_t* __restrict__ bins, const Bracket* __restrict__ brackets, RefinePoints* __restrict__ points,
+    const double rel_tol, const int32_t max_iter, double* __restrict__ levels, int32_t* __restrict__ status,
     int32_t* __restrict__ iterations, int32_t* __restrict__ steps)
 {
     static_assert(K == 1 || K == 2, "one or two levels per block");
@@ -1657,33 +1797,39 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
     const PotentialRecord& P = pots[task.potential];
     const int32_t lane = static_cast<int32_t>(threadIdx.x);
     const BandWindow* W = windows + task.potential * kBands;
-    if (lane == 0) {
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            const int32_t l = task.first + q;
-            const int64_t out = P.level_offset + l;
-            Refiner& r = ctl.r[q];
-            r = Refiner{};
-            ctl.live[q] = 0;
-            ctl.st[q] = 0;
-            if (l >= P.level_count)
-                continue;
+    if (lane < K) {
+        const int32_t q = lane;
+        const int32_t l = task.first + q;
+        const int64_t out = P.level_offset + l;
+        Refiner& r = ctl.r[q];
+        r = Refiner{};
+        ctl.live[q] = 0;
+        ctl.st[q] = 0;
+        ctl.pts[q].n = 0;
+        if (l < P.level_count) {
             if (P.flags != 0) {
                 write_level(levels, status, iterations, out, __builtin_nan(""), 3, 0);
-                continue;
+            } else {
+                r.v = P.level_min + l;
+                r.b = brackets[out];
+                r.start_model();
+                if (!r.b.found) {
+                    write_level(levels, status, iterations, out, __builtin_nan(""), 1, 0);
+                } else {
+                    ctl.live[q] = 1;
+                    ctl.pts[q] = points[out];
+                    steps[out] = W[r.b.band].hi - W[r.b.band].lo - 1;
+                }
             }
-            r.v = P.level_min + l;
-            r.b = brackets[out];
-            r.start_model();
-            if (!r.b.found) {
-                write_level(levels, status, iterations, out, __builtin_nan(""), 1, 0);
-                continue;
-            }
-            ctl.live[q] = 1;
-            steps[out] = W[r.b.band].hi - W[r.b.band].lo - 1;
         }
     }
     __syncthreads();
+    // The model transform of the interpolation points, one lane per point.
+    if (lane < K * kPts) {
+        const int32_t q = lane / kPts, i = lane % kPts;
+        if (ctl.live[q] && ctl.r[q].rate != 0.0 && i < ctl.pts[q].n)
+            point_model(ctl.pts[q], i, ctl.r[q].rate, ctl.r[q].e0);
+    }
     const double* uv_pot = uv + P.ut_offset;
     // A shot's geometry is rebuilt from its band window before each shot;
     // only each lane's segment bounds stay in registers between shots.
@@ -1700,14 +1846,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
         }
     }
     const StepConsts k = step_consts();
+    __syncthreads();
 
     for (;;) {
-        if (lane == 0) {
-            bool any = false;
-#pragma unroll
-            for (int q = 0; q < K; ++q) {
-                if (!ctl.live[q])
-                    continue;
+        if (lane < K) {
+            const int32_t q = lane;
+            if (ctl.live[q]) {
                 Refiner& r = ctl.r[q];
                 bool done = false;
                 if (r.converged(rel_tol, P.abs_tol)) {
@@ -1716,9 +1860,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
                     ctl.st[q] = 2;
                     done = true;
                 } else {
-                    ctl.em[q] = r.probe(rel_tol, P.abs_tol);
+                    ctl.em[q] = r.probe(rel_tol, P.abs_tol, ctl.pts[q]);
                     ctl.counted[q] = r.isolated ? 0 : 1;
-                    any = true;
                 }
                 if (done) {
                     ctl.live[q] = 0;
@@ -1726,14 +1869,16 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
                                 r.it);
                 }
             }
-            ctl.action = !any ? 0 : ((K == 2 && together && ctl.live[0] && ctl.live[K - 1]) ? 1 : 2);
         }
         __syncthreads();
-        const int32_t action = ctl.action;
-        if (action == 0)
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+            any = any || ctl.live[q] != 0;
+        if (!any)
             break;
         if constexpr (K == 2) {
-            if (action == 1) {
+            if (together && ctl.live[0] && ctl.live[1]) {
                 const double em0 = ctl.em[0], em1 = ctl.em[1];
                 const double e[2] = {P.beta * em0, P.beta * em1};
                 const bool counted[2] = {ctl.counted[0] != 0, ctl.counted[1] != 0};
@@ -1742,9 +1887,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
                 SegGeom<TABLE> g;
                 g.init(uv_pot, segtab, W[ctl.r[0].b.band], qb[0], lane, T);
                 segmented_shot<T, K, 2, TABLE>(sh, g, lane, 0, e, counted, P.stride, k, fw, fk, c);
-                if (lane == 0) {
-                    ctl.r[0].update(em0, fw[0], fk[0], c[0]);
-                    ctl.r[1].update(em1, fw[1], fk[1], c[1]);
+                if (lane < 2) {
+                    const int32_t q = lane;
+                    ctl.r[q].update(q == 0 ? em0 : em1, q == 0 ? fw[0] : fw[1], q == 0 ? fk[0] : fk[1],
+                                    q == 0 ? c[0] : c[1], ctl.pts[q]);
                 }
                 __syncthreads();
                 continue;
@@ -1763,7 +1909,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
             g.init(uv_pot, segtab, W[ctl.r[q].b.band], qb[q], lane, T);
             segmented_shot<T, K, 1, TABLE>(sh, g, lane, q, e, counted, P.stride, k, fw, fk, c);
             if (lane == 0)
-                ctl.r[q].update(em, fw[0], fk[0], c[0]);
+                ctl.r[q].update(em, fw[0], fk[0], c[0], ctl.pts[q]);
             __syncthreads();
         }
     }
@@ -1779,7 +1925,8 @@ __global__ __launch_bounds__(256) void bracket_kernel(const PotentialRecord* __r
                                                       const int32_t* __restrict__ nodes,
                                                       const double* __restrict__ far_end,
                                                       const int32_t* __restrict__ rescales,
-                                                      Bracket* __restrict__ brackets)
+                                                      Bracket* __restrict__ brackets,
+                                                      RefinePoints* __restrict__ points)
 {
     const int32_t t = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
     if (t >= n_tasks)
@@ -1788,8 +1935,9 @@ __global__ __launch_bounds__(256) void bracket_kernel(const PotentialRecord* __r
     const PotentialRecord& P = pots[task.potential];
     if (P.flags != 0 || task.first >= P.level_count)
         return;
-    brackets[P.level_offset + task.first] =
-        find_bracket(P, windows + task.potential * kBands, P.level_min + task.first, nodes, far_end, rescales);
+    const int64_t out = P.level_offset + task.first;
+    brackets[out] = find_bracket(P, windows + task.potential * kBands, P.level_min + task.first, nodes, far_end,
+                                 rescales, points[out]);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1836,19 +1984,20 @@ hipError_t launch_scan(hipStream_t stream, const double* ut, const PotentialReco
 
 hipError_t launch_brackets(hipStream_t stream, const PotentialRecord* pots, const BandWindow* windows,
                            const WaveTask* level_tasks, int32_t n_tasks, const int32_t* nodes,
-                           const double* far_end, const int32_t* rescales, Bracket* brackets)
+                           const double* far_end, const int32_t* rescales, Bracket* brackets,
+                           RefinePoints* points)
 {
     if (n_tasks == 0)
         return hipSuccess;
     hipLaunchKernelGGL(bracket_kernel, dim3(static_cast<unsigned>((n_tasks + 255) / 256)), dim3(256), 0, stream,
-                       pots, windows, level_tasks, n_tasks, nodes, far_end, rescales, brackets);
+                       pots, windows, level_tasks, n_tasks, nodes, far_end, rescales, brackets, points);
     return hipGetLastError();
 }
 
 hipError_t launch_refine(hipStream_t stream, int32_t levels_per_lane, const double* uv,
                          const PotentialRecord* pots, const BandWindow* windows, const WaveTask* tasks,
-                         int32_t n_tasks, int32_t waves_per_block, const Bracket* brackets, double rel_tol,
-                         int32_t max_iter, double* levels, int32_t* status, int32_t* iterations,
+                         int32_t n_tasks, int32_t waves_per_block, const Bracket* brackets, RefinePoints* points,
+                         double rel_tol, int32_t max_iter, double* levels, int32_t* status, int32_t* iterations,
                          int32_t* steps)
 {
     if (n_tasks == 0)
@@ -1857,10 +2006,10 @@ hipError_t launch_refine(hipStream_t stream, int32_t levels_per_lane, const doub
     const dim3 grid(static_cast<unsigned>(blocks)), block(static_cast<unsigned>(64 * waves_per_block));
     if (levels_per_lane == 2)
         hipLaunchKernelGGL(numerov_refine_kernel<2>, grid, block, 0, stream, uv, pots, windows, tasks,
-                           n_tasks, brackets, rel_tol, max_iter, levels, status, iterations, steps);
+                           n_tasks, brackets, points, rel_tol, max_iter, levels, status, iterations, steps);
     else if (levels_per_lane == 1)
         hipLaunchKernelGGL(numerov_refine_kernel<1>, grid, block, 0, stream, uv, pots, windows, tasks,
-                           n_tasks, brackets, rel_tol, max_iter, levels, status, iterations, steps);
+                           n_tasks, brackets, points, rel_tol, max_iter, levels, status, iterations, steps);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1874,8 +2023,8 @@ hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, int32_t le
                                    const double* segtab, const PotentialRecord* pots,
                                    const BandWindow* windows, const WaveTask* tasks, int32_t n_tasks,
                                    const int32_t* bins, int32_t max_bin, const Bracket* brackets,
-                                   double rel_tol, int32_t max_iter, double* levels, int32_t* status,
-                                   int32_t* iterations, int32_t* steps)
+                                   RefinePoints* points, double rel_tol, int32_t max_iter, double* levels,
+                                   int32_t* status, int32_t* iterations, int32_t* steps)
 {
     if (n_tasks == 0)
         return hipSuccess;
@@ -1884,7 +2033,7 @@ hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, int32_t le
     const dim3 grid(static_cast<unsigned>(bins == nullptr ? n_tasks : 8 * max_bin));
 #define EPSEON_SEGMENTED(T, TAB, K)                                                                   \
     hipLaunchKernelGGL((numerov_refine_segmented_kernel<T, TAB, K>), grid, dim3(T), 0, stream, uv,   \
-                       segtab, pots, windows, tasks, n_tasks, bins, brackets, rel_tol, max_iter,      \
+                       segtab, pots, windows, tasks, n_tasks, bins, brackets, points, rel_tol, max_iter, \
                        levels, status, iterations, steps)
     switch (lanes) {
     case 256:

@@ -49,6 +49,11 @@ class Bracket(ctypes.Structure):
         ("rate", ctypes.c_double),
         ("band", ctypes.c_int32),
         ("lo_foreign", ctypes.c_int32),
+        ("n_pts", ctypes.c_int32),
+        ("i_hi", ctypes.c_int32),
+        ("e_pt", ctypes.c_double * 8),
+        ("w_pt", ctypes.c_double * 8),
+        ("k_pt", ctypes.c_int32 * 8),
     ]
 
 

@@ -421,6 +421,10 @@ void eo_scan(const double* pt, int64_t n, double beta, double e_lo, double e_hi,
 
 /* --------------------------------------------------------------- bracket */
 
+/* Interpolation points of a level: at most EO_PTS (energy, F = w 2^k) pairs
+ * of its band window, kept sorted by energy. */
+#define EO_PTS 8
+
 typedef struct eo_bracket {
     double e_lo, e_hi;   /* bracket ends */
     double w_lo, w_hi;   /* far-end mantissas at the ends */
@@ -431,6 +435,11 @@ typedef struct eo_bracket {
     double rate;         /* b of the model F ~ (c0 + c1 E) e^(b E) (0: none) */
     int32_t band;        /* band of the upper end's scan group: the level's window */
     int32_t lo_foreign;  /* 1 when the lower end's value comes from another band's window */
+    int32_t n_pts;       /* scan points of the level's window near the bracket (<= EO_PTS) */
+    int32_t i_hi;        /* index of the upper end among them */
+    double e_pt[EO_PTS]; /* their energies, ascending */
+    double w_pt[EO_PTS]; /* far-end mantissas */
+    int32_t k_pt[EO_PTS]; /* binary exponents */
 } eo_bracket;
 
 /* ---- deterministic exp / log for the refinement model -----------------
@@ -582,6 +591,24 @@ void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t*
         out->k_lo = far_exp[bl] - rescales[j - 1] * 480;
         out->lo_valid = 1;
         out->lo_foreign = !eo_same_window(win, bl, band);
+    }
+    /* Interpolation points: scan points j - 4 .. j + 3 whose group shares the
+     * level's window, the lower end only when its value is of that window. */
+    out->n_pts = 0;
+    for (int64_t q = j - 4; q <= j + 3; ++q) {
+        if (q < 0 || q >= m)
+            continue;
+        if (q == j - 1 && !(out->lo_valid && !out->lo_foreign))
+            continue;
+        const int32_t bq = eo_group_band(q, m, e_lo, de, e_hi);
+        if (q != j && q != j - 1 && !eo_same_window(win, bq, band))
+            continue;
+        if (q == j)
+            out->i_hi = out->n_pts;
+        out->e_pt[out->n_pts] = eo_trial_energy(q, e_lo, de);
+        out->w_pt[out->n_pts] = w_last[q];
+        out->k_pt[out->n_pts] = far_exp[bq] - rescales[q] * 480;
+        ++out->n_pts;
     }
     /* Model rate from a third scan point in the same count regime: above
      * (j + 1, same count as j) or below (j - 2, same count as j - 1), all
@@ -856,15 +883,97 @@ void eo_shoot_segmented(const double* uv, int64_t n, double e, int32_t stride, i
 
 /* ---------------------------------------------------------------- refine */
 
+/* Interpolation points of one refinement, sorted by energy, the upper end at
+ * index ih (the lower end, when it has a value of the level's window, at
+ * ih - 1): the bracket's scan points, then every shot. */
+typedef struct {
+    double e[EO_PTS], w[EO_PTS];
+    int32_t k[EO_PTS];
+    int32_t n, ih;
+} eo_points;
+
+/* Insert a shot at em (inside the bracket: at index ih in energy order).  A
+ * full list first drops the outermost point farther from the bracket centre c
+ * (the last one on a tie). */
+static void eo_points_insert(eo_points* p, double c, double em, double w, int32_t k)
+{
+    if (p->n == EO_PTS) {
+        if (c - p->e[0] > p->e[p->n - 1] - c) {
+            for (int32_t i = 1; i < p->n; ++i) {
+                p->e[i - 1] = p->e[i];
+                p->w[i - 1] = p->w[i];
+                p->k[i - 1] = p->k[i];
+            }
+            --p->ih;
+        }
+        --p->n;
+    }
+    for (int32_t i = p->n; i > p->ih; --i) {
+        p->e[i] = p->e[i - 1];
+        p->w[i] = p->w[i - 1];
+        p->k[i] = p->k[i - 1];
+    }
+    p->e[p->ih] = em;
+    p->w[p->ih] = w;
+    p->k[p->ih] = k;
+    ++p->n;
+}
+
+/* Root in (0, 1) of the polynomial through the points in x = (E - el)/width,
+ * y = w 2^(k - max k) (Newton divided differences), by at most 8 Newton steps
+ * from x0, stopping once a step is at most 2^-40.  0 when there are fewer
+ * than 3 points, a derivative vanishes or a step leaves (0, 1). */
+static int32_t eo_points_root(const eo_points* p, double el, double width, double x0, double* root)
+{
+    const int32_t n = p->n;
+    if (n < 3)
+        return 0;
+    const double inv = 1.0 / width;
+    int32_t kmax = p->k[0];
+    for (int32_t i = 1; i < n; ++i)
+        kmax = p->k[i] > kmax ? p->k[i] : kmax;
+    double x[EO_PTS], c[EO_PTS];
+    for (int32_t i = 0; i < n; ++i) {
+        x[i] = (p->e[i] - el) * inv;
+        c[i] = ldexp(p->w[i], p->k[i] - kmax);
+    }
+    for (int32_t j = 1; j < n; ++j)
+        for (int32_t i = n - 1; i >= j; --i)
+            c[i] = (c[i] - c[i - 1]) / (x[i] - x[i - j]);
+    double t = x0;
+    for (int32_t it = 0; it < 8; ++it) {
+        double f = c[n - 1], d = 0.0;
+        for (int32_t i = n - 2; i >= 0; --i) {
+            const double u = t - x[i];
+            d = fma(d, u, f);
+            f = fma(f, u, c[i]);
+        }
+        if (!(d != 0.0))
+            return 0;
+        const double dt = f / d;
+        t = t - dt;
+        if (!(t > 0.0 && t < 1.0))
+            return 0;
+        if (fabs(dt) <= 0x1p-40)
+            break;
+    }
+    *root = t;
+    return 1;
+}
+
 /* Refine level v inside its bracket (the bracket ends carry the scan's
  * far-end values).  While the bracket holds more than level v (or its floor
  * has no shot) it is cut with counted shots at the count-interpolated point
  * (where level v would sit if the levels in it were evenly spaced, kept within
  * [1/8, 7/8] of the bracket); once it holds exactly level v
- * regula falsi with Anderson-Bjorck scaling runs on F(E) = w_{N-1} 2^k (or on
- * G = F e^(-b (E - E_hi)) when the scan gives a model rate b), each new
- * point kept at least tol/2 inside the bracket, and a probe's count follows
- * from the sign of F (count parity = sign bit of w_{N-1}).  Stops when the
+ * the probe is the root of the polynomial through the level's interpolation
+ * points (the scan points of its window near the bracket and every shot, at
+ * most EO_PTS, see eo_points_root) on F(E) = w_{N-1} 2^k (or on
+ * G = F e^(-b (E - E_hi)) when the scan gives a model rate b), started from
+ * the regula-falsi point of the ends (Anderson-Bjorck scaling), which is the
+ * probe when the root search fails; each probe is kept at least tol/2 inside
+ * the bracket, and its count follows from the sign of F (count parity = sign
+ * bit of w_{N-1}).  Stops when the
  * width is <= tol = rel_tol*max(|E_lo|,|E_hi|) + abs_tol; the level is the
  * final midpoint.  Shots are difference-form: sequential (segmented == 0) or
  * segmented over a block of `lanes` threads (segmented != 0).  Returns the
@@ -885,9 +994,19 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
      * nearly linear in E where F grows exponentially across the bracket;
      * sign(G) = sign(F), so counts and brackets are unchanged. */
     const double rate = br->rate, e0 = br->e_hi;
+    eo_points pts;
+    pts.n = br->n_pts;
+    pts.ih = br->i_hi;
+    for (int32_t i = 0; i < pts.n; ++i) {
+        pts.e[i] = br->e_pt[i];
+        pts.w[i] = br->w_pt[i];
+        pts.k[i] = br->k_pt[i];
+    }
     if (rate != 0.0) {
         eo_model_apply(rate, e0, el, &wl, &kl);
         eo_model_apply(rate, e0, eh, &wh, &kh);
+        for (int32_t i = 0; i < pts.n; ++i)
+            eo_model_apply(rate, e0, pts.e[i], &pts.w[i], &pts.k[i]);
     }
     for (;;) {
         const double width = eh - el;
@@ -908,7 +1027,11 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
         } else if (isolated) {
             const double r = ldexp(wh / wl, kh - kl);
             const double s = 1.0 / (1.0 - r);
-            em = fma(s, width, el);
+            double x;
+            if (eo_points_root(&pts, el, width, fmin(fmax(s, 0.0), 1.0), &x))
+                em = fma(x, width, el);
+            else
+                em = fma(s, width, el);
             const double dd = 0.5 * tol;
             if (!(em >= el + dd))
                 em = el + dd;
@@ -936,6 +1059,9 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
         if (rate != 0.0)
             eo_model_apply(rate, e0, em, &w, &k);
         ++it;
+        eo_points_insert(&pts, el + 0.5 * width, em, w, k);
+        if (c <= v)
+            ++pts.ih;
         /* Anderson-Bjorck: when the same end is replaced twice in a row, the
          * kept end's value is scaled by m = 1 - F_new/F_replaced (0.5 when
          * m <= 0 or undefined). */

@@ -161,9 +161,11 @@ hipError_t arena_array(Arena& a, size_t count, T** out)
 // per level.
 constexpr int64_t kLanePerLevelMinLevels = int64_t(64) * 2048;
 
-// Automatic level pairs in the segmented refinement need at least this many
-// levels per compute unit (three paired blocks per SIMD).
+// Automatic level groups in the segmented refinement need at least this many
+// levels per compute unit: pairs (three blocks per SIMD), and groups of four
+// (two blocks per SIMD, with several rounds of blocks to balance the XCD bins).
 constexpr int64_t kPairMinLevelsPerCu = 24;
+constexpr int64_t kQuadMinLevelsPerCu = 96;
 
 // Automatic windows need at least this many scan wavefronts per SIMD.
 constexpr int64_t kWindowsMinScanWavesPerSimd = 8;
@@ -320,14 +322,14 @@ struct epseon_ctx {
     int32_t xcd_max_bin_band = 0;
     int32_t* d_xcd_level_bins = nullptr;
     int32_t xcd_max_level_bin = 0;
-    // Segmented refinement with two levels per block: tasks (potential,
-    // level l) for l = 0, 2, 4, ... of each potential, in task order and
-    // binned per XCD like the level tasks.
-    WaveTask* d_pair_tasks = nullptr;
-    WaveTask* d_xcd_pair_tasks = nullptr;
-    int32_t* d_xcd_pair_bins = nullptr;
-    int32_t xcd_max_pair_bin = 0;
-    int32_t n_pair_tasks = 0;
+    // Segmented refinement with G = 2 (index 0) or 4 (index 1) levels per
+    // block: tasks (potential, level l) for l = 0, G, 2G, ... of each
+    // potential, in task order and binned per XCD like the level tasks.
+    WaveTask* d_group_tasks[2] = {nullptr, nullptr};
+    WaveTask* d_xcd_group_tasks[2] = {nullptr, nullptr};
+    int32_t* d_xcd_group_bins[2] = {nullptr, nullptr};
+    int32_t xcd_max_group_bin[2] = {0, 0};
+    int32_t n_group_tasks[2] = {0, 0};
     int32_t segmented_levels = 0;  // tuning: levels per segmented block (0: automatic)
     int32_t last_refine_levels = 1;
     bool xcd_scan = true;
@@ -341,7 +343,7 @@ struct epseon_ctx {
     // Cumulative task counts and scan work over the potentials in task order
     // (longest grids first): entry k covers the first k potentials.
     std::vector<int32_t> cum_scan_tasks, cum_refine_tasks, cum_level_tasks, cum_lane1_tasks,
-        cum_lane2_tasks, cum_pair_tasks;
+        cum_lane2_tasks, cum_group_tasks[2];
     std::vector<double> cum_scan_work;
     double solve_split = 0.0;
     // The single-stream solve's launches recorded as a HIP graph (tuning
@@ -906,7 +908,7 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     for (int32_t k = 0; k < count; ++k)
         ctx->order_pos[static_cast<size_t>(order[static_cast<size_t>(k)])] = k;
     // Segmented refinement uses one 64-lane block per level.
-    std::vector<WaveTask> scan_tasks, refine_tasks, level_tasks, lane1_tasks, lane2_tasks, pair_tasks;
+    std::vector<WaveTask> scan_tasks, refine_tasks, level_tasks, lane1_tasks, lane2_tasks, group_tasks[2];
     std::vector<PsiTask> vjp_lists[4], psi_tasks;
     ctx->cum_psi_tasks.assign(1, 0);
     ctx->cum_lane1_tasks.assign(1, 0);
@@ -914,7 +916,8 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     ctx->cum_scan_tasks.assign(1, 0);
     ctx->cum_refine_tasks.assign(1, 0);
     ctx->cum_level_tasks.assign(1, 0);
-    ctx->cum_pair_tasks.assign(1, 0);
+    ctx->cum_group_tasks[0].assign(1, 0);
+    ctx->cum_group_tasks[1].assign(1, 0);
     ctx->cum_scan_work.assign(1, 0.0);
     for (int32_t p : order) {
         const int32_t groups = (ctx->rec[p].level_count + 63) / 64;
@@ -941,8 +944,9 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
             refine_tasks.push_back({p, l});
         for (int32_t l = 0; l < r.level_count; ++l)
             level_tasks.push_back({p, l});
-        for (int32_t l = 0; l < r.level_count; l += 2)
-            pair_tasks.push_back({p, l});
+        for (int32_t g = 0; g < 2; ++g)
+            for (int32_t l = 0; l < r.level_count; l += 2 << g)
+                group_tasks[g].push_back({p, l});
         if (r.level_count > 64)
             for (int32_t l = 0; l < r.level_count; l += 128)
                 lane2_tasks.push_back({p, l});
@@ -953,7 +957,8 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
         ctx->cum_scan_tasks.push_back(static_cast<int32_t>(scan_tasks.size()));
         ctx->cum_refine_tasks.push_back(static_cast<int32_t>(refine_tasks.size()));
         ctx->cum_level_tasks.push_back(static_cast<int32_t>(level_tasks.size()));
-        ctx->cum_pair_tasks.push_back(static_cast<int32_t>(pair_tasks.size()));
+        for (int32_t g = 0; g < 2; ++g)
+            ctx->cum_group_tasks[g].push_back(static_cast<int32_t>(group_tasks[g].size()));
         ctx->cum_scan_work.push_back(
             ctx->cum_scan_work.back() +
             (r.flags == EPSEON_POT_OK ? static_cast<double>(r.n_points) * r.n_energies : 0.0));
@@ -969,21 +974,28 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     }
     for (size_t t = 0; t < level_tasks.size(); ++t)
         level_task_work[t] = static_cast<double>(ctx->rec[static_cast<size_t>(level_tasks[t].potential)].n_points);
-    std::vector<double> pair_task_work(pair_tasks.size());
-    for (size_t t = 0; t < pair_tasks.size(); ++t) {
-        const PotentialRecord& r = ctx->rec[static_cast<size_t>(pair_tasks[t].potential)];
-        pair_task_work[t] = static_cast<double>(r.n_points) * std::min(2, r.level_count - pair_tasks[t].first);
+    std::vector<double> group_task_work[2];
+    for (int32_t g = 0; g < 2; ++g) {
+        group_task_work[g].resize(group_tasks[g].size());
+        for (size_t t = 0; t < group_tasks[g].size(); ++t) {
+            const PotentialRecord& r = ctx->rec[static_cast<size_t>(group_tasks[g][t].potential)];
+            group_task_work[g][t] =
+                static_cast<double>(r.n_points) * std::min(2 << g, r.level_count - group_tasks[g][t].first);
+        }
     }
-    std::vector<int32_t> xcd_bins, xcd_level_bins, xcd_pair_bins;
+    std::vector<int32_t> xcd_bins, xcd_level_bins, xcd_group_bins[2];
     const std::vector<WaveTask> xcd_tasks =
         xcd_binned(scan_tasks, ctx->cum_scan_tasks, scan_task_work, false, xcd_bins, ctx->xcd_max_bin);
     const std::vector<WaveTask> xcd_level_tasks = xcd_binned(level_tasks, ctx->cum_level_tasks, level_task_work,
                                                              false, xcd_level_bins, ctx->xcd_max_level_bin);
-    const std::vector<WaveTask> xcd_pair_tasks = xcd_binned(pair_tasks, ctx->cum_pair_tasks, pair_task_work, false,
-                                                            xcd_pair_bins, ctx->xcd_max_pair_bin);
+    std::vector<WaveTask> xcd_group_tasks[2];
+    for (int32_t g = 0; g < 2; ++g)
+        xcd_group_tasks[g] = xcd_binned(group_tasks[g], ctx->cum_group_tasks[g], group_task_work[g], false,
+                                        xcd_group_bins[g], ctx->xcd_max_group_bin[g]);
     ctx->n_refine_tasks = static_cast<int32_t>(refine_tasks.size());
     ctx->n_level_tasks = static_cast<int32_t>(level_tasks.size());
-    ctx->n_pair_tasks = static_cast<int32_t>(pair_tasks.size());
+    for (int32_t g = 0; g < 2; ++g)
+        ctx->n_group_tasks[g] = static_cast<int32_t>(group_tasks[g].size());
 
     const size_t np = static_cast<size_t>(std::max<int64_t>(pts, 1));
     const size_t ne = static_cast<size_t>(std::max<int64_t>(en, 1));
@@ -1012,9 +1024,11 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     EPSEON_HIP_TRY(arena_array(a, size_t(9), &ctx->d_xcd_bins));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(xcd_level_tasks.size(), 1), &ctx->d_xcd_level_tasks));
     EPSEON_HIP_TRY(arena_array(a, size_t(9), &ctx->d_xcd_level_bins));
-    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(pair_tasks.size(), 1), &ctx->d_pair_tasks));
-    EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(xcd_pair_tasks.size(), 1), &ctx->d_xcd_pair_tasks));
-    EPSEON_HIP_TRY(arena_array(a, size_t(9), &ctx->d_xcd_pair_bins));
+    for (int32_t g = 0; g < 2; ++g) {
+        EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(group_tasks[g].size(), 1), &ctx->d_group_tasks[g]));
+        EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(xcd_group_tasks[g].size(), 1), &ctx->d_xcd_group_tasks[g]));
+        EPSEON_HIP_TRY(arena_array(a, size_t(9), &ctx->d_xcd_group_bins[g]));
+    }
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(refine_tasks.size(), 1), &ctx->d_refine_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(level_tasks.size(), 1), &ctx->d_level_tasks));
     EPSEON_HIP_TRY(arena_array(a, std::max<size_t>(lane1_tasks.size(), 1), &ctx->d_lane1_tasks));
@@ -1070,15 +1084,18 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
                                       ctx->stream));
     EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_level_bins, xcd_level_bins.data(), 9 * sizeof(int32_t),
                                   hipMemcpyHostToDevice, ctx->stream));
-    if (!pair_tasks.empty())
-        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_pair_tasks, pair_tasks.data(), pair_tasks.size() * sizeof(WaveTask),
+    for (int32_t g = 0; g < 2; ++g) {
+        if (!group_tasks[g].empty())
+            EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_group_tasks[g], group_tasks[g].data(),
+                                          group_tasks[g].size() * sizeof(WaveTask), hipMemcpyHostToDevice,
+                                          ctx->stream));
+        if (!xcd_group_tasks[g].empty())
+            EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_group_tasks[g], xcd_group_tasks[g].data(),
+                                          xcd_group_tasks[g].size() * sizeof(WaveTask), hipMemcpyHostToDevice,
+                                          ctx->stream));
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_group_bins[g], xcd_group_bins[g].data(), 9 * sizeof(int32_t),
                                       hipMemcpyHostToDevice, ctx->stream));
-    if (!xcd_pair_tasks.empty())
-        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_pair_tasks, xcd_pair_tasks.data(),
-                                      xcd_pair_tasks.size() * sizeof(WaveTask), hipMemcpyHostToDevice,
-                                      ctx->stream));
-    EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_pair_bins, xcd_pair_bins.data(), 9 * sizeof(int32_t),
-                                  hipMemcpyHostToDevice, ctx->stream));
+    }
     EPSEON_HIP_TRY(hipMemcpyAsync(ctx->d_xcd_bins, xcd_bins.data(), 9 * sizeof(int32_t),
                                   hipMemcpyHostToDevice, ctx->stream));
     if (!refine_tasks.empty())
@@ -1277,9 +1294,10 @@ int32_t segmented_lanes(const epseon_ctx* ctx)
     return lanes;
 }
 
-// Levels per segmented block: two in 64-lane blocks (each table load then
-// feeds both levels' shots) once the batch has at least kPairMinLevelsPerCu
-// levels per compute unit, so that the pairs still give every SIMD several
+// Levels per segmented block: in 64-lane blocks four (each table load then
+// feeds the shots of every level of the group in the same window) once the
+// batch has at least kQuadMinLevelsPerCu levels per compute unit, two from
+// kPairMinLevelsPerCu, so that the groups still give every SIMD several
 // blocks; else one (unless fixed by the tuning).
 int32_t segmented_levels(const epseon_ctx* ctx, int32_t lanes)
 {
@@ -1287,6 +1305,8 @@ int32_t segmented_levels(const epseon_ctx* ctx, int32_t lanes)
         return 1;
     if (ctx->segmented_levels != 0)
         return ctx->segmented_levels;
+    if (ctx->total_levels >= kQuadMinLevelsPerCu * ctx->compute_units)
+        return 4;
     return ctx->total_levels >= kPairMinLevelsPerCu * ctx->compute_units ? 2 : 1;
 }
 
@@ -1391,17 +1411,19 @@ hipError_t run_refine(epseon_ctx* ctx, hipStream_t stream, hipStream_t stream_b,
     const bool whole = ctx->xcd_scan && k0 == 0 && k1 == ctx->cum_level_tasks.size() - 1;
     const int32_t lpb = segmented_levels(ctx, lanes);
     ctx->last_refine_levels = lpb;
-    if (lpb == 2) {
-        const int32_t p0 = ctx->cum_pair_tasks[k0];
-        const int32_t p1 = ctx->cum_pair_tasks[k1];
+    if (lpb == 2 || lpb == 4) {
+        const int32_t g = lpb == 2 ? 0 : 1;
+        const int32_t p0 = ctx->cum_group_tasks[g][k0];
+        const int32_t p1 = ctx->cum_group_tasks[g][k1];
         if (whole)
-            return launch_refine_segmented(stream, lanes, 2, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_xcd_pair_tasks,
-                                           p1 - p0, ctx->d_xcd_pair_bins, ctx->xcd_max_pair_bin, ctx->d_brackets,
-                                           ctx->d_points, rel_tol, max_iter, ctx->d_levels, ctx->d_status,
-                                           ctx->d_iters, ctx->d_steps);
-        return launch_refine_segmented(stream, lanes, 2, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_pair_tasks + p0,
-                                       p1 - p0, nullptr, 0, ctx->d_brackets, ctx->d_points, rel_tol, max_iter,
-                                       ctx->d_levels, ctx->d_status, ctx->d_iters, ctx->d_steps);
+            return launch_refine_segmented(stream, lanes, lpb, ctx->d_uv, tab, ctx->d_rec, win,
+                                           ctx->d_xcd_group_tasks[g], p1 - p0, ctx->d_xcd_group_bins[g],
+                                           ctx->xcd_max_group_bin[g], ctx->d_brackets, ctx->d_points, rel_tol,
+                                           max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters, ctx->d_steps);
+        return launch_refine_segmented(stream, lanes, lpb, ctx->d_uv, tab, ctx->d_rec, win,
+                                       ctx->d_group_tasks[g] + p0, p1 - p0, nullptr, 0, ctx->d_brackets,
+                                       ctx->d_points, rel_tol, max_iter, ctx->d_levels, ctx->d_status, ctx->d_iters,
+                                       ctx->d_steps);
     }
     if (whole)
         return launch_refine_segmented(stream, lanes, 1, ctx->d_uv, tab, ctx->d_rec, win, ctx->d_xcd_level_tasks,
@@ -1710,8 +1732,9 @@ int epseon_set_tuning(epseon_ctx* ctx, const epseon_tuning* t)
         return fail(EPSEON_ERR_INVALID, "windows must be 0, 1 or 2");
     if (t->graph != 0 && t->graph != 1)
         return fail(EPSEON_ERR_INVALID, "graph must be 0 or 1");
-    if (t->segmented_levels < 0 || t->segmented_levels > 2)
-        return fail(EPSEON_ERR_INVALID, "segmented_levels must be 0, 1 or 2");
+    if (t->segmented_levels != 0 && t->segmented_levels != 1 && t->segmented_levels != 2 &&
+        t->segmented_levels != 4)
+        return fail(EPSEON_ERR_INVALID, "segmented_levels must be 0, 1, 2 or 4");
     ctx->refine_mode = t->refine_mode;
     ctx->levels_per_lane = t->levels_per_lane;
     ctx->waves_per_block = t->scan_waves_per_block;
@@ -1819,12 +1842,17 @@ int epseon_autotune(epseon_ctx* ctx, double rel_tol, int32_t max_iter, int32_t f
                 if (const int rc = trial(c); rc != EPSEON_OK)
                     return rc;
             }
-        // Level pairs leave every level's arithmetic unchanged.
+        // Level groups leave every level's arithmetic unchanged.
         if ((best.segmented_lanes == 0 ? segmented_lanes(ctx) : best.segmented_lanes) == 64) {
-            epseon_tuning c = best;
-            c.segmented_levels = 3 - best.segmented_levels;
-            if (const int rc = trial(c); rc != EPSEON_OK)
-                return rc;
+            const int32_t current = best.segmented_levels;
+            for (int32_t levels : {1, 2, 4}) {
+                if (levels == current)
+                    continue;
+                epseon_tuning c = best;
+                c.segmented_levels = levels;
+                if (const int rc = trial(c); rc != EPSEON_OK)
+                    return rc;
+            }
         }
     }
     for (double split : {0.15, 0.3, 0.5}) {

@@ -874,15 +874,17 @@ __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const 
 
 // Insert a shot at em (inside the bracket: at index ih in energy order).  A
 // full list first drops the outermost point farther from the bracket centre c
-// (the last one on a tie).
+// (the last one on a tie).  Every slot is addressed by a constant index
+// (predicated moves), so a list held in registers stays there.
 __device__ __forceinline__ void points_insert(RefinePoints& p, const double c, const double em, const double w,
                                               const int32_t k)
 {
     int32_t n = p.n;
     int32_t ih = p.ih;
     if (n == kPts) {
-        if (c - p.e[0] > p.e[n - 1] - c) {
-            for (int32_t i = 1; i < n; ++i) {
+        if (c - p.e[0] > p.e[kPts - 1] - c) {
+#pragma unroll
+            for (int32_t i = 1; i < kPts; ++i) {
                 p.e[i - 1] = p.e[i];
                 p.w[i - 1] = p.w[i];
                 p.k[i - 1] = p.k[i];
@@ -891,14 +893,21 @@ __device__ __forceinline__ void points_insert(RefinePoints& p, const double c, c
         }
         --n;
     }
-    for (int32_t i = n; i > ih; --i) {
-        p.e[i] = p.e[i - 1];
-        p.w[i] = p.w[i - 1];
-        p.k[i] = p.k[i - 1];
-    }
-    p.e[ih] = em;
-    p.w[ih] = w;
-    p.k[ih] = k;
+    // Slots ih .. n - 1 move up by one (from the top down), slot ih takes the shot.
+#pragma unroll
+    for (int32_t i = kPts - 1; i >= 1; --i)
+        if (i > ih && i <= n) {
+            p.e[i] = p.e[i - 1];
+            p.w[i] = p.w[i - 1];
+            p.k[i] = p.k[i - 1];
+        }
+#pragma unroll
+    for (int32_t i = 0; i < kPts; ++i)
+        if (i == ih) {
+            p.e[i] = em;
+            p.w[i] = w;
+            p.k[i] = k;
+        }
     p.n = n + 1;
     p.ih = ih;
 }
@@ -977,7 +986,7 @@ __device__ __forceinline__ bool points_root(const RefinePoints& p, const double 
 struct Refiner {
     Bracket b;
     int32_t v, side, it;
-    bool isolated, reshoot;
+    int32_t isolated, reshoot;
     double rate, e0;
 
     // Regula falsi runs on G = F e^(-b (E - E_hi0)) when the scan gave a
@@ -1239,11 +1248,13 @@ __device__ __forceinline__ void walk_tail(const double (&buf)[kWalkChunk], const
 // (DATA_FORMAT = 4, i.e. 32-bit elements; everything else zero).
 constexpr int32_t kBufferResourceWord3 = 0x00020000;
 
-// Grid values of one segment, value j being the j-th step's point.
-// RowSource reads the natural uv table (lane-private addresses); ColumnSource
-// reads the lane's column of the segment table through a wave-uniform buffer
-// resource (row r at byte off + r * 512), so each load instruction of a
-// wavefront covers 512 contiguous bytes and needs one offset register.
+// Grid values of one segment, value j being the j-th step's point (j
+// wave-uniform).  RowSource reads the natural uv table (lane-private
+// addresses); ColumnSource reads the lane's column of the segment table
+// through a wave-uniform buffer resource (row r at byte off + r * 512: the
+// lane's column offset in the vector offset, the row's in the scalar offset),
+// so each load instruction of a wavefront covers 512 contiguous bytes and
+// needs no address arithmetic on the vector ALU.
 struct RowSource {
     const double* p;
     __device__ __forceinline__ double at(const int32_t j) const { return p[j]; }
@@ -1252,12 +1263,13 @@ struct RowSource {
 struct ColumnSource {
     __amdgpu_buffer_rsrc_t rs;
     int32_t off;
-    __device__ __forceinline__ double at(const int32_t j) const
+    __device__ __forceinline__ double row(const int32_t r) const
     {
         typedef int32_t v2i __attribute__((ext_vector_type(2)));
-        const v2i x = __builtin_amdgcn_raw_buffer_load_b64(rs, off + j * 512, 0, 0);
+        const v2i x = __builtin_amdgcn_raw_buffer_load_b64(rs, off, r * 512, 0);
         return __builtin_bit_cast(double, x);
     }
+    __device__ __forceinline__ double at(const int32_t j) const { return row(j + 1); }
 };
 
 template <class Src>
@@ -1268,20 +1280,22 @@ __device__ __forceinline__ void load_chunk(double (&buf)[kWalkChunk], const Src&
         buf[k] = src.at(j + k);
 }
 
-// Visit the `len` values src.at(0 .. len - 1) in order, body(value, index).
-// Three kWalkChunk-value VGPR buffers rotate so that every load is issued two
-// chunks before its use; vector-memory loads return in order, so the waits
-// are partial.  Also reads the 2 kWalkChunk values that follow the segment
-// (the tables are padded).
+// Visit the `len` values src.at(0 .. len - 1) in order, body(value, index);
+// ulen is wave-uniform with ulen <= len <= ulen + 1 (segment lengths differ by
+// at most one), so the step index stays wave-uniform and only the last step
+// is predicated per lane.  Three kWalkChunk-value VGPR buffers rotate so that
+// every load is issued two chunks before its use; vector-memory loads return
+// in order, so the waits are partial.  Also reads the 2 kWalkChunk values
+// that follow the segment (the tables are padded).
 template <class Src, class Body>
-__device__ __forceinline__ void walk_segment(const Src& src, const int32_t len, Body& body)
+__device__ __forceinline__ void walk_segment(const Src& src, const int32_t len, const int32_t ulen, Body& body)
 {
     constexpr int32_t C = kWalkChunk;
     double A[C], B[C], D[C];
     load_chunk(A, src, 0);
     load_chunk(B, src, C);
     int32_t j = 0;
-    for (; j + 3 * C <= len; j += 3 * C) {
+    for (; j + 3 * C <= ulen; j += 3 * C) {
         load_chunk(D, src, j + 2 * C);
         walk_chunk(A, j, body);
         load_chunk(A, src, j + 3 * C);
@@ -1289,7 +1303,7 @@ __device__ __forceinline__ void walk_segment(const Src& src, const int32_t len, 
         load_chunk(B, src, j + 4 * C);
         walk_chunk(D, j + 2 * C, body);
     }
-    const int32_t rem = len - j;
+    const int32_t rem = ulen - j;
     if (rem >= 2 * C) {
         load_chunk(D, src, j + 2 * C);
         walk_chunk(A, j, body);
@@ -1341,16 +1355,18 @@ struct MatrixBody {
     }
 };
 
-// Two levels' transfer matrices through one segment of the same window: the
-// two bodies share every grid value, so each table load feeds both (the walk
-// of one level is bound by those loads, not by its FP64 work).  Each member's
-// arithmetic is that of a lone MatrixBody.
-struct MatrixPair {
-    MatrixBody x, y;
+// KK levels' transfer matrices through one segment of the same window: the
+// bodies share every grid value, so each table load feeds all of them (the
+// walk of one or two levels is bound by those loads, not by its FP64 work).
+// Each member's arithmetic is that of a lone MatrixBody.
+template <int KK>
+struct MatrixGroup {
+    MatrixBody m[KK];
     __device__ __forceinline__ void operator()(const double u, const int32_t idx)
     {
-        x(u, idx);
-        y(u, idx);
+#pragma unroll
+        for (int q = 0; q < KK; ++q)
+            m[q](u, idx);
     }
 };
 
@@ -1388,15 +1404,6 @@ struct CountBody {
             d = __builtin_ldexp(d, -sh);
             wm = __builtin_ldexp(wm, -sh);
         }
-    }
-};
-
-struct CountPair {
-    CountBody x, y;
-    __device__ __forceinline__ void operator()(const double u, const int32_t idx)
-    {
-        x(u, idx);
-        y(u, idx);
     }
 };
 
@@ -1448,16 +1455,16 @@ __global__ __launch_bounds__(256) void segment_table_kernel(const double* __rest
 
 namespace {
 
-// Shared memory of the segmented refinement, one slot per level of a block:
-// the segment matrices (entries and binary exponents), the chained start
-// vectors and segment node counts of counted shots, and the joined counts.
-template <int T, int K>
+// Shared memory of the segmented refinement (one shot at a time): the
+// segment matrices (entries and binary exponents), the chained start vectors
+// and segment node counts of a counted shot, and the joined count.
+template <int T>
 struct SegShared {
-    double m[K][4][T];
-    int32_t e[K][T];
-    double start[K][2][T];
-    SegCount cnt[K][T];
-    int32_t c[K];
+    double m[4][T];
+    int32_t e[T];
+    double start[2][T];
+    SegCount cnt[T];
+    int32_t c;
 };
 
 // One level's shot geometry in a T-thread block: its band window (Dirichlet
@@ -1466,21 +1473,22 @@ struct SegShared {
 template <bool TABLE>
 struct SegGeom {
     const double* u;
-    int32_t lo, hi, seg_row, n, qn, segs, q0, q1;
+    int32_t lo, hi, seg_row, n, qn, segs, q0, q1, ulen;
     bool active;
     double u_start;
     ColumnSource cs;
 
-    // Lane `lane`'s steps [q0, q1) of the window (64-bit divisions: done once
-    // per level, not per shot).
+    // Lane `lane`'s steps [q0, q1) of the window (once per level):
+    // floor(s q / sg) = s (q / sg) + floor(s (q mod sg) / sg), so 32-bit
+    // arithmetic gives segment_start's values (s (q mod sg) < 2^16).
     static __device__ __forceinline__ int2 bounds(const BandWindow& B, const int32_t lane, const int32_t T)
     {
         const int32_t q = B.hi - B.lo - 1;
         const int32_t sg = min(T, max(1, q / 32));
         if (lane >= sg)
             return make_int2(0, 0);
-        return make_int2(static_cast<int32_t>(segment_start(q, sg, lane)),
-                         static_cast<int32_t>(segment_start(q, sg, lane + 1)));
+        const int32_t base = q / sg, rem = q % sg;
+        return make_int2(lane * base + (lane * rem) / sg, (lane + 1) * base + ((lane + 1) * rem) / sg);
     }
 
     __device__ __forceinline__ void init(const double* uv_pot, const double* segtab, const BandWindow& B,
@@ -1496,6 +1504,8 @@ struct SegGeom {
         active = lane < segs;
         q0 = qb.x;
         q1 = qb.y;
+        // Every segment has qn / segs or qn / segs + 1 steps.
+        ulen = __builtin_amdgcn_readfirstlane(qn / segs);
         cs = ColumnSource{};
         if constexpr (TABLE) {
             // With the segment table, lane s reads column s (row r = point
@@ -1510,8 +1520,8 @@ struct SegGeom {
             cs.rs = __builtin_amdgcn_make_buffer_rsrc(
                 reinterpret_cast<double*>((static_cast<uint64_t>(bhi) << 32) | blo), static_cast<int16_t>(0),
                 bytes, kBufferResourceWord3);
-            cs.off = lane * 8 + 512;
-            u_start = cs.at(-1);
+            cs.off = lane * 8;
+            u_start = cs.row(0);
         } else {
             u_start = u[q0];
         }
@@ -1521,9 +1531,9 @@ struct SegGeom {
     __device__ __forceinline__ void walk(Body& body) const
     {
         if constexpr (TABLE)
-            walk_segment(cs, q1 - q0, body);
+            walk_segment(cs, q1 - q0, ulen, body);
         else
-            walk_segment(RowSource{u + q0 + 1}, q1 - q0, body);
+            walk_segment(RowSource{u + q0 + 1}, q1 - q0, ulen, body);
     }
 };
 
@@ -1543,53 +1553,91 @@ __device__ __forceinline__ SegMat normalized(const MatrixBody& mb)
                   __builtin_ldexp(mb.db, -s), mb.ex + s};
 }
 
-template <int T, int K>
-__device__ __forceinline__ void store_seg(SegShared<T, K>& sh, const int32_t slot, const int32_t lane,
-                                          const SegMat& a)
+template <int T>
+__device__ __forceinline__ void store_seg(SegShared<T>& sh, const int32_t lane, const SegMat& a)
 {
-    sh.m[slot][0][lane] = a.m00;
-    sh.m[slot][1][lane] = a.m01;
-    sh.m[slot][2][lane] = a.m10;
-    sh.m[slot][3][lane] = a.m11;
-    sh.e[slot][lane] = a.e;
+    sh.m[0][lane] = a.m00;
+    sh.m[1][lane] = a.m01;
+    sh.m[2][lane] = a.m10;
+    sh.m[3][lane] = a.m11;
+    sh.e[lane] = a.e;
 }
 
-// The binary-tree product of a wavefront's segment matrices in registers:
-// at d = 1, 2, 4, ... lane i (a multiple of 2d) takes M_{i+d} M_i, the
-// partner's matrix arriving by a cross-lane shuffle; the same operations as
-// tree_step.  Lane 0 ends with the product of all segments.
+// Value of lane + D (lanes whose partner lies outside the wavefront get an
+// unused value): data-parallel moves within 16-lane rows for D < 16, a
+// swizzle within 32-lane halves for D = 16, lane 32 for D = 32 (used by lane
+// 0 only).
+template <int D>
+__device__ __forceinline__ int32_t lane_plus(const int32_t x)
+{
+    if constexpr (D < 16)
+        return __builtin_amdgcn_mov_dpp(x, 0x100 + D, 0xF, 0xF, false);  // row_shl:D
+    else if constexpr (D == 16)
+        return __builtin_amdgcn_ds_swizzle(x, 0x401F);  // lane ^ 16 within 32 lanes
+    else
+        return __builtin_amdgcn_readlane(x, 32);
+}
+
+template <int D>
+__device__ __forceinline__ double lane_plus(const double x)
+{
+    const int2 v = __builtin_bit_cast(int2, x);
+    return __builtin_bit_cast(double, make_int2(lane_plus<D>(v.x), lane_plus<D>(v.y)));
+}
+
+// One level of the binary-tree product in registers: lane i (a multiple of
+// 2D with i + D < segs) takes M_{i+D} M_i; the same operations as tree_step.
+template <int D>
+__device__ __forceinline__ void tree_round(SegMat& a, const int32_t lane, const int32_t segs)
+{
+    const double b00 = lane_plus<D>(a.m00), b01 = lane_plus<D>(a.m01);
+    const double b10 = lane_plus<D>(a.m10), b11 = lane_plus<D>(a.m11);
+    const int32_t be = lane_plus<D>(a.e);
+    if ((lane & (2 * D - 1)) == 0 && lane + D < segs) {
+        const double c00 = __builtin_fma(b00, a.m00, b01 * a.m10);
+        const double c01 = __builtin_fma(b00, a.m01, b01 * a.m11);
+        const double c10 = __builtin_fma(b10, a.m00, b11 * a.m10);
+        const double c11 = __builtin_fma(b10, a.m01, b11 * a.m11);
+        const int32_t s = frexp_exponent(fmax(fmax(fabs(c00), fabs(c01)), fmax(fabs(c10), fabs(c11))));
+        a.m00 = __builtin_ldexp(c00, -s);
+        a.m01 = __builtin_ldexp(c01, -s);
+        a.m10 = __builtin_ldexp(c10, -s);
+        a.m11 = __builtin_ldexp(c11, -s);
+        a.e = be + a.e + s;
+    }
+}
+
+// The binary-tree product of a wavefront's segment matrices in registers
+// (d = 1, 2, 4, ... while d < segs); lane 0 ends with the product of all
+// segments.  Every lane of the wavefront takes part (the cross-lane moves
+// read all lanes).
 __device__ __forceinline__ void tree_in_registers(SegMat& a, const int32_t lane, const int32_t segs)
 {
-    for (int32_t d = 1; d < segs; d *= 2) {
-        const double b00 = __shfl_down(a.m00, d, 64), b01 = __shfl_down(a.m01, d, 64);
-        const double b10 = __shfl_down(a.m10, d, 64), b11 = __shfl_down(a.m11, d, 64);
-        const int32_t be = __shfl_down(a.e, d, 64);
-        if ((lane & (2 * d - 1)) == 0 && lane + d < segs) {
-            const double c00 = __builtin_fma(b00, a.m00, b01 * a.m10);
-            const double c01 = __builtin_fma(b00, a.m01, b01 * a.m11);
-            const double c10 = __builtin_fma(b10, a.m00, b11 * a.m10);
-            const double c11 = __builtin_fma(b10, a.m01, b11 * a.m11);
-            const int32_t s = frexp_exponent(fmax(fmax(fabs(c00), fabs(c01)), fmax(fabs(c10), fabs(c11))));
-            a.m00 = __builtin_ldexp(c00, -s);
-            a.m01 = __builtin_ldexp(c01, -s);
-            a.m10 = __builtin_ldexp(c10, -s);
-            a.m11 = __builtin_ldexp(c11, -s);
-            a.e = be + a.e + s;
-        }
-    }
+    if (segs > 1)
+        tree_round<1>(a, lane, segs);
+    if (segs > 2)
+        tree_round<2>(a, lane, segs);
+    if (segs > 4)
+        tree_round<4>(a, lane, segs);
+    if (segs > 8)
+        tree_round<8>(a, lane, segs);
+    if (segs > 16)
+        tree_round<16>(a, lane, segs);
+    if (segs > 32)
+        tree_round<32>(a, lane, segs);
 }
 
 // Start vector of every segment of a counted shot: the matrices chained from
 // (w_1, d_1) = (1, 1), renormalised after each segment (one lane).
-template <int T, int K>
-__device__ __forceinline__ void chain_starts(SegShared<T, K>& sh, const int32_t slot, const int32_t segs)
+template <int T>
+__device__ __forceinline__ void chain_starts(SegShared<T>& sh, const int32_t segs)
 {
     double v0 = 1.0, v1 = 1.0;
     for (int32_t s = 0; s < segs; ++s) {
-        sh.start[slot][0][s] = v0;
-        sh.start[slot][1][s] = v1;
-        const double a = __builtin_fma(sh.m[slot][0][s], v0, sh.m[slot][1][s] * v1);
-        const double b = __builtin_fma(sh.m[slot][2][s], v0, sh.m[slot][3][s] * v1);
+        sh.start[0][s] = v0;
+        sh.start[1][s] = v1;
+        const double a = __builtin_fma(sh.m[0][s], v0, sh.m[1][s] * v1);
+        const double b = __builtin_fma(sh.m[2][s], v0, sh.m[3][s] * v1);
         const int32_t x = frexp_exponent(fmax(fabs(a), fabs(b)));
         v0 = __builtin_ldexp(a, -x);
         v1 = __builtin_ldexp(b, -x);
@@ -1597,33 +1645,32 @@ __device__ __forceinline__ void chain_starts(SegShared<T, K>& sh, const int32_t 
 }
 
 // One level of the binary-tree product: M_lane <- M_{lane+d} M_lane.
-template <int T, int K>
-__device__ __forceinline__ void tree_step(SegShared<T, K>& sh, const int32_t slot, const int32_t lane,
-                                          const int32_t d)
+template <int T>
+__device__ __forceinline__ void tree_step(SegShared<T>& sh, const int32_t lane, const int32_t d)
 {
-    const double a00 = sh.m[slot][0][lane + d], a01 = sh.m[slot][1][lane + d];
-    const double a10 = sh.m[slot][2][lane + d], a11 = sh.m[slot][3][lane + d];
-    const double b00 = sh.m[slot][0][lane], b01 = sh.m[slot][1][lane];
-    const double b10 = sh.m[slot][2][lane], b11 = sh.m[slot][3][lane];
+    const double a00 = sh.m[0][lane + d], a01 = sh.m[1][lane + d];
+    const double a10 = sh.m[2][lane + d], a11 = sh.m[3][lane + d];
+    const double b00 = sh.m[0][lane], b01 = sh.m[1][lane];
+    const double b10 = sh.m[2][lane], b11 = sh.m[3][lane];
     const double c00 = __builtin_fma(a00, b00, a01 * b10);
     const double c01 = __builtin_fma(a00, b01, a01 * b11);
     const double c10 = __builtin_fma(a10, b00, a11 * b10);
     const double c11 = __builtin_fma(a10, b01, a11 * b11);
     const int32_t s = frexp_exponent(fmax(fmax(fabs(c00), fabs(c01)), fmax(fabs(c10), fabs(c11))));
-    sh.m[slot][0][lane] = __builtin_ldexp(c00, -s);
-    sh.m[slot][1][lane] = __builtin_ldexp(c01, -s);
-    sh.m[slot][2][lane] = __builtin_ldexp(c10, -s);
-    sh.m[slot][3][lane] = __builtin_ldexp(c11, -s);
-    sh.e[slot][lane] = sh.e[slot][lane + d] + sh.e[slot][lane] + s;
+    sh.m[0][lane] = __builtin_ldexp(c00, -s);
+    sh.m[1][lane] = __builtin_ldexp(c01, -s);
+    sh.m[2][lane] = __builtin_ldexp(c10, -s);
+    sh.m[3][lane] = __builtin_ldexp(c11, -s);
+    sh.e[lane] = sh.e[lane + d] + sh.e[lane] + s;
 }
 
 // Node count of a counted shot from its segments' counts (one lane).
-template <int T, int K>
-__device__ __forceinline__ int32_t join_counts(const SegShared<T, K>& sh, const int32_t slot, const int32_t segs)
+template <int T>
+__device__ __forceinline__ int32_t join_counts(const SegShared<T>& sh, const int32_t segs)
 {
     int32_t total = 0, prev = 0;
     for (int32_t s = 0; s < segs; ++s) {
-        const SegCount sc = sh.cnt[slot][s];
+        const SegCount sc = sh.cnt[s];
         if (sc.has) {
             total += sc.internal + (sc.first ^ prev);
             prev = sc.last;
@@ -1632,124 +1679,80 @@ __device__ __forceinline__ int32_t join_counts(const SegShared<T, K>& sh, const 
     return total;
 }
 
-// One segmented shot of KK levels (1, or 2 sharing the window of g): every
-// lane builds its segment's matrix for each energy e[k] (slot k of the
-// shared memory), then the tree product gives the far end F = fw 2^fk of
-// each; for a counted slot the segments are walked again from their chained
-// start vectors and the node count is joined across segment borders.  In
-// 64-lane blocks (one wavefront) the tree runs in registers and the
-// matrices go to LDS only for a counted shot's chain.
-template <int T, int K, int KK, bool TABLE>
-__device__ __forceinline__ void segmented_shot(SegShared<T, K>& sh, const SegGeom<TABLE>& g, const int32_t lane,
-                                               const int32_t slot0, const double (&e)[KK],
-                                               const bool (&counted)[KK], const int32_t stride,
+// One segmented shot of KK levels (1, 2 or 4 sharing the window of g; only a
+// lone level's shot may be counted): every lane builds its segment's matrix
+// for each energy e[k], then the tree product gives the far end F = fw 2^fk
+// of each.  A counted shot stores the matrices in LDS, chains them (lane 0)
+// for the segment start vectors, walks every segment again and joins the
+// node count across segment borders.  In 64-lane blocks (one wavefront) the
+// tree runs in registers.
+template <int T, int KK, bool TABLE>
+__device__ __forceinline__ void segmented_shot(SegShared<T>& sh, const SegGeom<TABLE>& g, const int32_t lane,
+                                               const double (&e)[KK], const bool counted, const int32_t stride,
                                                const StepConsts& k, double (&fw)[KK], int32_t (&fk)[KK],
-                                               int32_t (&c)[KK])
+                                               int32_t& c)
 {
+    static_assert(KK == 1 || KK == 2 || KK == 4, "1, 2 or 4 levels per shot");
     const int32_t segs = g.segs;
-    bool any_counted = false;
-#pragma unroll
-    for (int q = 0; q < KK; ++q)
-        any_counted = any_counted || counted[q];
     SegMat a[KK];
 #pragma unroll
     for (int q = 0; q < KK; ++q)
         a[q] = SegMat{1.0, 0.0, 0.0, 1.0, 0};
     if (g.active) {
-        if constexpr (KK == 1) {
-            MatrixBody mb{e[0], e[0] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10};
-            g.walk(mb);
-            a[0] = normalized(mb);
-        } else {
-            MatrixPair mp{{e[0], e[0] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10},
-                          {e[1], e[1] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10}};
-            g.walk(mp);
-            a[0] = normalized(mp.x);
-            a[1] = normalized(mp.y);
-        }
+        MatrixGroup<KK> mg;
+#pragma unroll
+        for (int q = 0; q < KK; ++q)
+            mg.m[q] = MatrixBody{e[q], e[q] - g.u_start, 1.0, 0.0, 1.0, 0.0, 1.0, -1.0, 0, k.c10};
+        g.walk(mg);
+#pragma unroll
+        for (int q = 0; q < KK; ++q)
+            a[q] = normalized(mg.m[q]);
     }
     constexpr bool kRegisterTree = T == 64;
-    if (!kRegisterTree || any_counted) {
-        if (g.active) {
-#pragma unroll
-            for (int q = 0; q < KK; ++q)
-                if (!kRegisterTree || counted[q])
-                    store_seg(sh, slot0 + q, lane, a[q]);
-        }
+    if (!kRegisterTree || counted) {
+        if (g.active)
+            store_seg(sh, lane, a[0]);
         __syncthreads();
     }
-    if (any_counted) {
-        if (lane == 0) {
-#pragma unroll
-            for (int q = 0; q < KK; ++q)
-                if (counted[q])
-                    chain_starts(sh, slot0 + q, segs);
-        }
+    if (counted) {
+        if (lane == 0)
+            chain_starts(sh, segs);
         __syncthreads();
     }
     if constexpr (kRegisterTree) {
 #pragma unroll
         for (int q = 0; q < KK; ++q) {
             tree_in_registers(a[q], lane, segs);
-            fw[q] = __shfl(__builtin_fma(a[q].m00, 1.0, a[q].m01 * 1.0), 0, 64);
-            fk[q] = __shfl(a[q].e, 0, 64);
+            // Lane 0's product, to every lane.
+            const int2 f = __builtin_bit_cast(int2, __builtin_fma(a[q].m00, 1.0, a[q].m01 * 1.0));
+            fw[q] = __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(f.x, 0),
+                                                         __builtin_amdgcn_readlane(f.y, 0)));
+            fk[q] = __builtin_amdgcn_readlane(a[q].e, 0);
         }
     } else {
         for (int32_t d = 1; d < segs; d *= 2) {
-            if ((lane & (2 * d - 1)) == 0 && lane + d < segs) {
-#pragma unroll
-                for (int q = 0; q < KK; ++q)
-                    tree_step(sh, slot0 + q, lane, d);
-            }
+            if ((lane & (2 * d - 1)) == 0 && lane + d < segs)
+                tree_step(sh, lane, d);
             __syncthreads();
         }
-#pragma unroll
-        for (int q = 0; q < KK; ++q) {
-            fw[q] = __builtin_fma(sh.m[slot0 + q][0][0], 1.0, sh.m[slot0 + q][1][0] * 1.0);
-            fk[q] = sh.e[slot0 + q][0];
-        }
+        fw[0] = __builtin_fma(sh.m[0][0], 1.0, sh.m[1][0] * 1.0);
+        fk[0] = sh.e[0];
     }
-#pragma unroll
-    for (int q = 0; q < KK; ++q)
-        c[q] = 0;
-    if (!any_counted)
+    c = 0;
+    if (!counted)
         return;
     if (g.active) {
-        const auto count_body = [&](const int q) {
-            const double w0 = sh.start[slot0 + q][0][lane], d0 = sh.start[slot0 + q][1][lane];
-            return CountBody{e[q], e[q] - g.u_start, w0, d0, w0 - d0, g.q0, stride - 1,
-                             (g.qn & (stride - 1)) != 0 ? g.qn - 1 : -1, {0, 0, 0, 0}, k.c10};
-        };
-        if constexpr (KK == 2) {
-            if (counted[0] && counted[1]) {
-                CountPair cp{count_body(0), count_body(1)};
-                g.walk(cp);
-                sh.cnt[slot0][lane] = cp.x.sc;
-                sh.cnt[slot0 + 1][lane] = cp.y.sc;
-            } else {
-                const int q = counted[0] ? 0 : 1;
-                CountBody cb = count_body(q);
-                g.walk(cb);
-                sh.cnt[slot0 + q][lane] = cb.sc;
-            }
-        } else {
-            CountBody cb = count_body(0);
-            g.walk(cb);
-            sh.cnt[slot0][lane] = cb.sc;
-        }
+        const double w0 = sh.start[0][lane], d0 = sh.start[1][lane];
+        CountBody cb{e[0], e[0] - g.u_start, w0, d0, w0 - d0, g.q0, stride - 1,
+                     (g.qn & (stride - 1)) != 0 ? g.qn - 1 : -1, {0, 0, 0, 0}, k.c10};
+        g.walk(cb);
+        sh.cnt[lane] = cb.sc;
     }
     __syncthreads();
-    if (lane == 0) {
-#pragma unroll
-        for (int q = 0; q < KK; ++q)
-            if (counted[q])
-                sh.c[slot0 + q] = join_counts(sh, slot0 + q, segs);
-    }
+    if (lane == 0)
+        sh.c = join_counts(sh, segs);
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < KK; ++q)
-        if (counted[q])
-            c[q] = sh.c[slot0 + q];
+    c = sh.c;
 }
 
 }  // namespace
@@ -1767,28 +1770,32 @@ struct SegControl {
     int32_t counted[K];
 };
 
-// Segmented refinement: one T-thread block per level (K = 1) or per pair of
-// levels l, l + 1 (K = 2).  Lane s owns steps [q_s, q_{s+1}) of a level's
-// window (S = clamp(Q/32, 1, T) segments) and builds that segment's 2x2
-// difference-form transfer matrix (columns (w, d) = (1, 0) and (0, 1),
-// renormalised every 128 steps and at the end).  The far-end value is the
-// binary-tree product of the matrices (M_i <- M_{i+d} M_i at d = 1, 2, 4, ...)
-// applied to (w_1, d_1) = (1, 1).  Counted shots chain the matrices
-// sequentially (lane 0) for the segment start vectors and re-run every
-// segment.  The two levels of a pair whose shots use the same window walk it
-// together, each grid value loaded once for both; otherwise (and once one
-// of them has converged) each shoots alone.  Every level's arithmetic is
-// that of a block of its own.
+// Segmented refinement: one T-thread block per level (K = 1) or per group of
+// K = 2 or 4 consecutive levels (64-lane blocks).  Lane s owns steps
+// [q_s, q_{s+1}) of a level's window (S = clamp(Q/32, 1, T) segments) and
+// builds that segment's 2x2 difference-form transfer matrix (columns
+// (w, d) = (1, 0) and (0, 1), renormalised every 128 steps and at the end).
+// The far-end value is the binary-tree product of the matrices
+// (M_i <- M_{i+d} M_i at d = 1, 2, 4, ...) applied to (w_1, d_1) = (1, 1).
+// Counted shots chain the matrices sequentially (lane 0) for the segment
+// start vectors and re-run every segment.  In each round the live levels of
+// a group whose shots use the same window walk it together (up to four per
+// walk, three padded with a copy of the first), each grid value loaded once
+// for all of them; counted shots and levels of other windows shoot apart.
+// Every level's arithmetic is that of a block of its own.
 template <int T, bool TABLE, int K>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void numerov_refine_segmented_kernel(
-    const double* __restrict__ uv, const double* __restrict__ segtab, const PotentialRecord* __restrict__ pots,
-    const BandWindow* __restrict__ windows, const WaveTask* __restrict__ tasks, const int32_t n_tasks,
-    const int32_t* __restrict__ bins, const Bracket* __restrict__ brackets, RefinePoints* __restrict__ points,
-    const double rel_tol, const int32_t max_iter, double* __restrict__ levels, int32_t* __restrict__ status,
-    int32_t* __restrict__ iterations, int32_t* __restrict__ steps)
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(K == 4 ? 2 : 3))) void
+numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __restrict__ segtab,
+                                const PotentialRecord* __restrict__ pots, const BandWindow* __restrict__ windows,
+                                const WaveTask* __restrict__ tasks, const int32_t n_tasks,
+                                const int32_t* __restrict__ bins, const Bracket* __restrict__ brackets,
+                                RefinePoints* __restrict__ points, const double rel_tol, const int32_t max_iter,
+                                double* __restrict__ levels, int32_t* __restrict__ status,
+                                int32_t* __restrict__ iterations, int32_t* __restrict__ steps)
 {
-    static_assert(K == 1 || K == 2, "one or two levels per block");
-    __shared__ SegShared<T, K> sh;
+    static_assert(K == 1 || K == 2 || K == 4, "1, 2 or 4 levels per block");
+    static_assert(K == 1 || T == 64, "level groups need 64-lane blocks");
+    __shared__ SegShared<T> sh;
     __shared__ SegControl<K> ctl;
 
     WaveTask task;
@@ -1831,18 +1838,21 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
             point_model(ctl.pts[q], i, ctl.r[q].rate, ctl.r[q].e0);
     }
     const double* uv_pot = uv + P.ut_offset;
-    // A shot's geometry is rebuilt from its band window before each shot;
-    // only each lane's segment bounds stay in registers between shots.
+    // Each level's window and segment bounds stay in registers between
+    // shots; wid[q] is the first level of the group with the same window.
     int2 qb[K];
+    int32_t band[K], wid[K];
 #pragma unroll
-    for (int q = 0; q < K; ++q)
-        qb[q] = ctl.live[q] ? SegGeom<TABLE>::bounds(W[ctl.r[q].b.band], lane, T) : make_int2(0, 0);
-    bool together = false;
-    if constexpr (K == 2) {
-        if (ctl.live[0] && ctl.live[1]) {
-            const BandWindow& a = W[ctl.r[0].b.band];
-            const BandWindow& b = W[ctl.r[1].b.band];
-            together = a.lo == b.lo && a.hi == b.hi && a.seg_row == b.seg_row;
+    for (int q = 0; q < K; ++q) {
+        band[q] = __builtin_amdgcn_readfirstlane(ctl.live[q] ? ctl.r[q].b.band : 0);
+        qb[q] = ctl.live[q] ? SegGeom<TABLE>::bounds(W[band[q]], lane, T) : make_int2(0, 0);
+        wid[q] = q;
+#pragma unroll
+        for (int o = q - 1; o >= 0; --o) {
+            const BandWindow& a = W[band[o]];
+            const BandWindow& b = W[band[q]];
+            if (a.lo == b.lo && a.hi == b.hi && a.seg_row == b.seg_row)
+                wid[q] = o;
         }
     }
     const StepConsts k = step_consts();
@@ -1850,9 +1860,10 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
 
     for (;;) {
         if (lane < K) {
+            // The level's state is copied to registers for the step and back.
             const int32_t q = lane;
             if (ctl.live[q]) {
-                Refiner& r = ctl.r[q];
+                Refiner r = ctl.r[q];
                 bool done = false;
                 if (r.converged(rel_tol, P.abs_tol)) {
                     done = true;
@@ -1860,8 +1871,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
                     ctl.st[q] = 2;
                     done = true;
                 } else {
-                    ctl.em[q] = r.probe(rel_tol, P.abs_tol, ctl.pts[q]);
+                    const RefinePoints pts = ctl.pts[q];
+                    ctl.em[q] = r.probe(rel_tol, P.abs_tol, pts);
                     ctl.counted[q] = r.isolated ? 0 : 1;
+                    // The fields a probe sets.
+                    ctl.r[q].isolated = r.isolated;
+                    ctl.r[q].reshoot = r.reshoot;
+                    ctl.r[q].side = r.side;
                 }
                 if (done) {
                     ctl.live[q] = 0;
@@ -1871,45 +1887,106 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void num
             }
         }
         __syncthreads();
-        bool any = false;
-#pragma unroll
-        for (int q = 0; q < K; ++q)
-            any = any || ctl.live[q] != 0;
-        if (!any)
-            break;
-        if constexpr (K == 2) {
-            if (together && ctl.live[0] && ctl.live[1]) {
-                const double em0 = ctl.em[0], em1 = ctl.em[1];
-                const double e[2] = {P.beta * em0, P.beta * em1};
-                const bool counted[2] = {ctl.counted[0] != 0, ctl.counted[1] != 0};
-                double fw[2];
-                int32_t fk[2], c[2];
-                SegGeom<TABLE> g;
-                g.init(uv_pot, segtab, W[ctl.r[0].b.band], qb[0], lane, T);
-                segmented_shot<T, K, 2, TABLE>(sh, g, lane, 0, e, counted, P.stride, k, fw, fk, c);
-                if (lane < 2) {
-                    const int32_t q = lane;
-                    ctl.r[q].update(q == 0 ? em0 : em1, q == 0 ? fw[0] : fw[1], q == 0 ? fk[0] : fk[1],
-                                    q == 0 ? c[0] : c[1], ctl.pts[q]);
-                }
-                __syncthreads();
-                continue;
-            }
-        }
+        // Levels still to shoot this round (bit q), and those whose shot is counted.
+        uint32_t pending = 0, cmask = 0;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            if (!ctl.live[q])
-                continue;
-            const double em = ctl.em[q];
-            const double e[1] = {P.beta * em};
-            const bool counted[1] = {ctl.counted[q] != 0};
-            double fw[1];
-            int32_t fk[1], c[1];
+            pending |= (ctl.live[q] != 0 ? 1u : 0u) << q;
+            cmask |= (ctl.counted[q] != 0 ? 1u : 0u) << q;
+        }
+        pending = __builtin_amdgcn_readfirstlane(pending);
+        cmask = __builtin_amdgcn_readfirstlane(cmask) & pending;
+        if (pending == 0)
+            break;
+        while (pending != 0) {
+            const int32_t q0 = __builtin_ctz(pending);
+            // Level q0's window and bounds (selected, not indexed, so that
+            // the arrays stay in registers).
+            int32_t band0 = band[0], wid0 = wid[0];
+            int2 qb0 = qb[0];
+#pragma unroll
+            for (int q = 1; q < K; ++q)
+                if (q == q0) {
+                    band0 = band[q];
+                    wid0 = wid[q];
+                    qb0 = qb[q];
+                }
+            // The members (bit q): q0 alone when its shot is counted, else
+            // the uncounted pending levels of its window.
+            uint32_t mm = 1u << q0;
+            if (!((cmask >> q0) & 1u)) {
+#pragma unroll
+                for (int q = 0; q < K; ++q)
+                    if (q > q0 && (((pending & ~cmask) >> q) & 1u) && wid[q] == wid0)
+                        mm |= 1u << q;
+            }
+            pending &= ~mm;
+            const int32_t n = __builtin_popcount(mm);
+            const int32_t kk = n >= 3 ? 4 : n;
+            // Member i's level (the first member again past the last).
+            int32_t mem[4];
+            {
+                uint32_t rest = mm;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    mem[i] = rest != 0 ? __builtin_ctz(rest) : q0;
+                    rest &= rest - 1;
+                }
+            }
             SegGeom<TABLE> g;
-            g.init(uv_pot, segtab, W[ctl.r[q].b.band], qb[q], lane, T);
-            segmented_shot<T, K, 1, TABLE>(sh, g, lane, q, e, counted, P.stride, k, fw, fk, c);
-            if (lane == 0)
-                ctl.r[q].update(em, fw[0], fk[0], c[0], ctl.pts[q]);
+            g.init(uv_pot, segtab, W[band0], qb0, lane, T);
+            int32_t c = 0;
+            double fw[4];
+            int32_t fk[4];
+            if (kk == 4) {
+                if constexpr (K == 4) {
+                    const double e[4] = {P.beta * ctl.em[mem[0]], P.beta * ctl.em[mem[1]], P.beta * ctl.em[mem[2]],
+                                         P.beta * ctl.em[mem[3]]};
+                    double f4[4];
+                    int32_t k4[4];
+                    segmented_shot<T, 4, TABLE>(sh, g, lane, e, false, P.stride, k, f4, k4, c);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        fw[i] = f4[i];
+                        fk[i] = k4[i];
+                    }
+                }
+            } else if (kk == 2) {
+                if constexpr (K >= 2) {
+                    const double e[2] = {P.beta * ctl.em[mem[0]], P.beta * ctl.em[mem[1]]};
+                    double f2[2];
+                    int32_t k2[2];
+                    segmented_shot<T, 2, TABLE>(sh, g, lane, e, false, P.stride, k, f2, k2, c);
+                    fw[0] = f2[0];
+                    fw[1] = f2[1];
+                    fk[0] = k2[0];
+                    fk[1] = k2[1];
+                }
+            } else {
+                const double e[1] = {P.beta * ctl.em[q0]};
+                double f1[1];
+                int32_t k1[1];
+                segmented_shot<T, 1, TABLE>(sh, g, lane, e, ((cmask >> q0) & 1u) != 0, P.stride, k, f1, k1, c);
+                fw[0] = f1[0];
+                fk[0] = k1[0];
+            }
+            // Lane i takes member i's result.
+            if (lane < n) {
+                double wi = fw[0];
+                int32_t ki = fk[0], qi = mem[0];
+#pragma unroll
+                for (int i = 1; i < 4; ++i)
+                    if (lane == i) {
+                        wi = fw[i];
+                        ki = fk[i];
+                        qi = mem[i];
+                    }
+                Refiner r = ctl.r[qi];
+                RefinePoints pts = ctl.pts[qi];
+                r.update(ctl.em[qi], wi, ki, c, pts);
+                ctl.r[qi] = r;
+                ctl.pts[qi] = pts;
+            }
             __syncthreads();
         }
     }
@@ -2016,7 +2093,8 @@ hipError_t launch_refine(hipStream_t stream, int32_t levels_per_lane, const doub
 }
 
 // segtab != nullptr selects the segment tables for 64-lane blocks.
-// levels_per_block 2 (64-lane blocks only) takes tasks of two levels each.
+// levels_per_block 2 or 4 (64-lane blocks only) takes tasks of that many
+// consecutive levels each.
 // bins == nullptr: tasks[0, n_tasks) in order; else `bins` (device, 9
 // offsets) splits the list per XCD and `max_bin` is the largest bin's size.
 hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, int32_t levels_per_block, const double* uv,
@@ -2028,7 +2106,7 @@ hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, int32_t le
 {
     if (n_tasks == 0)
         return hipSuccess;
-    if (levels_per_block != 1 && !(levels_per_block == 2 && lanes == 64))
+    if (levels_per_block != 1 && !((levels_per_block == 2 || levels_per_block == 4) && lanes == 64))
         return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(bins == nullptr ? n_tasks : 8 * max_bin));
 #define EPSEON_SEGMENTED(T, TAB, K)                                                                   \
@@ -2043,7 +2121,12 @@ hipError_t launch_refine_segmented(hipStream_t stream, int32_t lanes, int32_t le
         EPSEON_SEGMENTED(128, false, 1);
         break;
     case 64:
-        if (levels_per_block == 2) {
+        if (levels_per_block == 4) {
+            if (segtab != nullptr)
+                EPSEON_SEGMENTED(64, true, 4);
+            else
+                EPSEON_SEGMENTED(64, false, 4);
+        } else if (levels_per_block == 2) {
             if (segtab != nullptr)
                 EPSEON_SEGMENTED(64, true, 2);
             else

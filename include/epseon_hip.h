@@ -272,11 +272,12 @@ typedef struct epseon_tuning {
                                      one.  Pipelined (solve_split > 0) and paired-lane solves
                                      always launch directly.  Results are bit-identical either
                                      way. */
-    int32_t segmented_levels;     /* segmented mode with 64-lane blocks: levels per block.  2: the
-                                     levels l, l + 1 of a potential share a block, and while both
-                                     shoot in the same window every grid value is loaded once for
-                                     both; 1: one level per block; 0 (default): 2 when the batch has
-                                     at least 24 levels per compute unit, else 1.  Results are
+    int32_t segmented_levels;     /* segmented mode with 64-lane blocks: levels per block.  4 or 2:
+                                     that many consecutive levels of a potential share a block, and
+                                     those that shoot in the same window in a round walk it
+                                     together, every grid value loaded once for all of them; 1: one
+                                     level per block; 0 (default): 4 when the batch has at least 96
+                                     levels per compute unit, 2 from 24, else 1.  Results are
                                      bit-identical either way. */
     double solve_split;           /* see epseon_set_solve_split */
 } epseon_tuning;

@@ -55,7 +55,7 @@ def _compare(prob, nodes, far, resc, lv, it, levels, seg, lanes, windows=None):
     np.testing.assert_array_equal(it[ok], oit[ok])
 
 
-@pytest.mark.parametrize("mode", ["lane", "segmented", "segmented-pairs"])
+@pytest.mark.parametrize("mode", ["lane", "segmented", "segmented-pairs", "segmented-quads"])
 def test_batched_subset_parity(capi, mode):
     specs = batched(4096)[:40]
     _log(f"case: first 40 potentials of the batched workload at full size, {mode} refine")
@@ -68,9 +68,11 @@ def test_batched_subset_parity(capi, mode):
         t = ctx.tuning()
         t.refine_mode = capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE
         t.windows = int(seg)
-        # Two levels per 64-lane block (each table load feeding both shots);
-        # every level's arithmetic is that of a block of its own.
-        t.segmented_levels = 2 if mode == "segmented-pairs" else 1
+        # Two or four levels per 64-lane block (each table load feeding the
+        # shots of every level of the group in the same window); every
+        # level's arithmetic is that of a block of its own.
+        lpb = {"segmented-pairs": 2, "segmented-quads": 4}.get(mode, 1)
+        t.segmented_levels = lpb
         ctx.set_tuning(t)
         ctx.load([_pot(capi, s, t, c) for s, t, c in zip(specs, tables, counts)])
         ctx.solve(1e-10, 200)
@@ -79,7 +81,7 @@ def test_batched_subset_parity(capi, mode):
         lanes = ctx.stats().refine_lanes
         assert ctx.stats().scan_windows == int(seg)
         if seg:
-            assert ctx.stats().refine_levels_per_block == (2 if mode == "segmented-pairs" else 1)
+            assert ctx.stats().refine_levels_per_block == lpb
         flags = [ctx.potential_info(i).flags for i in range(len(specs))]
     eo = lo = 0
     for i, s in enumerate(specs):
@@ -111,7 +113,7 @@ def test_batched_full_scale(capi):
          f"refine {stats.refine_ms:.2f} ms, levels converged {stats.levels_converged}, "
          f"unresolved potentials {int(flags.sum())}")
     assert stats.refine_mode == capi.REFINE_SEGMENTED and stats.refine_lanes == 64 and stats.scan_windows == 1
-    assert stats.refine_levels_per_block == 2
+    assert stats.refine_levels_per_block == 4
     assert np.all(st != 2), "a level hit the iteration cap"
     eo = lo = 0
     resolved_checked = 0
@@ -190,8 +192,8 @@ def test_result_preserving_tunings_are_bit_identical(capi):
 
 def test_result_preserving_tunings_windowed(capi):
     """The same schedule knobs with windowed shots and segmented refinement,
-    one or two levels per block: bit-identical outputs, equal to the oracle's
-    windowed solve."""
+    one, two or four levels per block: bit-identical outputs, equal to the
+    oracle's windowed solve."""
     specs = batched(4096)[:300]
     _log("case: first 300 batched potentials, segmented refine with band windows, tunings varied")
     counts = [s.bound_levels() + 2 for s in specs]
@@ -200,7 +202,7 @@ def test_result_preserving_tunings_windowed(capi):
     with capi.Context(0) as ctx:
         ctx.load([_pot(capi, s, t, c) for s, t, c in zip(specs, tables, counts)])
         for wpb, split, xcd, lpb in ((1, 0.0, 1, 1), (2, 0.3, 0, 2), (4, 0.0, 0, 1), (1, 0.0, 1, 2),
-                                     (2, 0.0, 0, 2)):
+                                     (2, 0.0, 0, 2), (1, 0.0, 1, 4), (2, 0.3, 0, 4)):
             t = capi.Tuning(refine_mode=capi.REFINE_SEGMENTED, levels_per_lane=1, scan_waves_per_block=wpb,
                             segmented_lanes=64, xcd_scan=xcd, windows=1, segmented_levels=lpb,
                             solve_split=split)

@@ -430,7 +430,8 @@ int check_ctx(epseon_ctx* ctx)
 
 }  // namespace
 
-extern "C" {
+// The entry points below have C linkage from their declarations in
+// include/epseon_hip.h.
 
 const char* epseon_last_error(void) { return g_last_error.c_str(); }
 
@@ -2310,5 +2311,3 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
     }
     return EPSEON_OK;
 }
-
-}  // extern "C"

@@ -912,32 +912,40 @@ __device__ __forceinline__ void points_insert(RefinePoints& p, const double c, c
     p.ih = ih;
 }
 
-// Root in (0, 1) of the polynomial through the points in x = (E - el)/width,
-// y = w 2^(k - max k) (Newton divided differences), by at most 8 Newton steps
-// from x0, stopping once a step is at most 2^-40.  false when there are fewer
-// than 3 points, a derivative vanishes or a step leaves (0, 1).  The loops
-// run over all kPts slots (predicated on the count) so that the table stays
-// in registers.
-__device__ __forceinline__ bool points_root(const RefinePoints& p, const double el, const double width,
-                                            const double x0, double& root)
+// Interpolation table of a probe: x_i = (E_i - el)/width, and the Newton
+// divided differences c_i = y[x_0 .. x_i] of y_i = w_i 2^(k_i - max k).
+struct PolyTable {
+    double x[kPts], c[kPts];
+    int32_t n;
+};
+
+// The largest binary exponent of the points.
+__device__ __forceinline__ int32_t points_kmax(const RefinePoints& p)
 {
-    const int32_t n = p.n;
-    if (n < 3)
-        return false;
-    const double inv = 1.0 / width;
     int32_t kmax = p.k[0];
 #pragma unroll
     for (int32_t i = 1; i < kPts; ++i)
-        if (i < n)
+        if (i < p.n)
             kmax = p.k[i] > kmax ? p.k[i] : kmax;
-    double x[kPts], c[kPts];
+    return kmax;
+}
+
+// The table from the points (inv = 1/width), column by column:
+// c_i <- (c_i - c_{i-1}) / (x_i - x_{i-j}) for i = n - 1 down to j.  The
+// loops run over all kPts slots (predicated on the count) so that the table
+// stays in registers.
+__device__ __forceinline__ void poly_table(const RefinePoints& p, const double el, const double inv,
+                                           const int32_t kmax, PolyTable& t)
+{
+    const int32_t n = p.n;
+    t.n = n;
 #pragma unroll
     for (int32_t i = 0; i < kPts; ++i) {
-        x[i] = 0.0;
-        c[i] = 0.0;
+        t.x[i] = 0.0;
+        t.c[i] = 0.0;
         if (i < n) {
-            x[i] = (p.e[i] - el) * inv;
-            c[i] = __builtin_ldexp(p.w[i], p.k[i] - kmax);
+            t.x[i] = (p.e[i] - el) * inv;
+            t.c[i] = __builtin_ldexp(p.w[i], p.k[i] - kmax);
         }
     }
 #pragma unroll
@@ -945,34 +953,51 @@ __device__ __forceinline__ bool points_root(const RefinePoints& p, const double 
 #pragma unroll
         for (int32_t i = kPts - 1; i >= j; --i)
             if (i < n)
-                c[i] = (c[i] - c[i - 1]) / (x[i] - x[i - j]);
-    double top = c[2];
+                t.c[i] = (t.c[i] - t.c[i - 1]) / (t.x[i] - t.x[i - j]);
+}
+
+// Root in (0, 1) of the table's polynomial by at most 8 Newton steps from
+// x0, stopping once a step is at most 2^-40; false when a derivative
+// vanishes or a step leaves (0, 1).
+__device__ __forceinline__ bool poly_root(const PolyTable& t, const double x0, double& root)
+{
+    const int32_t n = t.n;
+    double top = t.c[2];
 #pragma unroll
     for (int32_t i = 3; i < kPts; ++i)
-        top = (i == n - 1) ? c[i] : top;
-    double t = x0;
+        top = (i == n - 1) ? t.c[i] : top;
+    double r = x0;
     for (int32_t it = 0; it < 8; ++it) {
         double f = top, d = 0.0;
 #pragma unroll
         for (int32_t i = kPts - 2; i >= 0; --i) {
             if (i <= n - 2) {
-                const double u = t - x[i];
+                const double u = r - t.x[i];
                 d = __builtin_fma(d, u, f);
-                f = __builtin_fma(f, u, c[i]);
+                f = __builtin_fma(f, u, t.c[i]);
             }
         }
         if (!(d != 0.0))
             return false;
         const double dt = f / d;
-        t = t - dt;
-        if (!(t > 0.0 && t < 1.0))
+        r = r - dt;
+        if (!(r > 0.0 && r < 1.0))
             return false;
         if (fabs(dt) <= 0x1p-40)
             break;
     }
-    root = t;
+    root = r;
     return true;
 }
+
+// A probe's plan: the probe itself (`em`) unless it is the root of the
+// interpolating polynomial (`root`), which starts from the regula-falsi
+// fraction s and falls back to it.
+struct ProbePlan {
+    double em, el, width, inv, s, tol;
+    int32_t kmax;
+    bool root;
+};
 
 // Refinement state machine shared by both refine kernels.  While the
 // bracket holds more than level v (or its floor has no shot) the probe is the
@@ -1008,40 +1033,67 @@ struct Refiner {
         return (b.eh - b.el) <= tol;
     }
 
-    __device__ __forceinline__ double probe(const double rel_tol, const double abs_tol, const RefinePoints& p)
+    // The next probe's plan (sets isolated, reshoot and side).
+    __device__ __forceinline__ ProbePlan plan(const double rel_tol, const double abs_tol, const RefinePoints& p)
     {
-        const double width = b.eh - b.el;
-        const double tol = __builtin_fma(rel_tol, fmax(fabs(b.el), fabs(b.eh)), abs_tol);
+        ProbePlan q{};
+        q.el = b.el;
+        q.width = b.eh - b.el;
+        q.tol = __builtin_fma(rel_tol, fmax(fabs(b.el), fabs(b.eh)), abs_tol);
+        q.root = false;
         isolated = b.lo_ok && b.cl == v && b.ch == v + 1;
         reshoot = isolated && b.lo_foreign;
-        double em;
         if (reshoot) {
             // The floor's value came from another band window: shoot it again
             // in this level's window (its count stays the scan's).
-            em = b.el;
+            q.em = b.el;
         } else if (isolated) {
             const double r = __builtin_ldexp(b.wh / b.wl, b.kh - b.kl);
-            const double s = 1.0 / (1.0 - r);
-            double x;
-            if (points_root(p, b.el, width, fmin(fmax(s, 0.0), 1.0), x))
-                em = __builtin_fma(x, width, b.el);
-            else
-                em = __builtin_fma(s, width, b.el);
-            const double d = 0.5 * tol;
-            if (!(em >= b.el + d))
-                em = b.el + d;
-            if (!(em <= b.eh - d))
-                em = b.eh - d;
+            q.s = 1.0 / (1.0 - r);
+            q.em = __builtin_fma(q.s, q.width, b.el);
+            if (p.n >= 3) {
+                q.root = true;
+                q.inv = 1.0 / q.width;
+                q.kmax = points_kmax(p);
+            }
         } else {
             // Several levels in the bracket (or no shot at its floor): probe
             // where level v would sit if the ch - cl levels were evenly
             // spaced, kept within [1/8, 7/8] of the bracket.
             double fr = (static_cast<double>(v - b.cl) + 0.5) / static_cast<double>(b.ch - b.cl);
             fr = fmin(fmax(fr, 0.125), 0.875);
-            em = __builtin_fma(fr, width, b.el);
+            q.em = __builtin_fma(fr, q.width, b.el);
             side = 0;
         }
+        return q;
+    }
+
+    // The probe from its plan and, for a root plan, the polynomial's table;
+    // an isolated bracket's probe is kept at least tol/2 inside.
+    __device__ __forceinline__ double finish(const ProbePlan& q, const PolyTable* t) const
+    {
+        double em = q.em;
+        if (!isolated || reshoot)
+            return em;
+        double x;
+        if (q.root && poly_root(*t, fmin(fmax(q.s, 0.0), 1.0), x))
+            em = __builtin_fma(x, q.width, q.el);
+        const double d = 0.5 * q.tol;
+        if (!(em >= b.el + d))
+            em = b.el + d;
+        if (!(em <= b.eh - d))
+            em = b.eh - d;
         return em;
+    }
+
+    // Plan, table and probe in one lane.
+    __device__ __forceinline__ double probe(const double rel_tol, const double abs_tol, const RefinePoints& p)
+    {
+        const ProbePlan q = plan(rel_tol, abs_tol, p);
+        PolyTable t;
+        if (q.root)
+            poly_table(p, q.el, q.inv, q.kmax, t);
+        return finish(q, &t);
     }
 
     // Take the probe's result; `c` is ignored when the bracket is isolated.
@@ -1764,11 +1816,68 @@ template <int K>
 struct SegControl {
     Refiner r[K];
     RefinePoints pts[K];
+    ProbePlan plan[K];
+    PolyTable tab[K];
     double em[K];
     int32_t st[K];
     int32_t live[K];
     int32_t counted[K];
+    int32_t root[K];
 };
+
+// Value of lane - J within its 16-lane row (J < 16; lanes whose source lies
+// outside the row get an unused value).
+template <int J>
+__device__ __forceinline__ double lane_minus(const double x)
+{
+    const int2 v = __builtin_bit_cast(int2, x);
+    return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_mov_dpp(v.x, 0x110 + J, 0xF, 0xF, false),
+                                                __builtin_amdgcn_mov_dpp(v.y, 0x110 + J, 0xF, 0xF, false)));
+}
+
+// The interpolation tables of the levels whose probe is a polynomial root,
+// built by kPts lanes per level (lane q kPts + i holds point i of level q):
+// the same operations as poly_table, one column per step.  Every lane of the
+// wavefront executes the cross-lane moves.
+template <int K>
+__device__ __forceinline__ void tables_in_lanes(SegControl<K>& ctl, const int32_t lane)
+{
+    const int32_t q = lane / kPts, i = lane % kPts;
+    const bool mine = lane < K * kPts && ctl.root[q < K ? q : 0] != 0 && q < K;
+    double x = 0.0, c = 0.0;
+    int32_t n = 0;
+    if (mine) {
+        const RefinePoints& p = ctl.pts[q];
+        const ProbePlan& pl = ctl.plan[q];
+        n = p.n;
+        if (i < n) {
+            x = (p.e[i] - pl.el) * pl.inv;
+            c = __builtin_ldexp(p.w[i], p.k[i] - pl.kmax);
+        }
+    }
+#define EPSEON_COLUMN(J)                                    \
+    {                                                       \
+        const double cp = lane_minus<1>(c);                 \
+        const double xp = lane_minus<J>(x);                 \
+        if (mine && i >= J && i < n)                        \
+            c = (c - cp) / (x - xp);                        \
+    }
+    EPSEON_COLUMN(1)
+    EPSEON_COLUMN(2)
+    EPSEON_COLUMN(3)
+    EPSEON_COLUMN(4)
+    EPSEON_COLUMN(5)
+    EPSEON_COLUMN(6)
+    EPSEON_COLUMN(7)
+#undef EPSEON_COLUMN
+    static_assert(kPts == 8, "one column step per point after the first");
+    if (mine) {
+        if (i == 0)
+            ctl.tab[q].n = n;
+        ctl.tab[q].x[i] = x;
+        ctl.tab[q].c[i] = c;
+    }
+}
 
 // Segmented refinement: one T-thread block per level (K = 1) or per group of
 // K = 2 or 4 consecutive levels (64-lane blocks).  Lane s owns steps
@@ -1859,9 +1968,11 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
     __syncthreads();
 
     for (;;) {
+        // The probes: each level's plan (lane q), the polynomial tables of
+        // the root plans (kPts lanes per level), then the probes (lane q).
         if (lane < K) {
-            // The level's state is copied to registers for the step and back.
             const int32_t q = lane;
+            ctl.root[q] = 0;
             if (ctl.live[q]) {
                 Refiner r = ctl.r[q];
                 bool done = false;
@@ -1871,10 +1982,11 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
                     ctl.st[q] = 2;
                     done = true;
                 } else {
-                    const RefinePoints pts = ctl.pts[q];
-                    ctl.em[q] = r.probe(rel_tol, P.abs_tol, pts);
+                    const ProbePlan pl = r.plan(rel_tol, P.abs_tol, ctl.pts[q]);
+                    ctl.plan[q] = pl;
+                    ctl.root[q] = pl.root ? 1 : 0;
                     ctl.counted[q] = r.isolated ? 0 : 1;
-                    // The fields a probe sets.
+                    // The fields a plan sets.
                     ctl.r[q].isolated = r.isolated;
                     ctl.r[q].reshoot = r.reshoot;
                     ctl.r[q].side = r.side;
@@ -1884,6 +1996,17 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
                     write_level(levels, status, iterations, P.level_offset + task.first + q, r.level(), ctl.st[q],
                                 r.it);
                 }
+            }
+        }
+        __syncthreads();
+        tables_in_lanes(ctl, lane);
+        __syncthreads();
+        if (lane < K) {
+            const int32_t q = lane;
+            if (ctl.live[q]) {
+                const Refiner r = ctl.r[q];
+                const PolyTable t = ctl.tab[q];
+                ctl.em[q] = r.finish(ctl.plan[q], &t);
             }
         }
         __syncthreads();

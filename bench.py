@@ -177,7 +177,7 @@ def cpu_baseline(all_cores_count: int = 256, one_core_count: int = 32) -> dict:
     }
 
 
-def per_potential_launches(capi, pots, rel_tol: float, sample: int = 64) -> dict:
+def per_potential_launches(capi, pots, rel_tol: float, sample: int = 256) -> dict:
     """The batched workload solved one potential at a time: each potential of
     a seeded sample in its own context (uploaded before timing), then a timed
     loop of solve + levels per potential."""

@@ -1274,24 +1274,23 @@ struct SegCount {
     int32_t internal, first, last, has;
 };
 
-// Values per VGPR buffer of the segment walks: three rotating buffers of 8
-// keep the walk's registers low enough for three wavefronts per SIMD.
-constexpr int kWalkChunk = 8;
-
-template <class Body>
-__device__ __forceinline__ void walk_chunk(const double (&buf)[kWalkChunk], const int32_t j, Body& body)
+// Values per VGPR buffer of the segment walks (Body::kChunk): three rotating
+// buffers of 8 keep the walk's registers low enough for three wavefronts per
+// SIMD (four-level walks run two per SIMD; buffers of 4 values would not
+// bring them to three).
+template <int C, class Body>
+__device__ __forceinline__ void walk_chunk(const double (&buf)[C], const int32_t j, Body& body)
 {
 #pragma unroll
-    for (int k = 0; k < kWalkChunk; ++k)
+    for (int k = 0; k < C; ++k)
         body(buf[k], j + k);
 }
 
-template <class Body>
-__device__ __forceinline__ void walk_tail(const double (&buf)[kWalkChunk], const int32_t j, const int32_t len,
-                                          Body& body)
+template <int C, class Body>
+__device__ __forceinline__ void walk_tail(const double (&buf)[C], const int32_t j, const int32_t len, Body& body)
 {
 #pragma unroll
-    for (int k = 0; k < kWalkChunk; ++k)
+    for (int k = 0; k < C; ++k)
         if (j + k < len)
             body(buf[k], j + k);
 }
@@ -1324,25 +1323,24 @@ struct ColumnSource {
     __device__ __forceinline__ double at(const int32_t j) const { return row(j + 1); }
 };
 
-template <class Src>
-__device__ __forceinline__ void load_chunk(double (&buf)[kWalkChunk], const Src& src, const int32_t j)
+template <int C, class Src>
+__device__ __forceinline__ void load_chunk(double (&buf)[C], const Src& src, const int32_t j)
 {
 #pragma unroll
-    for (int k = 0; k < kWalkChunk; ++k)
+    for (int k = 0; k < C; ++k)
         buf[k] = src.at(j + k);
 }
 
 // Visit the `len` values src.at(0 .. len - 1) in order, body(value, index);
 // ulen is wave-uniform with ulen <= len <= ulen + 1 (segment lengths differ by
 // at most one), so the step index stays wave-uniform and only the last step
-// is predicated per lane.  Three kWalkChunk-value VGPR buffers rotate so that
-// every load is issued two chunks before its use; vector-memory loads return
-// in order, so the waits are partial.  Also reads the 2 kWalkChunk values
-// that follow the segment (the tables are padded).
-template <class Src, class Body>
+// is predicated per lane.  Three C-value VGPR buffers rotate so that every
+// load is issued two chunks before its use; vector-memory loads return in
+// order, so the waits are partial.  Also reads the 2 C values that follow the
+// segment (the tables are padded).
+template <int C, class Src, class Body>
 __device__ __forceinline__ void walk_segment(const Src& src, const int32_t len, const int32_t ulen, Body& body)
 {
-    constexpr int32_t C = kWalkChunk;
     double A[C], B[C], D[C];
     load_chunk(A, src, 0);
     load_chunk(B, src, C);
@@ -1376,6 +1374,7 @@ __device__ __forceinline__ void walk_segment(const Src& src, const int32_t len, 
 // norm below 7.4), so 128 steps stay inside the normal range, and scaling by
 // a power of two is exact.
 struct MatrixBody {
+    static constexpr int kChunk = 8;
     double e, em;
     double wa, da, pa, wb, db, pb;
     int32_t ex;
@@ -1413,6 +1412,7 @@ struct MatrixBody {
 // Each member's arithmetic is that of a lone MatrixBody.
 template <int KK>
 struct MatrixGroup {
+    static constexpr int kChunk = 8;
     MatrixBody m[KK];
     __device__ __forceinline__ void operator()(const double u, const int32_t idx)
     {
@@ -1427,6 +1427,7 @@ struct MatrixGroup {
 // stride is a power of two: mask = stride - 1; last = qn - 1 when qn is not
 // a multiple of the stride, else -1).
 struct CountBody {
+    static constexpr int kChunk = 8;
     double e, em, w, d, wm;
     int32_t q0, mask, last;
     SegCount sc;
@@ -1583,9 +1584,9 @@ struct SegGeom {
     __device__ __forceinline__ void walk(Body& body) const
     {
         if constexpr (TABLE)
-            walk_segment(cs, q1 - q0, ulen, body);
+            walk_segment<Body::kChunk>(cs, q1 - q0, ulen, body);
         else
-            walk_segment(RowSource{u + q0 + 1}, q1 - q0, ulen, body);
+            walk_segment<Body::kChunk>(RowSource{u + q0 + 1}, q1 - q0, ulen, body);
     }
 };
 
@@ -1659,24 +1660,30 @@ __device__ __forceinline__ void tree_round(SegMat& a, const int32_t lane, const 
     }
 }
 
-// The binary-tree product of a wavefront's segment matrices in registers
-// (d = 1, 2, 4, ... while d < segs); lane 0 ends with the product of all
-// segments.  Every lane of the wavefront takes part (the cross-lane moves
-// read all lanes).
-__device__ __forceinline__ void tree_in_registers(SegMat& a, const int32_t lane, const int32_t segs)
+// The binary-tree products of a wavefront's segment matrices in registers,
+// one per level of the shot (d = 1, 2, 4, ... while d < segs; the levels'
+// rounds interleaved, so their dependency chains overlap); lane 0 ends with
+// each product of all segments.  Every lane of the wavefront takes part (the
+// cross-lane moves read all lanes).
+template <int D, int KK>
+__device__ __forceinline__ void tree_rounds(SegMat (&a)[KK], const int32_t lane, const int32_t segs)
 {
-    if (segs > 1)
-        tree_round<1>(a, lane, segs);
-    if (segs > 2)
-        tree_round<2>(a, lane, segs);
-    if (segs > 4)
-        tree_round<4>(a, lane, segs);
-    if (segs > 8)
-        tree_round<8>(a, lane, segs);
-    if (segs > 16)
-        tree_round<16>(a, lane, segs);
-    if (segs > 32)
-        tree_round<32>(a, lane, segs);
+    if (segs > D) {
+#pragma unroll
+        for (int q = 0; q < KK; ++q)
+            tree_round<D>(a[q], lane, segs);
+    }
+}
+
+template <int KK>
+__device__ __forceinline__ void trees_in_registers(SegMat (&a)[KK], const int32_t lane, const int32_t segs)
+{
+    tree_rounds<1>(a, lane, segs);
+    tree_rounds<2>(a, lane, segs);
+    tree_rounds<4>(a, lane, segs);
+    tree_rounds<8>(a, lane, segs);
+    tree_rounds<16>(a, lane, segs);
+    tree_rounds<32>(a, lane, segs);
 }
 
 // Start vector of every segment of a counted shot: the matrices chained from
@@ -1772,9 +1779,9 @@ __device__ __forceinline__ void segmented_shot(SegShared<T>& sh, const SegGeom<T
         __syncthreads();
     }
     if constexpr (kRegisterTree) {
+        trees_in_registers(a, lane, segs);
 #pragma unroll
         for (int q = 0; q < KK; ++q) {
-            tree_in_registers(a[q], lane, segs);
             // Lane 0's product, to every lane.
             const int2 f = __builtin_bit_cast(int2, __builtin_fma(a[q].m00, 1.0, a[q].m01 * 1.0));
             fw[q] = __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readlane(f.x, 0),

@@ -842,23 +842,52 @@ __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const 
         b.lo_ok = 1;
         b.lo_foreign = same_window(W, bl, band) ? 0 : 1;
     }
-    int32_t np = 0;
-    for (int32_t q = j - 4; q <= j + 3; ++q) {
-        if (q < 0 || q >= m)
-            continue;
-        if (q == j - 1 && !(b.lo_ok && !b.lo_foreign))
-            continue;
-        const int32_t bq = group_band(P, q);
-        if (q != j && q != j - 1 && !same_window(W, bq, band))
-            continue;
-        if (q == j)
-            pts.ih = np;
-        pts.e[np] = __builtin_fma(static_cast<double>(q + 1), P.de, P.e_lo);
-        pts.w[np] = far_end[P.energy_offset + q];
-        pts.k[np] = W[bq].far_exp - rescales[P.energy_offset + q] * kRescaleLog2;
-        ++np;
+    // Candidates j - 4 + s, compacted in registers (every slot addressed by a
+    // constant index), then the record is written whole.
+    double pe[kPts], pw[kPts];
+    int32_t pk[kPts];
+    int32_t np = 0, ih = 0;
+#pragma unroll
+    for (int32_t s = 0; s < kPts; ++s) {
+        pe[s] = 0.0;
+        pw[s] = 0.0;
+        pk[s] = 0;
+    }
+#pragma unroll
+    for (int32_t s = 0; s < kPts; ++s) {
+        const int32_t q = j - 4 + s;
+        bool ok = q >= 0 && q < m;
+        int32_t bq = band;
+        if (ok && q == j - 1)
+            ok = b.lo_ok && !b.lo_foreign;
+        else if (ok && q != j) {
+            bq = group_band(P, q);
+            ok = same_window(W, bq, band);
+        }
+        if (ok) {
+            const double e = __builtin_fma(static_cast<double>(q + 1), P.de, P.e_lo);
+            const double w = far_end[P.energy_offset + q];
+            const int32_t k = W[bq].far_exp - rescales[P.energy_offset + q] * kRescaleLog2;
+#pragma unroll
+            for (int32_t t = 0; t < kPts; ++t)
+                if (t == np) {
+                    pe[t] = e;
+                    pw[t] = w;
+                    pk[t] = k;
+                }
+            if (q == j)
+                ih = np;
+            ++np;
+        }
+    }
+#pragma unroll
+    for (int32_t t = 0; t < kPts; ++t) {
+        pts.e[t] = pe[t];
+        pts.w[t] = pw[t];
+        pts.k[t] = pk[t];
     }
     pts.n = np;
+    pts.ih = ih;
     b.rate = 0.0;
     if (b.lo_ok && !b.lo_foreign && b.cl == v && b.ch == v + 1) {
         const int64_t o = P.energy_offset;
@@ -1850,7 +1879,7 @@ template <int K>
 __device__ __forceinline__ void tables_in_lanes(SegControl<K>& ctl, const int32_t lane)
 {
     const int32_t q = lane / kPts, i = lane % kPts;
-    const bool mine = lane < K * kPts && ctl.root[q < K ? q : 0] != 0 && q < K;
+    const bool mine = q < K && ctl.root[q] != 0;
     double x = 0.0, c = 0.0;
     int32_t n = 0;
     if (mine) {

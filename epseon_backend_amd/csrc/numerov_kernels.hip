@@ -794,9 +794,9 @@ __device__ __forceinline__ void model_apply(const double b, const double e0, con
 // exists since every eigenvalue exceeds min V).  Exponents are binary.  When
 // the bracket holds exactly level v and a third scan point of the same count
 // regime exists above or below it, `rate` is the model rate b.  `pts` gets
-// the level's interpolation points: scan points j - 4 .. j + 3 (j the upper
-// end) whose group shares the level's window, the lower end only when its
-// value is of that window.
+// the level's interpolation points: scan points j - kPts/2 .. j + kPts/2 - 1
+// (j the upper end) whose group shares the level's window, the lower end only
+// when its value is of that window.
 // W: the potential's kBands band windows (those the scan used).
 __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const BandWindow* W,
                                                 const int32_t v, const int32_t* __restrict__ nodes,
@@ -842,8 +842,8 @@ __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const 
         b.lo_ok = 1;
         b.lo_foreign = same_window(W, bl, band) ? 0 : 1;
     }
-    // Candidates j - 4 + s, compacted in registers (every slot addressed by a
-    // constant index), then the record is written whole.
+    // Candidates j - kPts/2 + s, compacted in registers (every slot addressed
+    // by a constant index), then the record is written whole.
     double pe[kPts], pw[kPts];
     int32_t pk[kPts];
     int32_t np = 0, ih = 0;
@@ -855,7 +855,7 @@ __device__ __forceinline__ Bracket find_bracket(const PotentialRecord& P, const 
     }
 #pragma unroll
     for (int32_t s = 0; s < kPts; ++s) {
-        const int32_t q = j - 4 + s;
+        const int32_t q = j - kPts / 2 + s;
         bool ok = q >= 0 && q < m;
         int32_t bq = band;
         if (ok && q == j - 1)
@@ -1872,14 +1872,15 @@ __device__ __forceinline__ double lane_minus(const double x)
 }
 
 // The interpolation tables of the levels whose probe is a polynomial root,
-// built by kPts lanes per level (lane q kPts + i holds point i of level q):
-// the same operations as poly_table, one column per step.  Every lane of the
-// wavefront executes the cross-lane moves.
+// built by one 16-lane row per level (lane 16 q + i holds point i of level
+// q): the same operations as poly_table, one column per step.  Every lane of
+// the wavefront executes the cross-lane moves.
 template <int K>
 __device__ __forceinline__ void tables_in_lanes(SegControl<K>& ctl, const int32_t lane)
 {
-    const int32_t q = lane / kPts, i = lane % kPts;
-    const bool mine = q < K && ctl.root[q] != 0;
+    static_assert(kPts <= 16 && K * 16 <= 64, "a row of lanes per level");
+    const int32_t q = lane / 16, i = lane % 16;
+    const bool mine = q < K && i < kPts && ctl.root[q] != 0;
     double x = 0.0, c = 0.0;
     int32_t n = 0;
     if (mine) {
@@ -1905,8 +1906,12 @@ __device__ __forceinline__ void tables_in_lanes(SegControl<K>& ctl, const int32_
     EPSEON_COLUMN(5)
     EPSEON_COLUMN(6)
     EPSEON_COLUMN(7)
+    EPSEON_COLUMN(8)
+    EPSEON_COLUMN(9)
+    EPSEON_COLUMN(10)
+    EPSEON_COLUMN(11)
 #undef EPSEON_COLUMN
-    static_assert(kPts == 8, "one column step per point after the first");
+    static_assert(kPts == 12, "one column step per point after the first");
     if (mine) {
         if (i == 0)
             ctl.tab[q].n = n;
@@ -2005,7 +2010,7 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
 
     for (;;) {
         // The probes: each level's plan (lane q), the polynomial tables of
-        // the root plans (kPts lanes per level), then the probes (lane q).
+        // the root plans (a row of lanes per level), then the probes (lane q).
         if (lane < K) {
             const int32_t q = lane;
             ctl.root[q] = 0;

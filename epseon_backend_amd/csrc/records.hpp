@@ -202,14 +202,14 @@ static_assert(sizeof(Bracket) == 72, "bracket layout");
 // pass, updated by the refinement): at most kPts (energy, F = w 2^k) pairs of
 // the level's band window sorted by energy, the upper bracket end at index
 // ih (the lower end, when its value is of that window, at ih - 1).
-constexpr int32_t kPts = 8;
+constexpr int32_t kPts = 12;
 struct RefinePoints {
     double e[kPts];
     double w[kPts];
     int32_t k[kPts];
     int32_t n, ih;
 };
-static_assert(sizeof(RefinePoints) == 168, "points layout");
+static_assert(sizeof(RefinePoints) == 248, "points layout");
 
 // Work item of one wavefront: potential index and the first trial energy
 // (scan) or first level (refine) of its 64 lanes.

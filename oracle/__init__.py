@@ -51,9 +51,9 @@ class Bracket(ctypes.Structure):
         ("lo_foreign", ctypes.c_int32),
         ("n_pts", ctypes.c_int32),
         ("i_hi", ctypes.c_int32),
-        ("e_pt", ctypes.c_double * 8),
-        ("w_pt", ctypes.c_double * 8),
-        ("k_pt", ctypes.c_int32 * 8),
+        ("e_pt", ctypes.c_double * 12),
+        ("w_pt", ctypes.c_double * 12),
+        ("k_pt", ctypes.c_int32 * 12),
     ]
 
 

@@ -423,7 +423,7 @@ void eo_scan(const double* pt, int64_t n, double beta, double e_lo, double e_hi,
 
 /* Interpolation points of a level: at most EO_PTS (energy, F = w 2^k) pairs
  * of its band window, kept sorted by energy. */
-#define EO_PTS 8
+#define EO_PTS 12
 
 typedef struct eo_bracket {
     double e_lo, e_hi;   /* bracket ends */
@@ -592,10 +592,10 @@ void eo_bracket_level(const int32_t* nodes, const double* w_last, const int32_t*
         out->lo_valid = 1;
         out->lo_foreign = !eo_same_window(win, bl, band);
     }
-    /* Interpolation points: scan points j - 4 .. j + 3 whose group shares the
+    /* Interpolation points: scan points j - 6 .. j + 5 whose group shares the
      * level's window, the lower end only when its value is of that window. */
     out->n_pts = 0;
-    for (int64_t q = j - 4; q <= j + 3; ++q) {
+    for (int64_t q = j - EO_PTS / 2; q < j + EO_PTS / 2; ++q) {
         if (q < 0 || q >= m)
             continue;
         if (q == j - 1 && !(out->lo_valid && !out->lo_foreign))

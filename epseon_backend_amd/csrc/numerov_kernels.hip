@@ -1037,19 +1037,39 @@ struct ProbePlan {
 // fails; every probe is kept at least tol/2 inside, and the probe's count
 // follows from the sign of F.  Every shot joins the points.  Stops when the
 // width is <= rel_tol max(|E_lo|, |E_hi|) + abs_tol.
+// The model transform of interpolation point i (as of the bracket ends).
+__device__ __forceinline__ void point_model(RefinePoints& p, const int32_t i, const double rate, const double e0)
+{
+    double w = p.w[i];
+    int32_t k = p.k[i];
+    model_apply(rate, e0, p.e[i], w, k);
+    p.w[i] = w;
+    p.k[i] = k;
+}
+
+// The model transform of every interpolation point.
+__device__ __forceinline__ void points_model(RefinePoints& p, const double rate, const double e0)
+{
+    for (int32_t i = 0; i < p.n; ++i)
+        point_model(p, i, rate, e0);
+}
+
 struct Refiner {
     Bracket b;
     int32_t v, side, it;
-    int32_t isolated, reshoot;
+    int32_t isolated, reshoot, mid, refit;
     double rate, e0;
 
     // Regula falsi runs on G = F e^(-b (E - E_hi0)) when the scan gave a
     // model rate (nearly linear where F grows exponentially across the
     // bracket); sign(G) = sign(F), so counts and brackets are unchanged.
+    // Without one (refit set), the first isolated probe is the bracket's
+    // midpoint and the rate is fitted to the three equally spaced values.
     __device__ __forceinline__ void start_model()
     {
         rate = b.rate;
         e0 = b.eh;
+        refit = rate == 0.0 ? 1 : 0;
         if (rate != 0.0) {
             model_apply(rate, e0, b.el, b.wl, b.kl);
             model_apply(rate, e0, b.eh, b.wh, b.kh);
@@ -1062,7 +1082,7 @@ struct Refiner {
         return (b.eh - b.el) <= tol;
     }
 
-    // The next probe's plan (sets isolated, reshoot and side).
+    // The next probe's plan (sets isolated, reshoot, mid and side).
     __device__ __forceinline__ ProbePlan plan(const double rel_tol, const double abs_tol, const RefinePoints& p)
     {
         ProbePlan q{};
@@ -1072,10 +1092,13 @@ struct Refiner {
         q.root = false;
         isolated = b.lo_ok && b.cl == v && b.ch == v + 1;
         reshoot = isolated && b.lo_foreign;
+        mid = isolated && !reshoot && refit;
         if (reshoot) {
             // The floor's value came from another band window: shoot it again
             // in this level's window (its count stays the scan's).
             q.em = b.el;
+        } else if (mid) {
+            q.em = b.el + 0.5 * q.width;
         } else if (isolated) {
             const double r = __builtin_ldexp(b.wh / b.wl, b.kh - b.kl);
             q.s = 1.0 / (1.0 - r);
@@ -1102,7 +1125,7 @@ struct Refiner {
     __device__ __forceinline__ double finish(const ProbePlan& q, const PolyTable* t) const
     {
         double em = q.em;
-        if (!isolated || reshoot)
+        if (!isolated || reshoot || mid)
             return em;
         double x;
         if (q.root && poly_root(*t, fmin(fmax(q.s, 0.0), 1.0), x))
@@ -1126,14 +1149,32 @@ struct Refiner {
     }
 
     // Take the probe's result; `c` is ignored when the bracket is isolated.
-    __device__ __forceinline__ void update(const double em, double w, int32_t k, int32_t c, RefinePoints& p)
+    // True when a midpoint shot gave the level its rate: the caller then
+    // applies the transform to every interpolation point (the shot is
+    // inserted untransformed, so all of them take it alike).
+    __device__ __forceinline__ bool update(const double em, double w, int32_t k, int32_t c, RefinePoints& p)
     {
         if (isolated)
             c = reshoot ? v : ((static_cast<int32_t>(sign_bit(w)) == (v & 1)) ? v : v + 1);
+        bool fitted = false;
+        if (mid) {
+            // The rate through (el, em, eh), em above the root when c > v;
+            // the ends take the transform from here on.
+            refit = 0;
+            const double br = model_rate(b.wl, b.kl, w, k, b.wh, b.kh, c > v, 0.5 * (b.eh - b.el));
+            if (br != 0.0) {
+                rate = br;
+                model_apply(rate, e0, b.el, b.wl, b.kl);
+                model_apply(rate, e0, b.eh, b.wh, b.kh);
+                fitted = true;
+            }
+        }
+        const double wr = w;
+        const int32_t kr = k;
         if (rate != 0.0)
             model_apply(rate, e0, em, w, k);
         ++it;
-        points_insert(p, b.el + 0.5 * (b.eh - b.el), em, w, k);
+        points_insert(p, b.el + 0.5 * (b.eh - b.el), em, fitted ? wr : w, fitted ? kr : k);
         if (c <= v)
             ++p.ih;
         // Anderson-Bjorck: when the same end is replaced twice in a row, the
@@ -1162,20 +1203,11 @@ struct Refiner {
             b.kh = k;
             side = isolated ? 1 : 0;
         }
+        return fitted;
     }
 
     __device__ __forceinline__ double level() const { return b.el + 0.5 * (b.eh - b.el); }
 };
-
-// The model transform of interpolation point i (as of the bracket ends).
-__device__ __forceinline__ void point_model(RefinePoints& p, const int32_t i, const double rate, const double e0)
-{
-    double w = p.w[i];
-    int32_t k = p.k[i];
-    model_apply(rate, e0, p.e[i], w, k);
-    p.w[i] = w;
-    p.k[i] = k;
-}
 
 __device__ __forceinline__ void write_level(double* levels, int32_t* status, int32_t* iterations,
                                             const int64_t out, const double lv, const int32_t st,
@@ -1234,11 +1266,8 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
             write_level(levels, status, iterations, out[q], __builtin_nan(""), 1, 0);
             continue;
         }
-        if (r[q].rate != 0.0) {
-            RefinePoints& pq = points[out[q]];
-            for (int32_t i = 0; i < pq.n; ++i)
-                point_model(pq, i, r[q].rate, r[q].e0);
-        }
+        if (r[q].rate != 0.0)
+            points_model(points[out[q]], r[q].rate, r[q].e0);
         live[q] = true;
     }
     const double* u = uv + P.ut_offset + W[0].lo;
@@ -1268,17 +1297,18 @@ __global__ __launch_bounds__(256) void numerov_refine_kernel(
             double w;
             int32_t k;
             const int32_t c = shoot<DiffForm>(u, n, P.beta * em[0], P.stride, w, k);
-            r[0].update(em[0], w, k * kRescaleLog2, c, points[out[0]]);
+            if (r[0].update(em[0], w, k * kRescaleLog2, c, points[out[0]]))
+                points_model(points[out[0]], r[0].rate, r[0].e0);
         } else {
             const double ea = live[0] ? em[0] : em[1];
             const double eb = live[1] ? em[1] : em[0];
             DiffPair s;
             DiffPair::Counter c;
             shoot<DiffPair>(u, n, DiffPair::Energy{P.beta * ea, P.beta * eb}, P.stride, s, c);
-            if (live[0])
-                r[0].update(em[0], s.x.w, s.x.rescales * kRescaleLog2, c.a.nodes, points[out[0]]);
-            if (live[1])
-                r[1].update(em[1], s.y.w, s.y.rescales * kRescaleLog2, c.b.nodes, points[out[1]]);
+            if (live[0] && r[0].update(em[0], s.x.w, s.x.rescales * kRescaleLog2, c.a.nodes, points[out[0]]))
+                points_model(points[out[0]], r[0].rate, r[0].e0);
+            if (live[1] && r[1].update(em[1], s.y.w, s.y.rescales * kRescaleLog2, c.b.nodes, points[out[1]]))
+                points_model(points[out[1]], r[1].rate, r[1].e0);
         }
     }
 }
@@ -1859,6 +1889,7 @@ struct SegControl {
     int32_t live[K];
     int32_t counted[K];
     int32_t root[K];
+    int32_t remodel[K];
 };
 
 // Value of lane - J within its 16-lane row (J < 16; lanes whose source lies
@@ -1962,6 +1993,7 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
         r = Refiner{};
         ctl.live[q] = 0;
         ctl.st[q] = 0;
+        ctl.remodel[q] = 0;
         ctl.pts[q].n = 0;
         if (l < P.level_count) {
             if (P.flags != 0) {
@@ -1974,6 +2006,7 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
                     write_level(levels, status, iterations, out, __builtin_nan(""), 1, 0);
                 } else {
                     ctl.live[q] = 1;
+                    ctl.remodel[q] = r.rate != 0.0 ? 1 : 0;
                     ctl.pts[q] = points[out];
                     steps[out] = W[r.b.band].hi - W[r.b.band].lo - 1;
                 }
@@ -1981,12 +2014,6 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
         }
     }
     __syncthreads();
-    // The model transform of the interpolation points, one lane per point.
-    if (lane < K * kPts) {
-        const int32_t q = lane / kPts, i = lane % kPts;
-        if (ctl.live[q] && ctl.r[q].rate != 0.0 && i < ctl.pts[q].n)
-            point_model(ctl.pts[q], i, ctl.r[q].rate, ctl.r[q].e0);
-    }
     const double* uv_pot = uv + P.ut_offset;
     // Each level's window and segment bounds stay in registers between
     // shots; wid[q] is the first level of the group with the same window.
@@ -2006,14 +2033,23 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
         }
     }
     const StepConsts k = step_consts();
-    __syncthreads();
 
     for (;;) {
+        // The model transform of the interpolation points of levels that
+        // have just been given a rate (at the start, or by a midpoint shot),
+        // one lane per point.
+        if (lane < K * kPts) {
+            const int32_t q = lane / kPts, i = lane % kPts;
+            if (ctl.remodel[q] && i < ctl.pts[q].n)
+                point_model(ctl.pts[q], i, ctl.r[q].rate, ctl.r[q].e0);
+        }
+        __syncthreads();
         // The probes: each level's plan (lane q), the polynomial tables of
         // the root plans (a row of lanes per level), then the probes (lane q).
         if (lane < K) {
             const int32_t q = lane;
             ctl.root[q] = 0;
+            ctl.remodel[q] = 0;
             if (ctl.live[q]) {
                 Refiner r = ctl.r[q];
                 bool done = false;
@@ -2030,6 +2066,7 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
                     // The fields a plan sets.
                     ctl.r[q].isolated = r.isolated;
                     ctl.r[q].reshoot = r.reshoot;
+                    ctl.r[q].mid = r.mid;
                     ctl.r[q].side = r.side;
                 }
                 if (done) {
@@ -2147,7 +2184,7 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
                     }
                 Refiner r = ctl.r[qi];
                 RefinePoints pts = ctl.pts[qi];
-                r.update(ctl.em[qi], wi, ki, c, pts);
+                ctl.remodel[qi] = r.update(ctl.em[qi], wi, ki, c, pts) ? 1 : 0;
                 ctl.r[qi] = r;
                 ctl.pts[qi] = pts;
             }

@@ -992,8 +992,12 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
     int32_t it = 0;
     /* With a model rate b, regula falsi runs on G = F e^(-b (E - E_hi0)),
      * nearly linear in E where F grows exponentially across the bracket;
-     * sign(G) = sign(F), so counts and brackets are unchanged. */
-    const double rate = br->rate, e0 = br->e_hi;
+     * sign(G) = sign(F), so counts and brackets are unchanged.  Without one
+     * (refit set), the first isolated probe is the bracket's midpoint and the
+     * rate is fitted to the three equally spaced values it gives. */
+    double rate = br->rate;
+    const double e0 = br->e_hi;
+    int32_t refit = rate == 0.0 ? 1 : 0;
     eo_points pts;
     pts.n = br->n_pts;
     pts.ih = br->i_hi;
@@ -1020,10 +1024,13 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
         double em;
         const int32_t isolated = lo_ok && cl == v && ch == v + 1;
         const int32_t reshoot = isolated && lo_foreign;
+        const int32_t mid = isolated && !reshoot && refit;
         if (reshoot) {
             /* The floor's value came from another band's window: shoot it
              * again in this level's window (its count stays the scan's). */
             em = el;
+        } else if (mid) {
+            em = el + 0.5 * width;
         } else if (isolated) {
             const double r = ldexp(wh / wl, kh - kl);
             const double s = 1.0 / (1.0 - r);
@@ -1056,6 +1063,19 @@ int32_t eo_refine_level(const double* uv, int64_t n, double beta, int32_t stride
         }
         if (isolated)
             c = reshoot ? v : (((int32_t)eo_signbit(w) == (v & 1)) ? v : v + 1);
+        if (mid) {
+            /* The rate through (el, em, eh), em above the root when c > v;
+             * the ends and the points take the transform from here on. */
+            refit = 0;
+            const double b = eo_model_rate(wl, kl, w, k, wh, kh, c > v ? 1 : 0, 0.5 * width);
+            if (b != 0.0) {
+                rate = b;
+                eo_model_apply(rate, e0, el, &wl, &kl);
+                eo_model_apply(rate, e0, eh, &wh, &kh);
+                for (int32_t i = 0; i < pts.n; ++i)
+                    eo_model_apply(rate, e0, pts.e[i], &pts.w[i], &pts.k[i]);
+            }
+        }
         if (rate != 0.0)
             eo_model_apply(rate, e0, em, &w, &k);
         ++it;

@@ -64,14 +64,22 @@ __device__ __forceinline__ StepConsts step_consts()
     return k;
 }
 
+// Node counter: each sample's sign bit is shifted into `bits` (newest in bit
+// 0; one funnel shift per sample), and the sign changes are counted in
+// batches.  flush(2^m - 1) adds those among the m newest samples and the
+// sample before them (bit m; zero-initialised bits stand for the positive
+// start), so m <= 31 samples may wait between flushes.
 struct SignCounter {
     int32_t nodes;
-    uint32_t last;
+    uint32_t bits;
     __device__ __forceinline__ void sample(const double w)
     {
-        const uint32_t sb = sign_bit(w);
-        nodes += static_cast<int32_t>(sb ^ last);
-        last = sb;
+        const uint32_t hi = static_cast<uint32_t>(static_cast<uint64_t>(__double_as_longlong(w)) >> 32);
+        bits = __builtin_amdgcn_alignbit(bits, hi, 31u);
+    }
+    __device__ __forceinline__ void flush(const uint32_t mask)
+    {
+        nodes += __builtin_popcount((bits ^ (bits >> 1)) & mask);
     }
 };
 
@@ -161,6 +169,11 @@ struct DiffPair {
     };
     struct Counter {
         SignCounter a, b;
+        __device__ __forceinline__ void flush(const uint32_t mask)
+        {
+            a.flush(mask);
+            b.flush(mask);
+        }
     };
     DiffForm x, y;
     __device__ __forceinline__ void init(const Energy e, const double u0)
@@ -192,6 +205,12 @@ struct DiffPair {
 // order, so every wait is lgkmcnt(0)).
 __device__ __forceinline__ void fence_schedule() { __builtin_amdgcn_sched_barrier(0); }
 
+// Steps between node-count flushes at stride B (at most 16 samples wait).
+template <int B>
+constexpr int kFlushSteps = B >= 8 ? 128 : 16 * B;
+
+// Step Q of the 128-step loop body: the sign is sampled after every B-th
+// step and the counter flushed after every kFlushSteps<B>-th.
 template <int B, int Q, class Form>
 __device__ __forceinline__ void step_at(const double u, const typename Form::Energy e,
                                         const StepConsts& k, Form& s, typename Form::Counter& c)
@@ -199,6 +218,8 @@ __device__ __forceinline__ void step_at(const double u, const typename Form::Ene
     s.step(u, e, k);
     if constexpr ((Q + 1) % B == 0)
         s.sample(c);
+    if constexpr ((Q + 1) % kFlushSteps<B> == 0)
+        c.flush((1u << (kFlushSteps<B> / B)) - 1u);
 }
 
 template <int B, int Q0, class Form, int... Is>
@@ -209,23 +230,23 @@ __device__ __forceinline__ void steps_1_15(const double (&buf)[16], const typena
     (step_at<B, Q0 + Is + 1>(buf[Is + 1], e, k, s, c), ...);
 }
 
-// One 32-step super-block fed from the two 16-point SGPR buffers: each
-// buffer's scalar loads issue right after the first step of the other buffer,
-// so a load has 15 steps to land.  With R the rescale test follows the 17th
-// step.
-template <int B, bool R, class Form>
+// One 32-step super-block (number S of the loop body) fed from the two
+// 16-point SGPR buffers: each buffer's scalar loads issue right after the
+// first step of the other buffer, so a load has 15 steps to land.  With R
+// the rescale test follows the 17th step.
+template <int B, bool R, int S, class Form>
 __device__ __forceinline__ void super_block(const double*& p, double (&a)[16], double (&b)[16],
                                             const typename Form::Energy e, const StepConsts& k, Form& s,
                                             typename Form::Counter& c)
 {
-    step_at<B, 0>(a[0], e, k, s, c);
+    step_at<B, 32 * S>(a[0], e, k, s, c);
     fence_schedule();
 #pragma unroll
     for (int q = 0; q < 16; ++q)
         b[q] = p[16 + q];
     fence_schedule();
-    steps_1_15<B, 0>(a, e, k, s, c, std::make_integer_sequence<int, 15>{});
-    step_at<B, 16>(b[0], e, k, s, c);
+    steps_1_15<B, 32 * S>(a, e, k, s, c, std::make_integer_sequence<int, 15>{});
+    step_at<B, 32 * S + 16>(b[0], e, k, s, c);
     if constexpr (R)
         s.rescale();
     fence_schedule();
@@ -233,7 +254,7 @@ __device__ __forceinline__ void super_block(const double*& p, double (&a)[16], d
     for (int q = 0; q < 16; ++q)
         a[q] = p[32 + q];
     fence_schedule();
-    steps_1_15<B, 16>(b, e, k, s, c, std::make_integer_sequence<int, 15>{});
+    steps_1_15<B, 32 * S + 16>(b, e, k, s, c, std::make_integer_sequence<int, 15>{});
     p += 32;
 }
 
@@ -263,18 +284,18 @@ __device__ __forceinline__ void shoot_fixed(const double* __restrict__ u, const 
         a[q] = p[q];
     constexpr bool odd = Form::kRescaleEvery == 2;
     for (int32_t g = 0; g < (nsb >> 2); ++g) {
-        super_block<B, false>(p, a, b, e, k, s, c);
-        super_block<B, odd>(p, a, b, e, k, s, c);
-        super_block<B, false>(p, a, b, e, k, s, c);
-        super_block<B, true>(p, a, b, e, k, s, c);
+        super_block<B, false, 0>(p, a, b, e, k, s, c);
+        super_block<B, odd, 1>(p, a, b, e, k, s, c);
+        super_block<B, false, 2>(p, a, b, e, k, s, c);
+        super_block<B, true, 3>(p, a, b, e, k, s, c);
     }
     const int32_t left = nsb & 3;
     if (left >= 1)
-        super_block<B, false>(p, a, b, e, k, s, c);
+        super_block<B, false, 0>(p, a, b, e, k, s, c);
     if (left >= 2)
-        super_block<B, odd>(p, a, b, e, k, s, c);
+        super_block<B, odd, 1>(p, a, b, e, k, s, c);
     if (left >= 3)
-        super_block<B, false>(p, a, b, e, k, s, c);
+        super_block<B, false, 2>(p, a, b, e, k, s, c);
     const int32_t rem = steps - (nsb << 5);
     for (int32_t q = 0; q < rem; ++q) {
         s.step(p[q], e, k);
@@ -283,6 +304,10 @@ __device__ __forceinline__ void shoot_fixed(const double* __restrict__ u, const 
     }
     if (rem % B != 0)
         s.sample(c);
+    // Samples since the last flush: those of the remaining super-blocks past
+    // the last flush point and of the tail (at most 16).
+    const int32_t pend = ((32 * left) % kFlushSteps<B>) / B + rem / B + (rem % B != 0 ? 1 : 0);
+    c.flush((1u << pend) - 1u);
 }
 
 template <class Form>

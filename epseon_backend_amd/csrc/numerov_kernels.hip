@@ -1011,7 +1011,7 @@ __device__ __forceinline__ void poly_table(const RefinePoints& p, const double e
 }
 
 // Root in (0, 1) of the table's polynomial by at most 8 Newton steps from
-// x0, stopping once a step is at most 2^-40; false when a derivative
+// x0, stopping once a step is at most 2^-30; false when a derivative
 // vanishes or a step leaves (0, 1).
 __device__ __forceinline__ bool poly_root(const PolyTable& t, const double x0, double& root)
 {
@@ -1037,7 +1037,7 @@ __device__ __forceinline__ bool poly_root(const PolyTable& t, const double x0, d
         r = r - dt;
         if (!(r > 0.0 && r < 1.0))
             return false;
-        if (fabs(dt) <= 0x1p-40)
+        if (fabs(dt) <= 0x1p-30)
             break;
     }
     root = r;
@@ -1458,7 +1458,6 @@ __device__ __forceinline__ void walk_segment(const Src& src, const int32_t len, 
 // norm below 7.4), so 128 steps stay inside the normal range, and scaling by
 // a power of two is exact.
 struct MatrixBody {
-    static constexpr int kChunk = 8;
     double e, em;
     double wa, da, pa, wb, db, pb;
     int32_t ex;

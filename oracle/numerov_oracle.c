@@ -921,7 +921,7 @@ static void eo_points_insert(eo_points* p, double c, double em, double w, int32_
 
 /* Root in (0, 1) of the polynomial through the points in x = (E - el)/width,
  * y = w 2^(k - max k) (Newton divided differences), by at most 8 Newton steps
- * from x0, stopping once a step is at most 2^-40.  0 when there are fewer
+ * from x0, stopping once a step is at most 2^-30.  0 when there are fewer
  * than 3 points, a derivative vanishes or a step leaves (0, 1). */
 static int32_t eo_points_root(const eo_points* p, double el, double width, double x0, double* root)
 {
@@ -954,7 +954,7 @@ static int32_t eo_points_root(const eo_points* p, double el, double width, doubl
         t = t - dt;
         if (!(t > 0.0 && t < 1.0))
             return 0;
-        if (fabs(dt) <= 0x1p-40)
+        if (fabs(dt) <= 0x1p-30)
             break;
     }
     *root = t;

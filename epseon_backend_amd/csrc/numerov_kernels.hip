@@ -781,8 +781,10 @@ __device__ __forceinline__ double model_log(const double x)
 
 // Rate b of F(E) = (c0 + c1 (E - E_b)) e^(b (E - E_b)) through equally spaced
 // scan points a < b < c: with x = e^(b de), F_a/F_b x + F_c/F_b / x = 2, one
-// positive root either way the third point lies.  0 when no fit.
-__device__ __forceinline__ double model_rate(const double wa, const int32_t ka, const double wb,
+// positive root either way the third point lies.  0 when no fit.  Not
+// inlined: its square root and logarithm would raise the register peak of
+// the refinement kernels that call it once per level.
+__device__ __attribute__((noinline)) double model_rate(const double wa, const int32_t ka, const double wb,
                                              const int32_t kb, const double wc, const int32_t kc,
                                              const bool above, const double de)
 {
@@ -1176,13 +1178,15 @@ struct Refiner {
     // Take the probe's result; `c` is ignored when the bracket is isolated.
     // True when a midpoint shot gave the level its rate: the caller then
     // applies the transform to every interpolation point (the shot is
-    // inserted untransformed, so all of them take it alike).
+    // inserted untransformed, so all of them take it alike).  kMidShot false
+    // leaves out the midpoint's rate fit, for callers that never pass one.
+    template <bool kMidShot = true>
     __device__ __forceinline__ bool update(const double em, double w, int32_t k, int32_t c, RefinePoints& p)
     {
         if (isolated)
             c = reshoot ? v : ((static_cast<int32_t>(sign_bit(w)) == (v & 1)) ? v : v + 1);
         bool fitted = false;
-        if (mid) {
+        if (kMidShot && mid) {
             // The rate through (el, em, eh), em above the root when c > v;
             // the ends take the transform from here on.
             refit = 0;
@@ -1914,6 +1918,10 @@ struct SegControl {
     int32_t counted[K];
     int32_t root[K];
     int32_t remodel[K];
+    // A midpoint shot's result, taken at the start of the next round.
+    double held_w[K];
+    int32_t held_k[K];
+    int32_t held[K];
 };
 
 // Value of lane - J within its 16-lane row (J < 16; lanes whose source lies
@@ -2018,6 +2026,7 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
         ctl.live[q] = 0;
         ctl.st[q] = 0;
         ctl.remodel[q] = 0;
+        ctl.held[q] = 0;
         ctl.pts[q].n = 0;
         if (l < P.level_count) {
             if (P.flags != 0) {
@@ -2059,6 +2068,16 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
     const StepConsts k = step_consts();
 
     for (;;) {
+        // Midpoint shots of the last round (lane q): their rate fit is kept
+        // out of the walks' update phase, where registers are scarce.
+        if (lane < K) {
+            const int32_t q = lane;
+            if (ctl.held[q]) {
+                ctl.held[q] = 0;
+                ctl.remodel[q] = ctl.r[q].update(ctl.em[q], ctl.held_w[q], ctl.held_k[q], 0, ctl.pts[q]) ? 1 : 0;
+            }
+        }
+        __syncthreads();
         // The model transform of the interpolation points of levels that
         // have just been given a rate (at the start, or by a midpoint shot),
         // one lane per point.
@@ -2206,11 +2225,17 @@ numerov_refine_segmented_kernel(const double* __restrict__ uv, const double* __r
                         ki = fk[i];
                         qi = mem[i];
                     }
-                Refiner r = ctl.r[qi];
-                RefinePoints pts = ctl.pts[qi];
-                ctl.remodel[qi] = r.update(ctl.em[qi], wi, ki, c, pts) ? 1 : 0;
-                ctl.r[qi] = r;
-                ctl.pts[qi] = pts;
+                if (ctl.r[qi].mid) {
+                    ctl.held_w[qi] = wi;
+                    ctl.held_k[qi] = ki;
+                    ctl.held[qi] = 1;
+                } else {
+                    Refiner r = ctl.r[qi];
+                    RefinePoints pts = ctl.pts[qi];
+                    r.update<false>(ctl.em[qi], wi, ki, c, pts);
+                    ctl.r[qi] = r;
+                    ctl.pts[qi] = pts;
+                }
             }
             __syncthreads();
         }

@@ -28,7 +28,7 @@
 // bit-identical to the C oracle.
 //
 // Kernels in this file, in launch order: potential statistics and table
-// shifts (load), band windows (each shot stops 20 guaranteed e-folds past its
+// shifts (load), band windows (each shot stops 15 guaranteed e-folds past its
 // band top's outermost turning points), segment tables (load), the scan, the
 // bracket pass (one thread per level), and the refinement: lane mode (one
 // level per lane over the whole grid) or segmented mode (one block per level,

@@ -153,7 +153,7 @@ struct PotentialStats {
 // g(x) = 2^(floor((k-1)/2) - 1) for x = m 2^k, m in [1/2, 1).  A window's
 // Dirichlet points sit where the sum of g from the outermost allowed point
 // (uv_i <= e) first reaches kWindowDepth, or at the grid ends; moving a wall
-// from there to the grid end changes the levels by an e^-40-order fraction
+// from there to the grid end changes the levels by an e^-30-order fraction
 // of a spacing.  The sum is kept in 2^-40 fixed point (exact, any order).
 // The energy window [e_lo, e_hi] splits into kBands bands with tops
 // E_b = fma(b + 1, (e_hi - e_lo) / kBands, e_lo); a band's window is the
@@ -161,7 +161,7 @@ struct PotentialStats {
 // window of the band holding the group's highest energy; a level is refined
 // in the window of its bracket's upper scan point.
 constexpr int kBands = 8;
-constexpr int kWindowDepth = 20;
+constexpr int kWindowDepth = 15;
 
 // One band window of a potential: Dirichlet points lo, hi (on the trimmed
 // grid) and the far-end scale of its n = hi - lo + 1 point scan shots

@@ -257,7 +257,7 @@ typedef struct epseon_tuning {
     int32_t segmented_lanes;      /* segmented mode: 0 (from the batch size), 64, 128 or 256 */
     int32_t xcd_scan;             /* 1: whole-batch scans and segmented refinements bin potentials per
                                      XCD (default), 0: in order */
-    int32_t windows;              /* 1: shots stop 20 guaranteed e-folds past the outermost turning
+    int32_t windows;              /* 1: shots stop 15 guaranteed e-folds past the outermost turning
                                      points of their energy band's top (EPSEON_BANDS bands per
                                      potential); 0: every shot spans the trimmed grid; 2 (default):
                                      1 when the scan has at least 8 wavefronts (64 trial energies

@@ -305,8 +305,9 @@ void eo_trajectory(const double* ut, int64_t n, double e, double* w_out)
  * the recurrence's dominant root lambda has lambda + 1/lambda >= 2 + x).
  * Moving a Dirichlet point into that region from depth S to depth S' > S
  * changes the eigenvalues only by a relative e^(-2S)-order amount, so shots
- * may stop where the guaranteed depth reaches EO_WINDOW_DEPTH = 20 e-folds
- * (e^-40 ~ 4e-18 of a level spacing).  The depth is summed in 2^-40 fixed
+ * may stop where the guaranteed depth reaches EO_WINDOW_DEPTH = 15 e-folds
+ * (e^-30 ~ 1e-13 of a level spacing; the bound g below is below the true
+ * rate by up to a factor 2, so the true depth is larger).  The depth is summed in 2^-40 fixed
  * point from the exact lower bound g(x) = 2^(floor((k-1)/2) - 1) of
  * acosh(1 + x/2) for x = m 2^k, m in [1/2, 1), x < 6 (which the wall trim
  * guarantees), so the sum is exact and independent of order.
@@ -324,7 +325,7 @@ void eo_trajectory(const double* ut, int64_t n, double e, double* w_out)
  * and a level is refined in the window of the group of its bracket's upper
  * scan point.  Windows off = every band's window is the whole grid. */
 #define EO_BANDS 8
-#define EO_WINDOW_DEPTH 20
+#define EO_WINDOW_DEPTH 15
 
 static int64_t eo_growth_term(double x)
 {

@@ -1,6 +1,6 @@
 """Shot windows of the oracle (numerov_oracle.c, windows section).
 
-A shot stops where it has gone at least 20 e-folds into the classically
+A shot stops where it has gone at least 15 e-folds into the classically
 forbidden region past its outermost turning points (on both sides), and the
 scan and the refinement use per-energy-band windows.  Pinned here:
 
@@ -25,7 +25,7 @@ from epseon_backend_amd.workloads import batched, double_wells
 from tests.helpers import morse_problem, problem_from_spec
 
 K = oracle.K_CM
-DEPTH = 20.0
+DEPTH = 15.0
 
 
 def growth_bound(x: np.ndarray) -> np.ndarray:
@@ -106,7 +106,7 @@ def _dense_levels(v: np.ndarray, h: float, mu: float) -> np.ndarray:
 def test_window_leaves_dense_eigenvalues_unchanged():
     """Heavy Morse on a coarse grid: every band's window cuts deep into both
     walls, yet the levels below the band top agree with the whole-grid
-    problem's to rounding (the truncation effect is ~e^-40 of a spacing)."""
+    problem's to rounding (the truncation effect is ~e^-30 of a spacing)."""
     prob = morse_problem(mu=63.452, de=12440.0, a=1.8676, re=2.666, r_min=2.2, r_max=5.0, n=1400, m=512)
     first, last, ok = prob.trimmed
     assert ok and first == 0
@@ -115,7 +115,7 @@ def test_window_leaves_dense_eigenvalues_unchanged():
     assert win[0][1] - win[0][0] + 1 < prob.n // 2
     # Dense-solver rounding: a few ulp of the operator norm (~ the largest
     # eigenvalue).  The check has teeth: windows cut where the bound's sum
-    # reaches 4 instead of 20 move these levels by ~3e-8 cm^-1 (2: ~7e-4).
+    # reaches 4 instead of 15 move these levels by ~3e-8 cm^-1 (2: ~7e-4).
     atol = 64 * np.finfo(float).eps * np.abs(full).max()
     assert atol < 1e-8
     for b, (lo, hi) in enumerate(win):

@@ -513,15 +513,45 @@ __device__ __forceinline__ bool same_window(const BandWindow* W, const int32_t a
     return W[a].lo == W[b].lo && W[a].hi == W[b].hi;
 }
 
-// Lower bound g(x) of the per-step growth in 2^-40 fixed point.
+// floor(sqrt(n)) for n < 2^53 (the double square root, then corrected).
+__device__ __forceinline__ uint64_t isqrt_exact(const uint64_t n)
+{
+    uint64_t r = static_cast<uint64_t>(__builtin_sqrt(static_cast<double>(n)));
+    while (r * r > n)
+        --r;
+    while ((r + 1) * (r + 1) <= n)
+        ++r;
+    return r;
+}
+
+// Lower bound g(x) of the per-step growth acosh(1 + x/2) in 2^-40 fixed
+// point, within 9 % of it: with x = m 2^k (m in [1/2, 1)) and T the top 16
+// bits of m's significand, x >= T 2^(k-16), and sqrt(T 2^(k-16)) 2^40 =
+// sqrt(T 2^r) 2^h (k + 64 = 2 h + r) is taken as isqrt(T 2^(r+32)) 2^(h-16),
+// floored, times floor(65536 acosh(1 + y/2)/sqrt(y)) at y = 2^k (the ratio
+// falls with y; entry -5 serves below) over 65536.  Integer operations only
+// after frexp.  0 below 2^-65; x >= 8 counts as just below 8.
 __device__ __forceinline__ int64_t growth_term(const double x)
 {
+    constexpr int64_t kRatio[9] = {65450, 65366, 65199, 64871, 64242, 63073, 61029, 57761, 53116};
     int k = 0;
-    (void)__builtin_frexp(x, &k);
-    const int32_t s = ((k - 1) >> 1) - 1 + 40;
-    if (s < 0)
+    const double m = __builtin_frexp(x, &k);
+    if (!(x > 0.0) || k < -64)
         return 0;
-    return int64_t(1) << (s > 48 ? 48 : s);
+    uint64_t t = static_cast<uint64_t>(__builtin_ldexp(m, 16));
+    if (k > 3) {
+        k = 3;
+        t = 0xFFFF;
+    }
+    const int32_t e = k + 64, h = e >> 1;
+    const uint64_t r = isqrt_exact(t << ((e & 1) + 32));
+    const uint64_t sq = h >= 16 ? r << (h - 16) : r >> (16 - h);
+    int64_t f = kRatio[0];
+#pragma unroll
+    for (int32_t i = 1; i < 9; ++i)
+        if (k == i - 5)
+            f = kRatio[i];
+    return static_cast<int64_t>((sq * static_cast<uint64_t>(f)) >> 16);
 }
 
 __device__ __forceinline__ int64_t wave_inclusive_sum(int64_t x)

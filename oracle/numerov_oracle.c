@@ -306,16 +306,15 @@ void eo_trajectory(const double* ut, int64_t n, double e, double* w_out)
  * Moving a Dirichlet point into that region from depth S to depth S' > S
  * changes the eigenvalues only by a relative e^(-2S)-order amount, so shots
  * may stop where the guaranteed depth reaches EO_WINDOW_DEPTH = 15 e-folds
- * (e^-30 ~ 1e-13 of a level spacing; the bound g below is below the true
- * rate by up to a factor 2, so the true depth is larger).  The depth is summed in 2^-40 fixed
- * point from the exact lower bound g(x) = 2^(floor((k-1)/2) - 1) of
- * acosh(1 + x/2) for x = m 2^k, m in [1/2, 1), x < 6 (which the wall trim
- * guarantees), so the sum is exact and independent of order.
+ * (e^-30 ~ 1e-13 of a level spacing).  The depth is summed in 2^-40 fixed
+ * point from an integer lower bound g(x) of acosh(1 + x/2) within 9 % of it
+ * (eo_growth_term; x < 6, which the wall trim guarantees), so the sum is
+ * exact and independent of order.
  *
  * Window of a probe e (= beta E) on the n-point trimmed grid: i_first and
  * i_last are the first and last points with uv_i <= e; hi is the first
- * i > i_last at which sum_{i_last < j <= i} g(x_j) >= 20 (else n - 1), lo the
- * last i < i_first at which sum_{i <= j < i_first} g(x_j) >= 20 (else 0).
+ * i > i_last at which sum_{i_last < j <= i} g(x_j) >= 15 (else n - 1), lo the
+ * last i < i_first at which sum_{i <= j < i_first} g(x_j) >= 15 (else 0).
  *
  * Energy bands: the window [e_lo, e_hi] splits into EO_BANDS bands with tops
  * E_b = fma(b + 1, (e_hi - e_lo)/EO_BANDS, e_lo); band b's window is the
@@ -327,16 +326,43 @@ void eo_trajectory(const double* ut, int64_t n, double e, double* w_out)
 #define EO_BANDS 8
 #define EO_WINDOW_DEPTH 15
 
+/* floor(65536 acosh(1 + x/2)/sqrt(x)) at x = 2^k, k = -5..3: the ratio only
+ * falls with x, so entry k bounds it on [2^(k-1), 2^k) and entry -5 below. */
+static const int64_t eo_growth_ratio[9] = {65450, 65366, 65199, 64871, 64242, 63073, 61029, 57761, 53116};
+
+/* floor(sqrt(n)) for n < 2^53 (the double square root, then corrected). */
+static uint64_t eo_isqrt(uint64_t n)
+{
+    uint64_t r = (uint64_t)sqrt((double)n);
+    while (r * r > n)
+        --r;
+    while ((r + 1) * (r + 1) <= n)
+        ++r;
+    return r;
+}
+
+/* Lower bound of acosh(1 + x/2) in 2^-40 fixed point, x > 0: with
+ * x = m 2^k (m in [1/2, 1)), T = the top 16 bits of m's 53-bit significand,
+ * x >= T 2^(k-16); sqrt(T 2^(k-16)) 2^40 = sqrt(T 2^r) 2^h (k + 64 = 2 h + r)
+ * is taken as isqrt(T 2^(r+32)) 2^(h-16), floored, and scaled by the ratio
+ * table.  Integer operations only after frexp, so any two implementations
+ * agree.  0 below 2^-65; x >= 8 counts as just below 8. */
 static int64_t eo_growth_term(double x)
 {
     int k;
-    (void)frexp(x, &k);
-    int s = ((k - 1) >> 1) - 1 + 40;
-    if (s < 0)
+    const double m = frexp(x, &k);
+    if (!(x > 0.0) || k < -64)
         return 0;
-    if (s > 48)
-        s = 48;
-    return (int64_t)1 << s;
+    uint64_t t = (uint64_t)ldexp(m, 16);
+    if (k > 3) {
+        k = 3;
+        t = 0xFFFF;
+    }
+    const int32_t e = k + 64, h = e >> 1;
+    const uint64_t r = eo_isqrt(t << ((e & 1) + 32));
+    const uint64_t sq = h >= 16 ? r << (h - 16) : r >> (16 - h);
+    const int64_t f = eo_growth_ratio[(k < -5 ? -5 : k) + 5];
+    return (int64_t)((sq * (uint64_t)f) >> 16);
 }
 
 void eo_window(const double* uv, int64_t n, double e, int64_t* lo, int64_t* hi)

@@ -28,9 +28,47 @@ K = oracle.K_CM
 DEPTH = 15.0
 
 
+# floor(65536 acosh(1 + y/2)/sqrt(y)) at y = 2^k, k = -5..3.
+RATIO = np.array([65450, 65366, 65199, 64871, 64242, 63073, 61029, 57761, 53116], dtype=np.uint64)
+
+
+def growth_fixed(x: np.ndarray) -> np.ndarray:
+    """The oracle's growth bound g(x) in 2^-40 units (numerov_oracle.c,
+    eo_growth_term), restated with numpy integers."""
+    x = np.asarray(x, dtype=np.float64)
+    m, k = np.frexp(x)
+    k = k.astype(np.int64)
+    t = np.floor(np.ldexp(m, 16)).astype(np.uint64)
+    big = k > 3
+    k = np.where(big, 3, k)
+    t = np.where(big, np.uint64(0xFFFF), t)
+    e = k + 64
+    h = e >> 1
+    n = t << ((e & 1) + 32).astype(np.uint64)
+    r = np.floor(np.sqrt(n.astype(np.float64))).astype(np.uint64)
+    for _ in range(2):
+        r = np.where(r * r > n, r - np.uint64(1), r)
+        r = np.where((r + np.uint64(1)) * (r + np.uint64(1)) <= n, r + np.uint64(1), r)
+    up = np.clip(h - 16, 0, 63).astype(np.uint64)
+    down = np.clip(16 - h, 0, 63).astype(np.uint64)
+    sq = np.where(h >= 16, r << up, r >> down)
+    f = RATIO[np.clip(k, -5, 3) + 5]
+    g = (sq * f) >> np.uint64(16)
+    return np.where((x > 0) & (k >= -64), g, 0).astype(np.int64)
+
+
 def growth_bound(x: np.ndarray) -> np.ndarray:
-    _, k = np.frexp(x)
-    return np.ldexp(1.0, (k - 1) // 2 - 1)
+    return np.ldexp(growth_fixed(x).astype(np.float64), -40)
+
+
+def test_growth_ratio_table():
+    """The table holds floor(65536 acosh(1 + y/2)/sqrt(y)) at y = 2^k."""
+    import mpmath
+
+    mpmath.mp.dps = 40
+    for i, k in enumerate(range(-5, 4)):
+        y = mpmath.mpf(2) ** k
+        assert int(RATIO[i]) == int(mpmath.floor(65536 * mpmath.acosh(1 + y / 2) / mpmath.sqrt(y)))
 
 
 def test_growth_bound_is_below_the_recurrence_growth():
@@ -39,8 +77,13 @@ def test_growth_bound_is_below_the_recurrence_growth():
     g = growth_bound(x)
     a = 2.0 * np.arcsinh(0.5 * np.sqrt(x))  # acosh(1 + x/2), accurate for small x
     assert np.all(g <= a)
-    # Within a factor 4 of sqrt(x) (so the windows stay tight for small x).
-    assert np.all(g > 0.25 * np.sqrt(x) * (1 - 1e-15))
+    # Within 9 % of it wherever the 2^-40 resolution allows (so the windows
+    # stay tight); one ulp of x below each binade edge is the worst case.
+    fine = x >= 2.0**-40
+    assert np.all(g[fine] > 0.91 * a[fine])
+    edges = np.nextafter(2.0 ** np.arange(-40, 3), 0.0)
+    edges = edges[edges < 6.0]
+    assert np.all(growth_bound(edges) > 0.91 * 2.0 * np.arcsinh(0.5 * np.sqrt(edges)))
     # Constant-f Numerov recurrence f y+ = (12 - 10 f) y - f y-, f = 1 - x/12:
     # its roots satisfy lambda + 1/lambda = 12/f - 10 = 2 + x/f >= 2 + x, so
     # ln(lambda) = acosh(1 + x/(2f)) >= acosh(1 + x/2).
@@ -54,12 +97,13 @@ def _check_window(uv: np.ndarray, e: float) -> tuple[int, int]:
     allowed = np.nonzero(uv <= e)[0]
     assert lo <= allowed[0] and allowed[-1] <= hi
     x = 12.0 * (uv - e)
+    depth = int(DEPTH) << 40
     if hi < len(uv) - 1:
-        right = growth_bound(x[allowed[-1] + 1 : hi + 1])
-        assert right.sum() >= DEPTH > right[:-1].sum()
+        right = growth_fixed(x[allowed[-1] + 1 : hi + 1])
+        assert right.sum() >= depth > right[:-1].sum()
     if lo > 0:
-        left = growth_bound(x[lo : allowed[0]])
-        assert left.sum() >= DEPTH > left[1:].sum()
+        left = growth_fixed(x[lo : allowed[0]])
+        assert left.sum() >= depth > left[1:].sum()
     return lo, hi
 
 
@@ -115,7 +159,8 @@ def test_window_leaves_dense_eigenvalues_unchanged():
     assert win[0][1] - win[0][0] + 1 < prob.n // 2
     # Dense-solver rounding: a few ulp of the operator norm (~ the largest
     # eigenvalue).  The check has teeth: windows cut where the bound's sum
-    # reaches 4 instead of 15 move these levels by ~3e-8 cm^-1 (2: ~7e-4).
+    # reaches 4 instead of 15 move these levels by ~2e-2 cm^-1 (8: ~6e-6, 10: ~1e-7;
+    # e^-2 per unit of depth).
     atol = 64 * np.finfo(float).eps * np.abs(full).max()
     assert atol < 1e-8
     for b, (lo, hi) in enumerate(win):

@@ -160,7 +160,7 @@ class Tuning(ctypes.Structure):
 
 
 TUNE_KEEP_RESULTS = 1
-BANDS = 8  # energy bands of the shot windows (EPSEON_BANDS)
+BANDS = 16  # energy bands of the shot windows (EPSEON_BANDS)
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int32_p = ctypes.POINTER(ctypes.c_int32)

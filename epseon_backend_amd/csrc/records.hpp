@@ -160,7 +160,7 @@ struct PotentialStats {
 // window at its top.  The scan shoots a group of 64 trial energies in the
 // window of the band holding the group's highest energy; a level is refined
 // in the window of its bracket's upper scan point.
-constexpr int kBands = 8;
+constexpr int kBands = 16;
 constexpr int kWindowDepth = 15;
 
 // One band window of a potential: Dirichlet points lo, hi (on the trimmed

@@ -193,7 +193,7 @@ int epseon_potential_info_get(epseon_ctx* ctx, int32_t p, epseon_potential_info*
 
 /* Energy bands per potential: the window [e_lo, e_hi] splits into bands with
  * tops E_b = fma(b + 1, (e_hi - e_lo) / EPSEON_BANDS, e_lo). */
-#define EPSEON_BANDS 8
+#define EPSEON_BANDS 16
 
 /* Shot windows of potential p for the last scan (or, before any scan, for
  * the current tuning): for each band b, the Dirichlet points lo[b] and hi[b]

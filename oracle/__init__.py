@@ -30,7 +30,7 @@ K_CM = 16.857629191640175  # hbar^2/(2 amu A^2) in cm^-1
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 _lp = ctypes.POINTER(ctypes.c_int64)
-BANDS = 8  # energy bands of the shot windows (numerov_oracle.c, EO_BANDS)
+BANDS = 16  # energy bands of the shot windows (numerov_oracle.c, EO_BANDS)
 _LIB: ctypes.CDLL | None = None
 
 

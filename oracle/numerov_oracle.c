@@ -323,7 +323,7 @@ void eo_trajectory(const double* ut, int64_t n, double e, double* w_out)
  * its highest energy (the first b with E_b >= E_top, else the last band),
  * and a level is refined in the window of the group of its bracket's upper
  * scan point.  Windows off = every band's window is the whole grid. */
-#define EO_BANDS 8
+#define EO_BANDS 16
 #define EO_WINDOW_DEPTH 15
 
 /* floor(65536 acosh(1 + x/2)/sqrt(x)) at x = 2^k, k = -5..3: the ratio only

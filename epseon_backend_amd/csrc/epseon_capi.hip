@@ -390,9 +390,13 @@ struct epseon_ctx {
     // Position of each potential in task order.
     std::vector<int32_t> order_pos;
 
-    // Host copies of the last refine's status and iteration arrays, filled
-    // by epseon_get_levels so that epseon_get_stats needs no extra copy.
-    std::vector<int32_t> h_status, h_iters;
+    // Host views of the last refine's levels, status, iterations and steps:
+    // pointers into the pinned read-back block, set by fetch_levels so that
+    // epseon_get_stats needs no extra copy (valid while host_levels_valid).
+    const double* h_levels = nullptr;
+    const int32_t* h_status = nullptr;
+    const int32_t* h_iters = nullptr;
+    const int32_t* h_steps = nullptr;
     bool host_levels_valid = false;
     // Levels, status, iterations and steps live in one device block
     // (d_level_block, level_block_bytes) read back by one copy into pinned
@@ -401,13 +405,11 @@ struct epseon_ctx {
     size_t level_block_bytes = 0;
     void* h_pinned = nullptr;
     size_t h_pinned_bytes = 0;
-    std::vector<double> h_levels;
 
     bool loaded = false, scanned = false, refined = false, normalized = false;
     bool scan_timed = false, refine_timed = false, normalize_timed = false, vjp_timed = false;
     double scan_point_updates = 0.0;       // of the last scan
     double scan_pu_full = 0.0, scan_pu_band = 0.0;
-    std::vector<int32_t> h_steps;
     std::vector<BandWindow> h_win_full, h_win_band;
 
     explicit epseon_ctx(size_t block_bytes) : arena(block_bytes) {}
@@ -1886,14 +1888,12 @@ int epseon_ctx_synchronize(epseon_ctx* ctx)
 namespace {
 
 // Read the level block back (one copy into pinned memory, then a stream
-// synchronisation) into h_levels, h_status, h_iters and h_steps.
+// synchronisation); h_levels, h_status, h_iters and h_steps point into it.
 int fetch_levels(epseon_ctx* ctx)
 {
     const size_t n = static_cast<size_t>(ctx->total_levels);
-    ctx->h_levels.resize(n);
-    ctx->h_status.resize(n);
-    ctx->h_iters.resize(n);
-    ctx->h_steps.resize(n);
+    ctx->h_levels = nullptr;
+    ctx->h_status = ctx->h_iters = ctx->h_steps = nullptr;
     if (n > 0) {
         const size_t bytes = ctx->level_block_bytes;
         if (ctx->h_pinned_bytes < bytes) {
@@ -1911,13 +1911,13 @@ int fetch_levels(epseon_ctx* ctx)
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (n > 0) {
         const char* h = static_cast<const char*>(ctx->h_pinned);
-        std::memcpy(ctx->h_levels.data(), h, n * sizeof(double));
-        std::memcpy(ctx->h_status.data(), h + (reinterpret_cast<char*>(ctx->d_status) - ctx->d_level_block),
-                    n * sizeof(int32_t));
-        std::memcpy(ctx->h_iters.data(), h + (reinterpret_cast<char*>(ctx->d_iters) - ctx->d_level_block),
-                    n * sizeof(int32_t));
-        std::memcpy(ctx->h_steps.data(), h + (reinterpret_cast<char*>(ctx->d_steps) - ctx->d_level_block),
-                    n * sizeof(int32_t));
+        const auto at = [&](const void* d) {
+            return h + (static_cast<const char*>(d) - static_cast<const char*>(ctx->d_level_block));
+        };
+        ctx->h_levels = reinterpret_cast<const double*>(h);
+        ctx->h_status = reinterpret_cast<const int32_t*>(at(ctx->d_status));
+        ctx->h_iters = reinterpret_cast<const int32_t*>(at(ctx->d_iters));
+        ctx->h_steps = reinterpret_cast<const int32_t*>(at(ctx->d_steps));
     }
     ctx->host_levels_valid = true;
     return EPSEON_OK;
@@ -2148,11 +2148,11 @@ int epseon_get_levels(epseon_ctx* ctx, double* levels, int32_t* status, int32_t*
     if (const int rc = fetch_levels(ctx); rc != EPSEON_OK)
         return rc;
     if (n > 0 && levels)
-        std::memcpy(levels, ctx->h_levels.data(), n * sizeof(double));
+        std::memcpy(levels, ctx->h_levels, n * sizeof(double));
     if (n > 0 && status)
-        std::memcpy(status, ctx->h_status.data(), n * sizeof(int32_t));
+        std::memcpy(status, ctx->h_status, n * sizeof(int32_t));
     if (n > 0 && iterations)
-        std::memcpy(iterations, ctx->h_iters.data(), n * sizeof(int32_t));
+        std::memcpy(iterations, ctx->h_iters, n * sizeof(int32_t));
     return EPSEON_OK;
 }
 
@@ -2271,8 +2271,8 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
         if (!ctx->host_levels_valid)
             if (const int rc = fetch_levels(ctx); rc != EPSEON_OK)
                 return rc;
-        const std::vector<int32_t>& it = ctx->h_iters;
-        const std::vector<int32_t>& st = ctx->h_status;
+        const int32_t* it = ctx->h_iters;
+        const int32_t* st = ctx->h_status;
         double pu = 0.0;
         int64_t ok = 0;
         for (size_t p = 0; p < ctx->rec.size(); ++p) {

@@ -45,6 +45,8 @@ hipError_t launch_refine_segmented(hipStream_t, int32_t, int32_t, const double*,
 hipError_t launch_segment_tables(hipStream_t, const double*, const PotentialRecord*, const TableJob*,
                                  int32_t, int32_t, double*);
 hipError_t launch_band_windows(hipStream_t, const double*, const PotentialRecord*, int32_t, int2*);
+hipError_t launch_level_summary(hipStream_t, const PotentialRecord*, int32_t, const int32_t*, const int32_t*,
+                                const int32_t*, int64_t*);
 hipError_t launch_normalize(hipStream_t, const double*, const double*, const PotentialRecord*,
                             const PsiTask*, int32_t, const WaveTask*, int32_t, const double*, const int32_t*,
                             PsiSegMatrix*, PsiCheckpoint*, LevelNorm*);
@@ -390,14 +392,19 @@ struct epseon_ctx {
     // Position of each potential in task order.
     std::vector<int32_t> order_pos;
 
-    // Host views of the last refine's levels, status, iterations and steps:
-    // pointers into the pinned read-back block, set by fetch_levels so that
-    // epseon_get_stats needs no extra copy (valid while host_levels_valid).
+    // Host views of the last refine's levels, status and iterations:
+    // pointers into the pinned read-back block, set by fetch_levels (valid
+    // while host_levels_valid).
     const double* h_levels = nullptr;
     const int32_t* h_status = nullptr;
     const int32_t* h_iters = nullptr;
-    const int32_t* h_steps = nullptr;
     bool host_levels_valid = false;
+    // The last refine's summary (level_summary_kernel: refinement
+    // point-updates, converged levels, eigenfunction steps), read back into
+    // the pinned tail of the level block by epseon_get_stats.
+    int64_t* d_summary = nullptr;
+    int64_t h_summary[3] = {0, 0, 0};
+    bool summary_valid = false;
     // Levels, status, iterations and steps live in one device block
     // (d_level_block, level_block_bytes) read back by one copy into pinned
     // host memory.
@@ -1040,16 +1047,18 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_far));
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_resc));
     {
-        // One block: levels (f64), status, iterations, steps (i32), each
-        // 256-byte aligned.
+        // One block: levels (f64), status, iterations (i32), each 256-byte
+        // aligned, read back whole; then the steps (i32) and the summary
+        // (3 i64), which stay on the device.
         const size_t a8 = (nl * sizeof(double) + 255) & ~size_t(255);
         const size_t a4 = (nl * sizeof(int32_t) + 255) & ~size_t(255);
-        ctx->level_block_bytes = a8 + 3 * a4;
-        EPSEON_HIP_TRY(arena_array(a, ctx->level_block_bytes, &ctx->d_level_block));
+        ctx->level_block_bytes = a8 + 2 * a4;
+        EPSEON_HIP_TRY(arena_array(a, ctx->level_block_bytes + a4 + 256, &ctx->d_level_block));
         ctx->d_levels = reinterpret_cast<double*>(ctx->d_level_block);
         ctx->d_status = reinterpret_cast<int32_t*>(ctx->d_level_block + a8);
         ctx->d_iters = reinterpret_cast<int32_t*>(ctx->d_level_block + a8 + a4);
         ctx->d_steps = reinterpret_cast<int32_t*>(ctx->d_level_block + a8 + 2 * a4);
+        ctx->d_summary = reinterpret_cast<int64_t*>(ctx->d_level_block + a8 + 3 * a4);
     }
     EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_norms));
     EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_grad_levels));
@@ -1552,6 +1561,7 @@ int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     ctx->normalized = false;
     ctx->normalize_timed = ctx->vjp_timed = false;
     ctx->host_levels_valid = false;
+    ctx->summary_valid = false;
     return EPSEON_OK;
 }
 
@@ -1632,6 +1642,7 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
         ctx->normalized = false;
         ctx->normalize_timed = ctx->vjp_timed = false;
         ctx->host_levels_valid = false;
+        ctx->summary_valid = false;
         ctx->last_split_used = 0.0;
         ctx->solve_timed = true;
         return EPSEON_OK;
@@ -1673,6 +1684,7 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     ctx->normalized = false;
     ctx->normalize_timed = ctx->vjp_timed = false;
     ctx->host_levels_valid = false;
+    ctx->summary_valid = false;
     ctx->solve_timed = true;
     ctx->last_split_used = ctx->cum_scan_work[k] / std::max(ctx->cum_scan_work.back(), 1.0);
     return EPSEON_OK;
@@ -1888,14 +1900,15 @@ int epseon_ctx_synchronize(epseon_ctx* ctx)
 namespace {
 
 // Read the level block back (one copy into pinned memory, then a stream
-// synchronisation); h_levels, h_status, h_iters and h_steps point into it.
+// synchronisation); h_levels, h_status and h_iters point into it.  The
+// pinned buffer holds 256 more bytes for the summary (epseon_get_stats).
 int fetch_levels(epseon_ctx* ctx)
 {
     const size_t n = static_cast<size_t>(ctx->total_levels);
     ctx->h_levels = nullptr;
-    ctx->h_status = ctx->h_iters = ctx->h_steps = nullptr;
+    ctx->h_status = ctx->h_iters = nullptr;
     if (n > 0) {
-        const size_t bytes = ctx->level_block_bytes;
+        const size_t bytes = ctx->level_block_bytes + 256;
         if (ctx->h_pinned_bytes < bytes) {
             if (ctx->h_pinned != nullptr) {
                 EPSEON_HIP_TRY(hipHostFree(ctx->h_pinned));
@@ -1905,8 +1918,8 @@ int fetch_levels(epseon_ctx* ctx)
             EPSEON_HIP_TRY(hipHostMalloc(&ctx->h_pinned, bytes, hipHostMallocDefault));
             ctx->h_pinned_bytes = bytes;
         }
-        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->h_pinned, ctx->d_level_block, bytes, hipMemcpyDeviceToHost,
-                                      ctx->stream));
+        EPSEON_HIP_TRY(hipMemcpyAsync(ctx->h_pinned, ctx->d_level_block, ctx->level_block_bytes,
+                                      hipMemcpyDeviceToHost, ctx->stream));
     }
     EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (n > 0) {
@@ -1917,7 +1930,6 @@ int fetch_levels(epseon_ctx* ctx)
         ctx->h_levels = reinterpret_cast<const double*>(h);
         ctx->h_status = reinterpret_cast<const int32_t*>(at(ctx->d_status));
         ctx->h_iters = reinterpret_cast<const int32_t*>(at(ctx->d_iters));
-        ctx->h_steps = reinterpret_cast<const int32_t*>(at(ctx->d_steps));
     }
     ctx->host_levels_valid = true;
     return EPSEON_OK;
@@ -2268,36 +2280,32 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
             EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
             stats->refine_ms = ms;
         }
-        if (!ctx->host_levels_valid)
-            if (const int rc = fetch_levels(ctx); rc != EPSEON_OK)
-                return rc;
-        const int32_t* it = ctx->h_iters;
-        const int32_t* st = ctx->h_status;
-        double pu = 0.0;
-        int64_t ok = 0;
-        for (size_t p = 0; p < ctx->rec.size(); ++p) {
-            const PotentialRecord& r = ctx->rec[p];
-            for (int32_t l = 0; l < r.level_count; ++l) {
-                const size_t o = static_cast<size_t>(r.level_offset + l);
-                if (it[o] > 0)
-                    pu += static_cast<double>(it[o]) * static_cast<double>(ctx->h_steps[o]);
-                ok += st[o] == EPSEON_LEVEL_OK;
+        if (!ctx->summary_valid) {
+            // The summary comes from the device (three integer sums over the
+            // levels), into the pinned buffer's tail.
+            const size_t bytes = ctx->level_block_bytes + 256;
+            if (ctx->h_pinned_bytes < bytes) {
+                if (const int rc = fetch_levels(ctx); rc != EPSEON_OK)
+                    return rc;
             }
+            EPSEON_HIP_TRY(launch_level_summary(ctx->stream, ctx->d_rec, static_cast<int32_t>(ctx->rec.size()),
+                                                ctx->d_status, ctx->d_iters, ctx->d_steps, ctx->d_summary));
+            EPSEON_HIP_TRY(hipMemcpyAsync(static_cast<char*>(ctx->h_pinned) + ctx->level_block_bytes,
+                                          ctx->d_summary, 3 * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                          ctx->stream));
+            EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
+            std::memcpy(ctx->h_summary, static_cast<const char*>(ctx->h_pinned) + ctx->level_block_bytes,
+                        sizeof ctx->h_summary);
+            ctx->summary_valid = true;
         }
-        stats->refine_point_updates = pu;
-        stats->levels_converged = ok;
+        stats->refine_point_updates = static_cast<double>(ctx->h_summary[0]);
+        stats->levels_converged = ctx->h_summary[1];
         stats->refine_mode = ctx->last_refine_mode;
         stats->refine_lanes = ctx->last_refine_lanes;
         stats->refine_levels_per_block = ctx->last_refine_mode == EPSEON_REFINE_SEGMENTED ? ctx->last_refine_levels : 1;
         // Eigenfunction passes shoot every converged level once over its
         // trimmed grid (n - 2 steps split between the two inward shots).
-        double shot_pu = 0.0;
-        for (size_t p = 0; p < ctx->rec.size(); ++p) {
-            const PotentialRecord& r = ctx->rec[p];
-            for (int32_t l = 0; l < r.level_count; ++l)
-                if (st[static_cast<size_t>(r.level_offset + l)] == EPSEON_LEVEL_OK)
-                    shot_pu += static_cast<double>(r.n_points - 2);
-        }
+        const double shot_pu = static_cast<double>(ctx->h_summary[2]);
         if (ctx->normalize_timed) {
             EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
             stats->normalize_ms = ms;

@@ -2435,6 +2435,61 @@ hipError_t launch_segment_tables(hipStream_t stream, const double* uv, const Pot
     return hipGetLastError();
 }
 
+// Refinement summary for the statistics, one wavefront per potential adding
+// its sums to out (zeroed first): out[0] = sum of iterations x steps over
+// levels that shot, out[1] = converged levels, out[2] = sum over converged
+// levels of their grid's n - 2 steps.  Integer sums, so the result does not
+// depend on the order.
+__global__ __launch_bounds__(64) void level_summary_kernel(const PotentialRecord* __restrict__ pots,
+                                                           const int32_t n_pots, const int32_t* __restrict__ status,
+                                                           const int32_t* __restrict__ iterations,
+                                                           const int32_t* __restrict__ steps,
+                                                           unsigned long long* __restrict__ out)
+{
+    const int32_t p = static_cast<int32_t>(blockIdx.x);
+    if (p >= n_pots)
+        return;
+    const PotentialRecord& r = pots[p];
+    int64_t sum[3] = {0, 0, 0};
+    for (int32_t l = static_cast<int32_t>(threadIdx.x); l < r.level_count; l += 64) {
+        const int64_t o = r.level_offset + l;
+        const int32_t it = iterations[o];
+        if (it > 0)
+            sum[0] += static_cast<int64_t>(it) * steps[o];
+        if (status[o] == 0) {
+            sum[1] += 1;
+            sum[2] += r.n_points - 2;
+        }
+    }
+#pragma unroll
+    for (int32_t q = 0; q < 3; ++q) {
+        int64_t x = sum[q];
+#pragma unroll
+        for (int32_t d = 32; d >= 1; d /= 2)
+            x += __shfl_xor(x, d, 64);
+        if (threadIdx.x == 0 && x != 0)
+            atomicAdd(out + q, static_cast<unsigned long long>(x));
+    }
+}
+
+__global__ void zero_summary_kernel(unsigned long long* __restrict__ out)
+{
+    if (threadIdx.x < 3)
+        out[threadIdx.x] = 0;
+}
+
+hipError_t launch_level_summary(hipStream_t stream, const PotentialRecord* pots, int32_t n_pots,
+                                const int32_t* status, const int32_t* iterations, const int32_t* steps,
+                                int64_t* out)
+{
+    unsigned long long* o = reinterpret_cast<unsigned long long*>(out);
+    hipLaunchKernelGGL(zero_summary_kernel, dim3(1), dim3(64), 0, stream, o);
+    if (n_pots > 0)
+        hipLaunchKernelGGL(level_summary_kernel, dim3(static_cast<unsigned>(n_pots)), dim3(64), 0, stream, pots,
+                           n_pots, status, iterations, steps, o);
+    return hipGetLastError();
+}
+
 hipError_t launch_band_windows(hipStream_t stream, const double* uv, const PotentialRecord* pots,
                                int32_t n_pots, int2* out)
 {

@@ -533,9 +533,12 @@ def main() -> int:
     if args.workload == "batched" and not args.no_autotune:
         tuning = ctx.autotune(args.rel_tol, 200).as_dict()
 
+    # Result arrays reused by every step (levels(out=...)).
+    n_levels = max(sum(p.level_count for p in pots), 1)
+    out = (np.zeros(n_levels, np.float64), np.zeros(n_levels, np.int32), np.zeros(n_levels, np.int32))
     for _ in range(args.warmup):
         ctx.solve(args.rel_tol, 200)
-        ctx.levels()
+        ctx.levels(out)
 
     solve_ms: list[float] = []
     solve_split = 0.0
@@ -546,7 +549,7 @@ def main() -> int:
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.solve(args.rel_tol, 200)
-        lv, st, it = ctx.levels()
+        lv, st, it = ctx.levels(out)
         s = ctx.stats()
         solve_ms.append(s.solve_ms)
         solve_split = s.solve_split

@@ -442,12 +442,24 @@ class Context:
     def synchronize(self) -> None:
         _check(self._lib.epseon_ctx_synchronize(self._h))
 
-    def levels(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
-        """(levels, status, iterations), concatenated in potential order."""
+    def levels(
+        self, out: tuple[np.ndarray, np.ndarray, np.ndarray] | None = None
+    ) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(levels, status, iterations), concatenated in potential order.
+        ``out`` may hold three C-contiguous arrays (float64, int32, int32) of
+        at least the total level count to fill instead of new ones, so that a
+        loop of solves reuses its result memory."""
         n = sum(self.level_counts)
-        lv = np.empty(max(n, 1), dtype=np.float64)
-        st = np.empty(max(n, 1), dtype=np.int32)
-        it = np.empty(max(n, 1), dtype=np.int32)
+        if out is None:
+            lv = np.empty(max(n, 1), dtype=np.float64)
+            st = np.empty(max(n, 1), dtype=np.int32)
+            it = np.empty(max(n, 1), dtype=np.int32)
+        else:
+            lv, st, it = out
+            for a, dt in ((lv, np.float64), (st, np.int32), (it, np.int32)):
+                if a.dtype != dt or not a.flags.c_contiguous or a.size < max(n, 1):
+                    raise ValueError("levels(out=...): arrays must be C-contiguous float64, int32, int32 "
+                                     f"with at least {max(n, 1)} elements")
         _check(
             self._lib.epseon_get_levels(
                 self._h,

@@ -142,3 +142,37 @@ def test_graph_follows_reloads(capi):
                 ctx.solve(1e-10, 200)
                 assert ctx.stats().solve_graph == (1 if rep > 0 else 0)
                 _equal(_results(ctx), want)
+
+
+def test_levels_into_reused_arrays_and_summary(capi):
+    """levels(out=...) fills caller arrays (larger ones keep their tail) with
+    the same values as fresh ones; the statistics' device summary counts the
+    converged levels and matches between graph replays and eager solves."""
+    specs = batched(4096)[300:340]
+    counts = [s.bound_levels() + 2 for s in specs]
+    n = sum(counts)
+    _log(f"case: {len(specs)} potentials, {n} levels, reused result arrays")
+    with capi.Context(0) as ctx, capi.Context(0) as ref:
+        _tune(ctx, capi, True, 1, graph=1)
+        _tune(ref, capi, True, 1, graph=0)
+        ctx.load(_pots(capi, specs, counts))
+        ref.load(_pots(capi, specs, counts))
+        ref.solve(1e-10, 200)
+        want = ref.levels()
+        want_stats = ref.stats()
+        out = (np.full(n + 5, -1.0), np.full(n + 5, -7, np.int32), np.full(n + 5, -9, np.int32))
+        for rep in range(3):
+            ctx.solve(1e-10, 200)
+            got = ctx.levels(out)
+            for g, w in zip(got, want):
+                np.testing.assert_array_equal(np.asarray(g)[:n].view(np.int64 if g.dtype == np.float64 else np.int32),
+                                              np.asarray(w).view(np.int64 if w.dtype == np.float64 else np.int32))
+            assert np.all(out[0][n:] == -1.0) and np.all(out[1][n:] == -7) and np.all(out[2][n:] == -9)
+            s = ctx.stats()
+            assert s.levels_converged == int(np.sum(want[1] == capi.LEVEL_OK))
+            assert s.refine_point_updates == want_stats.refine_point_updates > 0
+            assert s.levels_converged == want_stats.levels_converged
+        with pytest.raises(ValueError):
+            ctx.levels((np.empty(n), np.empty(n, np.int64), np.empty(n, np.int32)))
+        with pytest.raises(ValueError):
+            ctx.levels((np.empty(n - 1), np.empty(n, np.int32), np.empty(n, np.int32)))

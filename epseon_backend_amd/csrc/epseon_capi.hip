@@ -399,12 +399,11 @@ struct epseon_ctx {
     const int32_t* h_status = nullptr;
     const int32_t* h_iters = nullptr;
     bool host_levels_valid = false;
-    // The last refine's summary (level_summary_kernel: refinement
-    // point-updates, converged levels, eigenfunction steps), read back into
-    // the pinned tail of the level block by epseon_get_stats.
+    // The last refine's summary (level_summary_kernel, enqueued after every
+    // refinement: refinement point-updates, converged levels, eigenfunction
+    // steps), read back with the levels.
     int64_t* d_summary = nullptr;
     int64_t h_summary[3] = {0, 0, 0};
-    bool summary_valid = false;
     // Levels, status, iterations and steps live in one device block
     // (d_level_block, level_block_bytes) read back by one copy into pinned
     // host memory.
@@ -1047,18 +1046,18 @@ int load_batch(epseon_ctx* ctx, int32_t count, const epseon_potential_desc* desc
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_far));
     EPSEON_HIP_TRY(arena_array(a, ne, &ctx->d_resc));
     {
-        // One block: levels (f64), status, iterations (i32), each 256-byte
-        // aligned, read back whole; then the steps (i32) and the summary
-        // (3 i64), which stay on the device.
+        // One block: levels (f64), status, iterations (i32) and the summary
+        // (3 i64), each 256-byte aligned, read back whole; then the steps
+        // (i32), which stay on the device.
         const size_t a8 = (nl * sizeof(double) + 255) & ~size_t(255);
         const size_t a4 = (nl * sizeof(int32_t) + 255) & ~size_t(255);
-        ctx->level_block_bytes = a8 + 2 * a4;
-        EPSEON_HIP_TRY(arena_array(a, ctx->level_block_bytes + a4 + 256, &ctx->d_level_block));
+        ctx->level_block_bytes = a8 + 2 * a4 + 256;
+        EPSEON_HIP_TRY(arena_array(a, ctx->level_block_bytes + a4, &ctx->d_level_block));
         ctx->d_levels = reinterpret_cast<double*>(ctx->d_level_block);
         ctx->d_status = reinterpret_cast<int32_t*>(ctx->d_level_block + a8);
         ctx->d_iters = reinterpret_cast<int32_t*>(ctx->d_level_block + a8 + a4);
-        ctx->d_steps = reinterpret_cast<int32_t*>(ctx->d_level_block + a8 + 2 * a4);
-        ctx->d_summary = reinterpret_cast<int64_t*>(ctx->d_level_block + a8 + 3 * a4);
+        ctx->d_summary = reinterpret_cast<int64_t*>(ctx->d_level_block + a8 + 2 * a4);
+        ctx->d_steps = reinterpret_cast<int32_t*>(ctx->d_level_block + ctx->level_block_bytes);
     }
     EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_norms));
     EPSEON_HIP_TRY(arena_array(a, nl, &ctx->d_grad_levels));
@@ -1473,6 +1472,14 @@ size_t split_point(const epseon_ctx* ctx, double fraction)
     return k;
 }
 
+// The refinement summary of the statistics, on the context's stream after
+// every refinement (outside any recorded graph).
+hipError_t enqueue_summary(epseon_ctx* ctx)
+{
+    return launch_level_summary(ctx->stream, ctx->d_rec, static_cast<int32_t>(ctx->rec.size()), ctx->d_status,
+                                ctx->d_iters, ctx->d_steps, ctx->d_summary);
+}
+
 // Record the single-stream solve (scan, brackets, refinement of the whole
 // batch) on the context's stream as a graph and instantiate it.  The
 // kernels take only device pointers of this load and by-value arguments;
@@ -1556,12 +1563,12 @@ int epseon_refine(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
                                   n_potentials(ctx)));
     }
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
+    EPSEON_HIP_TRY(enqueue_summary(ctx));
     ctx->refined = true;
     ctx->refine_timed = true;
     ctx->normalized = false;
     ctx->normalize_timed = ctx->vjp_timed = false;
     ctx->host_levels_valid = false;
-    ctx->summary_valid = false;
     return EPSEON_OK;
 }
 
@@ -1637,12 +1644,12 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
             EPSEON_HIP_TRY(run_refine(ctx, ctx->stream, ctx->stream, mode, lanes, rel_tol, max_iter, 0, np));
             EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve1, ctx->stream));
         }
+        EPSEON_HIP_TRY(enqueue_summary(ctx));
         ctx->scanned = ctx->refined = true;
         ctx->scan_timed = ctx->refine_timed = false;
         ctx->normalized = false;
         ctx->normalize_timed = ctx->vjp_timed = false;
         ctx->host_levels_valid = false;
-        ctx->summary_valid = false;
         ctx->last_split_used = 0.0;
         ctx->solve_timed = true;
         return EPSEON_OK;
@@ -1679,12 +1686,12 @@ int epseon_solve(epseon_ctx* ctx, double rel_tol, int32_t max_iter)
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev_join, ctx->stream2));
     EPSEON_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     EPSEON_HIP_TRY(hipEventRecord(ctx->ev_solve1, ctx->stream));
+    EPSEON_HIP_TRY(enqueue_summary(ctx));
     ctx->scanned = ctx->refined = true;
     ctx->scan_timed = ctx->refine_timed = false;
     ctx->normalized = false;
     ctx->normalize_timed = ctx->vjp_timed = false;
     ctx->host_levels_valid = false;
-    ctx->summary_valid = false;
     ctx->solve_timed = true;
     ctx->last_split_used = ctx->cum_scan_work[k] / std::max(ctx->cum_scan_work.back(), 1.0);
     return EPSEON_OK;
@@ -1900,15 +1907,15 @@ int epseon_ctx_synchronize(epseon_ctx* ctx)
 namespace {
 
 // Read the level block back (one copy into pinned memory, then a stream
-// synchronisation); h_levels, h_status and h_iters point into it.  The
-// pinned buffer holds 256 more bytes for the summary (epseon_get_stats).
+// synchronisation); h_levels, h_status and h_iters point into it, and the
+// summary is copied out of it.
 int fetch_levels(epseon_ctx* ctx)
 {
     const size_t n = static_cast<size_t>(ctx->total_levels);
     ctx->h_levels = nullptr;
     ctx->h_status = ctx->h_iters = nullptr;
     if (n > 0) {
-        const size_t bytes = ctx->level_block_bytes + 256;
+        const size_t bytes = ctx->level_block_bytes;
         if (ctx->h_pinned_bytes < bytes) {
             if (ctx->h_pinned != nullptr) {
                 EPSEON_HIP_TRY(hipHostFree(ctx->h_pinned));
@@ -1930,6 +1937,7 @@ int fetch_levels(epseon_ctx* ctx)
         ctx->h_levels = reinterpret_cast<const double*>(h);
         ctx->h_status = reinterpret_cast<const int32_t*>(at(ctx->d_status));
         ctx->h_iters = reinterpret_cast<const int32_t*>(at(ctx->d_iters));
+        std::memcpy(ctx->h_summary, at(ctx->d_summary), sizeof ctx->h_summary);
     }
     ctx->host_levels_valid = true;
     return EPSEON_OK;
@@ -2280,24 +2288,11 @@ int epseon_get_stats(epseon_ctx* ctx, epseon_stats* stats)
             EPSEON_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
             stats->refine_ms = ms;
         }
-        if (!ctx->summary_valid) {
-            // The summary comes from the device (three integer sums over the
-            // levels), into the pinned buffer's tail.
-            const size_t bytes = ctx->level_block_bytes + 256;
-            if (ctx->h_pinned_bytes < bytes) {
-                if (const int rc = fetch_levels(ctx); rc != EPSEON_OK)
-                    return rc;
-            }
-            EPSEON_HIP_TRY(launch_level_summary(ctx->stream, ctx->d_rec, static_cast<int32_t>(ctx->rec.size()),
-                                                ctx->d_status, ctx->d_iters, ctx->d_steps, ctx->d_summary));
-            EPSEON_HIP_TRY(hipMemcpyAsync(static_cast<char*>(ctx->h_pinned) + ctx->level_block_bytes,
-                                          ctx->d_summary, 3 * sizeof(int64_t), hipMemcpyDeviceToHost,
-                                          ctx->stream));
-            EPSEON_HIP_TRY(hipStreamSynchronize(ctx->stream));
-            std::memcpy(ctx->h_summary, static_cast<const char*>(ctx->h_pinned) + ctx->level_block_bytes,
-                        sizeof ctx->h_summary);
-            ctx->summary_valid = true;
-        }
+        // The summary (three integer sums over the levels, computed on the
+        // device after the refinement) comes back with the levels.
+        if (!ctx->host_levels_valid)
+            if (const int rc = fetch_levels(ctx); rc != EPSEON_OK)
+                return rc;
         stats->refine_point_updates = static_cast<double>(ctx->h_summary[0]);
         stats->levels_converged = ctx->h_summary[1];
         stats->refine_mode = ctx->last_refine_mode;

@@ -2435,30 +2435,32 @@ hipError_t launch_segment_tables(hipStream_t stream, const double* uv, const Pot
     return hipGetLastError();
 }
 
-// Refinement summary for the statistics, one wavefront per potential adding
-// its sums to out (zeroed first): out[0] = sum of iterations x steps over
-// levels that shot, out[1] = converged levels, out[2] = sum over converged
-// levels of their grid's n - 2 steps.  Integer sums, so the result does not
-// depend on the order.
+// Refinement summary for the statistics, one wavefront per kSummaryPots
+// potentials adding its sums to out (zeroed first): out[0] = sum of
+// iterations x steps over levels that shot, out[1] = converged levels,
+// out[2] = sum over converged levels of their grid's n - 2 steps.  Integer
+// sums, so the result does not depend on the order.
+constexpr int32_t kSummaryPots = 16;
 __global__ __launch_bounds__(64) void level_summary_kernel(const PotentialRecord* __restrict__ pots,
                                                            const int32_t n_pots, const int32_t* __restrict__ status,
                                                            const int32_t* __restrict__ iterations,
                                                            const int32_t* __restrict__ steps,
                                                            unsigned long long* __restrict__ out)
 {
-    const int32_t p = static_cast<int32_t>(blockIdx.x);
-    if (p >= n_pots)
-        return;
-    const PotentialRecord& r = pots[p];
+    const int32_t p0 = static_cast<int32_t>(blockIdx.x) * kSummaryPots;
+    const int32_t p1 = min(n_pots, p0 + kSummaryPots);
     int64_t sum[3] = {0, 0, 0};
-    for (int32_t l = static_cast<int32_t>(threadIdx.x); l < r.level_count; l += 64) {
-        const int64_t o = r.level_offset + l;
-        const int32_t it = iterations[o];
-        if (it > 0)
-            sum[0] += static_cast<int64_t>(it) * steps[o];
-        if (status[o] == 0) {
-            sum[1] += 1;
-            sum[2] += r.n_points - 2;
+    for (int32_t p = p0; p < p1; ++p) {
+        const PotentialRecord& r = pots[p];
+        for (int32_t l = static_cast<int32_t>(threadIdx.x); l < r.level_count; l += 64) {
+            const int64_t o = r.level_offset + l;
+            const int32_t it = iterations[o];
+            if (it > 0)
+                sum[0] += static_cast<int64_t>(it) * steps[o];
+            if (status[o] == 0) {
+                sum[1] += 1;
+                sum[2] += r.n_points - 2;
+            }
         }
     }
 #pragma unroll
@@ -2485,8 +2487,8 @@ hipError_t launch_level_summary(hipStream_t stream, const PotentialRecord* pots,
     unsigned long long* o = reinterpret_cast<unsigned long long*>(out);
     hipLaunchKernelGGL(zero_summary_kernel, dim3(1), dim3(64), 0, stream, o);
     if (n_pots > 0)
-        hipLaunchKernelGGL(level_summary_kernel, dim3(static_cast<unsigned>(n_pots)), dim3(64), 0, stream, pots,
-                           n_pots, status, iterations, steps, o);
+        hipLaunchKernelGGL(level_summary_kernel, dim3(static_cast<unsigned>((n_pots + kSummaryPots - 1) / kSummaryPots)),
+                           dim3(64), 0, stream, pots, n_pots, status, iterations, steps, o);
     return hipGetLastError();
 }
 

@@ -172,6 +172,26 @@ def test_levels_into_reused_arrays_and_summary(capi):
             assert s.levels_converged == int(np.sum(want[1] == capi.LEVEL_OK))
             assert s.refine_point_updates == want_stats.refine_point_updates > 0
             assert s.levels_converged == want_stats.levels_converged
+        # The summary restated on the host: iterations x window steps of every
+        # level that shot (segmented shots walk their band's window), and the
+        # trimmed grid's steps of every converged level.
+        br = ctx.brackets()
+        _, st, it = want
+        pu = 0
+        shot = 0
+        o = 0
+        for p, c in enumerate(counts):
+            lo, hi, _ = ctx.band_windows(p)
+            info = ctx.potential_info(p)
+            for l in range(c):
+                if it[o + l] > 0:
+                    pu += int(it[o + l]) * int(hi[br["band"][o + l]] - lo[br["band"][o + l]] - 1)
+                if st[o + l] == capi.LEVEL_OK:
+                    shot += int(info.point_count) - 2
+            o += c
+        assert want_stats.refine_point_updates == pu
+        ctx.normalize()
+        assert ctx.stats().normalize_point_updates == shot
         with pytest.raises(ValueError):
             ctx.levels((np.empty(n), np.empty(n, np.int64), np.empty(n, np.int32)))
         with pytest.raises(ValueError):

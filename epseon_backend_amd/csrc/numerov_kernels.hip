@@ -636,7 +636,7 @@ __device__ __forceinline__ int32_t fringe_end(const double* __restrict__ u, cons
 // pass over the table finds the first and last allowed point (uv_i <= e_b)
 // of all kBands band tops at once; then the wavefronts walk the 2 kBands
 // fringes.  Writes (lo, hi) per band (whole grid for rejected potentials).
-constexpr int32_t kBandWaves = 8;
+constexpr int32_t kBandWaves = 16;
 __global__ __launch_bounds__(64 * kBandWaves) void band_window_kernel(const double* __restrict__ uv,
                                                           const PotentialRecord* __restrict__ pots,
                                                           const int32_t n_pots, int2* __restrict__ out)

@@ -304,3 +304,47 @@ def test_brackets_match_oracle(capi, windows):
         eo += s.m
         lo += counts[i]
     assert n_found > 1000
+
+
+@pytest.mark.parametrize("mode", ["lane", "segmented"])
+def test_every_node_stride_matches_oracle(capi, mode):
+    """Two batch potentials of each node-sampling stride (1, 2, 4, 8, 16 and
+    32 steps between sign samples; the counter packs the samples' sign bits
+    and flushes them on a stride-dependent schedule) bit-exact against the
+    oracle, whole-grid lane shots or windowed segmented shots."""
+    specs = batched(4096)
+    picked: dict[int, list[int]] = {}
+    for i, s in enumerate(specs):
+        prob = problem_from_spec(s)
+        if prob.usable and len(picked.setdefault(prob.stride, [])) < 2:
+            picked[prob.stride].append(i)
+        if len(picked) == 6 and all(len(v) == 2 for v in picked.values()):
+            break
+    assert sorted(picked) == [1, 2, 4, 8, 16, 32]
+    idx = [i for b in sorted(picked) for i in picked[b]]
+    sub = [specs[i] for i in idx]
+    _log(f"case: {mode}, potentials {idx} (two per stride)")
+    counts = [s.bound_levels() + 2 for s in sub]
+    tables = [s.table() for s in sub]
+    seg = mode == "segmented"
+    with capi.Context(0) as ctx:
+        t = ctx.tuning()
+        t.refine_mode = capi.REFINE_SEGMENTED if seg else capi.REFINE_LANE
+        t.levels_per_lane = 1
+        t.windows = 1 if seg else 0
+        t.segmented_lanes = 64
+        t.solve_split = 0.0
+        ctx.set_tuning(t)
+        ctx.load([_pot(capi, s, tb, c) for s, tb, c in zip(sub, tables, counts)])
+        ctx.solve(1e-10, 200)
+        nodes, far, resc = ctx.scan_results()
+        lv, st, it = ctx.levels()
+        lanes = ctx.stats().refine_lanes
+    eo = lo = 0
+    for i, s in enumerate(sub):
+        prob = problem_from_spec(s)
+        prob.v = tables[i]
+        _compare(prob, nodes[eo : eo + s.m], far[eo : eo + s.m], resc[eo : eo + s.m],
+                 lv[lo : lo + counts[i]], it[lo : lo + counts[i]], counts[i], seg, lanes)
+        eo += s.m
+        lo += counts[i]
